@@ -1,0 +1,556 @@
+// C ABI of idefix_amd (declared in include/idefix_amd.h): context management, dispatch
+// of the runtime configuration to the compiled kernel configurations, and the host-side
+// sequencing of a Runge-Kutta stage and cycle (timeIntegrator.cpp:246-414).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/idefix_amd.h"
+#include "launch.hpp"
+
+namespace idfx {
+// defined by the instantiation units
+void register_hd1(Ops *t);
+void register_hd2(Ops *t);
+void register_hd3(Ops *t);
+void register_mhd2_f64(Ops *t);
+void register_mhd2_f32(Ops *t);
+void register_mhd3_plm_f64(Ops *t);
+void register_mhd3_ppm_f64(Ops *t);
+void register_mhd3_f32(Ops *t);
+
+Ops *ops_table() {
+  static Ops *table = [] {
+    Ops *t = new Ops[kOpsTableSize];
+    register_hd1(t);
+    register_hd2(t);
+    register_hd3(t);
+    register_mhd2_f64(t);
+    register_mhd2_f32(t);
+    register_mhd3_plm_f64(t);
+    register_mhd3_ppm_f64(t);
+    register_mhd3_f32(t);
+    return t;
+  }();
+  return table;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ULL; }
+__global__ void k_fill(double *p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+}  // namespace idfx
+
+using namespace idfx;
+
+struct idfx_ctx {
+  idfx_desc d;
+  GridDesc g;
+  const Ops *ops = nullptr;
+  int np_tot[3], nghost[3], nvar, vsn[3];
+  long long n3 = 0, nvs = 0;
+  std::vector<void *> allocs;
+  EmfArrays<double> e64{};
+  EmfArrays<float> e32{};
+  double *cMax = nullptr;      // flux-only mode
+  double *Flux = nullptr;      // FluxRiemann scratch for the unfused stage path
+  double *InvDt = nullptr;     // used when the caller passes no InvDt
+  double *Uc0 = nullptr, *Vs0 = nullptr;  // Runge-Kutta "begin" state
+  unsigned long long *dtmax = nullptr, *divb = nullptr;  // device scalars
+  unsigned long long *host = nullptr;                    // pinned readback
+  hipEvent_t ev_dt = nullptr;
+  int target_waves = 8192;
+  bool fp32 = false;
+  const void *emf() const { return fp32 ? (const void *)&e32 : (const void *)&e64; }
+};
+
+namespace {
+
+int check(hipError_t e) { return e == hipSuccess ? IDFX_OK : IDFX_E_LAUNCH; }
+#define IDFX_TRY(x)                  \
+  do {                               \
+    int _s = (x);                    \
+    if (_s != IDFX_OK) return _s;    \
+  } while (0)
+
+template <typename T>
+int alloc_n(idfx_ctx *c, T **p, long long n) {
+  void *q = nullptr;
+  if (hipMalloc(&q, sizeof(T) * (size_t)std::max<long long>(n, 1)) != hipSuccess) return IDFX_E_ALLOC;
+  c->allocs.push_back(q);
+  *p = static_cast<T *>(q);
+  // scratch starts at zero, as Kokkos views do
+  if (hipMemset(q, 0, sizeof(T) * (size_t)std::max<long long>(n, 1)) != hipSuccess) return IDFX_E_ALLOC;
+  return IDFX_OK;
+}
+
+template <typename T>
+int alloc_emf(idfx_ctx *c, EmfArrays<T> &e) {
+  const int dim = c->d.dim;
+  const int emf = c->d.emf;
+  const long long n = c->n3;
+  IDFX_TRY(alloc_n(c, &e.ez, n));
+  IDFX_TRY(alloc_n(c, &e.ezi, n));
+  IDFX_TRY(alloc_n(c, &e.ezj, n));
+  if (dim == 3) {
+    IDFX_TRY(alloc_n(c, &e.ex, n));
+    IDFX_TRY(alloc_n(c, &e.ey, n));
+    IDFX_TRY(alloc_n(c, &e.exj, n));
+    IDFX_TRY(alloc_n(c, &e.exk, n));
+    IDFX_TRY(alloc_n(c, &e.eyi, n));
+    IDFX_TRY(alloc_n(c, &e.eyk, n));
+  } else if (c->d.comps == 3) {
+    IDFX_TRY(alloc_n(c, &e.eyi, n));
+    IDFX_TRY(alloc_n(c, &e.exj, n));
+  }
+  if (emf == IDFX_EMF_UCT_CONTACT) {
+    IDFX_TRY(alloc_n(c, &e.svx, n));
+    if (dim >= 2) IDFX_TRY(alloc_n(c, &e.svy, n));
+    if (dim >= 3) IDFX_TRY(alloc_n(c, &e.svz, n));
+  }
+  if (emf == IDFX_EMF_UCT_HLL || emf == IDFX_EMF_UCT_HLLD) {
+    IDFX_TRY(alloc_n(c, &e.axL, n));
+    IDFX_TRY(alloc_n(c, &e.axR, n));
+    IDFX_TRY(alloc_n(c, &e.dxL, n));
+    IDFX_TRY(alloc_n(c, &e.dxR, n));
+    if (dim >= 2) {
+      IDFX_TRY(alloc_n(c, &e.ayL, n));
+      IDFX_TRY(alloc_n(c, &e.ayR, n));
+      IDFX_TRY(alloc_n(c, &e.dyL, n));
+      IDFX_TRY(alloc_n(c, &e.dyR, n));
+    }
+    if (dim >= 3) {
+      IDFX_TRY(alloc_n(c, &e.azL, n));
+      IDFX_TRY(alloc_n(c, &e.azR, n));
+      IDFX_TRY(alloc_n(c, &e.dzL, n));
+      IDFX_TRY(alloc_n(c, &e.dzR, n));
+    }
+  }
+  return IDFX_OK;
+}
+
+SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const double *Vs,
+                     double *Uc, double *InvDt, int flags) {
+  SweepArgs a{};
+  a.Vc = Vc;
+  a.Vs = Vs;
+  a.Uc = Uc;
+  a.Uc0 = c->Uc0;
+  a.InvDt = InvDt ? InvDt : c->InvDt;
+  a.Flux = nullptr;
+  a.cMax = c->cMax;
+  a.dt = dt;
+  a.emf = c->d.emf;
+  a.flags = flags;
+  a.dtmax = c->dtmax;
+  for (int d = 0; d < 3; d++) a.ext[d] = 0;
+  if (c->d.mhd) {
+    // face range extension across DIR required by CT (hlldMHD.hpp:32-49)
+    int perp = 1;
+    if (c->d.emf == IDFX_EMF_UCT_HLL || c->d.emf == IDFX_EMF_UCT_HLLD) perp = c->nghost[dir];
+    for (int d = 0; d < c->d.dim; d++) a.ext[d] = (d == dir) ? 0 : perp;
+  }
+  return a;
+}
+
+bool needs_unfused(const idfx_ctx *c) {
+  // the 1-D Roe flux keeps the previous flux at strong shocks (roeHD.hpp:315-328)
+  return c->d.dim == 1 && !c->d.mhd && c->d.solver == IDFX_SOLVER_ROE;
+}
+
+int do_sweep_unfused(idfx_ctx *c, int dir, double dt, const double *Vc, const double *Vs,
+                     double *Uc, double *InvDt, double *Flux, hipStream_t s) {
+  SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, 0);
+  a.Flux = Flux;
+  IDFX_TRY(check(c->ops->sweep(dir, false, c->g, a, c->emf(), c->target_waves, s)));
+  IDFX_TRY(check(c->ops->correct_flux(dir, c->g, Flux, s)));
+  IDFX_TRY(check(c->ops->rhs(dir, c->g, Flux, c->cMax, Uc, a.InvDt, dt, s)));
+  return IDFX_OK;
+}
+
+int read_u64(idfx_ctx *c, unsigned long long *dev, unsigned long long *out, hipStream_t s) {
+  IDFX_TRY(check(hipMemcpyAsync(c->host, dev, sizeof(unsigned long long), hipMemcpyDeviceToHost, s)));
+  IDFX_TRY(check(hipStreamSynchronize(s)));
+  *out = *c->host;
+  return IDFX_OK;
+}
+
+double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *idfx_strerror(int s) {
+  switch (s) {
+    case IDFX_OK: return "ok";
+    case IDFX_E_ARGS: return "invalid argument or unsupported configuration";
+    case IDFX_E_LAUNCH: return "HIP launch or runtime error";
+    case IDFX_E_NAN: return "Nan found after integration cycle";
+    case IDFX_E_DT_SMALL: return "dt is too small";
+    case IDFX_E_FIRST_DT: return "Your guessed first_dt is too large";
+    case IDFX_E_ALLOC: return "device allocation failed";
+    default: return "unknown status";
+  }
+}
+
+int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
+                    const double *dx3, idfx_ctx **out) {
+  if (!desc || !out) return IDFX_E_ARGS;
+  *out = nullptr;
+  const idfx_desc &d = *desc;
+  if (d.dim < 1 || d.dim > 3 || d.comps < d.dim || d.comps > 3) return IDFX_E_ARGS;
+  if (d.recon != 1 && d.recon != 2 && d.recon != 4) return IDFX_E_ARGS;
+  if (d.mhd && d.solver != IDFX_SOLVER_HLLD) return IDFX_E_ARGS;  // Roe MHD: not built
+  if (!d.mhd && d.solver != IDFX_SOLVER_HLLC && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
+  if (d.emf < 0 || d.emf > 4) return IDFX_E_ARGS;
+  const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver, d.precision == IDFX_PREC_FP32);
+  if (key < 0 || key >= kOpsTableSize) return IDFX_E_ARGS;
+  const Ops *ops = &ops_table()[key];
+  if (!ops->valid) return IDFX_E_ARGS;
+  idfx_ctx *c = new idfx_ctx();
+  c->d = d;
+  c->ops = ops;
+  c->fp32 = d.precision == IDFX_PREC_FP32;
+  const double *dxh[3] = {dx1, dx2, dx3};
+  for (int q = 0; q < 3; q++) {
+    const bool act = q < d.dim;
+    if (act && d.nx[q] < 1) {
+      delete c;
+      return IDFX_E_ARGS;
+    }
+    c->nghost[q] = act ? (d.recon < 4 ? 2 : 3) : 0;
+    c->np_tot[q] = (act ? d.nx[q] : 1) + 2 * c->nghost[q];
+    c->vsn[q] = c->np_tot[q] + (act ? 1 : 0);
+    c->g.nt[q] = c->np_tot[q];
+    c->g.beg[q] = c->nghost[q];
+    c->g.end[q] = c->nghost[q] + (act ? d.nx[q] : 1);
+    c->g.vsn[q] = c->vsn[q];
+  }
+  c->nvar = ops->nvar;
+  c->n3 = (long long)c->np_tot[0] * c->np_tot[1] * c->np_tot[2];
+  c->nvs = d.mhd ? (long long)d.dim * c->vsn[0] * c->vsn[1] * c->vsn[2] : 0;
+  c->g.sv = c->n3;
+  c->g.svs = (long long)c->vsn[0] * c->vsn[1] * c->vsn[2];
+  c->g.gamma = d.gamma;
+  int st = IDFX_OK;
+  for (int q = 0; q < 3 && st == IDFX_OK; q++) {
+    double *p = nullptr;
+    st = alloc_n(c, &p, c->np_tot[q]);
+    if (st != IDFX_OK) break;
+    std::vector<double> h(c->np_tot[q], 1.0);
+    if (dxh[q]) std::memcpy(h.data(), dxh[q], sizeof(double) * c->np_tot[q]);
+    if (hipMemcpy(p, h.data(), sizeof(double) * c->np_tot[q], hipMemcpyHostToDevice) != hipSuccess)
+      st = IDFX_E_LAUNCH;
+    c->g.dx[q] = p;
+  }
+  if (st == IDFX_OK && d.mhd) st = c->fp32 ? alloc_emf(c, c->e32) : alloc_emf(c, c->e64);
+  if (st == IDFX_OK) st = alloc_n(c, &c->cMax, c->n3);
+  if (st == IDFX_OK) st = alloc_n(c, &c->InvDt, c->n3);
+  if (st == IDFX_OK) st = alloc_n(c, &c->Uc0, c->n3 * c->nvar);
+  if (st == IDFX_OK && d.mhd) st = alloc_n(c, &c->Vs0, c->nvs);
+  if (st == IDFX_OK && needs_unfused(c)) st = alloc_n(c, &c->Flux, c->n3 * c->nvar);
+  if (st == IDFX_OK) st = alloc_n(c, &c->dtmax, 1);
+  if (st == IDFX_OK) st = alloc_n(c, &c->divb, 1);
+  if (st == IDFX_OK) {
+    void *h = nullptr;
+    if (hipHostMalloc(&h, 64, hipHostMallocDefault) != hipSuccess) st = IDFX_E_ALLOC;
+    c->host = static_cast<unsigned long long *>(h);
+  }
+  if (st == IDFX_OK && hipEventCreateWithFlags(&c->ev_dt, hipEventDisableTiming) != hipSuccess)
+    st = IDFX_E_LAUNCH;
+  if (st == IDFX_OK) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    c->target_waves = ncu * 4 * 8;
+  }
+  if (st != IDFX_OK) {
+    idfx_ctx_destroy(c);
+    return st;
+  }
+  *out = c;
+  return IDFX_OK;
+}
+
+int idfx_ctx_destroy(idfx_ctx *c) {
+  if (!c) return IDFX_OK;
+  hipDeviceSynchronize();
+  for (void *p : c->allocs) hipFree(p);
+  if (c->host) hipHostFree(c->host);
+  if (c->ev_dt) hipEventDestroy(c->ev_dt);
+  delete c;
+  return IDFX_OK;
+}
+
+int idfx_ctx_layout(const idfx_ctx *c, int *np_tot, int *nghost, int *nvar, int *vs_shape) {
+  if (!c) return IDFX_E_ARGS;
+  for (int q = 0; q < 3; q++) {
+    if (np_tot) np_tot[q] = c->np_tot[q];
+    if (nghost) nghost[q] = c->nghost[q];
+    if (vs_shape) vs_shape[q] = c->vsn[q];
+  }
+  if (nvar) *nvar = c->nvar;
+  return IDFX_OK;
+}
+
+int idfx_ctx_array(idfx_ctx *c, int which, double **ptr) {
+  if (!c || !ptr || c->fp32) return IDFX_E_ARGS;
+  if (which == 24) {
+    *ptr = c->cMax;
+    return IDFX_OK;
+  }
+  double *const arr[24] = {c->e64.ex,  c->e64.ey,  c->e64.ez,  c->e64.ezi, c->e64.ezj, c->e64.exj,
+                           c->e64.exk, c->e64.eyi, c->e64.eyk, c->e64.svx, c->e64.svy, c->e64.svz,
+                           c->e64.axL, c->e64.axR, c->e64.ayL, c->e64.ayR, c->e64.azL, c->e64.azR,
+                           c->e64.dxL, c->e64.dxR, c->e64.dyL, c->e64.dyR, c->e64.dzL, c->e64.dzR};
+  if (which < 0 || which >= 24) return IDFX_E_ARGS;
+  *ptr = arr[which];
+  return *ptr ? IDFX_OK : IDFX_E_ARGS;
+}
+
+int idfx_set_boundaries(idfx_ctx *c, double *Vc, double *Vs, void *stream) {
+  if (!c || !Vc || (c->d.mhd && !Vs)) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  for (int dir = 0; dir < c->d.dim; dir++) {
+    IDFX_TRY(check(c->ops->boundary(c->g, Vc, Vs, dir, c->d.lbound[dir], c->d.rbound[dir], s)));
+    if (c->d.mhd) {
+      const int recL = c->d.lbound[dir] != IDFX_BC_PERIODIC && c->d.lbound[dir] != IDFX_BC_USERDEF;
+      const int recR = c->d.rbound[dir] != IDFX_BC_PERIODIC && c->d.rbound[dir] != IDFX_BC_USERDEF;
+      if (recL || recR) IDFX_TRY(check(c->ops->normal_field(c->g, Vs, dir, recL, recR, s)));
+    }
+  }
+  if (c->d.mhd) IDFX_TRY(check(c->ops->vc_field(c->g, Vs, Vc, s)));
+  return IDFX_OK;
+}
+
+int idfx_prim_to_cons(idfx_ctx *c, const double *Vc, double *Uc, void *stream) {
+  if (!c || !Vc || !Uc) return IDFX_E_ARGS;
+  return check(c->ops->prim_to_cons(c->g, Vc, Uc, (hipStream_t)stream));
+}
+
+int idfx_cons_to_prim(idfx_ctx *c, double *Uc, double *Vc, const double *Vs, void *stream) {
+  if (!c || !Uc || !Vc || (c->d.mhd && !Vs)) return IDFX_E_ARGS;
+  const int lo[3] = {0, 0, 0};
+  return check(c->ops->cons_to_prim(c->g, Uc, nullptr, Vc, Vs, lo, c->np_tot, 2, 0.0, 0.0,
+                                    (hipStream_t)stream));
+}
+
+int idfx_reconstruct_vc_field(idfx_ctx *c, const double *Vs, double *X, void *stream) {
+  if (!c || !Vs || !X) return IDFX_E_ARGS;
+  return check(c->ops->vc_field(c->g, Vs, X, (hipStream_t)stream));
+}
+
+int idfx_reset_stage(idfx_ctx *c, double *InvDt, void *stream) {
+  if (!c) return IDFX_E_ARGS;
+  double *p = InvDt ? InvDt : c->InvDt;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(c->n3)), dim3(256), 0, (hipStream_t)stream, p, c->n3, 0.0);
+  return check(hipGetLastError());
+}
+
+int idfx_calc_flux(idfx_ctx *c, int dir, const double *Vc, const double *Vs, double *Flux,
+                   void *stream) {
+  if (!c || !Vc || !Flux || dir < 0 || dir >= c->d.dim || (c->d.mhd && !Vs) || c->fp32)
+    return IDFX_E_ARGS;
+  SweepArgs a = sweep_args(c, dir, 0.0, Vc, Vs, nullptr, nullptr, 0);
+  a.Flux = Flux;
+  return check(c->ops->sweep(dir, false, c->g, a, c->emf(), c->target_waves, (hipStream_t)stream));
+}
+
+int idfx_calc_rhs(idfx_ctx *c, int dir, double dt, double *Flux, double *Uc, double *InvDt,
+                  void *stream) {
+  if (!c || !Flux || !Uc || dir < 0 || dir >= c->d.dim) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  IDFX_TRY(check(c->ops->correct_flux(dir, c->g, Flux, s)));
+  return check(c->ops->rhs(dir, c->g, Flux, c->cMax, Uc, InvDt ? InvDt : c->InvDt, dt, s));
+}
+
+int idfx_sweep(idfx_ctx *c, int dir, double dt, const double *Vc, const double *Vs, double *Uc,
+               double *InvDt, int flags, void *stream) {
+  if (!c || !Vc || !Uc || dir < 0 || dir >= c->d.dim || (c->d.mhd && !Vs)) return IDFX_E_ARGS;
+  if (needs_unfused(c)) return IDFX_E_ARGS;
+  const int f = flags & (SW_INIT_UC | SW_SAVE_UC0 | SW_INVDT_SET | SW_INVDT_ADD);
+  SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, f);
+  return check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, (hipStream_t)stream));
+}
+
+int idfx_corner_emf(idfx_ctx *c, const double *Vc, const double *Vs, void *stream) {
+  if (!c || !Vc || !Vs || !c->d.mhd) return IDFX_E_ARGS;
+  return check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, (hipStream_t)stream));
+}
+
+int idfx_evolve_mag_field(idfx_ctx *c, double dt, double *Vs, void *stream) {
+  if (!c || !Vs || !c->d.mhd) return IDFX_E_ARGS;
+  return check(c->ops->evolve_b(c->g, Vs, nullptr, c->emf(), dt, 0, 0.0, 0.0, (hipStream_t)stream));
+}
+
+int idfx_evolve_stage(idfx_ctx *c, double t, double dt, const double *Vc, double *Vs, double *Uc,
+                      double *InvDt, void *stream) {
+  (void)t;
+  if (!c || !Vc || !Uc || (c->d.mhd && !Vs)) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  for (int dir = 0; dir < c->d.dim; dir++) {
+    if (needs_unfused(c)) {
+      IDFX_TRY(do_sweep_unfused(c, dir, dt, Vc, Vs, Uc, InvDt, c->Flux, s));
+    } else {
+      SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, SW_INVDT_ADD);
+      IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
+    }
+  }
+  if (c->d.mhd && c->d.dim >= 2) {
+    IDFX_TRY(check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, s)));
+    IDFX_TRY(check(c->ops->evolve_b(c->g, Vs, nullptr, c->emf(), dt, 0, 0.0, 0.0, s)));
+    IDFX_TRY(check(c->ops->vc_field(c->g, Vs, Uc, s)));
+  }
+  return IDFX_OK;
+}
+
+int idfx_compute_timestep(idfx_ctx *c, const double *InvDt, double *dtmin, void *stream) {
+  if (!c || !dtmin) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, c->dtmax);
+  hipLaunchKernelGGL(k_invdt_max, dim3(1024), dim3(256), 0, s, c->g, InvDt ? InvDt : c->InvDt,
+                     c->dtmax);
+  IDFX_TRY(check(hipGetLastError()));
+  unsigned long long b;
+  IDFX_TRY(read_u64(c, c->dtmax, &b, s));
+  *dtmin = 1.0 / bits_to_double(b);  // min(1/x) == 1/max(x) under correct rounding
+  return IDFX_OK;
+}
+
+int idfx_add_and_store(idfx_ctx *c, double wl, double wr, double *U, const double *U0,
+                       long long n, void *stream) {
+  if (!c || !U || !U0 || n < 0) return IDFX_E_ARGS;
+  hipLaunchKernelGGL(k_add_and_store, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, U,
+                     U0, n, wl, wr);
+  return check(hipGetLastError());
+}
+
+int idfx_check_divb(idfx_ctx *c, const double *Vs, double *divb, void *stream) {
+  if (!c || !Vs || !divb || !c->d.mhd) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, c->divb);
+  IDFX_TRY(check(c->ops->divb(c->g, Vs, c->divb, s)));
+  unsigned long long b;
+  IDFX_TRY(read_u64(c, c->divb, &b, s));
+  *divb = bits_to_double(b);
+  return IDFX_OK;
+}
+
+// One RK stage on the fused path.  When newdt_host is given in stage 0, the InvDt
+// maximum is copied to pinned memory asynchronously and ctx->ev_dt records it.
+static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double *Vc, double *Vs,
+                         double *Uc, double *InvDt, int skip_boundaries, int write_uc,
+                         bool want_dt, hipStream_t s) {
+  static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
+  static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
+  if (!skip_boundaries) IDFX_TRY(idfx_set_boundaries(c, Vc, Vs, s));
+  double wc = 0.0, w0 = 0.0;
+  if (stage > 0) {
+    wc = WC[nstages - 2][stage - 1];
+    w0 = W0[nstages - 2][stage - 1];
+  }
+  const int dim = c->d.dim;
+  if (want_dt) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, c->dtmax);
+  for (int dir = 0; dir < dim; dir++) {
+    int flags = 0;
+    if (dir == 0) {
+      flags |= SW_INIT_UC;
+      if (stage == 0 && nstages > 1) flags |= SW_SAVE_UC0;
+    }
+    if (stage == 0) {
+      flags |= (dir == 0) ? SW_INVDT_SET : SW_INVDT_ADD;
+      if (dir == dim - 1 && want_dt) flags |= SW_DT_REDUCE;
+    }
+    if (needs_unfused(c)) {
+      // reference sequence: PrimToCons, (begin copy), CalcFlux, CalcRightHandSide
+      if (dir == 0) {
+        IDFX_TRY(check(c->ops->prim_to_cons(c->g, Vc, Uc, s)));
+        if (stage == 0 && nstages > 1)
+          IDFX_TRY(check(hipMemcpyAsync(c->Uc0, Uc, sizeof(double) * c->n3 * c->nvar,
+                                        hipMemcpyDeviceToDevice, s)));
+        if (stage == 0)
+          hipLaunchKernelGGL(k_fill, dim3(blocks_for(c->n3)), dim3(256), 0, s,
+                             InvDt ? InvDt : c->InvDt, c->n3, 0.0);
+      }
+      IDFX_TRY(do_sweep_unfused(c, dir, dt, Vc, Vs, Uc, InvDt, c->Flux, s));
+      if (dir == dim - 1 && want_dt)
+        hipLaunchKernelGGL(k_invdt_max, dim3(1024), dim3(256), 0, s, c->g,
+                           InvDt ? InvDt : c->InvDt, c->dtmax);
+    } else {
+      SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, flags);
+      IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
+    }
+  }
+  if (c->d.mhd && dim >= 2) {
+    IDFX_TRY(check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, s)));
+    const int mode = nstages > 1 ? (stage == 0 ? 1 : 2) : 0;
+    IDFX_TRY(check(c->ops->evolve_b(c->g, Vs, c->Vs0, c->emf(), dt, mode, wc, w0, s)));
+  } else if (c->d.mhd && stage > 0) {
+    // 1-D MHD keeps Vs constant; AddAndStore still applies (a no-op on equal values)
+  }
+  if (want_dt) {
+    IDFX_TRY(check(hipMemcpyAsync(c->host, c->dtmax, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s)));
+    IDFX_TRY(check(hipEventRecord(c->ev_dt, s)));
+  }
+  const int lo[3] = {c->g.beg[0], c->g.beg[1], c->g.beg[2]};
+  const int hi[3] = {c->g.end[0], c->g.end[1], c->g.end[2]};
+  const int flags = (stage > 0 ? 1 : 0) | (write_uc ? 2 : 0);
+  IDFX_TRY(check(c->ops->cons_to_prim(c->g, Uc, c->Uc0, Vc, Vs, lo, hi, flags, wc, w0, s)));
+  return IDFX_OK;
+}
+
+int idfx_stage(idfx_ctx *c, int stage, int nstages, double dt, double cfl, double *Vc, double *Vs,
+               double *Uc, double *InvDt, int skip_boundaries, int write_uc, double *newdt_host,
+               void *stream) {
+  if (!c || !Vc || !Uc || (c->d.mhd && !Vs) || nstages < 1 || nstages > 3 || stage < 0 ||
+      stage >= nstages)
+    return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  const bool want_dt = stage == 0 && newdt_host != nullptr;
+  IDFX_TRY(stage_enqueue(c, stage, nstages, dt, Vc, Vs, Uc, InvDt, skip_boundaries, write_uc,
+                         want_dt, s));
+  if (want_dt) {
+    IDFX_TRY(check(hipEventSynchronize(c->ev_dt)));
+    *newdt_host = cfl * (1.0 / bits_to_double(*c->host));
+  }
+  return IDFX_OK;
+}
+
+int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long ncycles, double *t,
+               double *dt, double *Vc, double *Vs, double *Uc, double *InvDt, void *stream) {
+  if (!c || !t || !dt || nstages < 1 || nstages > 3) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  const double t0 = *t;
+  for (int stage = 0; stage < nstages; stage++) {
+    IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0, stage == nstages - 1,
+                           stage == 0, s));
+  }
+  // the dt readback of stage 0 completes while the later stages still run
+  IDFX_TRY(check(hipEventSynchronize(c->ev_dt)));
+  const double newdt = cfl * (1.0 / bits_to_double(*c->host));
+  *t = t0 + *dt;
+  int status = IDFX_OK;
+  if (newdt > cfl_max_var * *dt) {
+    *dt = cfl_max_var * *dt;
+  } else {
+    if (ncycles == 0 && newdt < 0.5 * *dt) status = IDFX_E_FIRST_DT;
+    *dt = newdt;
+  }
+  if (status == IDFX_OK && *dt < 1e-15) status = IDFX_E_DT_SMALL;
+  if (status == IDFX_OK && !(newdt == newdt)) status = IDFX_E_NAN;
+  return status;
+}
+
+}  // extern "C"

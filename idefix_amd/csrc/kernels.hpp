@@ -1,0 +1,1013 @@
+// gfx950 kernels of the per-stage finite-volume update.
+//
+// Layout is the reference's LayoutRight ([var][k][j][i], i fastest) for Vc/Uc and
+// [dir][k][j][i] with one extra face along each active direction for Vs
+// (src/fluid/fluid.hpp:529-560).  Kernels index with the ghost-inclusive np_tot extents.
+//
+// Sweeps.  The x sweep gives one face per lane: a wave covers 64 faces and updates the
+// 63 cells between them, taking the right-hand flux from the neighbouring lane with a
+// cross-lane shuffle.  The y and z sweeps march: a lane owns one x column and walks a
+// segment of cells along the sweep direction, carrying the stencil window, the
+// reconstructed state of the previous cell and the previous face flux in registers, so
+// every cell is reconstructed once and every face solved once (plus one face per
+// segment).  In FUSED mode the flux divergence goes straight into Uc in FP64 and the
+// intercell flux never reaches HBM; in flux-only mode the kernel writes FluxRiemann
+// like the reference (needed when a user flux hook is enrolled).
+#pragma once
+
+#include "physics.hpp"
+
+namespace idfx {
+
+struct GridDesc {
+  int nt[3];    // np_tot per direction (i, j, k)
+  int beg[3];   // first interior index
+  int end[3];   // one past the last interior index
+  int vsn[3];   // Vs extents (np_tot + 1 along active directions)
+  long long sv;   // stride between variables of Vc/Uc
+  long long svs;  // stride between components of Vs
+  const double *dx[3];
+  double gamma;
+};
+
+template <typename T>
+struct EmfArrays {
+  T *ex, *ey, *ez;               // corner (edge) EMFs
+  T *ezi, *ezj, *exj, *exk, *eyi, *eyk;  // face EMFs (Et/Eb of each sweep)
+  T *svx, *svy, *svz;            // uct_contact upwind weights
+  T *axL, *axR, *ayL, *ayR, *azL, *azR;  // uct_hll(d) coefficients
+  T *dxL, *dxR, *dyL, *dyR, *dzL, *dzR;
+};
+
+// sweep flags
+enum : int {
+  SW_INIT_UC = 1,      // Uc = PrimToCons(Vc) + rhs   (else Uc += rhs)
+  SW_SAVE_UC0 = 2,     // Uc0 = PrimToCons(Vc)        (the RK "begin" state)
+  SW_INVDT_SET = 4,    // InvDt = term                (first direction after ResetStage)
+  SW_INVDT_ADD = 8,    // InvDt += term
+  SW_DT_REDUCE = 16,   // fold max(InvDt) of the updated cells into *dtmax
+  SW_NO_INVDT_STORE = 32,  // keep InvDt in registers only (requires SW_INVDT_SET)
+};
+
+struct SweepArgs {
+  const double *Vc;
+  const double *Vs;
+  double *Uc;
+  double *Uc0;
+  double *InvDt;
+  double *Flux;   // flux-only mode
+  double *cMax;   // flux-only mode
+  double dt;
+  int emf;
+  int flags;
+  int ext[3];     // transverse extension of the face range (CT needs)
+  unsigned long long *dtmax;  // bit pattern of max(InvDt) (non-negative doubles)
+};
+
+__device__ __forceinline__ long long idx3(const GridDesc &g, int k, int j, int i) {
+  return ((long long)k * g.nt[1] + j) * g.nt[0] + i;
+}
+__device__ __forceinline__ long long idxs(const GridDesc &g, int n, int k, int j, int i) {
+  return (((long long)n * g.vsn[2] + k) * g.vsn[1] + j) * g.vsn[0] + i;
+}
+
+// Cartesian volume and areas, makeGeometry.cpp:67-70, :146-147, :178-179, :201-202
+template <int DIM>
+__device__ __forceinline__ double cell_dV(const GridDesc &g, int k, int j, int i) {
+  double v = g.dx[0][i];
+  if constexpr (DIM >= 2) v = v * g.dx[1][j];
+  if constexpr (DIM >= 3) v = v * g.dx[2][k];
+  return v;
+}
+template <int DIM, int DIR>
+__device__ __forceinline__ double face_A(const GridDesc &g, int k, int j, int i) {
+  double a;
+  if constexpr (DIR == IDIR) {
+    a = 1.0;
+    if constexpr (DIM >= 2) a = a * g.dx[1][j];
+    if constexpr (DIM >= 3) a = a * g.dx[2][k];
+  } else if constexpr (DIR == JDIR) {
+    a = g.dx[0][i];
+    if constexpr (DIM >= 2) a = a * 1.0;
+    if constexpr (DIM >= 3) a = a * g.dx[2][k];
+  } else {
+    a = g.dx[0][i];
+    if constexpr (DIM >= 2) a = a * g.dx[1][j];
+    if constexpr (DIM >= 3) a = a * 1.0;
+  }
+  return a;
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ T *et_of(const EmfArrays<T> &e) {
+  return DIR == 0 ? e.ezi : DIR == 1 ? e.ezj : e.eyk;
+}
+template <typename T, int DIR>
+__device__ __forceinline__ T *eb_of(const EmfArrays<T> &e) {
+  return DIR == 0 ? e.eyi : DIR == 1 ? e.exj : e.exk;
+}
+
+// Store the face-EMF data of one face (the K_Store* of storeFlux.hpp).
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void store_face_emf(const EmfArrays<T> &e, long long p, int emf,
+                                               const FaceEmf<T> &E) {
+  if constexpr (C::MHD) {
+    if (emf <= EMF_UCT_CONTACT) {
+      if constexpr (C::COMP > 1) {
+        et_of<T, DIR>(e)[p] = E.Et;
+        if constexpr (C::DIM == 3) eb_of<T, DIR>(e)[p] = E.Eb;
+      }
+      if (emf == EMF_UCT_CONTACT) (DIR == 0 ? e.svx : DIR == 1 ? e.svy : e.svz)[p] = E.sv;
+    } else {
+      if constexpr (C::COMP > 1) {
+        et_of<T, DIR>(e)[p] = E.Et;
+        if constexpr (C::COMP >= 3) eb_of<T, DIR>(e)[p] = E.Eb;
+      }
+      (DIR == 0 ? e.axL : DIR == 1 ? e.ayL : e.azL)[p] = E.aL;
+      (DIR == 0 ? e.axR : DIR == 1 ? e.ayR : e.azR)[p] = E.aR;
+      (DIR == 0 ? e.dxL : DIR == 1 ? e.dyL : e.dzL)[p] = E.dL;
+      (DIR == 0 ? e.dxR : DIR == 1 ? e.dyR : e.dzR)[p] = E.dR;
+    }
+  }
+}
+
+// Per-cell reconstruction from a stencil window centred on the cell (PLM: 3 cells,
+// PPM: 5 cells).  Returns the cell's left-face value `vl` (the right state of its left
+// face) and right-face value `vr` (the left state of its right face), exactly as
+// ExtrapolatePrimVar computes them for the two faces (extrapolateToFaces.hpp:88-124,
+// :247-306).
+template <class C, typename T>
+__device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr) {
+  if constexpr (C::ORDER == RECON_CONSTANT) {
+    vl = w[0];
+    vr = w[0];
+  } else if constexpr (C::ORDER == RECON_PLM) {
+    // w[0] = V(c-1), w[1] = V(c), w[2] = V(c+1)
+    const T dvm = w[1] - w[0];
+    const T dvp = w[2] - w[1];
+    const T dv = vanleer(dvp, dvm);
+    vl = w[1] - T(0.5) * dv;
+    vr = w[1] + T(0.5) * dv;
+  } else {
+    // w[0..4] = V(c-2) .. V(c+2)
+    ppm_states(w[0], w[1], w[2], w[3], w[4], vl, vr);
+    if (nv == C::RHO || nv == C::ENG) {
+      if (vr <= T(0)) {
+        const T dv = vanleer(w[3] - w[2], w[2] - w[1]);
+        vr = w[2] + T(0.5) * dv;
+      }
+      if (vl <= T(0)) {
+        const T dv = vanleer(w[3] - w[2], w[2] - w[1]);
+        vl = w[2] - T(0.5) * dv;
+      }
+    }
+  }
+}
+template <class C>
+constexpr int window() {
+  return C::ORDER == RECON_CONSTANT ? 1 : (C::ORDER == RECON_PLM ? 3 : 5);
+}
+
+// Conserved state at the start of the sweep for one cell: PrimToCons(Vc) (FP64).
+template <class C>
+__device__ __forceinline__ void cell_p2c(const GridDesc &g, const double *Vc, long long p,
+                                         double *u) {
+  double v[C::NVAR];
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) v[n] = Vc[n * g.sv + p];
+  prim2cons<C>(v, u, g.gamma);
+}
+
+// Block-level max of non-negative doubles by bit pattern, one atomic per block.
+__device__ __forceinline__ void block_max_atomic(unsigned long long v, unsigned long long *dst) {
+  __shared__ unsigned long long red[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = red[q] > m ? red[q] : m;
+    atomicMax(dst, m);
+  }
+}
+
+// Update of one cell by the flux difference along DIR (calcRightHandSide.hpp:288-549,
+// Cartesian, with the area correction of :156-194 applied to both faces first).
+template <class C, int DIR, typename T>
+__device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs &a, int k,
+                                              int j, int i, const T *Fm, const T *Fp,
+                                              T cmaxm, T cmaxp) {
+  const long long p = idx3(g, k, j, i);
+  const double dtdV = a.dt / cell_dV<C::DIM>(g, k, j, i);
+  const double A = face_A<C::DIM, DIR>(g, k, j, i);
+  double u[C::NVAR];
+  if (a.flags & (SW_INIT_UC | SW_SAVE_UC0)) {
+    double u0[C::NVAR];
+    cell_p2c<C>(g, a.Vc, p, u0);
+    if (a.flags & SW_SAVE_UC0) {
+#pragma unroll
+      for (int n = 0; n < C::NVAR; n++)
+        if (!C::is_ct_field(n)) a.Uc0[n * g.sv + p] = u0[n];
+    }
+    if (a.flags & SW_INIT_UC) {
+#pragma unroll
+      for (int n = 0; n < C::NVAR; n++) u[n] = u0[n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < C::NVAR; n++)
+        if (!C::is_ct_field(n)) u[n] = a.Uc[n * g.sv + p];
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < C::NVAR; n++)
+      if (!C::is_ct_field(n)) u[n] = a.Uc[n * g.sv + p];
+  }
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) {
+    if (C::is_ct_field(n)) continue;
+    const double fp = (double)Fp[n] * A;
+    const double fm = (double)Fm[n] * A;
+    const double rhs = -dtdV * (fp - fm);
+    a.Uc[n * g.sv + p] = u[n] + rhs;
+  }
+  const int ig = DIR == 0 ? i : DIR == 1 ? j : k;
+  const double dl = g.dx[DIR][ig];
+  const double term = 0.5 * ((double)cmaxp + (double)cmaxm) / (dl);
+  double inv = 0.0;
+  if (a.flags & SW_INVDT_SET) {
+    inv = term;
+    if (!(a.flags & SW_NO_INVDT_STORE)) a.InvDt[p] = inv;
+  } else if (a.flags & SW_INVDT_ADD) {
+    inv = a.InvDt[p] + term;
+    a.InvDt[p] = inv;
+  }
+  return inv;
+}
+
+// ------------------------------------------------------------------------------------
+// x sweep: one face per lane (64 faces / 63 cells per wave)
+// ------------------------------------------------------------------------------------
+template <class C, typename T, bool FUSED>
+__global__ __launch_bounds__(256) void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
+                                                 int nseg) {
+  constexpr int NV = C::NVAR;
+  constexpr int H = window<C>() / 2;
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nj = g.end[1] - g.beg[1] + 2 * a.ext[1];
+  const int nk = g.end[2] - g.beg[2] + 2 * a.ext[2];
+  const bool wave_ok = w < nseg * nj * nk;
+  unsigned long long red = 0;
+  if (wave_ok) {
+    const int seg = w % nseg;
+    const int r = w / nseg;
+    const int j = g.beg[1] - a.ext[1] + r % nj;
+    const int k = g.beg[2] - a.ext[2] + r / nj;
+    const int i = g.beg[0] + seg * 63 + lane;  // face i-1/2
+    const bool face_ok = i <= g.end[0];
+    const int ic = face_ok ? i : g.end[0];
+    const long long row = idx3(g, k, j, 0);
+
+    T vL[NV], vR[NV];
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      T wl[window<C>()], wr[window<C>()];
+#pragma unroll
+      for (int s = 0; s < window<C>(); s++) {
+        wl[s] = (T)a.Vc[n * g.sv + row + ic - 1 - H + s];
+        wr[s] = (T)a.Vc[n * g.sv + row + ic - H + s];
+      }
+      T dummy;
+      recon_cell<C>(wl, n, dummy, vL[n]);
+      recon_cell<C>(wr, n, vR[n], dummy);
+    }
+    if constexpr (C::MHD) {
+      vL[C::BX1] = (T)a.Vs[idxs(g, 0, k, j, ic)];
+      vR[C::BX1] = vL[C::BX1];
+    }
+    T F[NV], cmax;
+    FaceEmf<T> E;
+    const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
+    const bool own_face = face_ok && (lane < 63 || i == g.end[0]);
+    const long long p = row + i;
+    if (own_face) store_face_emf<C, IDIR>(e, p, a.emf, E);
+    if constexpr (!FUSED) {
+      if (own_face) {
+        if (wrote) {
+#pragma unroll
+          for (int n = 0; n < NV; n++) a.Flux[n * g.sv + p] = (double)F[n];
+        }
+        a.cMax[p] = (double)cmax;
+      }
+    } else {
+      T Fp[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) Fp[n] = __shfl_down(F[n], 1);
+      const T cmaxp = __shfl_down(cmax, 1);
+      const bool cell_ok = lane < 63 && i < g.end[0] && j >= g.beg[1] && j < g.end[1] &&
+                           k >= g.beg[2] && k < g.end[2];
+      if (cell_ok) {
+        const double inv = cell_update<C, IDIR>(g, a, k, j, i, F, Fp, cmax, cmaxp);
+        red = __double_as_longlong(inv);
+      }
+    }
+  }
+  if constexpr (FUSED) {
+    if (a.flags & SW_DT_REDUCE) block_max_atomic(red, a.dtmax);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// y / z sweeps: marching along DIR with the stencil window in registers
+// ------------------------------------------------------------------------------------
+struct MarchGeom {
+  int ilo, ihi;   // x range (interior +- extension)
+  int tlo, thi;   // range of the other transverse direction
+  int ts;         // cells per segment along DIR
+  int nseg;       // segments along DIR
+  int nxb;        // 64-wide x blocks
+};
+
+template <class C, typename T, int DIR, bool FUSED>
+__global__ __launch_bounds__(256) void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
+                                                     MarchGeom m) {
+  static_assert(DIR == JDIR || DIR == KDIR, "marching sweep is for y and z");
+  constexpr int NV = C::NVAR;
+  constexpr int NW = window<C>();
+  constexpr int H = NW / 2;
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int ntr = m.thi - m.tlo;
+  unsigned long long red = 0;
+  if (w < m.nxb * ntr * m.nseg) {
+    const int xb = w % m.nxb;
+    const int rest = w / m.nxb;
+    const int seg = rest % m.nseg;
+    const int tr = m.tlo + rest / m.nseg;
+    const int i = m.ilo + xb * 64 + lane;
+    const bool lane_ok = i < m.ihi;
+    const int ic = lane_ok ? i : m.ihi - 1;
+    const int c0 = g.beg[DIR] + seg * m.ts;
+    const int c1 = min(c0 + m.ts, g.end[DIR]);
+    // stride along DIR and base offset of the column
+    const long long sd = (DIR == JDIR) ? g.nt[0] : (long long)g.nt[0] * g.nt[1];
+    const int jj = (DIR == JDIR) ? 0 : tr;  // j for the z sweep
+    const int kk = (DIR == JDIR) ? tr : 0;  // k for the y sweep
+    const long long col = idx3(g, kk, jj, ic);  // index with DIR coordinate = 0
+    const bool rhs_lane = lane_ok && i >= g.beg[0] && i < g.end[0] &&
+                          ((DIR == JDIR) ? (tr >= g.beg[2] && tr < g.end[2])
+                                         : (tr >= g.beg[1] && tr < g.end[1]));
+    // Vs normal component: stride along DIR in the Vs array
+    const long long vsd = (DIR == JDIR) ? g.vsn[0] : (long long)g.vsn[0] * g.vsn[1];
+    const long long vcol = idxs(g, DIR, kk, jj, ic);
+
+    T win[NW][NV];
+#pragma unroll
+    for (int s = 0; s < NW; s++)
+#pragma unroll
+      for (int n = 0; n < NV; n++)
+        win[s][n] = (T)a.Vc[n * g.sv + col + (long long)(c0 - 1 - H + s) * sd];
+    // right-face value of cell c0-1: the left state of the first face
+    T carry[NV], Fprev[NV], cprev = 0;
+#pragma unroll
+    for (int n = 0; n < NV; n++) Fprev[n] = 0;
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      T wv[NW];
+#pragma unroll
+      for (int s = 0; s < NW; s++) wv[s] = win[s][n];
+      T dummy;
+      recon_cell<C>(wv, n, dummy, carry[n]);
+    }
+    for (int f = c0; f <= c1; f++) {
+      // shift the window to cell f and load its leading row
+#pragma unroll
+      for (int s = 0; s < NW - 1; s++)
+#pragma unroll
+        for (int n = 0; n < NV; n++) win[s][n] = win[s + 1][n];
+#pragma unroll
+      for (int n = 0; n < NV; n++)
+        win[NW - 1][n] = (T)a.Vc[n * g.sv + col + (long long)(f + H) * sd];
+      T vL[NV], vR[NV], vr_next[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        T wv[NW];
+#pragma unroll
+        for (int s = 0; s < NW; s++) wv[s] = win[s][n];
+        recon_cell<C>(wv, n, vR[n], vr_next[n]);
+        vL[n] = carry[n];
+      }
+      if constexpr (C::MHD) {
+        constexpr int BXn = C::template BXn<DIR>();
+        vL[BXn] = (T)a.Vs[vcol + (long long)f * vsd];
+        vR[BXn] = vL[BXn];
+      }
+      T F[NV], cmax;
+      FaceEmf<T> E;
+      const bool wrote = riemann<C, DIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
+      const long long p = col + (long long)f * sd;
+      const bool own_face = lane_ok && (f < c1 || c1 == g.end[DIR]);
+      if (own_face) store_face_emf<C, DIR>(e, p, a.emf, E);
+      if constexpr (!FUSED) {
+        if (own_face) {
+          if (wrote) {
+#pragma unroll
+            for (int n = 0; n < NV; n++) a.Flux[n * g.sv + p] = (double)F[n];
+          }
+          a.cMax[p] = (double)cmax;
+        }
+      } else {
+        if (f > c0 && rhs_lane) {
+          const int cf = f - 1;
+          const int ci = ic, cj = (DIR == JDIR) ? cf : jj, ck = (DIR == JDIR) ? kk : cf;
+          const double inv = cell_update<C, DIR>(g, a, ck, cj, ci, Fprev, F, cprev, cmax);
+          const unsigned long long b = __double_as_longlong(inv);
+          red = b > red ? b : red;
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        carry[n] = vr_next[n];
+        Fprev[n] = F[n];
+      }
+      cprev = cmax;
+    }
+  }
+  if constexpr (FUSED) {
+    if (a.flags & SW_DT_REDUCE) block_max_atomic(red, a.dtmax);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// CalcRightHandSide from a materialised FluxRiemann (flux-only mode):
+// "Correct Flux" then "CalcRightHandSide" (calcRightHandSide.hpp:556-596).
+// ------------------------------------------------------------------------------------
+template <class C, int DIR>
+__global__ __launch_bounds__(256) void k_correct_flux(GridDesc g, double *Flux) {
+  constexpr int io = DIR == 0, jo = DIR == 1, ko = DIR == 2;
+  const int ni = g.end[0] + io - g.beg[0], nj = g.end[1] + jo - g.beg[1];
+  const int nk = g.end[2] + ko - g.beg[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = g.beg[0] + t % ni;
+  const int j = g.beg[1] + (t / ni) % nj;
+  const int k = g.beg[2] + t / ((long long)ni * nj);
+  const double A = face_A<C::DIM, DIR>(g, k, j, i);
+  const long long p = idx3(g, k, j, i);
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) Flux[n * g.sv + p] = Flux[n * g.sv + p] * A;
+}
+
+template <class C, int DIR>
+__global__ __launch_bounds__(256) void k_rhs(GridDesc g, const double *Flux, const double *cMax,
+                                             double *Uc, double *InvDt, double dt) {
+  constexpr int io = DIR == 0, jo = DIR == 1, ko = DIR == 2;
+  const int ni = g.end[0] - g.beg[0], nj = g.end[1] - g.beg[1], nk = g.end[2] - g.beg[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = g.beg[0] + t % ni;
+  const int j = g.beg[1] + (t / ni) % nj;
+  const int k = g.beg[2] + t / ((long long)ni * nj);
+  const long long p = idx3(g, k, j, i);
+  const long long pp = idx3(g, k + ko, j + jo, i + io);
+  const double dtdV = dt / cell_dV<C::DIM>(g, k, j, i);
+  const int ig = io * i + jo * j + ko * k;
+  const double dl = g.dx[DIR][ig];
+  InvDt[p] = InvDt[p] + 0.5 * (cMax[pp] + cMax[p]) / (dl);
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) {
+    if (C::is_ct_field(n)) continue;
+    const double rhs = -dtdV * (Flux[n * g.sv + pp] - Flux[n * g.sv + p]);
+    Uc[n * g.sv + p] = Uc[n * g.sv + p] + rhs;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Constrained transport: corner EMFs (calcCornerEmf.hpp, calcRiemannEmf.hpp)
+// ------------------------------------------------------------------------------------
+// cell-centred E = -v x B of one cell (CalcCellCenteredEMF, calcCornerEmf.hpp:84-126)
+template <class C, typename T>
+__device__ __forceinline__ void cell_emf(const GridDesc &g, const double *Vc, int k, int j,
+                                         int i, T &E1, T &E2, T &E3) {
+  const long long p = idx3(g, k, j, i);
+  T vx1 = (T)Vc[C::MX1 * g.sv + p], vx2 = 0, vx3 = 0;
+  T bx1 = (T)Vc[C::BX1 * g.sv + p], bx2 = 0, bx3 = 0;
+  if constexpr (C::COMP >= 2) {
+    vx2 = (T)Vc[C::MX2 * g.sv + p];
+    bx2 = (T)Vc[C::BX2 * g.sv + p];
+  }
+  if constexpr (C::COMP >= 3) {
+    vx3 = (T)Vc[C::MX3 * g.sv + p];
+    bx3 = (T)Vc[C::BX3 * g.sv + p];
+  }
+  E1 = (vx3 * bx2 - vx2 * bx3);
+  E2 = (vx1 * bx3 - vx3 * bx1);
+  E3 = (vx2 * bx1 - vx1 * bx2);
+}
+
+// One edge term of CalcRiemannAverage (calcRiemannEmf.hpp:62-428): B and the upwinded
+// transverse velocity reconstructed (MC limiter, or PPM for ORDER 4) from the left and
+// right cells of the edge.  b[s], v[s] hold the face values at offset s-3 from the
+// edge's left cell (s = 0..6 covers -3..3).
+template <class C, typename T>
+__device__ __forceinline__ T edge_term(const T *b, const T *v, T aL, T aR, T dL, T dR,
+                                       bool plus) {
+  T bL, bR, vL, vR, d;
+  if constexpr (C::ORDER == RECON_PPM) {
+    ppm_states(b[1], b[2], b[3], b[4], b[5], d, bL);
+    ppm_states(b[2], b[3], b[4], b[5], b[6], bR, d);
+    ppm_states(v[1], v[2], v[3], v[4], v[5], d, vL);
+    ppm_states(v[2], v[3], v[4], v[5], v[6], vR, d);
+  } else {
+    d = mclim(b[4] - b[3], b[3] - b[2]);
+    bL = b[3] + T(0.5) * d;
+    d = mclim(b[5] - b[4], b[4] - b[3]);
+    bR = b[4] - T(0.5) * d;
+    d = mclim(v[4] - v[3], v[3] - v[2]);
+    vL = v[3] + T(0.5) * d;
+    d = mclim(v[5] - v[4], v[4] - v[3]);
+    vR = v[4] - T(0.5) * d;
+  }
+  const T phi = dR * bR - dL * bL;
+  return plus ? (aL * vL * bL + aR * vR * bR) + phi : (aL * vL * bL + aR * vR * bR) - phi;
+}
+
+template <class C, typename T>
+__global__ __launch_bounds__(256) void k_corner_emf(GridDesc g, const double *Vc,
+                                                    const double *Vs, EmfArrays<T> e, int emf) {
+  constexpr int D3 = C::DIM == 3;
+  const int ni = g.end[0] + 1 - g.beg[0], nj = g.end[1] + 1 - g.beg[1];
+  const int nk = g.end[2] + D3 - g.beg[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = g.beg[0] + t % ni;
+  const int j = g.beg[1] + (t / ni) % nj;
+  const int k = g.beg[2] + t / ((long long)ni * nj);
+  const long long p = idx3(g, k, j, i);
+  auto at = [&](const T *arr, int kk, int jj, int ii) { return arr[idx3(g, kk, jj, ii)]; };
+  if (emf == EMF_ARITHMETIC || emf == EMF_UCT0) {
+    // CalcArithmeticAverage (:62-78); uct0 first rewrites the face EMFs (:150-172)
+    const T w = T(0.25);
+    auto mod = [&](const T *arr, int kk, int jj, int ii, int comp, int dk, int dj, int di) {
+      T v = at(arr, kk, jj, ii);
+      if (emf == EMF_UCT0) {
+        T a1, a2, a3, b1, b2, b3;
+        cell_emf<C, T>(g, Vc, kk - dk, jj - dj, ii - di, a1, a2, a3);
+        cell_emf<C, T>(g, Vc, kk, jj, ii, b1, b2, b3);
+        const T ea = comp == 1 ? a1 : comp == 2 ? a2 : a3;
+        const T eb = comp == 1 ? b1 : comp == 2 ? b2 : b3;
+        v *= T(2);
+        v -= T(0.5) * (ea + eb);
+      }
+      return v;
+    };
+    if constexpr (C::DIM == 3) {
+      e.ex[p] = w * (mod(e.exj, k, j, i, 1, 0, 1, 0) + mod(e.exj, k - 1, j, i, 1, 0, 1, 0) +
+                     mod(e.exk, k, j, i, 1, 1, 0, 0) + mod(e.exk, k, j - 1, i, 1, 1, 0, 0));
+      e.ey[p] = w * (mod(e.eyi, k, j, i, 2, 0, 0, 1) + mod(e.eyi, k - 1, j, i, 2, 0, 0, 1) +
+                     mod(e.eyk, k, j, i, 2, 1, 0, 0) + mod(e.eyk, k, j, i - 1, 2, 1, 0, 0));
+    }
+    e.ez[p] = w * (mod(e.ezi, k, j, i, 3, 0, 0, 1) + mod(e.ezi, k, j - 1, i, 3, 0, 0, 1) +
+                   mod(e.ezj, k, j, i, 3, 0, 1, 0) + mod(e.ezj, k, j, i - 1, 3, 0, 1, 0));
+  } else if (emf == EMF_UCT_CONTACT) {
+    // CalcContactAverage, calcCornerEmf.hpp:228-282
+    T E1[2][2][2], E2[2][2][2], E3[2][2][2];  // [dk][dj][di] for cells (k-1+dk, j-1+dj, i-1+di)
+#pragma unroll
+    for (int dk = 0; dk < 1 + D3; dk++)
+#pragma unroll
+      for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+        for (int di = 0; di < 2; di++)
+          cell_emf<C, T>(g, Vc, k - D3 + dk, j - 1 + dj, i - 1 + di, E1[dk][dj][di],
+                         E2[dk][dj][di], E3[dk][dj][di]);
+    const int K = D3;  // index of cell k in the [dk] axis
+    const T ez_l2 = (1 - at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i) - E3[K][0][1]) +
+                    (at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i - 1) - E3[K][0][0]);
+    const T ez_r2 = (1 - at(e.svx, k, j, i)) * (at(e.ezj, k, j, i) - E3[K][1][1]) +
+                    (at(e.svx, k, j, i)) * (at(e.ezj, k, j, i - 1) - E3[K][1][0]);
+    const T ez_l1 = (1 - at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j, i) - E3[K][1][0]) +
+                    (at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j - 1, i) - E3[K][0][0]);
+    const T ez_r1 = (1 - at(e.svy, k, j, i)) * (at(e.ezi, k, j, i) - E3[K][1][1]) +
+                    (at(e.svy, k, j, i)) * (at(e.ezi, k, j - 1, i) - E3[K][0][1]);
+    e.ez[p] = T(0.25) * (ez_l2 + ez_r2 + ez_l1 + ez_r1 + at(e.ezi, k, j, i) +
+                         at(e.ezi, k, j - 1, i) + at(e.ezj, k, j, i) + at(e.ezj, k, j, i - 1));
+    if constexpr (C::DIM == 3) {
+      const T ex_l3 = (1 - at(e.svy, k - 1, j, i)) * (at(e.exk, k, j, i) - E1[0][1][1]) +
+                      (at(e.svy, k - 1, j, i)) * (at(e.exk, k, j - 1, i) - E1[0][0][1]);
+      const T ex_r3 = (1 - at(e.svy, k, j, i)) * (at(e.exk, k, j, i) - E1[1][1][1]) +
+                      (at(e.svy, k, j, i)) * (at(e.exk, k, j - 1, i) - E1[1][0][1]);
+      const T ex_l2 = (1 - at(e.svz, k, j - 1, i)) * (at(e.exj, k, j, i) - E1[1][0][1]) +
+                      (at(e.svz, k, j - 1, i)) * (at(e.exj, k - 1, j, i) - E1[0][0][1]);
+      const T ex_r2 = (1 - at(e.svz, k, j, i)) * (at(e.exj, k, j, i) - E1[1][1][1]) +
+                      (at(e.svz, k, j, i)) * (at(e.exj, k - 1, j, i) - E1[0][1][1]);
+      e.ex[p] = T(0.25) * (ex_l3 + ex_r3 + ex_l2 + ex_r2 + at(e.exj, k, j, i) +
+                           at(e.exj, k - 1, j, i) + at(e.exk, k, j, i) + at(e.exk, k, j - 1, i));
+      const T ey_l3 = (1 - at(e.svx, k - 1, j, i)) * (at(e.eyk, k, j, i) - E2[0][1][1]) +
+                      (at(e.svx, k - 1, j, i)) * (at(e.eyk, k, j, i - 1) - E2[0][1][0]);
+      const T ey_r3 = (1 - at(e.svx, k, j, i)) * (at(e.eyk, k, j, i) - E2[1][1][1]) +
+                      (at(e.svx, k, j, i)) * (at(e.eyk, k, j, i - 1) - E2[1][1][0]);
+      const T ey_l1 = (1 - at(e.svz, k, j, i - 1)) * (at(e.eyi, k, j, i) - E2[1][1][0]) +
+                      (at(e.svz, k, j, i - 1)) * (at(e.eyi, k - 1, j, i) - E2[0][1][0]);
+      const T ey_r1 = (1 - at(e.svz, k, j, i)) * (at(e.eyi, k, j, i) - E2[1][1][1]) +
+                      (at(e.svz, k, j, i)) * (at(e.eyi, k - 1, j, i) - E2[0][1][1]);
+      e.ey[p] = T(0.25) * (ey_l3 + ey_r3 + ey_l1 + ey_r1 + at(e.eyi, k, j, i) +
+                           at(e.eyi, k - 1, j, i) + at(e.eyk, k, j, i) + at(e.eyk, k, j, i - 1));
+    }
+  } else {
+    // uct_hll / uct_hlld: CalcRiemannAverage
+    // stencil slots used: MC limiter -1..2, PPM -2..3 around the edge's left cell
+    constexpr int SLO = C::ORDER == RECON_PPM ? 1 : 2;
+    constexpr int SHI = C::ORDER == RECON_PPM ? 6 : 5;
+    const int im = i - 1, jm = j - 1, km = k - 1;
+    T b[7], v[7], aL, aR, dL, dR;
+    auto vs = [&](int n, int kk, int jj, int ii) { return (T)Vs[idxs(g, n, kk, jj, ii)]; };
+    // ez from x faces: B = BX2s along x at y-face j; v = ezj along x
+    aL = T(0.5) * (at(e.axL, k, jm, i) + at(e.axL, k, jm + 1, i));
+    aR = T(0.5) * (at(e.axR, k, jm, i) + at(e.axR, k, jm + 1, i));
+    dL = T(0.5) * (at(e.dxL, k, jm, i) + at(e.dxL, k, jm + 1, i));
+    dR = T(0.5) * (at(e.dxR, k, jm, i) + at(e.dxR, k, jm + 1, i));
+#pragma unroll
+    for (int s = SLO; s <= SHI; s++) {
+      b[s] = vs(1, k, j, im + s - 3);
+      v[s] = at(e.ezj, k, j, im + s - 3);
+    }
+    T ezv = edge_term<C>(b, v, aL, aR, dL, dR, true);
+    T eyv = 0, exv = 0;
+    if constexpr (C::DIM == 3) {
+      aL = T(0.5) * (at(e.axL, km, j, i) + at(e.axL, km + 1, j, i));
+      aR = T(0.5) * (at(e.axR, km, j, i) + at(e.axR, km + 1, j, i));
+      dL = T(0.5) * (at(e.dxL, km, j, i) + at(e.dxL, km + 1, j, i));
+      dR = T(0.5) * (at(e.dxR, km, j, i) + at(e.dxR, km + 1, j, i));
+#pragma unroll
+      for (int s = SLO; s <= SHI; s++) {
+        b[s] = vs(2, k, j, im + s - 3);
+        v[s] = at(e.eyk, k, j, im + s - 3);
+      }
+      eyv = edge_term<C>(b, v, aL, aR, dL, dR, false);
+      aL = T(0.5) * (at(e.ayL, km, j, i) + at(e.ayL, km + 1, j, i));
+      aR = T(0.5) * (at(e.ayR, km, j, i) + at(e.ayR, km + 1, j, i));
+      dL = T(0.5) * (at(e.dyL, km, j, i) + at(e.dyL, km + 1, j, i));
+      dR = T(0.5) * (at(e.dyR, km, j, i) + at(e.dyR, km + 1, j, i));
+#pragma unroll
+      for (int s = SLO; s <= SHI; s++) {
+        b[s] = vs(2, k, jm + s - 3, i);
+        v[s] = at(e.exk, k, jm + s - 3, i);
+      }
+      exv = edge_term<C>(b, v, aL, aR, dL, dR, true);
+    }
+    aL = T(0.5) * (at(e.ayL, k, j, im) + at(e.ayL, k, j, im + 1));
+    aR = T(0.5) * (at(e.ayR, k, j, im) + at(e.ayR, k, j, im + 1));
+    dL = T(0.5) * (at(e.dyL, k, j, im) + at(e.dyL, k, j, im + 1));
+    dR = T(0.5) * (at(e.dyR, k, j, im) + at(e.dyR, k, j, im + 1));
+#pragma unroll
+    for (int s = SLO; s <= SHI; s++) {
+      b[s] = vs(0, k, jm + s - 3, i);
+      v[s] = at(e.ezi, k, jm + s - 3, i);
+    }
+    ezv += edge_term<C>(b, v, aL, aR, dL, dR, false);
+    if constexpr (C::DIM == 3) {
+      aL = T(0.5) * (at(e.azL, k, j, im) + at(e.azL, k, j, im + 1));
+      aR = T(0.5) * (at(e.azR, k, j, im) + at(e.azR, k, j, im + 1));
+      dL = T(0.5) * (at(e.dzL, k, j, im) + at(e.dzL, k, j, im + 1));
+      dR = T(0.5) * (at(e.dzR, k, j, im) + at(e.dzR, k, j, im + 1));
+#pragma unroll
+      for (int s = SLO; s <= SHI; s++) {
+        b[s] = vs(0, km + s - 3, j, i);
+        v[s] = at(e.eyi, km + s - 3, j, i);
+      }
+      eyv += edge_term<C>(b, v, aL, aR, dL, dR, true);
+      aL = T(0.5) * (at(e.azL, k, jm, i) + at(e.azL, k, jm + 1, i));
+      aR = T(0.5) * (at(e.azR, k, jm, i) + at(e.azR, k, jm + 1, i));
+      dL = T(0.5) * (at(e.dzL, k, jm, i) + at(e.dzL, k, jm + 1, i));
+      dR = T(0.5) * (at(e.dzR, k, jm, i) + at(e.dzR, k, jm + 1, i));
+#pragma unroll
+      for (int s = SLO; s <= SHI; s++) {
+        b[s] = vs(1, km + s - 3, j, i);
+        v[s] = at(e.exj, km + s - 3, j, i);
+      }
+      exv += edge_term<C>(b, v, aL, aR, dL, dR, false);
+      e.ex[p] = exv;
+      e.ey[p] = eyv;
+    }
+    e.ez[p] = ezv;
+  }
+}
+
+// EvolveMagField, evolveMagField.hpp:53-132 (Cartesian).  mode 0: Vs += rhs;
+// mode 1: Vs0 = Vs then Vs += rhs (stage 0 of a multi-stage integrator);
+// mode 2: Vs = wc*(Vs + rhs) + w0*Vs0 (AddAndStore of a later stage).
+template <class C, typename T>
+__global__ __launch_bounds__(256) void k_evolve_b(GridDesc g, double *Vs, double *Vs0,
+                                                  EmfArrays<T> e, double dt, int mode, double wc,
+                                                  double w0) {
+  constexpr int D3 = C::DIM == 3;
+  const int ni = g.end[0] + 1 - g.beg[0], nj = g.end[1] + 1 - g.beg[1];
+  const int nk = g.end[2] + D3 - g.beg[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = g.beg[0] + t % ni;
+  const int j = g.beg[1] + (t / ni) % nj;
+  const int k = g.beg[2] + t / ((long long)ni * nj);
+  auto E = [&](const T *arr, int kk, int jj, int ii) { return (double)arr[idx3(g, kk, jj, ii)]; };
+  double rhsx1 = 0.0 - dt / g.dx[1][j] * (E(e.ez, k, j + 1, i) - E(e.ez, k, j, i));
+  double rhsx2 = dt / g.dx[0][i] * (E(e.ez, k, j, i + 1) - E(e.ez, k, j, i));
+  double rhsx3 = 0.0;
+  if constexpr (D3) {
+    rhsx1 = rhsx1 + dt / g.dx[2][k] * (E(e.ey, k + 1, j, i) - E(e.ey, k, j, i));
+    rhsx2 = rhsx2 - dt / g.dx[2][k] * (E(e.ex, k + 1, j, i) - E(e.ex, k, j, i));
+    rhsx3 = -dt / g.dx[0][i] * (E(e.ey, k, j, i + 1) - E(e.ey, k, j, i)) +
+            dt / g.dx[1][j] * (E(e.ex, k, j + 1, i) - E(e.ex, k, j, i));
+  }
+  double rhs[3] = {rhsx1, rhsx2, rhsx3};
+#pragma unroll
+  for (int n = 0; n < C::DIM; n++) {
+    const long long q = idxs(g, n, k, j, i);
+    const double old = Vs[q];
+    const double upd = old + rhs[n];
+    if (mode == 1) Vs0[q] = old;
+    Vs[q] = (mode == 2) ? wc * upd + w0 * Vs0[q] : upd;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Pointwise conversions
+// ------------------------------------------------------------------------------------
+// ConvertPrimToCons over a box (convertConsToPrim.hpp:152-189)
+template <class C>
+__global__ __launch_bounds__(256) void k_prim_to_cons(GridDesc g, const double *Vc, double *Uc) {
+  const long long n3 = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  double u[C::NVAR];
+  cell_p2c<C>(g, Vc, p, u);
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) Uc[n * g.sv + p] = u[n];
+}
+
+// ReconstructVcField (boundary.hpp:388-404) + AddAndStore of the cell state
+// (stateContainer.cpp:75-104) + ConvertConsToPrim (convertConsToPrim.hpp:105-149), over
+// the box [lo, hi).  flags: 1 = combine Uc = wc*Uc + w0*Uc0 first, 2 = write Uc back.
+template <class C>
+__global__ __launch_bounds__(256) void k_cons_to_prim(GridDesc g, double *Uc, const double *Uc0,
+                                                      double *Vc, const double *Vs, int lo0,
+                                                      int lo1, int lo2, int n0, int n1, int n2,
+                                                      int flags, double wc, double w0) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n0 * n1 * n2) return;
+  const int i = lo0 + t % n0;
+  const int j = lo1 + (t / n0) % n1;
+  const int k = lo2 + t / ((long long)n0 * n1);
+  const long long p = idx3(g, k, j, i);
+  double u[C::NVAR], v[C::NVAR];
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) {
+    if (C::is_ct_field(n)) continue;
+    u[n] = Uc[n * g.sv + p];
+    if (flags & 1) u[n] = wc * u[n] + w0 * Uc0[n * g.sv + p];
+  }
+  if constexpr (C::MHD) {
+    u[C::BX1] = 0.5 * (Vs[idxs(g, 0, k, j, i)] + Vs[idxs(g, 0, k, j, i + 1)]);
+    if constexpr (C::DIM >= 2)
+      u[C::BX2] = 0.5 * (Vs[idxs(g, 1, k, j, i)] + Vs[idxs(g, 1, k, j + 1, i)]);
+    if constexpr (C::DIM >= 3)
+      u[C::BX3] = 0.5 * (Vs[idxs(g, 2, k, j, i)] + Vs[idxs(g, 2, k + 1, j, i)]);
+  }
+  cons2prim<C>(u, v, g.gamma);
+#pragma unroll
+  for (int n = 0; n < C::NVAR; n++) Vc[n * g.sv + p] = v[n];
+  if (flags & 2) {
+#pragma unroll
+    for (int n = 0; n < C::NVAR; n++) Uc[n * g.sv + p] = u[n];
+  }
+}
+
+// ReconstructVcField over all cells (boundary.hpp:388-404)
+template <class C>
+__global__ __launch_bounds__(256) void k_vc_field(GridDesc g, const double *Vs, double *X) {
+  const long long n3 = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  const int i = p % g.nt[0];
+  const int j = (p / g.nt[0]) % g.nt[1];
+  const int k = p / ((long long)g.nt[0] * g.nt[1]);
+  X[C::BX1 * g.sv + p] = 0.5 * (Vs[idxs(g, 0, k, j, i)] + Vs[idxs(g, 0, k, j, i + 1)]);
+  if constexpr (C::DIM >= 2)
+    X[C::BX2 * g.sv + p] = 0.5 * (Vs[idxs(g, 1, k, j, i)] + Vs[idxs(g, 1, k, j + 1, i)]);
+  if constexpr (C::DIM >= 3)
+    X[C::BX3 * g.sv + p] = 0.5 * (Vs[idxs(g, 2, k, j, i)] + Vs[idxs(g, 2, k + 1, j, i)]);
+}
+
+// AddAndStore over whole arrays (stateContainer.cpp:91-98)
+static __global__ __launch_bounds__(256) void k_add_and_store(double *U, const double *U0, long long n,
+                                                       double wl, double wr) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) U[p] = wl * U[p] + wr * U0[p];
+}
+
+// max over the interior of InvDt (bit pattern), for ComputeTimestep (dataBlock.cpp:333-364)
+static __global__ __launch_bounds__(256) void k_invdt_max(GridDesc g, const double *InvDt,
+                                                   unsigned long long *out) {
+  const int ni = g.end[0] - g.beg[0], nj = g.end[1] - g.beg[1], nk = g.end[2] - g.beg[2];
+  const long long n = (long long)ni * nj * nk;
+  unsigned long long m = 0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = g.beg[0] + t % ni;
+    const int j = g.beg[1] + (t / ni) % nj;
+    const int k = g.beg[2] + t / ((long long)ni * nj);
+    const unsigned long long b = __double_as_longlong(InvDt[idx3(g, k, j, i)]);
+    m = b > m ? b : m;
+  }
+  block_max_atomic(m, out);
+}
+
+// CheckDivB (checkDivB.hpp:14-68): max over the interior, bit pattern
+template <class C>
+__global__ __launch_bounds__(256) void k_divb_max(GridDesc g, const double *Vs,
+                                                  unsigned long long *out) {
+  const int ni = g.end[0] - g.beg[0], nj = g.end[1] - g.beg[1], nk = g.end[2] - g.beg[2];
+  const long long n = (long long)ni * nj * nk;
+  unsigned long long m = 0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = g.beg[0] + t % ni;
+    const int j = g.beg[1] + (t / ni) % nj;
+    const int k = g.beg[2] + t / ((long long)ni * nj);
+    auto vs = [&](int c, int kk, int jj, int ii) { return Vs[idxs(g, c, kk, jj, ii)]; };
+    const double A1 = face_A<C::DIM, 0>(g, k, j, i);
+    double dB = A1 * vs(0, k, j, i + 1) - A1 * vs(0, k, j, i);
+    double d1 = 0.5 * (A1 + A1);
+    double B1 = 0.5 * (vs(0, k, j, i + 1) + vs(0, k, j, i));
+    double am = fabs(B1) * d1;
+    if constexpr (C::DIM >= 2) {
+      const double A2 = face_A<C::DIM, 1>(g, k, j, i);
+      const double dB2 = A2 * vs(1, k, j + 1, i) - A2 * vs(1, k, j, i);
+      const double d2 = 0.5 * (A2 + A2);
+      const double B2 = 0.5 * (vs(1, k, j + 1, i) + vs(1, k, j, i));
+      am = am + fabs(B2) * d2;
+      dB = dB + dB2;
+    }
+    if constexpr (C::DIM >= 3) {
+      const double A3 = face_A<C::DIM, 2>(g, k, j, i);
+      const double dB3 = A3 * vs(2, k + 1, j, i) - A3 * vs(2, k, j, i);
+      const double d3 = 0.5 * (A3 + A3);
+      const double B3 = 0.5 * (vs(2, k + 1, j, i) + vs(2, k, j, i));
+      am = am + fabs(B3) * d3;
+      dB = dB + dB3;
+    }
+    double amplitude = 1e-40;
+    amplitude += am;
+    const double r = fabs(dB) / amplitude;
+    const unsigned long long b = __double_as_longlong(r);
+    m = b > m ? b : m;
+  }
+  block_max_atomic(m, out);
+}
+
+// ------------------------------------------------------------------------------------
+// Boundaries (boundary.hpp): one launch per direction fills both sides.
+// ------------------------------------------------------------------------------------
+enum BcType { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2, BC_USERDEF = 3 };
+
+// Ghost cells of Vc along `dir`, and the tangential Vs components (all components for
+// periodic), for the sides whose type is not user-defined.
+template <class C>
+__global__ __launch_bounds__(256) void k_boundary(GridDesc g, double *Vc, double *Vs, int dir,
+                                                  int lbc, int rbc) {
+  // enumerate ghost sites of the cell array and of the three face arrays in one grid
+  const int ng = g.beg[dir];
+  const int nint = g.end[dir] - g.beg[dir];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // slab sizes (both sides) for cells and for each face component
+  constexpr int NF = C::MHD ? C::DIM : 0;  // face-centred components to fill
+  long long sz[4];
+  int ext[4][3];
+  for (int f = -1; f < NF; f++) {
+    for (int d = 0; d < 3; d++) {
+      const int extra = (f == d) ? 1 : 0;
+      ext[f + 1][d] = (d == dir) ? ng : g.nt[d] + extra;
+    }
+    sz[f + 1] = 2LL * ext[f + 1][0] * ext[f + 1][1] * ext[f + 1][2];
+  }
+  int f = -1;
+  for (; f < NF; f++) {
+    if (t < sz[f + 1]) break;
+    t -= sz[f + 1];
+  }
+  if (f >= NF) return;
+  const int *e3 = ext[f + 1];
+  const long long half = sz[f + 1] / 2;
+  const int side = t >= half ? 1 : 0;
+  if (side) t -= half;
+  int q[3];
+  q[0] = t % e3[0];
+  q[1] = (t / e3[0]) % e3[1];
+  q[2] = t / ((long long)e3[0] * e3[1]);
+  const int bc = side ? rbc : lbc;
+  if (bc == BC_USERDEF) return;
+  // position along dir (BoundaryForAll / BoundaryForX?s ranges, boundary.hpp:969-1099)
+  const int nxd = nint + ((f == dir) ? 1 : 0);
+  q[dir] = side ? (ng + nxd + q[dir]) : q[dir];
+  int r[3] = {q[0], q[1], q[2]};
+  if (f < 0) {
+    for (int n = 0; n < C::NVAR; n++) {
+      double val;
+      if (bc == BC_PERIODIC) {
+        r[dir] = ng + (q[dir] + ng * (nint - 1)) % nint;
+        val = Vc[n * g.sv + idx3(g, r[2], r[1], r[0])];
+      } else if (bc == BC_OUTFLOW) {
+        r[dir] = ng + side * (nint - 1);
+        val = Vc[n * g.sv + idx3(g, r[2], r[1], r[0])];
+        const int sign = 1 - 2 * side;
+        if (n == C::MX1 + dir && sign * val >= 0.0) val = 0.0;
+      } else {
+        r[dir] = 2 * (ng + side * nint) - q[dir] - 1;
+        const double sign =
+            (n == C::MX1 + dir || (C::MHD && n >= C::BX1 && n != C::BX1 + dir)) ? -1.0 : 1.0;
+        val = sign * Vc[n * g.sv + idx3(g, r[2], r[1], r[0])];
+      }
+      Vc[n * g.sv + idx3(g, q[2], q[1], q[0])] = val;
+    }
+  } else {
+    if (bc == BC_PERIODIC) {
+      r[dir] = ng + (q[dir] + ng * (nint - 1)) % nint;
+      Vs[idxs(g, f, q[2], q[1], q[0])] = Vs[idxs(g, f, r[2], r[1], r[0])];
+    } else if (f != dir) {
+      if (bc == BC_OUTFLOW) {
+        r[dir] = ng + side * (nint - 1);
+        Vs[idxs(g, f, q[2], q[1], q[0])] = Vs[idxs(g, f, r[2], r[1], r[0])];
+      } else {
+        r[dir] = 2 * (ng + side * nint) - q[dir] - 1;
+        Vs[idxs(g, f, q[2], q[1], q[0])] = -Vs[idxs(g, f, r[2], r[1], r[0])];
+      }
+    }
+  }
+}
+
+// ReconstructNormalField (boundary.hpp:407-530), Cartesian without axis: one thread per
+// transverse site, sequential over the ghost layers.
+template <class C>
+__global__ __launch_bounds__(256) void k_normal_field(GridDesc g, double *Vs, int dir, int recL,
+                                                      int recR) {
+  const int o1 = (dir == 0) ? 1 : 0;
+  const int o2 = (dir == 2) ? 1 : 2;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nt[o1] * g.nt[o2]) return;
+  int idx[3];
+  idx[o1] = t % g.nt[o1];
+  idx[o2] = t / g.nt[o1];
+  auto vs = [&](int c, int kk, int jj, int ii) -> double & { return Vs[idxs(g, c, kk, jj, ii)]; };
+  auto A = [&](int d, int kk, int jj, int ii) {
+    if (d == 0) return face_A<C::DIM, 0>(g, kk, jj, ii);
+    if (d == 1) return face_A<C::DIM, 1>(g, kk, jj, ii);
+    return face_A<C::DIM, 2>(g, kk, jj, ii);
+  };
+  auto trans = [&](int k, int j, int i) {
+    if (dir == 0) {
+      double s = 0.0;
+      if constexpr (C::DIM >= 2) {
+        s = s + A(1, k, j + 1, i) * vs(1, k, j + 1, i);
+        s = s - A(1, k, j, i) * vs(1, k, j, i);
+      }
+      if constexpr (C::DIM >= 3) {
+        s = s + A(2, k + 1, j, i) * vs(2, k + 1, j, i);
+        s = s - A(2, k, j, i) * vs(2, k, j, i);
+      }
+      return s;
+    } else if (dir == 1) {
+      double s = A(0, k, j, i + 1) * vs(0, k, j, i + 1) - A(0, k, j, i) * vs(0, k, j, i);
+      if constexpr (C::DIM >= 3)
+        s = s + 1 * (A(2, k + 1, j, i) * vs(2, k + 1, j, i) - A(2, k, j, i) * vs(2, k, j, i));
+      return s;
+    } else {
+      double s = A(0, k, j, i + 1) * vs(0, k, j, i + 1) - A(0, k, j, i) * vs(0, k, j, i);
+      s = s + A(1, k, j + 1, i) * vs(1, k, j + 1, i);
+      s = s - A(1, k, j, i) * vs(1, k, j, i);
+      return s;
+    }
+  };
+  const int nstart = g.beg[dir] - 1, nend = g.end[dir], nd = g.nt[dir];
+  if (recL) {
+    for (int q = nstart; q >= 0; q--) {
+      idx[dir] = q;
+      const int k = idx[2], j = idx[1], i = idx[0];
+      const int kp = k + (dir == 2), jp = j + (dir == 1), ip = i + (dir == 0);
+      vs(dir, k, j, i) = 1.0 / A(dir, k, j, i) * (A(dir, kp, jp, ip) * vs(dir, kp, jp, ip) + (trans(k, j, i)));
+    }
+  }
+  if (recR) {
+    for (int q = nend; q < nd; q++) {
+      idx[dir] = q;
+      const int k = idx[2], j = idx[1], i = idx[0];
+      const int kp = k + (dir == 2), jp = j + (dir == 1), ip = i + (dir == 0);
+      vs(dir, kp, jp, ip) = 1.0 / A(dir, kp, jp, ip) * (A(dir, k, j, i) * vs(dir, k, j, i) - (trans(k, j, i)));
+    }
+  }
+}
+
+}  // namespace idfx

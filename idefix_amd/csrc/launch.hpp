@@ -1,0 +1,264 @@
+// Host-side launchers for the kernels of kernels.hpp, gathered per compile-time
+// configuration into a type-erased operation table that the C ABI dispatches on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace idfx {
+
+// Type-erased operations of one (configuration, compute type) pair.
+struct Ops {
+  bool valid = false;
+  int dim = 0, nvar = 0;
+  hipError_t (*sweep)(int dir, bool fused, const GridDesc &g, const SweepArgs &a,
+                      const void *emf, int target_waves, hipStream_t s) = nullptr;
+  hipError_t (*corner_emf)(const GridDesc &g, const double *Vc, const double *Vs,
+                           const void *emf, int scheme, hipStream_t s) = nullptr;
+  hipError_t (*evolve_b)(const GridDesc &g, double *Vs, double *Vs0, const void *emf,
+                         double dt, int mode, double wc, double w0, hipStream_t s) = nullptr;
+  hipError_t (*prim_to_cons)(const GridDesc &g, const double *Vc, double *Uc,
+                             hipStream_t s) = nullptr;
+  hipError_t (*cons_to_prim)(const GridDesc &g, double *Uc, const double *Uc0, double *Vc,
+                             const double *Vs, const int *lo, const int *hi, int flags,
+                             double wc, double w0, hipStream_t s) = nullptr;
+  hipError_t (*vc_field)(const GridDesc &g, const double *Vs, double *X,
+                         hipStream_t s) = nullptr;
+  hipError_t (*boundary)(const GridDesc &g, double *Vc, double *Vs, int dir, int lbc, int rbc,
+                         hipStream_t s) = nullptr;
+  hipError_t (*normal_field)(const GridDesc &g, double *Vs, int dir, int recL, int recR,
+                             hipStream_t s) = nullptr;
+  hipError_t (*correct_flux)(int dir, const GridDesc &g, double *Flux, hipStream_t s) = nullptr;
+  hipError_t (*rhs)(int dir, const GridDesc &g, const double *Flux, const double *cMax,
+                    double *Uc, double *InvDt, double dt, hipStream_t s) = nullptr;
+  hipError_t (*divb)(const GridDesc &g, const double *Vs, unsigned long long *out,
+                     hipStream_t s) = nullptr;
+};
+
+inline unsigned blocks_for(long long n, int bs = 256) {
+  return (unsigned)std::max<long long>(1, (n + bs - 1) / bs);
+}
+
+template <class C, typename T, int DIR, bool FUSED>
+hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
+                            int target_waves, hipStream_t s) {
+  if constexpr (DIR == IDIR) {
+    const int nint = g.end[0] - g.beg[0];
+    const int nseg = (nint + 62) / 63;
+    const long long nj = g.end[1] - g.beg[1] + 2 * a.ext[1];
+    const long long nk = g.end[2] - g.beg[2] + 2 * a.ext[2];
+    const long long waves = (long long)nseg * nj * nk;
+    hipLaunchKernelGGL((k_sweep_x<C, T, FUSED>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g,
+                       a, e, nseg);
+  } else {
+    MarchGeom m;
+    m.ilo = g.beg[0] - a.ext[0];
+    m.ihi = g.end[0] + a.ext[0];
+    constexpr int OT = (DIR == JDIR) ? 2 : 1;  // the other transverse direction
+    m.tlo = g.beg[OT] - a.ext[OT];
+    m.thi = g.end[OT] + a.ext[OT];
+    m.nxb = (m.ihi - m.ilo + 63) / 64;
+    const int nd = g.end[DIR] - g.beg[DIR];
+    const long long columns = (long long)m.nxb * (m.thi - m.tlo);
+    // segment length: enough waves to fill the chip, long enough to amortise the
+    // extra face and the window priming of each segment
+    int ts = 64;
+    while (ts > 8 && columns * ((nd + ts - 1) / ts) < target_waves) ts /= 2;
+    m.ts = std::min(ts, nd);
+    m.nseg = (nd + m.ts - 1) / m.ts;
+    const long long waves = columns * m.nseg;
+    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED>), dim3(blocks_for(waves, 4)), dim3(256),
+                       0, s, g, a, e, m);
+  }
+  return hipGetLastError();
+}
+
+template <class C, typename T>
+hipError_t launch_sweep(int dir, bool fused, const GridDesc &g, const SweepArgs &a,
+                        const void *emf, int target_waves, hipStream_t s) {
+  const EmfArrays<T> &e = *static_cast<const EmfArrays<T> *>(emf);
+  if (dir == 0) {
+    if (fused) return launch_sweep_dir<C, T, 0, true>(g, a, e, target_waves, s);
+    if constexpr (std::is_same<T, double>::value)
+      return launch_sweep_dir<C, T, 0, false>(g, a, e, target_waves, s);
+  }
+  if constexpr (C::DIM >= 2) {
+    if (dir == 1) {
+      if (fused) return launch_sweep_dir<C, T, 1, true>(g, a, e, target_waves, s);
+      if constexpr (std::is_same<T, double>::value)
+        return launch_sweep_dir<C, T, 1, false>(g, a, e, target_waves, s);
+    }
+  }
+  if constexpr (C::DIM >= 3) {
+    if (dir == 2) {
+      if (fused) return launch_sweep_dir<C, T, 2, true>(g, a, e, target_waves, s);
+      if constexpr (std::is_same<T, double>::value)
+        return launch_sweep_dir<C, T, 2, false>(g, a, e, target_waves, s);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class C, typename T>
+hipError_t launch_corner_emf(const GridDesc &g, const double *Vc, const double *Vs,
+                             const void *emf, int scheme, hipStream_t s) {
+  if constexpr (C::MHD && C::DIM >= 2) {
+    const long long n = (long long)(g.end[0] + 1 - g.beg[0]) * (g.end[1] + 1 - g.beg[1]) *
+                        (g.end[2] + (C::DIM == 3) - g.beg[2]);
+    hipLaunchKernelGGL((k_corner_emf<C, T>), dim3(blocks_for(n)), dim3(256), 0, s, g, Vc, Vs,
+                       *static_cast<const EmfArrays<T> *>(emf), scheme);
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
+template <class C, typename T>
+hipError_t launch_evolve_b(const GridDesc &g, double *Vs, double *Vs0, const void *emf, double dt,
+                           int mode, double wc, double w0, hipStream_t s) {
+  if constexpr (C::MHD && C::DIM >= 2) {
+    const long long n = (long long)(g.end[0] + 1 - g.beg[0]) * (g.end[1] + 1 - g.beg[1]) *
+                        (g.end[2] + (C::DIM == 3) - g.beg[2]);
+    hipLaunchKernelGGL((k_evolve_b<C, T>), dim3(blocks_for(n)), dim3(256), 0, s, g, Vs, Vs0,
+                       *static_cast<const EmfArrays<T> *>(emf), dt, mode, wc, w0);
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
+template <class C>
+hipError_t launch_prim_to_cons(const GridDesc &g, const double *Vc, double *Uc, hipStream_t s) {
+  const long long n = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+  hipLaunchKernelGGL((k_prim_to_cons<C>), dim3(blocks_for(n)), dim3(256), 0, s, g, Vc, Uc);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_cons_to_prim(const GridDesc &g, double *Uc, const double *Uc0, double *Vc,
+                               const double *Vs, const int *lo, const int *hi, int flags,
+                               double wc, double w0, hipStream_t s) {
+  const int n0 = hi[0] - lo[0], n1 = hi[1] - lo[1], n2 = hi[2] - lo[2];
+  const long long n = (long long)n0 * n1 * n2;
+  hipLaunchKernelGGL((k_cons_to_prim<C>), dim3(blocks_for(n)), dim3(256), 0, s, g, Uc, Uc0, Vc,
+                     Vs, lo[0], lo[1], lo[2], n0, n1, n2, flags, wc, w0);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_vc_field(const GridDesc &g, const double *Vs, double *X, hipStream_t s) {
+  if constexpr (C::MHD) {
+    const long long n = (long long)g.nt[0] * g.nt[1] * g.nt[2];
+    hipLaunchKernelGGL((k_vc_field<C>), dim3(blocks_for(n)), dim3(256), 0, s, g, Vs, X);
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
+template <class C>
+hipError_t launch_boundary(const GridDesc &g, double *Vc, double *Vs, int dir, int lbc, int rbc,
+                           hipStream_t s) {
+  constexpr int NF = C::MHD ? C::DIM : 0;
+  long long total = 0;
+  for (int f = -1; f < NF; f++) {
+    long long sz = 2;
+    for (int d = 0; d < 3; d++) sz *= (d == dir) ? g.beg[dir] : g.nt[d] + ((f == d) ? 1 : 0);
+    total += sz;
+  }
+  hipLaunchKernelGGL((k_boundary<C>), dim3(blocks_for(total)), dim3(256), 0, s, g, Vc, Vs, dir,
+                     lbc, rbc);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_normal_field(const GridDesc &g, double *Vs, int dir, int recL, int recR,
+                               hipStream_t s) {
+  if constexpr (C::MHD) {
+    const int o1 = (dir == 0) ? 1 : 0;
+    const int o2 = (dir == 2) ? 1 : 2;
+    const long long n = (long long)g.nt[o1] * g.nt[o2];
+    hipLaunchKernelGGL((k_normal_field<C>), dim3(blocks_for(n)), dim3(256), 0, s, g, Vs, dir,
+                       recL, recR);
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
+template <class C, int DIR>
+hipError_t launch_correct_rhs_dir(bool rhs, const GridDesc &g, double *Flux, const double *cMax,
+                                  double *Uc, double *InvDt, double dt, hipStream_t s) {
+  constexpr int io = DIR == 0, jo = DIR == 1, ko = DIR == 2;
+  if (!rhs) {
+    const long long n = (long long)(g.end[0] + io - g.beg[0]) * (g.end[1] + jo - g.beg[1]) *
+                        (g.end[2] + ko - g.beg[2]);
+    hipLaunchKernelGGL((k_correct_flux<C, DIR>), dim3(blocks_for(n)), dim3(256), 0, s, g, Flux);
+  } else {
+    const long long n = (long long)(g.end[0] - g.beg[0]) * (g.end[1] - g.beg[1]) *
+                        (g.end[2] - g.beg[2]);
+    hipLaunchKernelGGL((k_rhs<C, DIR>), dim3(blocks_for(n)), dim3(256), 0, s, g, Flux, cMax, Uc,
+                       InvDt, dt);
+  }
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_correct_flux(int dir, const GridDesc &g, double *Flux, hipStream_t s) {
+  if (dir == 0) return launch_correct_rhs_dir<C, 0>(false, g, Flux, nullptr, nullptr, nullptr, 0, s);
+  if constexpr (C::DIM >= 2)
+    if (dir == 1) return launch_correct_rhs_dir<C, 1>(false, g, Flux, nullptr, nullptr, nullptr, 0, s);
+  if constexpr (C::DIM >= 3)
+    if (dir == 2) return launch_correct_rhs_dir<C, 2>(false, g, Flux, nullptr, nullptr, nullptr, 0, s);
+  return hipErrorInvalidValue;
+}
+
+template <class C>
+hipError_t launch_rhs(int dir, const GridDesc &g, const double *Flux, const double *cMax,
+                      double *Uc, double *InvDt, double dt, hipStream_t s) {
+  double *F = const_cast<double *>(Flux);
+  if (dir == 0) return launch_correct_rhs_dir<C, 0>(true, g, F, cMax, Uc, InvDt, dt, s);
+  if constexpr (C::DIM >= 2)
+    if (dir == 1) return launch_correct_rhs_dir<C, 1>(true, g, F, cMax, Uc, InvDt, dt, s);
+  if constexpr (C::DIM >= 3)
+    if (dir == 2) return launch_correct_rhs_dir<C, 2>(true, g, F, cMax, Uc, InvDt, dt, s);
+  return hipErrorInvalidValue;
+}
+
+template <class C>
+hipError_t launch_divb(const GridDesc &g, const double *Vs, unsigned long long *out,
+                       hipStream_t s) {
+  if constexpr (C::MHD) {
+    hipLaunchKernelGGL((k_divb_max<C>), dim3(1024), dim3(256), 0, s, g, Vs, out);
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
+template <class C, typename T>
+Ops make_ops() {
+  Ops o;
+  o.valid = true;
+  o.dim = C::DIM;
+  o.nvar = C::NVAR;
+  o.sweep = &launch_sweep<C, T>;
+  o.corner_emf = &launch_corner_emf<C, T>;
+  o.evolve_b = &launch_evolve_b<C, T>;
+  o.prim_to_cons = &launch_prim_to_cons<C>;
+  o.cons_to_prim = &launch_cons_to_prim<C>;
+  o.vc_field = &launch_vc_field<C>;
+  o.boundary = &launch_boundary<C>;
+  o.normal_field = &launch_normal_field<C>;
+  o.correct_flux = &launch_correct_flux<C>;
+  o.rhs = &launch_rhs<C>;
+  o.divb = &launch_divb<C>;
+  return o;
+}
+
+// Registry key: one entry per built configuration and precision.
+inline int ops_key(int dim, int comps, int mhd, int order, int solver, int fp32) {
+  return ((((dim * 4 + comps) * 2 + mhd) * 8 + order) * 4 + solver) * 2 + fp32;
+}
+constexpr int kOpsTableSize = (((3 * 4 + 3) * 2 + 1) * 8 + 7) * 4 * 2 + 8;
+Ops *ops_table();  // defined in capi.hip
+
+}  // namespace idfx

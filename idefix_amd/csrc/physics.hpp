@@ -1,0 +1,718 @@
+// Device-side physics of the per-stage finite-volume update, written for gfx950.
+//
+// Every function is templated on the compile-time configuration `C` (dimensions,
+// components, MHD, reconstruction order, Riemann solver) and on the compute type `T`
+// (double for the plain FP64 method, float for the FP32-stencil mode).  The
+// floating-point operation order of each expression follows the reference function
+// cited beside it, so that FP64 results match the reference to the last bit when the
+// code is compiled without FMA contraction (-ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace idfx {
+
+constexpr int IDIR = 0, JDIR = 1, KDIR = 2;
+enum Recon { RECON_CONSTANT = 1, RECON_PLM = 2, RECON_PPM = 4 };
+enum Solver { SOLVER_HLLC = 1, SOLVER_HLLD = 2, SOLVER_ROE = 3 };
+enum EmfScheme { EMF_ARITHMETIC = 0, EMF_UCT0 = 1, EMF_UCT_CONTACT = 2, EMF_UCT_HLL = 3,
+                 EMF_UCT_HLLD = 4 };
+
+// Compile-time configuration; variable indices follow src/idefix.hpp:74-108.
+template <int DIM_, int COMP_, bool MHD_, int ORDER_, int SOLVER_>
+struct Cfg {
+  static constexpr int DIM = DIM_;
+  static constexpr int COMP = COMP_;
+  static constexpr bool MHD = MHD_;
+  static constexpr int ORDER = ORDER_;
+  static constexpr int SOLVER = SOLVER_;
+  static constexpr int RHO = 0, MX1 = 1;
+  static constexpr int MX2 = COMP >= 2 ? 2 : -1;
+  static constexpr int MX3 = COMP == 3 ? 3 : -1;
+  static constexpr int ENG = COMP + 1, PRS = COMP + 1;
+  static constexpr int BX1 = MHD ? COMP + 2 : -1;
+  static constexpr int BX2 = (MHD && COMP >= 2) ? COMP + 3 : -1;
+  static constexpr int BX3 = (MHD && COMP == 3) ? COMP + 4 : -1;
+  static constexpr int NVAR = MHD ? 2 * COMP + 2 : COMP + 2;
+  static constexpr int NG = ORDER < 4 ? 2 : 3;  // grid.cpp:60-65
+  // per-direction indices (hlldMHD.hpp:153-159)
+  template <int DIR> static constexpr int Xn() { return MX1 + DIR; }
+  template <int DIR> static constexpr int Xt() { return DIR == IDIR ? MX2 : MX1; }
+  template <int DIR> static constexpr int Xb() { return DIR == KDIR ? MX2 : MX3; }
+  template <int DIR> static constexpr int BXn() { return MHD ? BX1 + DIR : -1; }
+  template <int DIR> static constexpr int BXt() { return DIR == IDIR ? BX2 : BX1; }
+  template <int DIR> static constexpr int BXb() { return DIR == KDIR ? BX2 : BX3; }
+  // a variable that the cell-update loop skips (magnetic components handled by CT)
+  static constexpr bool is_ct_field(int n) {
+    return MHD && ((n == BX1) || (DIM >= 2 && n == BX2) || (DIM >= 3 && n == BX3));
+  }
+};
+
+// FMIN/FMAX/FABS of the reference are fmin/fmax/fabs (real_types.hpp:49-51)
+template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return fmin(a, b); }
+template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T fab(T a) { return fabs(a); }
+__device__ __forceinline__ float fmn(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float fab(float a) { return fabsf(a); }
+template <typename T> __device__ __forceinline__ T sqrt_(T a) { return sqrt(a); }
+__device__ __forceinline__ float sqrt_(float a) { return sqrtf(a); }
+
+// ------------------------------------------------------------------------------------
+// Slope limiters: src/fluid/RiemannSolver/slopeLimiter.hpp
+// ------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T vanleer(T dvp, T dvm) {  // slopeLimiter.hpp:54-57
+  return (dvp * dvm > T(0) ? T(2) * dvp * dvm / (dvp + dvm) : T(0));
+}
+template <typename T>
+__device__ __forceinline__ T mclim(T dvp, T dvm) {  // slopeLimiter.hpp:67-75
+  T dq = 0;
+  if (dvp * dvm > T(0)) {
+    T dqc = T(0.5) * (dvp + dvm);
+    T d2q = T(2) * (fab(dvp) < fab(dvm) ? dvp : dvm);
+    dq = fab(d2q) < fab(dqc) ? d2q : dqc;
+  }
+  return dq;
+}
+template <typename T>
+__device__ __forceinline__ int sgn(T v) { return (T(0) < v) - (v < T(0)); }
+
+template <typename T>
+__device__ __forceinline__ void limit_ppm_face(T vm1, T v0, T vp1, T vp2, T &vph) {
+  // limitPPMFaceValues, slopeLimiter.hpp:121-140
+  if ((vp1 - vph) * (vph - v0) < T(0)) {
+    const T deltaL = (vm1 - 2 * v0 + vp1);
+    const T deltaC = 3 * (v0 - 2 * vph + vp1);
+    const T deltaR = (v0 - 2 * vp1 + vp2);
+    T delta = 0;
+    if (sgn(deltaL) == sgn(deltaC) && sgn(deltaR) == sgn(deltaC)) {
+      const T Cc = T(1.25);
+      delta = Cc * fmn(fab(deltaL), fab(deltaR));
+      delta = sgn(deltaC) * fmn(delta, fab(deltaC));
+    }
+    vph = T(0.5) * (v0 + vp1) - delta / T(6.0);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void ppm_states(T vm2, T vm1, T v0, T vp1, T vp2, T &vl, T &vr) {
+  // getPPMStates, slopeLimiter.hpp:142-191
+  const int n = 2;
+  vr = T(7.0) / T(12.0) * (v0 + vp1) - T(1.0) / T(12.0) * (vm1 + vp2);
+  vl = T(7.0) / T(12.0) * (vm1 + v0) - T(1.0) / T(12.0) * (vm2 + vp1);
+  limit_ppm_face(vm2, vm1, v0, vp1, vl);
+  limit_ppm_face(vm1, v0, vp1, vp2, vr);
+  T d2qf = T(6.0) * (vl + vr - T(2.0) * v0);
+  T d2qc0 = vm1 + vp1 - T(2.0) * v0;
+  T d2qcp1 = v0 + vp2 - T(2.0) * vp1;
+  T d2qcm1 = vm2 + v0 - T(2.0) * vm1;
+  T d2q = 0;
+  if (sgn(d2qf) == sgn(d2qc0) && sgn(d2qf) == sgn(d2qcp1) && sgn(d2qf) == sgn(d2qcm1)) {
+    const T Cc = T(1.25);
+    d2q = fmn(fab(d2qc0), fab(d2qcp1));
+    d2q = Cc * fmn(fab(d2qcm1), d2q);
+    d2q = sgn(d2qf) * fmn(fab(d2qf), d2q);
+  }
+  T qmax = fmx(fmx(fab(vm1), fab(v0)), fab(vp1));
+  T rho = 0;
+  if (fab(d2qf) > T(1e-12) * qmax) rho = d2q / d2qf;
+  if (((vr - v0) * (v0 - vl) <= T(0)) || (vm1 - v0) * (v0 - vp1) <= T(0)) {
+    if (rho <= (T(1.0) - T(1e-12))) {
+      vl = v0 - rho * (v0 - vl);
+      vr = v0 + rho * (vr - v0);
+    }
+  } else {
+    if (fab(vr - v0) >= n * fab(v0 - vl)) vr = v0 + n * (v0 - vl);
+    if (fab(vl - v0) >= n * fab(v0 - vr)) vl = v0 + n * (v0 - vr);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Pointwise conversions: src/fluid/convertConsToPrim.hpp, eos_adiabatic.hpp:39-45
+// ------------------------------------------------------------------------------------
+template <class C, typename T>
+__device__ __forceinline__ void prim2cons(const T *v, T *u, T gamma) {
+  // K_PrimToCons, convertConsToPrim.hpp:69-100
+  u[C::RHO] = v[C::RHO];
+  u[C::MX1] = v[C::MX1] * v[C::RHO];
+  if constexpr (C::COMP >= 2) u[C::MX2] = v[C::MX2] * v[C::RHO];
+  if constexpr (C::COMP >= 3) u[C::MX3] = v[C::MX3] * v[C::RHO];
+  T v2 = v[C::MX1] * v[C::MX1];
+  if constexpr (C::COMP >= 2) v2 = v2 + v[C::MX2] * v[C::MX2];
+  if constexpr (C::COMP >= 3) v2 = v2 + v[C::MX3] * v[C::MX3];
+  const T eint = v[C::ENG] / (gamma - T(1.0));
+  if constexpr (C::MHD) {
+    u[C::BX1] = v[C::BX1];
+    if constexpr (C::COMP >= 2) u[C::BX2] = v[C::BX2];
+    if constexpr (C::COMP >= 3) u[C::BX3] = v[C::BX3];
+    T b2 = u[C::BX1] * u[C::BX1];
+    if constexpr (C::COMP >= 2) b2 = b2 + u[C::BX2] * u[C::BX2];
+    if constexpr (C::COMP >= 3) b2 = b2 + u[C::BX3] * u[C::BX3];
+    u[C::ENG] = eint + T(0.5) * v[C::RHO] * v2 + T(0.5) * b2;
+  } else {
+    u[C::ENG] = eint + T(0.5) * v[C::RHO] * v2;
+  }
+}
+
+// K_ConsToPrim, convertConsToPrim.hpp:15-67: always evaluated in FP64 (the
+// energy-to-pressure recovery suffers cancellation at low plasma beta).
+// SMALL_PRESSURE_FIX = 1e-5 (fluid_defs.hpp:23); the re-synced energy is local.
+template <class C>
+__device__ __forceinline__ void cons2prim(const double *u, double *v, double gamma) {
+  v[C::RHO] = u[C::RHO];
+  v[C::MX1] = u[C::MX1] / u[C::RHO];
+  if constexpr (C::COMP >= 2) v[C::MX2] = u[C::MX2] / u[C::RHO];
+  if constexpr (C::COMP >= 3) v[C::MX3] = u[C::MX3] / u[C::RHO];
+  double m2 = u[C::MX1] * u[C::MX1];
+  if constexpr (C::COMP >= 2) m2 = m2 + u[C::MX2] * u[C::MX2];
+  if constexpr (C::COMP >= 3) m2 = m2 + u[C::MX3] * u[C::MX3];
+  const double kin = 0.5 / u[C::RHO] * m2;
+  if constexpr (C::MHD) {
+    v[C::BX1] = u[C::BX1];
+    if constexpr (C::COMP >= 2) v[C::BX2] = u[C::BX2];
+    if constexpr (C::COMP >= 3) v[C::BX3] = u[C::BX3];
+    double b2 = u[C::BX1] * u[C::BX1];
+    if constexpr (C::COMP >= 2) b2 = b2 + u[C::BX2] * u[C::BX2];
+    if constexpr (C::COMP >= 3) b2 = b2 + u[C::BX3] * u[C::BX3];
+    const double mag = 0.5 * b2;
+    v[C::ENG] = (u[C::ENG] - kin - mag) * (gamma - 1.0);
+  } else {
+    v[C::ENG] = (u[C::ENG] - kin) * (gamma - 1.0);
+  }
+  if (v[C::ENG] <= 0.0) v[C::ENG] = 1.0e-5;
+}
+
+// ------------------------------------------------------------------------------------
+// K_Flux<Phys,DIR>, src/fluid/RiemannSolver/flux.hpp:28-95
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void kflux(T *F, const T *v, const T *u) {
+  constexpr int Xn = C::template Xn<DIR>();
+  F[C::RHO] = u[Xn];
+  if constexpr (C::MHD) {
+    constexpr int BXn = C::template BXn<DIR>();
+    constexpr int BXt = C::template BXt<DIR>();
+    constexpr int BXb = C::template BXb<DIR>();
+    constexpr int Xt = C::template Xt<DIR>();
+    constexpr int Xb = C::template Xb<DIR>();
+    F[C::MX1] = u[C::MX1] * v[Xn] - v[BXn] * v[C::BX1];
+    if constexpr (C::COMP >= 2) F[C::MX2] = u[C::MX2] * v[Xn] - v[BXn] * v[C::BX2];
+    if constexpr (C::COMP >= 3) F[C::MX3] = u[C::MX3] * v[Xn] - v[BXn] * v[C::BX3];
+    F[BXn] = T(0);
+    if constexpr (C::COMP >= 2) F[BXt] = v[Xn] * v[BXt] - v[BXn] * v[Xt];
+    if constexpr (C::COMP >= 3) F[BXb] = v[Xn] * v[BXb] - v[BXn] * v[Xb];
+    T b2 = v[C::BX1] * v[C::BX1];
+    if constexpr (C::COMP >= 2) b2 = b2 + v[C::BX2] * v[C::BX2];
+    if constexpr (C::COMP >= 3) b2 = b2 + v[C::BX3] * v[C::BX3];
+    const T ptot = v[C::ENG] + T(0.5) * b2;
+    T vb = v[C::MX1] * v[C::BX1];
+    if constexpr (C::COMP >= 2) vb = vb + v[C::MX2] * v[C::BX2];
+    if constexpr (C::COMP >= 3) vb = vb + v[C::MX3] * v[C::BX3];
+    F[C::ENG] = (u[C::ENG] + ptot) * v[Xn] - v[BXn] * vb;
+    F[Xn] += ptot;
+  } else {
+    F[C::MX1] = u[C::MX1] * v[Xn];
+    if constexpr (C::COMP >= 2) F[C::MX2] = u[C::MX2] * v[Xn];
+    if constexpr (C::COMP >= 3) F[C::MX3] = u[C::MX3] * v[Xn];
+    const T ptot = v[C::ENG];
+    F[C::ENG] = (u[C::ENG] + ptot) * v[Xn];
+    F[Xn] += ptot;
+  }
+}
+
+// Face data the constrained-transport step needs (storeFlux.hpp:14-208).
+template <typename T>
+struct FaceEmf {
+  T Et, Eb, sv, aL, aR, dL, dR;
+};
+
+// ------------------------------------------------------------------------------------
+// HllcHD, src/fluid/RiemannSolver/HDsolvers/hllcHD.hpp:35-163 (HAVE_ENERGY branch)
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hllc_hd(const T *vL, const T *vR, T *flux, T &cmax, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  T uL[NV], uR[NV], fL[NV], fR[NV];
+  const T cL = sqrt_(gamma * (vL[C::ENG] / vL[C::RHO]));
+  const T cR = sqrt_(gamma * (vR[C::ENG] / vR[C::RHO]));
+  const T cminL = vL[Xn] - cL, cmaxL = vL[Xn] + cL;
+  const T cminR = vR[Xn] - cR, cmaxR = vR[Xn] + cR;
+  const T SL = fmn(cminL, cminR);
+  const T SR = fmx(cmaxL, cmaxR);
+  cmax = fmx(fab(SL), fab(SR));
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fL, vL, uL);
+  kflux<C, DIR>(fR, vR, uR);
+  if (SL > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fL[n];
+  } else if (SR < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fR[n];
+  } else {
+    T usL[NV], usR[NV];
+    const T qL = vL[C::ENG] + uL[Xn] * (vL[Xn] - SL);
+    const T qR = vR[C::ENG] + uR[Xn] * (vR[Xn] - SR);
+    const T wL = vL[C::RHO] * (vL[Xn] - SL);
+    const T wR = vR[C::RHO] * (vR[Xn] - SR);
+    const T vs = (qR - qL) / (wR - wL);
+    usL[C::RHO] = uL[C::RHO] * (SL - vL[Xn]) / (SL - vs);
+    usR[C::RHO] = uR[C::RHO] * (SR - vR[Xn]) / (SR - vs);
+    usL[Xn] = usL[C::RHO] * vs;
+    usR[Xn] = usR[C::RHO] * vs;
+    if constexpr (C::COMP >= 2) {
+      usL[Xt] = usL[C::RHO] * vL[Xt];
+      usR[Xt] = usR[C::RHO] * vR[Xt];
+    }
+    if constexpr (C::COMP >= 3) {
+      usL[Xb] = usL[C::RHO] * vL[Xb];
+      usR[Xb] = usR[C::RHO] * vR[Xb];
+    }
+    usL[C::ENG] = uL[C::ENG] / vL[C::RHO] +
+                  (vs - vL[Xn]) * (vs + vL[C::ENG] / (vL[C::RHO] * (SL - vL[Xn])));
+    usR[C::ENG] = uR[C::ENG] / vR[C::RHO] +
+                  (vs - vR[Xn]) * (vs + vR[C::ENG] / (vR[C::RHO] * (SR - vR[Xn])));
+    usL[C::ENG] *= usL[C::RHO];
+    usR[C::ENG] *= usR[C::RHO];
+    if (vs >= T(0)) {
+#pragma unroll
+      for (int n = 0; n < NV; n++) flux[n] = fL[n] + SL * (usL[n] - uL[n]);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; n++) flux[n] = fR[n] + SR * (usR[n] - uR[n]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// RoeHD, src/fluid/RiemannSolver/HDsolvers/roeHD.hpp:55-364 (HAVE_ENERGY,
+// ROE_AVERAGE == 0).  Returns false where the reference leaves the flux unwritten
+// (strong shock in 1-D: the HLL switch exists only under DIMENSIONS > 1).
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ bool roe_hd(const T *vL, const T *vR, T *flux, T &cmax, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  T dv[NV], uL[NV], uR[NV], fL[NV], fR[NV], um[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) dv[n] = vR[n] - vL[n];
+  T a2L = sqrt_(gamma * (vL[C::ENG] / vL[C::RHO]));
+  T a2R = sqrt_(gamma * (vR[C::ENG] / vR[C::RHO]));
+  a2L = a2L * a2L;
+  a2R = a2R * a2R;
+  (void)a2L;
+  (void)a2R;
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fL, vL, uL);
+  kflux<C, DIR>(fR, vR, uR);
+  const T gamma_m1 = gamma - 1;
+#pragma unroll
+  for (int n = 0; n < NV; n++) um[n] = T(0.5) * (vR[n] + vL[n]);
+  const T a2 = gamma * um[C::ENG] / um[C::RHO];
+  const T a = sqrt_(a2);
+  T vel2 = um[C::MX1] * um[C::MX1];
+  if constexpr (C::COMP >= 2) vel2 = vel2 + um[C::MX2] * um[C::MX2];
+  if constexpr (C::COMP >= 3) vel2 = vel2 + um[C::MX3] * um[C::MX3];
+  const T h = T(0.5) * vel2 + a2 / gamma_m1;
+
+  // modes I0, I1, IE, I2, I3 = 0..4 (roeHD.hpp:20-27); Rc[var][mode]
+  T lambda[NV], eta[NV], Rc[NV][NV];
+#pragma unroll
+  for (int p = 0; p < NV; p++)
+#pragma unroll
+    for (int q = 0; q < NV; q++) Rc[p][q] = 0;
+  lambda[0] = um[Xn] - a;
+  eta[0] = T(0.5) / a2 * (dv[C::ENG] - dv[Xn] * um[C::RHO] * a);
+  Rc[C::RHO][0] = 1;
+  Rc[Xn][0] = um[Xn] - a;
+  if constexpr (C::COMP >= 2) Rc[Xt][0] = um[Xt];
+  if constexpr (C::COMP >= 3) Rc[Xb][0] = um[Xb];
+  Rc[C::ENG][0] = h - um[Xn] * a;
+  lambda[1] = um[Xn] + a;
+  eta[1] = T(0.5) / a2 * (dv[C::ENG] + dv[Xn] * um[C::RHO] * a);
+  Rc[C::RHO][1] = 1;
+  Rc[Xn][1] = um[Xn] + a;
+  if constexpr (C::COMP >= 2) Rc[Xt][1] = um[Xt];
+  if constexpr (C::COMP >= 3) Rc[Xb][1] = um[Xb];
+  Rc[C::ENG][1] = h + um[Xn] * a;
+  lambda[2] = um[Xn];
+  eta[2] = dv[C::RHO] - dv[C::ENG] / a2;
+  Rc[C::RHO][2] = 1;
+  Rc[C::MX1][2] = um[C::MX1];
+  if constexpr (C::COMP >= 2) Rc[C::MX2][2] = um[C::MX2];
+  if constexpr (C::COMP >= 3) Rc[C::MX3][2] = um[C::MX3];
+  Rc[C::ENG][2] = T(0.5) * vel2;
+  if constexpr (C::COMP > 1) {
+    lambda[3] = um[Xn];
+    eta[3] = um[C::RHO] * dv[Xt];
+    Rc[Xt][3] = 1;
+    Rc[C::ENG][3] = um[Xt];
+  }
+  if constexpr (C::COMP > 2) {
+    lambda[4] = um[Xn];
+    eta[4] = um[C::RHO] * dv[Xb];
+    Rc[Xb][4] = 1;
+    Rc[C::ENG][4] = um[Xb];
+  }
+  cmax = fab(um[Xn]) + a;
+  T scrh = fab(vL[C::ENG] - vR[C::ENG]);
+  scrh /= fmn(vL[C::ENG], vR[C::ENG]);
+  if (scrh > T(0.5) && (vR[Xn] < vL[Xn])) {
+    if constexpr (C::DIM > 1) {
+      const T bmin = fmn(T(0), lambda[0]);
+      const T bmax = fmx(T(0), lambda[1]);
+      const T scrh1 = T(1) / (bmax - bmin);
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        flux[n] = bmin * bmax * (uR[n] - uL[n]) + bmax * fL[n] - bmin * fR[n];
+        flux[n] *= scrh1;
+      }
+      return true;
+    } else {
+      return false;
+    }
+  }
+  T alambda[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) alambda[n] = fab(lambda[n]);
+  const T delta = T(1.e-7);
+  if (alambda[0] <= delta) alambda[0] = T(0.5) * lambda[0] * lambda[0] / delta + T(0.5) * delta;
+  if (alambda[1] <= delta) alambda[1] = T(0.5) * lambda[1] * lambda[1] / delta + T(0.5) * delta;
+#pragma unroll
+  for (int n = 0; n < NV; n++) {
+    T f = fL[n] + fR[n];
+#pragma unroll
+    for (int m = 0; m < NV; m++) f -= alambda[m] * eta[m] * Rc[n][m];
+    flux[n] = f * T(0.5);
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------------------------
+// HlldMHD, src/fluid/RiemannSolver/MHDsolvers/hlldMHD.hpp:147-589 (HAVE_ENERGY branch)
+// followed by the face-EMF stores of storeFlux.hpp:14-208.
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hlld_mhd(const T *vL, const T *vR, T *flux, T &cmax,
+                                         FaceEmf<T> &E, int emf, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  constexpr int BXn = C::template BXn<DIR>();
+  constexpr int BXt = C::template BXt<DIR>();
+  constexpr int BXb = C::template BXb<DIR>();
+  T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
+
+  // fast magnetosonic speeds, hlldMHD.hpp:183-226
+  T cL, cR;
+  {
+    T gpr = gamma * vL[C::ENG];
+    T b1 = vL[BXn], b2 = 0, b3 = 0;
+    if constexpr (CO >= 2) b2 = vL[BXt];
+    if constexpr (CO >= 3) b3 = vL[BXb];
+    T Btmag2 = b2 * b2 + b3 * b3;
+    T Bmag2 = b1 * b1 + Btmag2;
+    cL = gpr - Bmag2;
+    cL = gpr + Bmag2 + sqrt_(cL * cL + T(4) * gpr * Btmag2);
+    cL = sqrt_(T(0.5) * cL / vL[C::RHO]);
+  }
+  {
+    T gpr = gamma * vR[C::ENG];
+    T b1 = vR[BXn], b2 = 0, b3 = 0;
+    if constexpr (CO >= 2) b2 = vR[BXt];
+    if constexpr (CO >= 3) b3 = vR[BXb];
+    T Btmag2 = b2 * b2 + b3 * b3;
+    T Bmag2 = b1 * b1 + Btmag2;
+    cR = gpr - Bmag2;
+    cR = gpr + Bmag2 + sqrt_(cR * cR + T(4) * gpr * Btmag2);
+    cR = sqrt_(T(0.5) * cR / vR[C::RHO]);
+  }
+  const T cminL = vL[Xn] - cL, cmaxL = vL[Xn] + cL;
+  const T cminR = vR[Xn] - cR, cmaxR = vR[Xn] + cR;
+  const T sl = fmn(cminL, cminR);
+  const T sr = fmx(cmaxL, cmaxR);
+  cmax = fmx(fab(sl), fab(sr));
+
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fluxL, vL, uL);
+  kflux<C, DIR>(fluxR, vR, uR);
+
+  T b2L = vL[C::BX1] * vL[C::BX1];
+  if constexpr (CO >= 2) b2L = b2L + vL[C::BX2] * vL[C::BX2];
+  if constexpr (CO >= 3) b2L = b2L + vL[C::BX3] * vL[C::BX3];
+  T b2R = vR[C::BX1] * vR[C::BX1];
+  if constexpr (CO >= 2) b2R = b2R + vR[C::BX2] * vR[C::BX2];
+  if constexpr (CO >= 3) b2R = b2R + vR[C::BX3] * vR[C::BX3];
+  const T ptL = vL[C::ENG] + T(0.5) * b2L;
+  const T ptR = vR[C::ENG] + T(0.5) * b2R;
+
+  if (sl > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxL[n];
+  } else if (sr < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxR[n];
+  } else {
+    T usL[NV], usR[NV];
+    T scrh, scrhL, scrhR, duL, duR, sBx, Bx, SM, S1L, S1R, pts, sqrL, sqrR;
+    T vsL = 0, vsR = 0, wsL = 0, wsR = 0;
+    bool revert_to_hllc = false;
+    scrh = T(1) / (sr - sl);
+    Bx = (sr * vR[BXn] - sl * vL[BXn]) * scrh;
+    sBx = (Bx > T(0) ? T(1) : T(-1));
+    duL = sl - vL[Xn];
+    duR = sr - vR[Xn];
+    scrh = T(1) / (duR * uR[C::RHO] - duL * uL[C::RHO]);
+    SM = (duR * uR[Xn] - duL * uL[Xn] - ptR + ptL) * scrh;
+    pts = duR * uR[C::RHO] * ptL - duL * uL[C::RHO] * ptR +
+          vL[C::RHO] * vR[C::RHO] * duR * duL * (vR[Xn] - vL[Xn]);
+    pts *= scrh;
+    usL[C::RHO] = uL[C::RHO] * duL / (sl - SM);
+    usR[C::RHO] = uR[C::RHO] * duR / (sr - SM);
+    sqrL = sqrt_(usL[C::RHO]);
+    sqrR = sqrt_(usR[C::RHO]);
+    S1L = SM - fab(Bx) / sqrL;
+    S1R = SM + fab(Bx) / sqrR;
+    if ((S1L - sl) < T(1.e-4) * (SM - sl)) revert_to_hllc = true;
+    if ((S1R - sr) > -T(1.e-4) * (sr - SM)) revert_to_hllc = true;
+    if (revert_to_hllc) {
+      scrh = T(1) / (sr - sl);
+      T Uhll_n, Uhll_t = 0, Uhll_b = 0;
+      Uhll_n = (sr * uR[BXn] - sl * uL[BXn] + fluxL[BXn] - fluxR[BXn]) * scrh;
+      if constexpr (CO >= 2) Uhll_t = (sr * uR[BXt] - sl * uL[BXt] + fluxL[BXt] - fluxR[BXt]) * scrh;
+      if constexpr (CO >= 3) Uhll_b = (sr * uR[BXb] - sl * uL[BXb] + fluxL[BXb] - fluxR[BXb]) * scrh;
+      usL[BXn] = usR[BXn] = Uhll_n;
+      if constexpr (CO >= 2) usL[BXt] = usR[BXt] = Uhll_t;
+      if constexpr (CO >= 3) usL[BXb] = usR[BXb] = Uhll_b;
+      S1L = S1R = SM;
+    } else {
+      scrhL = (uL[C::RHO] * duL * duL - Bx * Bx) / (uL[C::RHO] * duL * (sl - SM) - Bx * Bx);
+      scrhR = (uR[C::RHO] * duR * duR - Bx * Bx) / (uR[C::RHO] * duR * (sr - SM) - Bx * Bx);
+      usL[BXn] = Bx;
+      if constexpr (CO >= 2) usL[BXt] = uL[BXt] * scrhL;
+      if constexpr (CO >= 3) usL[BXb] = uL[BXb] * scrhL;
+      usR[BXn] = Bx;
+      if constexpr (CO >= 2) usR[BXt] = uR[BXt] * scrhR;
+      if constexpr (CO >= 3) usR[BXb] = uR[BXb] * scrhR;
+    }
+    scrhL = Bx / (uL[C::RHO] * duL);
+    scrhR = Bx / (uR[C::RHO] * duR);
+    if constexpr (CO >= 2) {
+      vsL = vL[Xt] - scrhL * (usL[BXt] - uL[BXt]);
+      vsR = vR[Xt] - scrhR * (usR[BXt] - uR[BXt]);
+    }
+    if constexpr (CO >= 3) {
+      wsL = vL[Xb] - scrhL * (usL[BXb] - uL[BXb]);
+      wsR = vR[Xb] - scrhR * (usR[BXb] - uR[BXb]);
+    }
+    usL[Xn] = usL[C::RHO] * SM;
+    usR[Xn] = usR[C::RHO] * SM;
+    if constexpr (CO >= 2) {
+      usL[Xt] = usL[C::RHO] * vsL;
+      usR[Xt] = usR[C::RHO] * vsR;
+    }
+    if constexpr (CO >= 3) {
+      usL[Xb] = usL[C::RHO] * wsL;
+      usR[Xb] = usR[C::RHO] * wsR;
+    }
+    scrhL = vL[Xn] * Bx;
+    if constexpr (CO >= 2) scrhL = scrhL + vL[Xt] * uL[BXt];
+    if constexpr (CO >= 3) scrhL = scrhL + vL[Xb] * uL[BXb];
+    {
+      T s = SM * Bx;
+      if constexpr (CO >= 2) s = s + vsL * usL[BXt];
+      if constexpr (CO >= 3) s = s + wsL * usL[BXb];
+      scrhL -= s;
+    }
+    usL[C::ENG] = duL * uL[C::ENG] - ptL * vL[Xn] + pts * SM + Bx * scrhL;
+    usL[C::ENG] /= sl - SM;
+    scrhR = vR[Xn] * Bx;
+    if constexpr (CO >= 2) scrhR = scrhR + vR[Xt] * uR[BXt];
+    if constexpr (CO >= 3) scrhR = scrhR + vR[Xb] * uR[BXb];
+    {
+      T s = SM * Bx;
+      if constexpr (CO >= 2) s = s + vsR * usR[BXt];
+      if constexpr (CO >= 3) s = s + wsR * usR[BXb];
+      scrhR -= s;
+    }
+    usR[C::ENG] = duR * uR[C::ENG] - ptR * vR[Xn] + pts * SM + Bx * scrhR;
+    usR[C::ENG] /= sr - SM;
+
+    if (S1L >= T(0)) {
+#pragma unroll
+      for (int n = 0; n < NV; n++) flux[n] = fluxL[n] + sl * (usL[n] - uL[n]);
+    } else if (S1R <= T(0)) {
+#pragma unroll
+      for (int n = 0; n < NV; n++) flux[n] = fluxR[n] + sr * (usR[n] - uR[n]);
+    } else {
+      T vss = 0, wss = 0, ussl[NV], ussr[NV];
+      ussl[C::RHO] = usL[C::RHO];
+      ussr[C::RHO] = usR[C::RHO];
+      if constexpr (CO >= 2) {
+        vss = sqrL * vsL + sqrR * vsR + (usR[BXt] - usL[BXt]) * sBx;
+        vss /= sqrL + sqrR;
+      }
+      if constexpr (CO >= 3) {
+        wss = sqrL * wsL + sqrR * wsR + (usR[BXb] - usL[BXb]) * sBx;
+        wss /= sqrL + sqrR;
+      }
+      ussl[Xn] = ussl[C::RHO] * SM;
+      ussr[Xn] = ussr[C::RHO] * SM;
+      if constexpr (CO >= 2) {
+        ussl[Xt] = ussl[C::RHO] * vss;
+        ussr[Xt] = ussr[C::RHO] * vss;
+      }
+      if constexpr (CO >= 3) {
+        ussl[Xb] = ussl[C::RHO] * wss;
+        ussr[Xb] = ussr[C::RHO] * wss;
+      }
+      ussl[BXn] = ussr[BXn] = Bx;
+      if constexpr (CO >= 2) {
+        ussl[BXt] = sqrL * usR[BXt] + sqrR * usL[BXt] + sqrL * sqrR * (vsR - vsL) * sBx;
+        ussl[BXt] /= sqrL + sqrR;
+        ussr[BXt] = ussl[BXt];
+      }
+      if constexpr (CO >= 3) {
+        ussl[BXb] = sqrL * usR[BXb] + sqrR * usL[BXb] + sqrL * sqrR * (wsR - wsL) * sBx;
+        ussl[BXb] /= sqrL + sqrR;
+        ussr[BXb] = ussl[BXb];
+      }
+      scrhL = SM * Bx;
+      if constexpr (CO >= 2) scrhL = scrhL + vsL * usL[BXt];
+      if constexpr (CO >= 3) scrhL = scrhL + wsL * usL[BXb];
+      {
+        T s = SM * Bx;
+        if constexpr (CO >= 2) s = s + vss * ussl[BXt];
+        if constexpr (CO >= 3) s = s + wss * ussl[BXb];
+        scrhL -= s;
+      }
+      scrhR = SM * Bx;
+      if constexpr (CO >= 2) scrhR = scrhR + vsR * usR[BXt];
+      if constexpr (CO >= 3) scrhR = scrhR + wsR * usR[BXb];
+      {
+        T s = SM * Bx;
+        if constexpr (CO >= 2) s = s + vss * ussr[BXt];
+        if constexpr (CO >= 3) s = s + wss * ussr[BXb];
+        scrhR -= s;
+      }
+      ussl[C::ENG] = usL[C::ENG] - sqrL * scrhL * sBx;
+      ussr[C::ENG] = usR[C::ENG] + sqrR * scrhR * sBx;
+      if (SM >= T(0)) {
+#pragma unroll
+        for (int n = 0; n < NV; n++)
+          flux[n] = fluxL[n] + S1L * (ussl[n] - usL[n]) + sl * (usL[n] - uL[n]);
+      } else {
+#pragma unroll
+        for (int n = 0; n < NV; n++)
+          flux[n] = fluxR[n] + S1R * (ussr[n] - usR[n]) + sr * (usR[n] - uR[n]);
+      }
+    }
+  }
+
+  // face-EMF stores, hlldMHD.hpp:579-588 and storeFlux.hpp
+  T st, sb = T(1);  // hlldMHD.hpp:79-142
+  if constexpr (DIR == IDIR) { st = T(-1); sb = T(1); }
+  if constexpr (DIR == JDIR) { st = T(1); sb = (C::DIM == 3) ? T(-1) : T(1); }
+  if constexpr (DIR == KDIR) { st = T(-1); sb = T(1); }
+  if (emf <= EMF_UCT_CONTACT) {
+    // K_StoreEMF (arithmetic, uct0) and K_StoreContact
+    if constexpr (CO > 1) {
+      E.Et = st * flux[BXt];
+      if constexpr (C::DIM == 3) E.Eb = sb * flux[BXb];
+    }
+    if (emf == EMF_UCT_CONTACT) {
+      T s = T(0.5);
+      if (flux[C::RHO] > T(1.0e-6)) s = T(1);
+      if (flux[C::RHO] < -T(1.0e-6)) s = T(0);
+      E.sv = s;
+    }
+  } else if (emf == EMF_UCT_HLL) {
+    const T ar = fmx(T(0), sr);
+    const T al = fmn(T(0), sl);
+    const T scrh = T(1) / (ar - al);
+    if constexpr (CO > 1) {
+      E.Et = -st * (ar * vL[Xt] - al * vR[Xt]) * scrh;
+      if constexpr (CO >= 3) E.Eb = -sb * (ar * vL[Xb] - al * vR[Xb]) * scrh;
+    }
+    E.aL = ar * scrh;
+    E.aR = -al * scrh;
+    E.dR = -al * ar * scrh;
+    E.dL = E.dR;
+  } else {
+    // K_StoreHLLD, storeFlux.hpp:77-208 (HAVE_ENERGY branch)
+    bool revert_to_hll = false;
+    const T Bn = (sr * vR[BXn] - sl * vL[BXn]) / (sr - sl);
+    const T duL = sl - vL[Xn];
+    const T duR = sr - vR[Xn];
+    T scrh = T(1) / (duR * uR[C::RHO] - duL * uL[C::RHO]);
+    const T Sc = (duR * uR[Xn] - duL * uL[Xn] - ptR + ptL) * scrh;
+    const T usLRHO = uL[C::RHO] * duL / (sl - Sc);
+    const T usRRHO = uR[C::RHO] * duR / (sr - Sc);
+    const T sqrL = sqrt_(usLRHO);
+    const T sqrR = sqrt_(usRRHO);
+    const T SaL = Sc - fab(Bn) / sqrL;
+    const T SaR = Sc + fab(Bn) / sqrR;
+    if ((SaL - sl) < T(1e-4) * (Sc - sl)) revert_to_hll = true;
+    if ((SaR - sr) > -T(1e-4) * (sr - Sc)) revert_to_hll = true;
+    const T chiL = (vL[Xn] - Sc) * (sl - Sc) / (SaL + sl - T(2) * Sc);
+    const T chiR = (vR[Xn] - Sc) * (sr - Sc) / (SaR + sr - T(2) * Sc);
+    const T nuL = (SaL + sl) / (fab(SaL) + fab(sl));
+    const T nuR = (SaR + sr) / (fab(SaR) + fab(sr));
+    const T nuLR = (SaL + SaR) / (fab(SaL) + fab(SaR));
+    if (fab(SaR - SaL) > T(1.e-9) * fab(sr - sl)) {
+      E.dL = T(0.5) * (chiL * nuL - chiL * nuLR) + T(0.5) * (fab(SaL) - nuLR * SaL);
+      E.dR = T(0.5) * (chiR * nuR - chiR * nuLR) + T(0.5) * (fab(SaR) - nuLR * SaR);
+      E.aL = T(0.5) * (T(1) + nuLR);
+      E.aR = T(0.5) * (T(1) - nuLR);
+    } else {
+      E.dL = T(0.5) * chiL * nuL + T(0.5) * fab(SaL);
+      E.dR = T(0.5) * chiR * nuR + T(0.5) * fab(SaR);
+      E.aL = T(0.5);
+      E.aR = T(0.5);
+    }
+    const T ar = fmx(T(0), sr);
+    const T al = fmn(T(0), sl);
+    scrh = T(1) / (ar - al);
+    if (revert_to_hll) {
+      E.aL = ar * scrh;
+      E.aR = -al * scrh;
+      E.dR = -al * ar * scrh;
+      E.dL = E.dR;
+    }
+    if constexpr (CO > 1) {
+      E.Et = -st * (ar * vL[Xt] - al * vR[Xt]) * scrh;
+      if constexpr (CO >= 3) E.Eb = -sb * (ar * vL[Xb] - al * vR[Xb]) * scrh;
+    }
+  }
+}
+
+// Dispatch to the configured Riemann solver.  Returns false only in the 1-D Roe case
+// where the reference leaves the flux unwritten.
+template <class C, int DIR, typename T>
+__device__ __forceinline__ bool riemann(const T *vL, const T *vR, T *flux, T &cmax,
+                                        FaceEmf<T> &E, int emf, T gamma) {
+  if constexpr (C::MHD) {
+    static_assert(C::SOLVER == SOLVER_HLLD, "MHD solver not built");
+    hlld_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+    return true;
+  } else if constexpr (C::SOLVER == SOLVER_HLLC) {
+    hllc_hd<C, DIR>(vL, vR, flux, cmax, gamma);
+    return true;
+  } else {
+    return roe_hd<C, DIR>(vL, vR, flux, cmax, gamma);
+  }
+}
+
+}  // namespace idfx

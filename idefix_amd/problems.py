@@ -1,0 +1,91 @@
+"""Initial conditions of the workloads (the Setup::InitFlow of each problem).
+
+Each function fills host arrays Vc [nvar][nz][ny][nx] and Vs [dim][nz+1][ny+1][nx+1]
+from the cell coordinates of a grid object exposing `x`, `xl` (lists of per-direction
+arrays over np_tot), `np_tot`, `vs_shape`, `nvar`, `dim`, `comps`; both the GPU
+DataBlock and the CPU oracle provide them.  Variable order: RHO, VX1..VXc, PRS, BX1..BXc.
+"""
+import numpy as np
+
+
+def _coords(g, kind="x"):
+    if hasattr(g, "coord"):
+        return [g.coord(d, kind) for d in range(3)]
+    return list(getattr(g, kind))
+
+
+def _empty(g):
+    nx, ny, nz = g.np_tot
+    Vc = np.zeros((g.nvar, nz, ny, nx))
+    Vs = None
+    if getattr(g, "vs_shape", None) is not None and g.dim >= 1 and _is_mhd(g):
+        vx, vy, vz = g.vs_shape
+        Vs = np.zeros((g.dim, vz, vy, vx))
+    return Vc, Vs
+
+
+def _is_mhd(g):
+    return g.nvar == 2 * _comps(g) + 2
+
+
+def _comps(g):
+    if hasattr(g, "comps"):
+        return g.comps
+    return g.cfg.comps
+
+
+def orszag_tang(g):
+    """2-D Orszag-Tang vortex, test/MHD/OrszagTang/setup.cpp:31-51 (gamma 5/3)."""
+    Vc, Vs = _empty(g)
+    x, y = _coords(g)[0], _coords(g)[1]
+    nx, ny, nz = g.np_tot
+    X = x[None, None, :]
+    Y = y[None, :, None]
+    B0 = 1.0 / np.sqrt(4.0 * np.pi)
+    Vc[0] = 25.0 / (36.0 * np.pi)
+    Vc[3] = 5.0 / (12.0 * np.pi)
+    Vc[1] = -np.sin(2.0 * np.pi * Y) + 0 * X
+    Vc[2] = np.sin(2.0 * np.pi * X) + 0 * Y
+    Vs[0, :nz, :ny, :nx] = -B0 * np.sin(2.0 * np.pi * Y) + 0 * X
+    Vs[1, :nz, :ny, :nx] = B0 * np.sin(4.0 * np.pi * X) + 0 * Y
+    return Vc, Vs
+
+
+def sod(g, x0=0.5):
+    """1-D Sod shock tube, test/HD/sod/setup.cpp:35-38 (interface at x0)."""
+    Vc, _ = _empty(g)
+    x = _coords(g)[0][None, None, :]
+    Vc[0] = np.where(x > x0, 0.125, 1.0)
+    Vc[1] = 0.0
+    Vc[_comps(g) + 1] = np.where(x > x0, 0.1, 1.0)
+    return Vc, None
+
+
+def random_mhd(g, seed=0, amp=0.3):
+    """Smooth random perturbation of a uniform magnetised state (parity tests)."""
+    rng = np.random.default_rng(seed)
+    Vc, Vs = _empty(g)
+    comps = _comps(g)
+    shape = Vc.shape[1:]
+    Vc[0] = 1.0 + amp * rng.uniform(-1, 1, shape)
+    for c in range(comps):
+        Vc[1 + c] = amp * rng.uniform(-1, 1, shape)
+    Vc[comps + 1] = 0.6 + amp * rng.uniform(-1, 1, shape)
+    if Vs is not None:
+        for c in range(Vs.shape[0]):
+            Vs[c] = (0.4 - 0.1 * c) + amp * rng.uniform(-1, 1, Vs.shape[1:])
+        for c in range(Vs.shape[0], comps):
+            Vc[comps + 2 + c] = 0.2 + amp * rng.uniform(-1, 1, shape)
+    return Vc, Vs
+
+
+def random_hd(g, seed=0, amp=0.3):
+    rng = np.random.default_rng(seed)
+    Vc, _ = _empty(g)
+    comps = _comps(g)
+    shape = Vc.shape[1:]
+    Vc[0] = 1.0 + amp * rng.uniform(-1, 1, shape)
+    for c in range(comps):
+        Vc[1 + c] = amp * rng.uniform(-1, 1, shape)
+    Vc[comps + 1] = 0.6 + amp * rng.uniform(-1, 1, shape)
+    return Vc, None

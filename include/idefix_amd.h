@@ -1,0 +1,167 @@
+/*
+ * idefix_amd: C ABI of the MI355X-native per-stage finite-volume update.
+ *
+ * Every entry point takes raw device pointers in the reference's LayoutRight layout
+ * and returns an int status (IDFX_OK = 0).  Arrays (reference src/fluid/fluid.hpp:529-560):
+ *   Vc, Uc, FluxRiemann : double [nvar][np_tot[2]][np_tot[1]][np_tot[0]]
+ *   Vs                  : double [dim][np_tot[2]+KOFF][np_tot[1]+JOFF][np_tot[0]+IOFF]
+ *   InvDt               : double [np_tot[2]][np_tot[1]][np_tot[0]]
+ * with np_tot = nx + 2*nghost along active directions (nghost = 2 for PLM, 3 for PPM,
+ * src/grid.cpp:60-65) and 1 along the others; IOFF/JOFF/KOFF are 1 along active
+ * directions.  Variable order is src/idefix.hpp:74-108 (RHO, VX1..VXc, PRS, BX1..BXc).
+ *
+ * The library borrows these pointers and never frees them.  Scratch (face and corner
+ * EMFs, cMax, the Runge-Kutta "begin" state, the dt reduction scalar) is owned by the
+ * context.  All calls are asynchronous on the given HIP stream (a hipStream_t passed as
+ * void*, NULL = default stream) except those documented as blocking; calls on one
+ * context are not re-entrant.
+ */
+#ifndef IDEFIX_AMD_H_
+#define IDEFIX_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define IDFX_OK 0
+#define IDFX_E_ARGS 1        /* invalid argument or unsupported configuration */
+#define IDFX_E_LAUNCH 2      /* a HIP call failed */
+#define IDFX_E_NAN 3         /* NaN found (TimeIntegrator's "Nan found" runtime_error) */
+#define IDFX_E_DT_SMALL 4    /* dt < 1e-15 (timeIntegrator.cpp:401-405) */
+#define IDFX_E_FIRST_DT 5    /* first_dt too large (timeIntegrator.cpp:393-398) */
+#define IDFX_E_ALLOC 6       /* device allocation failed */
+
+/* [Hydro] solver (riemannSolver.hpp:97-140) */
+#define IDFX_SOLVER_HLLC 1
+#define IDFX_SOLVER_HLLD 2
+#define IDFX_SOLVER_ROE 3
+/* reconstruction, the reference's ORDER (CMakeLists.txt:173-213) */
+#define IDFX_RECON_CONSTANT 1
+#define IDFX_RECON_PLM 2
+#define IDFX_RECON_PPM 4
+/* [Hydro] emf (constrainedTransport.hpp:156-176) */
+#define IDFX_EMF_ARITHMETIC 0
+#define IDFX_EMF_UCT0 1
+#define IDFX_EMF_UCT_CONTACT 2
+#define IDFX_EMF_UCT_HLL 3
+#define IDFX_EMF_UCT_HLLD 4
+/* [Boundary] types (boundary.hpp:286-385); USERDEF sides are left to the caller */
+#define IDFX_BC_PERIODIC 0
+#define IDFX_BC_OUTFLOW 1
+#define IDFX_BC_REFLECTIVE 2
+#define IDFX_BC_USERDEF 3
+/* precision of the stencil, flux and EMF arithmetic (state stays FP64) */
+#define IDFX_PREC_FP64 0
+#define IDFX_PREC_FP32 1
+
+/* sweep flags (idfx_sweep) */
+#define IDFX_SWEEP_INIT_UC 1     /* Uc = PrimToCons(Vc) + rhs instead of Uc += rhs */
+#define IDFX_SWEEP_SAVE_UC0 2    /* also store PrimToCons(Vc) in the context's begin state */
+#define IDFX_SWEEP_INVDT_SET 4   /* InvDt = term (first direction after ResetStage) */
+#define IDFX_SWEEP_INVDT_ADD 8   /* InvDt += term */
+
+typedef struct idfx_ctx idfx_ctx;
+
+typedef struct {
+  int dim;        /* DIMENSIONS */
+  int comps;      /* COMPONENTS */
+  int mhd;        /* 0: hydro, 1: ideal MHD with constrained transport */
+  int nx[3];      /* interior cells per direction (1 for inactive directions) */
+  int recon;      /* IDFX_RECON_* */
+  int solver;     /* IDFX_SOLVER_* */
+  int emf;        /* IDFX_EMF_* */
+  int precision;  /* IDFX_PREC_* */
+  int lbound[3];  /* IDFX_BC_* per direction, left side */
+  int rbound[3];  /* IDFX_BC_* per direction, right side */
+  double gamma;   /* adiabatic index (eos_adiabatic.hpp:21) */
+} idfx_desc;
+
+/* Context: replaces the Fluid<Phys>/RiemannSolver/ConstrainedTransport constructors
+ * (fluid.hpp:263-755, riemannSolver.hpp:90-169, constrainedTransport.hpp:152-263).
+ * dx1..dx3 are host arrays of np_tot[d] cell widths (DataBlock::dx, gridHost.cpp:79-86). */
+int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
+                    const double *dx3, idfx_ctx **out);
+int idfx_ctx_destroy(idfx_ctx *ctx);
+/* np_tot[3], nghost[3], nvar, vs_shape[3] (i, j, k order) */
+int idfx_ctx_layout(const idfx_ctx *ctx, int *np_tot, int *nghost, int *nvar, int *vs_shape);
+const char *idfx_strerror(int status);
+/* Device pointer of a context-owned array for user hooks: which = 0..23 in the order
+ * ex ey ez ezi ezj exj exk eyi eyk svx svy svz axL axR ayL ayR azL azR dxL dxR dyL dyR
+ * dzL dzR (constrainedTransport.hpp members), 24 = cMax (fluid.hpp cMax).  FP64 only. */
+int idfx_ctx_array(idfx_ctx *ctx, int which, double **ptr);
+
+/* Boundary::SetBoundaries (boundary.hpp:239-282) for the built-in side types, with
+ * ReconstructNormalField (:407-530) and ReconstructVcField(Vc) (:388-404). */
+int idfx_set_boundaries(idfx_ctx *ctx, double *Vc, double *Vs, void *stream);
+
+/* Fluid::ConvertPrimToCons over all cells (convertConsToPrim.hpp:152-189). */
+int idfx_prim_to_cons(idfx_ctx *ctx, const double *Vc, double *Uc, void *stream);
+
+/* Fluid::ConvertConsToPrim over all cells, including ReconstructVcField(Uc)
+ * (convertConsToPrim.hpp:105-149). */
+int idfx_cons_to_prim(idfx_ctx *ctx, double *Uc, double *Vc, const double *Vs, void *stream);
+
+/* Boundary::ReconstructVcField(X) (boundary.hpp:388-404). */
+int idfx_reconstruct_vc_field(idfx_ctx *ctx, const double *Vs, double *X, void *stream);
+
+/* Fluid::ResetStage (enroll.hpp:116-129): InvDt = 0. */
+int idfx_reset_stage(idfx_ctx *ctx, double *InvDt, void *stream);
+
+/* RiemannSolver::CalcFlux<dir> (calcFlux.hpp:27-85): reconstruction + Riemann flux into
+ * FluxRiemann, cMax and the face EMFs of the context (unfused, for flux hooks). */
+int idfx_calc_flux(idfx_ctx *ctx, int dir, const double *Vc, const double *Vs, double *Flux,
+                   void *stream);
+
+/* Fluid::CalcRightHandSide<dir> (calcRightHandSide.hpp:556-596): Flux *= A, then
+ * Uc += -dt/dV (F(i+1)-F(i)) and InvDt += (cMax(i+1)+cMax(i))/(2 dl). */
+int idfx_calc_rhs(idfx_ctx *ctx, int dir, double dt, double *Flux, double *Uc, double *InvDt,
+                  void *stream);
+
+/* Fused CalcFlux<dir> + CalcRightHandSide<dir>: the flux never reaches memory.  flags
+ * are IDFX_SWEEP_*. */
+int idfx_sweep(idfx_ctx *ctx, int dir, double dt, const double *Vc, const double *Vs,
+               double *Uc, double *InvDt, int flags, void *stream);
+
+/* ConstrainedTransport::CalcCornerEMF (calcCornerEmf.hpp:14-38, calcRiemannEmf.hpp). */
+int idfx_corner_emf(idfx_ctx *ctx, const double *Vc, const double *Vs, void *stream);
+
+/* ConstrainedTransport::EvolveMagField (evolveMagField.hpp:14-135). */
+int idfx_evolve_mag_field(idfx_ctx *ctx, double dt, double *Vs, void *stream);
+
+/* Fluid::EvolveStage (evolveStage.hpp:40-90): every sweep, CT and ReconstructVcField(Uc),
+ * with Uc already holding the stage's initial conserved state. */
+int idfx_evolve_stage(idfx_ctx *ctx, double t, double dt, const double *Vc, double *Vs,
+                      double *Uc, double *InvDt, void *stream);
+
+/* DataBlock::ComputeTimestep (dataBlock.cpp:333-364): *dtmin = min over the interior of
+ * 1/InvDt.  Blocking (returns a host scalar). */
+int idfx_compute_timestep(idfx_ctx *ctx, const double *InvDt, double *dtmin, void *stream);
+
+/* StateContainer::AddAndStore (stateContainer.cpp:75-104): U = wl*U + wr*U0, n doubles. */
+int idfx_add_and_store(idfx_ctx *ctx, double wl, double wr, double *U, const double *U0,
+                       long long n, void *stream);
+
+/* Fluid::CheckDivB (checkDivB.hpp:14-68).  Blocking. */
+int idfx_check_divb(idfx_ctx *ctx, const double *Vs, double *divb, void *stream);
+
+/* One Runge-Kutta stage of TimeIntegrator::Cycle (timeIntegrator.cpp:275-350) on the
+ * fused path: built-in boundaries (unless skip_boundaries), fused sweeps with
+ * PrimToCons, CT with the stage combination of Vs, and the combined ConvertConsToPrim.
+ * After stage 0, *newdt_host = cfl * ComputeTimestep() (blocking in that stage only).
+ * Uc receives the combined conserved state when write_uc is set. */
+int idfx_stage(idfx_ctx *ctx, int stage, int nstages, double dt, double cfl, double *Vc,
+               double *Vs, double *Uc, double *InvDt, int skip_boundaries, int write_uc,
+               double *newdt_host, void *stream);
+
+/* TimeIntegrator::Cycle (timeIntegrator.cpp:246-414) on the fused path: all stages and
+ * the dt update (CFL_max_var, first-cycle check, dt < 1e-15).  *t and *dt are updated;
+ * ncycles is the index of this cycle. */
+int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long ncycles,
+               double *t, double *dt, double *Vc, double *Vs, double *Uc, double *InvDt,
+               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IDEFIX_AMD_H_ */

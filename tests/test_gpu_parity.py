@@ -1,0 +1,170 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bar: FP64 results within 1e-12 relative per cell of the oracle (absolute floor
+1e-12 x field scale), interior cells and interior faces, after each checked step.
+"""
+import numpy as np
+import pytest
+
+from idefix_amd import problems
+from tests.pairs import cfg_kwargs, compare_state, make_pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _cycle_both(o, g, ncycles):
+    for n in range(ncycles):
+        so = o.cycle()
+        g.cycle()
+        assert so == 0
+        assert g.t == pytest.approx(o.t, rel=1e-14, abs=0)
+        assert g.dt == pytest.approx(o.dt, rel=1e-12, abs=0), (n, g.dt, o.dt)
+
+
+MHD2D_EMFS = ["uct_hlld", "uct_contact", "arithmetic", "uct0", "uct_hll"]
+
+
+@pytest.mark.parametrize("emf", MHD2D_EMFS)
+@pytest.mark.parametrize("recon", ["plm", "ppm"])
+def test_orszag_tang_cycles(emf, recon):
+    kw = cfg_kwargs(2, (48, 40), emf=emf, recon=recon, cfl=0.6)
+    o, g = make_pair(kw, problems.orszag_tang)
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="OT %s %s" % (emf, recon))
+
+
+@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact"])
+@pytest.mark.parametrize("recon", ["plm", "ppm"])
+def test_mhd3d_cycles(emf, recon):
+    kw = cfg_kwargs(3, (16, 12, 10), emf=emf, recon=recon, cfl=0.4, nstages=3)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=3, amp=0.2))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="3D %s %s" % (emf, recon))
+
+
+@pytest.mark.parametrize("solver", ["hllc", "roe"])
+@pytest.mark.parametrize("recon", ["plm", "ppm"])
+def test_sod_cycles(solver, recon):
+    kw = cfg_kwargs(1, (1000,), mhd=False, solver=solver, recon=recon, bc="outflow",
+                    gamma=1.4, cfl=0.8)
+    o, g = make_pair(kw, problems.sod)
+    o.set_boundaries()
+    _cycle_both(o, g, 40)
+    compare_state(o, g, what="sod %s %s" % (solver, recon))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("solver", ["hllc", "roe"])
+def test_hd_multid_cycles(dim, solver):
+    nx = (40, 24) if dim == 2 else (16, 12, 10)
+    bc = ["outflow", "periodic"] if dim == 2 else "periodic"
+    kw = cfg_kwargs(dim, nx, mhd=False, solver=solver, recon="plm", bc=bc, gamma=1.4, cfl=0.5)
+    o, g = make_pair(kw, lambda gg: problems.random_hd(gg, seed=5, amp=0.3))
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="HD%dD %s" % (dim, solver))
+
+
+def test_outflow_mhd2d():
+    kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="outflow MHD")
+
+
+@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact"])
+def test_unfused_kernels_match_oracle(emf):
+    """CalcFlux (FluxRiemann, cMax, face EMFs), CalcRightHandSide, CalcCornerEMF and
+    EvolveMagField one at a time, as the reference sequences them."""
+    import torch
+    kw = cfg_kwargs(2, (40, 36), emf=emf)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=7, amp=0.25))
+    o.set_boundaries()
+    g.set_boundaries()
+    o.prim_to_cons()
+    g.prim_to_cons()
+    o.reset_stage()
+    g.reset_stage()
+    dt = 1e-3
+    flux = torch.zeros_like(g.Uc)
+    names = {"uct_hlld": [(3, "ezi"), (12, "axL"), (13, "axR"), (18, "dxL"), (19, "dxR")],
+             "uct_contact": [(3, "ezi"), (9, "svx")]}[emf]
+    ynames = {"ezi": "ezj", "axL": "ayL", "axR": "ayR", "dxL": "dyL", "dxR": "dyR",
+              "svx": "svy"}
+    yidx = {"ezj": 4, "ayL": 14, "ayR": 15, "dyL": 20, "dyR": 21, "svy": 10}
+    ext = 2 if emf == "uct_hlld" else 1
+    b, e = o.beg, o.end
+    for d in range(2):
+        o.calc_flux(d)
+        g.calc_flux(d, flux)
+        torch.cuda.synchronize()
+        # faces [beg, end+1) along d, [beg-ext, end+ext) across
+        sl = [slice(None), slice(0, 1), None, None]
+        if d == 0:
+            sl[2], sl[3] = slice(b[1] - ext, e[1] + ext), slice(b[0], e[0] + 1)
+        else:
+            sl[2], sl[3] = slice(b[1], e[1] + 1), slice(b[0] - ext, e[0] + ext)
+        assert rel_err(flux.cpu().numpy()[tuple(sl)], o.array("Flux")[tuple(sl)]) <= 1e-12
+        for which, name in names:
+            if d == 1:
+                name = ynames[name]
+                which = yidx[name]
+            got = g.ctx_array(which).cpu().numpy()[tuple(sl[1:])]
+            ref = o.array(name)[tuple(sl[1:])]
+            assert rel_err(got, ref) <= 1e-12, (d, name)
+        o.calc_rhs(d, dt)
+        g.calc_rhs(d, dt, flux)
+    torch.cuda.synchronize()
+    assert rel_err(o.interior(g.Uc.cpu().numpy()), o.interior(o.Uc)) <= 1e-12
+    assert rel_err(o.interior(g.InvDt.cpu().numpy()), o.interior(o.array("InvDt"))) <= 1e-12
+    o.calc_corner_emf()
+    g.calc_corner_emf()
+    torch.cuda.synchronize()
+    ez = g.ctx_array(2).cpu().numpy()[0, b[1]:e[1] + 1, b[0]:e[0] + 1]
+    assert rel_err(ez, o.array("ez")[0, b[1]:e[1] + 1, b[0]:e[0] + 1]) <= 1e-12
+    o.evolve_mag_field(dt)
+    g.evolve_mag_field(dt)
+    torch.cuda.synchronize()
+    compare_state(o, g, what="unfused stage pieces")
+
+
+def test_compute_timestep_and_divb():
+    kw = cfg_kwargs(2, (64, 48), emf="uct_hlld")
+    o, g = make_pair(kw, problems.orszag_tang)
+    o.set_boundaries()
+    for _ in range(3):
+        o.cycle()
+        g.cycle()
+    dg = g.check_divb()
+    do = o.check_divb()
+    assert dg == pytest.approx(do, rel=1e-6, abs=1e-15)
+    assert dg < 1e-12
+    # ComputeTimestep through the reference-mode entry points
+    g.set_boundaries()
+    g.prim_to_cons()
+    g.reset_stage()
+    g.evolve_stage(0.0, g.dt)
+    o.set_boundaries()
+    o.prim_to_cons()
+    o.reset_stage()
+    o.evolve_stage(0.0, o.dt)
+    assert g.compute_timestep() == o.compute_timestep()
+
+
+def test_fp32_stencil_close_to_fp64():
+    """FP32 stencil/flux/EMF arithmetic with FP64 accumulation and ConsToPrim stays
+    within single-precision error of the FP64 oracle over a few cycles."""
+    kw = cfg_kwargs(2, (64, 64), emf="uct_hlld")
+    o, g = make_pair(kw, problems.orszag_tang, precision="fp32")
+    o.set_boundaries()
+    for _ in range(5):
+        o.cycle()
+        g.cycle()
+    Vc_g, _ = g.download()
+    for n in range(o.nvar):
+        a = o.interior(Vc_g)[n]
+        b = o.interior(o.Vc)[n]
+        assert np.max(np.abs(a - b)) <= 1e-4 * max(1.0, np.max(np.abs(b)))
