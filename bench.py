@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the per-stage finite-volume update on one MI355X.
+
+Workload: 2-D ideal-MHD Orszag-Tang vortex at 2048^2 (the largest single-GPU size of
+that configuration): PLM van Leer, HLLD, constrained transport with UCT-HLLD corner
+EMFs, RK2, CFL 0.6, periodic boundaries, FP64.  A step is one full RK2 cycle of the
+whole grid (TimeIntegrator::Cycle); a zone-update is one cell advanced by one cycle.
+Inputs are synthetic (the analytic Orszag-Tang initial state), resident in HBM before
+the timed region starts.
+
+Prints ONE JSON line.  `value` is zone-updates/s of the plain FP64 method; the FP32
+stencil mode is reported beside it.  `roofline` is computed for the kernel with the
+largest share of the cycle, timed with HIP events on the stream it runs on;
+`cpu_baseline` times the CPU oracle (an OpenMP restatement of the same algorithm) on a
+bounded sample of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor peak
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--nx", type=int, default=2048)
+    p.add_argument("--no-fp32", action="store_true")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--kernel-reps", type=int, default=10)
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only run a plain 1 GiB device copy 10 times (counter calibration)")
+    return p.parse_args()
+
+
+def copy_probe():
+    """A plain device-to-device copy of known size: reads 1 GiB and writes 1 GiB per
+    launch, the yardstick for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8, "launches": 10}))
+
+
+def make_block(nx, precision):
+    from idefix_amd.datablock import DataBlock
+    from idefix_amd import problems
+    g = DataBlock(2, (nx, nx), mhd=True, recon="plm", solver="hlld", emf="uct_hlld",
+                  precision=precision, bc="periodic", gamma=5.0 / 3.0, nstages=2, cfl=0.6,
+                  first_dt=1e-4)
+    Vc, Vs = problems.orszag_tang(g)
+    g.upload(Vc, Vs)
+    return g
+
+
+def time_cycles(g, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        g.cycle()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.cycle()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(g, reps):
+    """Average duration of each kernel of a stage, HIP events on the launch stream."""
+    import torch
+    from idefix_amd import capi
+    s = torch.cuda.current_stream()
+    dt = g.dt
+    flux_flags = capi.SWEEP_INVDT_ADD
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(reps):
+            fn()
+        b.record(s)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    g.set_boundaries()
+    g.prim_to_cons()
+    out = {
+        "boundaries": timed(g.set_boundaries),
+        "sweep_x": timed(lambda: g.sweep(0, dt, flux_flags)),
+        "sweep_y": timed(lambda: g.sweep(1, dt, flux_flags)),
+        "corner_emf": timed(g.calc_corner_emf),
+        "evolve_b": timed(lambda: g.evolve_mag_field(0.0)),
+        "cons_to_prim": timed(g.cons_to_prim),
+    }
+    return out  # ms per launch
+
+
+def algorithmic_bytes(nx):
+    """Compulsory HBM bytes per launch of each stage kernel (each input read once, each
+    output written once; 2-D MHD, nvar 6, UCT-HLLD face data 5 words per face)."""
+    n = nx * nx
+    w = 8
+    return {
+        # Vc (6) + normal Vs (1) + Uc r/w (4+4) + face EMF data (5) + InvDt r/w (2)
+        "sweep_x": n * w * (6 + 1 + 8 + 5 + 2),
+        "sweep_y": n * w * (6 + 1 + 8 + 5 + 2),
+        # face data 2x5 + Vs 2 -> corner EMF 1
+        "corner_emf": n * w * (10 + 2 + 1),
+        # corner EMF 1 + Vs r/w 2x2
+        "evolve_b": n * w * (1 + 4),
+        # Uc (6 incl. B) + Vs 2 -> Vc 6 + Uc 6
+        "cons_to_prim": n * w * (4 + 2 + 6 + 6),
+        "boundaries": 0,
+    }
+
+
+def cpu_baseline(nx, seconds):
+    """CPU oracle on the host cores: Orszag-Tang at the same resolution, as many cycles as
+    fit in about `seconds`."""
+    import numpy as np
+    from oracle import oracle as O
+    from idefix_amd import problems
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or threads))
+    O.set_threads(threads)
+    cfg = O.make_cfg(2, mhd=True, recon="plm", solver="hlld", emf="uct_hlld", nstages=2,
+                     nx=(nx, nx), bc="periodic", gamma=5.0 / 3.0, cfl=0.6)
+    o = O.Oracle(cfg)
+    Vc, Vs = problems.orszag_tang(O.OracleGrid(o))
+    o.Vc[...] = Vc
+    o.Vs[...] = Vs
+    o.set_boundaries()
+    o.cycle()  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.cycle()
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds or n >= 1000:
+            break
+    rate = nx * nx * n / el
+    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": "Orszag-Tang %dx%d, PLM/HLLD/UCT-HLLD/RK2 FP64, %d cycles after 1 "
+                      "warm-up cycle (%.1f s)" % (nx, nx, n, el)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box "
+              "here has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
+    import torch
+    nx = args.nx
+    g = make_block(nx, "fp64")
+    el = time_cycles(g, args.steps, args.warmup)
+    zu = nx * nx * args.steps / el
+    ms = 1e3 * el / args.steps
+    kt = kernel_times(g, args.kernel_reps)
+    ab = algorithmic_bytes(nx)
+    # per-cycle share: two stages, each with one launch of every kernel
+    dom = max((k for k in kt if ab[k] > 0), key=lambda k: kt[k])
+    achieved = ab[dom] / (kt[dom] * 1e-3) / 1e9
+    del g
+    torch.cuda.empty_cache()
+    fp32 = None
+    if not args.no_fp32:
+        g32 = make_block(nx, "fp32")
+        el32 = time_cycles(g32, args.steps, args.warmup)
+        fp32 = nx * nx * args.steps / el32
+        del g32
+        torch.cuda.empty_cache()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(nx, args.cpu_seconds)
+    cycle_bytes = 5 * 6 * 8  # RK2: 5 state transfers of S = 6 words per zone-update
+    line = {
+        "metric": "zone-updates/s on 1 MI355X: plain FP64 vs FP32+learned at equal L1 error",
+        "value": zu,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D Orszag-Tang MHD %dx%d, PLM van Leer, HLLD, CT UCT-HLLD, "
+                               "RK2, CFL 0.6, periodic" % (nx, nx),
+                   "cells": nx * nx, "step": "one RK2 cycle",
+                   "parallelism": "1 GPU"},
+        "fp32_stencil_value": fp32,
+        "kernel_ms": kt,
+        "cycle_hbm_frac": cycle_bytes * zu / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -236,6 +236,21 @@ class Oracle:
         return a[..., b[2]:e[2], b[1]:e[1], b[0]:e[0]]
 
 
+class OracleGrid:
+    """Grid view of an oracle state (coordinates, extents) for the initial-condition
+    functions of idefix_amd.problems."""
+
+    def __init__(self, o):
+        self.o = o
+        self.np_tot = o.np_tot
+        self.vs_shape = o.vs_shape if o.cfg.mhd else None
+        self.nvar = o.nvar
+        self.dim = o.dim
+        self.comps = o.cfg.comps
+        self.x = [o.coord(d, "x") for d in range(3)]
+        self.xl = [o.coord(d, "xl") for d in range(3)]
+
+
 def set_threads(n):
     return lib().orc_nthreads(n)
 

@@ -3,6 +3,7 @@ the same configuration and the same initial data, plus the comparison helpers.""
 import numpy as np
 
 from oracle import oracle as O
+from oracle.oracle import OracleGrid  # noqa: F401
 
 # FP64 parity bar: 1e-12 relative per cell (absolute floor 1e-12 x field scale)
 RTOL = 1e-12
@@ -21,20 +22,6 @@ def make_oracle(kw):
                      xend=kw["xend"], bc=kw["bc"], gamma=kw["gamma"], cfl=kw["cfl"],
                      first_dt=kw["first_dt"])
     return O.Oracle(cfg)
-
-
-class OracleGrid:
-    """Grid view of an oracle state for problems.*()."""
-
-    def __init__(self, o):
-        self.o = o
-        self.np_tot = o.np_tot
-        self.vs_shape = o.vs_shape if o.cfg.mhd else None
-        self.nvar = o.nvar
-        self.dim = o.dim
-        self.comps = o.cfg.comps
-        self.x = [o.coord(d, "x") for d in range(3)]
-        self.xl = [o.coord(d, "xl") for d in range(3)]
 
 
 def make_pair(kw, ic, precision="fp64"):
