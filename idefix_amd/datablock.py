@@ -15,6 +15,30 @@ import numpy as np
 from . import capi
 
 
+def make_grid(dim, nx, xbeg, xend, nghost):
+    """Cell centres, left/right faces and widths per direction over np_tot cells, as
+    GridHost::MakeGrid builds them for uniform patches (gridHost.cpp:79-86) and for the
+    unused directions (gridHost.cpp:200-221)."""
+    x, xl, xr, dx = [], [], [], []
+    for d in range(3):
+        n = nx[d] + 2 * nghost[d]
+        if d < dim:
+            ps, pe = float(xbeg[d]), float(xend[d])
+            dxv = (pe - ps) / nx[d]
+            ii = np.arange(n) - nghost[d]
+            dxa = np.full(n, dxv)
+            dx.append(dxa)
+            x.append(ps + (ii + 0.5) * dxa)
+            xl.append(ps + ii * dxa)
+            xr.append(ps + (ii + 1) * dxa)
+        else:
+            dx.append(np.full(n, 0.5 - (-0.5)))
+            x.append(np.full(n, 0.5 * (-0.5 + 0.5)))
+            xl.append(np.full(n, -0.5))
+            xr.append(np.full(n, 0.5))
+    return x, xl, xr, dx
+
+
 class DataBlock:
     def __init__(self, dim, nx, xbeg=(0.0, 0.0, 0.0), xend=(1.0, 1.0, 1.0), comps=None,
                  mhd=True, recon="plm", solver="hlld", emf="uct_contact", precision="fp64",
@@ -40,24 +64,7 @@ class DataBlock:
         self.np_tot = [self.nx[d] + 2 * self.nghost[d] for d in range(3)]
         self.beg = list(self.nghost)
         self.end = [self.nghost[d] + self.nx[d] for d in range(3)]
-        # GridHost::MakeGrid, uniform patches
-        self.x, self.xl, self.xr, self.dx = [], [], [], []
-        for d in range(3):
-            n = self.np_tot[d]
-            if d < dim:
-                ps, pe = float(xbeg[d]), float(xend[d])
-                dxv = (pe - ps) / self.nx[d]
-                ii = np.arange(n) - self.nghost[d]
-                dxa = np.full(n, dxv)
-                self.dx.append(dxa)
-                self.x.append(ps + (ii + 0.5) * dxa)
-                self.xl.append(ps + ii * dxa)
-                self.xr.append(ps + (ii + 1) * dxa)
-            else:
-                self.dx.append(np.full(n, 0.5 - (-0.5)))
-                self.x.append(np.full(n, 0.5 * (-0.5 + 0.5)))
-                self.xl.append(np.full(n, -0.5))
-                self.xr.append(np.full(n, 0.5))
+        self.x, self.xl, self.xr, self.dx = make_grid(dim, self.nx, xbeg, xend, self.nghost)
         desc = capi.IdfxDesc()
         desc.dim, desc.comps, desc.mhd = dim, self.comps, int(self.mhd)
         for d in range(3):

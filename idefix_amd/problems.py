@@ -89,3 +89,52 @@ def random_hd(g, seed=0, amp=0.3):
         Vc[1 + c] = amp * rng.uniform(-1, 1, shape)
     Vc[comps + 1] = 0.6 + amp * rng.uniform(-1, 1, shape)
     return Vc, None
+
+
+def linear_fast_wave_2d(g, eps=1e-6, gamma=5.0 / 3.0, Lx=2.0, Ly=1.0):
+    """Oblique fast magnetosonic eigenmode on a periodic [0,Lx]x[0,Ly] box.
+
+    The background and the right eigenvector are those of the reference's linear-wave
+    test (test/MHD/LinearWaveTest/setup.cpp:27-70, Gardiner & Stone 2008): rho=1, p=1/gamma,
+    B=(1, 3/2, 0) in the frame of the wave, fast speed 2.  The wave vector
+    k = 2 pi (1/Lx, 1/Ly) makes the box hold exactly one wavelength along each axis.
+    Face fields come from a corner vector potential, so div B = 0 to round-off.
+    Returns (Vc, Vs, period)."""
+    Vc, Vs = _empty(g)
+    x, y = _coords(g)[0], _coords(g)[1]
+    xl, yl = _coords(g, "xl")[0], _coords(g, "xl")[1]
+    kx, ky = 2 * np.pi / Lx, 2 * np.pi / Ly
+    kk = np.hypot(kx, ky)
+    ct, st = kx / kk, ky / kk
+    R0 = 1.0 / (2.0 * np.sqrt(5.0))
+    Rrho, Rvpar, Rvperp, RBperp, RE = 2.0, 4.0, -2.0, 4.0, 9.0
+    Bpar, Bperp = 1.0, 1.5
+    Rp = (gamma - 1.0) * (RE - Bperp * RBperp)
+    X = x[None, None, :]
+    Y = y[None, :, None]
+    ph = np.cos(kx * X + ky * Y)
+    a = eps * R0
+    Vc[0] = 1.0 + a * Rrho * ph
+    vpar, vperp = a * Rvpar * ph, a * Rvperp * ph
+    Vc[1] = vpar * ct - vperp * st
+    Vc[2] = vpar * st + vperp * ct
+    Vc[3] = 1.0 / gamma + a * Rp * ph
+    # B = curl(Az z): Az = B0x y - B0y x - (a RBperp / |k|) sin(k.x)
+    B0x = Bpar * ct - Bperp * st
+    B0y = Bpar * st + Bperp * ct
+    amp = a * RBperp / kk
+
+    def Az(xx, yy):
+        return B0x * yy - B0y * xx - amp * np.sin(kx * xx + ky * yy)
+    nx, ny = len(x), len(y)
+    dxv = np.diff(xl)[0] if nx > 1 else 1.0
+    dyv = np.diff(yl)[0] if ny > 1 else 1.0
+    XL = xl[None, :]
+    YC = y[:, None]
+    Vs[0, 0, :ny, :nx] = (Az(XL, YC + 0.5 * dyv) - Az(XL, YC - 0.5 * dyv)) / dyv
+    XC = x[None, :]
+    YL = yl[:, None]
+    Vs[1, 0, :ny, :nx] = -(Az(XC + 0.5 * dxv, YL) - Az(XC - 0.5 * dxv, YL)) / dxv
+    cf = 2.0
+    period = (1.0 / np.hypot(1.0 / Lx, 1.0 / Ly)) / cf
+    return Vc, Vs, period

@@ -1,0 +1,258 @@
+"""Pins of the oracle's pointwise algebra by properties the mathematics fixes:
+limiter identities, PPM exactness on polynomials, conversion round trips, flux
+consistency with the textbook fluxes, upwinding, mirror symmetry and exact resolution of
+stationary contact discontinuities."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+rng = np.random.default_rng(1234)
+
+
+# ---------------- limiters ----------------
+def test_vanleer_properties():
+    for _ in range(2000):
+        a, b = rng.normal(size=2) * 10 ** rng.uniform(-3, 3)
+        v = O.vanleer(a, b)
+        if a * b <= 0:
+            assert v == 0.0
+            continue
+        # harmonic mean of the two slopes: 1/v = (1/a + 1/b)/2
+        assert 1.0 / v == pytest.approx(0.5 * (1.0 / a + 1.0 / b), rel=1e-13)
+        assert O.vanleer(b, a) == v
+        assert min(abs(a), abs(b)) * (1 - 1e-15) <= abs(v) <= 2 * min(abs(a), abs(b)) * (1 + 1e-15)
+    assert O.vanleer(0.3, 0.3) == 0.3  # linear data is reconstructed exactly
+
+
+def test_mc_limiter_is_minmod_of_three():
+    def minmod3(x, y, z):
+        if x > 0 and y > 0 and z > 0:
+            return min(x, y, z)
+        if x < 0 and y < 0 and z < 0:
+            return max(x, y, z)
+        return 0.0
+    for _ in range(2000):
+        a, b = rng.normal(size=2)
+        assert O.mclim(a, b) == pytest.approx(minmod3(2 * a, 2 * b, 0.5 * (a + b)), rel=1e-15, abs=0)
+
+
+def _cell_avg_poly(c, xl, xr):
+    # average of sum c_k x^k over [xl, xr]
+    return sum(ck * (xr ** (k + 1) - xl ** (k + 1)) / (k + 1) for k, ck in enumerate(c)) / (xr - xl)
+
+
+def test_ppm_exact_on_monotone_cubic():
+    """4th-order face interpolation is exact for cubics; on monotone data away from
+    extrema no limiter fires, so PPM returns the exact face values."""
+    for _ in range(200):
+        c = [rng.uniform(-1, 1), rng.uniform(1, 2), rng.uniform(-0.05, 0.05), rng.uniform(-0.01, 0.01)]
+        h = 0.01
+        x0 = rng.uniform(-0.2, 0.2)
+        avg = [_cell_avg_poly(c, x0 + (k - 0.5) * h, x0 + (k + 0.5) * h) for k in range(-2, 3)]
+        vl, vr = O.ppm_states(*avg)
+        exact_l = sum(ck * (x0 - 0.5 * h) ** k for k, ck in enumerate(c))
+        exact_r = sum(ck * (x0 + 0.5 * h) ** k for k, ck in enumerate(c))
+        assert vl == pytest.approx(exact_l, abs=1e-12)
+        assert vr == pytest.approx(exact_r, abs=1e-12)
+
+
+def test_ppm_no_new_extrema_at_step():
+    for _ in range(500):
+        lo, hi = sorted(rng.uniform(0.1, 2.0, 2))
+        k = rng.integers(0, 4)
+        v = [lo] * (k + 1) + [hi] * (4 - k)
+        vl, vr = O.ppm_states(*v)
+        assert lo - 1e-14 <= vl <= hi + 1e-14
+        assert lo - 1e-14 <= vr <= hi + 1e-14
+
+
+# ---------------- conversions ----------------
+def _cfg(dim, mhd, solver, comps=None, emf="uct_contact", gamma=5.0 / 3.0):
+    return O.make_cfg(dim, comps=comps, mhd=mhd, solver=solver, emf=emf, nx=(8, 8, 8), gamma=gamma)
+
+
+@pytest.mark.parametrize("mhd,comps", [(False, 1), (False, 3), (True, 2), (True, 3)])
+def test_prim_cons_round_trip(mhd, comps):
+    cfg = _cfg(comps, mhd, "hlld" if mhd else "hllc", comps=comps)
+    nv = 2 * comps + 2 if mhd else comps + 2
+    for _ in range(500):
+        v = np.zeros(8)
+        v[0] = rng.uniform(0.1, 10)
+        v[1:1 + comps] = rng.normal(size=comps)
+        v[comps + 1] = rng.uniform(0.1, 10)
+        if mhd:
+            v[comps + 2:comps + 2 + comps] = rng.normal(size=comps)
+        u = O.prim2cons(cfg, v)
+        w = O.cons2prim(cfg, u)
+        assert np.allclose(w[:nv], v[:nv], rtol=1e-13, atol=1e-14)
+        # E = p/(g-1) + rho v^2/2 + B^2/2
+        e = v[comps + 1] / (cfg.gamma - 1) + 0.5 * v[0] * np.sum(v[1:1 + comps] ** 2)
+        if mhd:
+            e += 0.5 * np.sum(v[comps + 2:comps + 2 + comps] ** 2)
+        assert u[comps + 1] == pytest.approx(e, rel=1e-14)
+
+
+def test_cons2prim_pressure_floor():
+    cfg = _cfg(3, True, "hlld", comps=3)
+    u = np.array([1.0, 1.0, 0.0, 0.0, 0.4, 0.1, 0.0, 0.0])  # E below kinetic energy
+    v = O.cons2prim(cfg, u)
+    assert v[4] == 1e-5
+
+
+# ---------------- fluxes ----------------
+def euler_flux(v, d, gamma, comps):
+    rho, vel, p = v[0], v[1:1 + comps], v[comps + 1]
+    E = p / (gamma - 1) + 0.5 * rho * np.sum(vel ** 2)
+    F = np.zeros(comps + 2)
+    F[0] = rho * vel[d]
+    F[1:1 + comps] = rho * vel * vel[d]
+    F[1 + d] += p
+    F[comps + 1] = (E + p) * vel[d]
+    return F
+
+
+def mhd_flux(v, d, gamma, comps=3):
+    rho, vel, p, B = v[0], v[1:4], v[4], v[5:8]
+    E = p / (gamma - 1) + 0.5 * rho * vel @ vel + 0.5 * B @ B
+    pt = p + 0.5 * B @ B
+    F = np.zeros(8)
+    F[0] = rho * vel[d]
+    F[1:4] = rho * vel * vel[d] - B * B[d]
+    F[1 + d] += pt
+    F[4] = (E + pt) * vel[d] - B[d] * (vel @ B)
+    F[5:8] = vel[d] * B - B[d] * vel
+    return F
+
+
+def _rand_hd(comps):
+    v = np.zeros(8)
+    v[0] = rng.uniform(0.2, 3)
+    v[1:1 + comps] = rng.normal(size=comps)
+    v[comps + 1] = rng.uniform(0.2, 3)
+    return v
+
+
+def _rand_mhd():
+    v = np.zeros(8)
+    v[0] = rng.uniform(0.2, 3)
+    v[1:4] = rng.normal(size=3)
+    v[4] = rng.uniform(0.2, 3)
+    v[5:8] = rng.normal(size=3)
+    return v
+
+
+@pytest.mark.parametrize("solver", ["hllc", "roe"])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hd_flux_consistency(solver, d):
+    cfg = _cfg(3, False, solver, comps=3, gamma=1.4)
+    for _ in range(200):
+        v = _rand_hd(3)
+        F, cmax, _, wrote = O.face_flux(cfg, d, v, v)
+        assert wrote
+        assert np.allclose(F[:5], euler_flux(v, d, 1.4, 3), rtol=1e-12, atol=1e-12)
+        a = np.sqrt(1.4 * v[4] / v[0])
+        assert cmax == pytest.approx(abs(v[1 + d]) + a, rel=1e-12)
+
+
+@pytest.mark.parametrize("emf", ["uct_contact", "uct_hlld"])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hlld_flux_consistency(emf, d):
+    cfg = _cfg(3, True, "hlld", comps=3, emf=emf)
+    for _ in range(200):
+        v = _rand_mhd()
+        F, cmax, _, _ = O.face_flux(cfg, d, v, v)
+        assert np.allclose(F, mhd_flux(v, d, 5.0 / 3.0), rtol=1e-11, atol=1e-11)
+
+
+def _mirror(v, d, mhd):
+    w = v.copy()
+    w[1 + d] = -w[1 + d]
+    if mhd:
+        # a reflection x_d -> -x_d flips the polar vector v_d and the axial B components
+        # other than B_d; the normal field is kept (it is shared by both sides)
+        for c in range(3):
+            if c != d:
+                w[5 + c] = -w[5 + c]
+    return w
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hllc_mirror_symmetry(d):
+    """Swapping and reflecting the two states reflects the flux: mass and energy fluxes
+    change sign, the normal momentum flux does not."""
+    cfg = _cfg(3, False, "hllc", comps=3, gamma=1.4)
+    for _ in range(300):
+        vL, vR = _rand_hd(3), _rand_hd(3)
+        F, _, _, _ = O.face_flux(cfg, d, vL, vR)
+        G, _, _, _ = O.face_flux(cfg, d, _mirror(vR, d, False), _mirror(vL, d, False))
+        assert G[0] == pytest.approx(-F[0], rel=1e-12, abs=1e-13)
+        assert G[1 + d] == pytest.approx(F[1 + d], rel=1e-12, abs=1e-13)
+        assert G[4] == pytest.approx(-F[4], rel=1e-12, abs=1e-13)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hlld_mirror_symmetry(d):
+    cfg = _cfg(3, True, "hlld", comps=3)
+    for _ in range(300):
+        vL, vR = _rand_mhd(), _rand_mhd()
+        vR[5 + d] = vL[5 + d]  # one normal field on the face
+        F, _, _, _ = O.face_flux(cfg, d, vL, vR)
+        G, _, _, _ = O.face_flux(cfg, d, _mirror(vR, d, True), _mirror(vL, d, True))
+        assert G[0] == pytest.approx(-F[0], rel=1e-11, abs=1e-12)
+        assert G[1 + d] == pytest.approx(F[1 + d], rel=1e-11, abs=1e-12)
+        assert G[4] == pytest.approx(-F[4], rel=1e-11, abs=1e-12)
+
+
+@pytest.mark.parametrize("solver", ["hllc", "roe"])
+def test_supersonic_upwinding(solver):
+    cfg = _cfg(3, False, solver, comps=3, gamma=1.4)
+    for _ in range(100):
+        vL, vR = _rand_hd(3), _rand_hd(3)
+        vL[1] = vR[1] = 10.0  # flow to the right faster than any wave
+        F, _, _, _ = O.face_flux(cfg, 0, vL, vR)
+        if solver == "hllc":
+            assert np.allclose(F[:5], euler_flux(vL, 0, 1.4, 3), rtol=1e-13)
+
+
+@pytest.mark.parametrize("solver", ["hllc", "hlld"])
+def test_stationary_contact_is_exact(solver):
+    """HLLC and HLLD resolve an isolated stationary contact exactly: zero mass flux and
+    the pressure (plus magnetic pressure) as the only momentum flux."""
+    mhd = solver == "hlld"
+    cfg = _cfg(3, mhd, solver, comps=3)
+    for _ in range(100):
+        p = rng.uniform(0.2, 2)
+        vL, vR = np.zeros(8), np.zeros(8)
+        vL[0], vR[0] = rng.uniform(0.2, 3, 2)
+        vL[4] = vR[4] = p
+        if mhd:
+            B = rng.normal(size=3)
+            B[0] = 0.0  # tangential field only: a pure contact / tangential discontinuity
+            vL[5:8] = vR[5:8] = B
+        F, _, _, _ = O.face_flux(cfg, 0, vL, vR)
+        assert abs(F[0]) < 1e-14
+        pt = p + (0.5 * vL[5:8] @ vL[5:8] if mhd else 0.0)
+        assert F[1] == pytest.approx(pt, rel=1e-13)
+        assert abs(F[4]) < 1e-13
+
+
+def test_hlld_emf_store_signs():
+    """Face EMF of the x sweep is -F(By) (= E_z = v_y B_x - v_x B_y), of the y sweep
+    +F(Bx) (storeFlux.hpp:14-29 with the signs of hlldMHD.hpp:79-142)."""
+    cfg = _cfg(3, True, "hlld", comps=3, emf="uct_contact")
+    for _ in range(100):
+        v = _rand_mhd()
+        F0, _, E0, _ = O.face_flux(cfg, 0, v, v)
+        F1, _, E1, _ = O.face_flux(cfg, 1, v, v)
+        ez = v[2] * v[5] - v[1] * v[6]
+        assert E0[0] == pytest.approx(ez, rel=1e-12, abs=1e-13)
+        assert E1[0] == pytest.approx(ez, rel=1e-12, abs=1e-13)
+
+
+def test_compute_timestep_is_brute_force_min():
+    cfg = O.make_cfg(2, mhd=True, solver="hlld", nx=(16, 12))
+    o = O.Oracle(cfg)
+    inv = o.array("InvDt")
+    inv[...] = rng.uniform(0.5, 5.0, inv.shape)
+    assert o.compute_timestep() == np.min(1.0 / o.interior(inv))
