@@ -21,6 +21,7 @@ void register_mhd2_f32(Ops *t);
 void register_mhd3_plm_f64(Ops *t);
 void register_mhd3_ppm_f64(Ops *t);
 void register_mhd3_f32(Ops *t);
+void register_mhd_roe(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -33,6 +34,7 @@ Ops *ops_table() {
     register_mhd3_plm_f64(t);
     register_mhd3_ppm_f64(t);
     register_mhd3_f32(t);
+    register_mhd_roe(t);
     return t;
   }();
   return table;
@@ -149,9 +151,12 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
   a.dtmax = c->dtmax;
   for (int d = 0; d < 3; d++) a.ext[d] = 0;
   if (c->d.mhd) {
-    // face range extension across DIR required by CT (hlldMHD.hpp:32-49)
+    // face range extension across DIR required by CT (hlldMHD.hpp:32-49); RoeMHD
+    // always extends by one cell (roeMHD.hpp:62-73)
     int perp = 1;
-    if (c->d.emf == IDFX_EMF_UCT_HLL || c->d.emf == IDFX_EMF_UCT_HLLD) perp = c->nghost[dir];
+    if ((c->d.emf == IDFX_EMF_UCT_HLL || c->d.emf == IDFX_EMF_UCT_HLLD) &&
+        c->d.solver != IDFX_SOLVER_ROE)
+      perp = c->nghost[dir];
     for (int d = 0; d < c->d.dim; d++) a.ext[d] = (d == dir) ? 0 : perp;
   }
   return a;
@@ -209,7 +214,7 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   const idfx_desc &d = *desc;
   if (d.dim < 1 || d.dim > 3 || d.comps < d.dim || d.comps > 3) return IDFX_E_ARGS;
   if (d.recon != 1 && d.recon != 2 && d.recon != 4) return IDFX_E_ARGS;
-  if (d.mhd && d.solver != IDFX_SOLVER_HLLD) return IDFX_E_ARGS;  // Roe MHD: not built
+  if (d.mhd && d.solver != IDFX_SOLVER_HLLD && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
   if (!d.mhd && d.solver != IDFX_SOLVER_HLLC && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
   if (d.emf < 0 || d.emf > 4) return IDFX_E_ARGS;
   const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver, d.precision == IDFX_PREC_FP32);

@@ -397,6 +397,95 @@ __device__ __forceinline__ bool roe_hd(const T *vL, const T *vR, T *flux, T &cma
   return true;
 }
 
+// Face data for constrained transport (storeFlux.hpp:14-208) with the signs of
+// hlldMHD.hpp:79-142 / roeMHD.hpp:106-170; shared by HLLD and Roe MHD.
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void store_face_emf(const T *vL, const T *vR, const T *uL,
+                                               const T *uR, const T *flux, T sl, T sr, T ptL,
+                                               T ptR, FaceEmf<T> &E, int emf) {
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  constexpr int BXn = C::template BXn<DIR>();
+  constexpr int BXt = C::template BXt<DIR>();
+  constexpr int BXb = C::template BXb<DIR>();
+  T st, sb = T(1);  // hlldMHD.hpp:79-142
+  if constexpr (DIR == IDIR) { st = T(-1); sb = T(1); }
+  if constexpr (DIR == JDIR) { st = T(1); sb = (C::DIM == 3) ? T(-1) : T(1); }
+  if constexpr (DIR == KDIR) { st = T(-1); sb = T(1); }
+  if (emf <= EMF_UCT_CONTACT) {
+    // K_StoreEMF (arithmetic, uct0) and K_StoreContact
+    if constexpr (CO > 1) {
+      E.Et = st * flux[BXt];
+      if constexpr (C::DIM == 3) E.Eb = sb * flux[BXb];
+    }
+    if (emf == EMF_UCT_CONTACT) {
+      T s = T(0.5);
+      if (flux[C::RHO] > T(1.0e-6)) s = T(1);
+      if (flux[C::RHO] < -T(1.0e-6)) s = T(0);
+      E.sv = s;
+    }
+  } else if (emf == EMF_UCT_HLL) {
+    const T ar = fmx(T(0), sr);
+    const T al = fmn(T(0), sl);
+    const T scrh = T(1) / (ar - al);
+    if constexpr (CO > 1) {
+      E.Et = -st * (ar * vL[Xt] - al * vR[Xt]) * scrh;
+      if constexpr (CO >= 3) E.Eb = -sb * (ar * vL[Xb] - al * vR[Xb]) * scrh;
+    }
+    E.aL = ar * scrh;
+    E.aR = -al * scrh;
+    E.dR = -al * ar * scrh;
+    E.dL = E.dR;
+  } else {
+    // K_StoreHLLD, storeFlux.hpp:77-208 (HAVE_ENERGY branch)
+    bool revert_to_hll = false;
+    const T Bn = (sr * vR[BXn] - sl * vL[BXn]) / (sr - sl);
+    const T duL = sl - vL[Xn];
+    const T duR = sr - vR[Xn];
+    T scrh = T(1) / (duR * uR[C::RHO] - duL * uL[C::RHO]);
+    const T Sc = (duR * uR[Xn] - duL * uL[Xn] - ptR + ptL) * scrh;
+    const T usLRHO = uL[C::RHO] * duL / (sl - Sc);
+    const T usRRHO = uR[C::RHO] * duR / (sr - Sc);
+    const T sqrL = sqrt_(usLRHO);
+    const T sqrR = sqrt_(usRRHO);
+    const T SaL = Sc - fab(Bn) / sqrL;
+    const T SaR = Sc + fab(Bn) / sqrR;
+    if ((SaL - sl) < T(1e-4) * (Sc - sl)) revert_to_hll = true;
+    if ((SaR - sr) > -T(1e-4) * (sr - Sc)) revert_to_hll = true;
+    const T chiL = (vL[Xn] - Sc) * (sl - Sc) / (SaL + sl - T(2) * Sc);
+    const T chiR = (vR[Xn] - Sc) * (sr - Sc) / (SaR + sr - T(2) * Sc);
+    const T nuL = (SaL + sl) / (fab(SaL) + fab(sl));
+    const T nuR = (SaR + sr) / (fab(SaR) + fab(sr));
+    const T nuLR = (SaL + SaR) / (fab(SaL) + fab(SaR));
+    if (fab(SaR - SaL) > T(1.e-9) * fab(sr - sl)) {
+      E.dL = T(0.5) * (chiL * nuL - chiL * nuLR) + T(0.5) * (fab(SaL) - nuLR * SaL);
+      E.dR = T(0.5) * (chiR * nuR - chiR * nuLR) + T(0.5) * (fab(SaR) - nuLR * SaR);
+      E.aL = T(0.5) * (T(1) + nuLR);
+      E.aR = T(0.5) * (T(1) - nuLR);
+    } else {
+      E.dL = T(0.5) * chiL * nuL + T(0.5) * fab(SaL);
+      E.dR = T(0.5) * chiR * nuR + T(0.5) * fab(SaR);
+      E.aL = T(0.5);
+      E.aR = T(0.5);
+    }
+    const T ar = fmx(T(0), sr);
+    const T al = fmn(T(0), sl);
+    scrh = T(1) / (ar - al);
+    if (revert_to_hll) {
+      E.aL = ar * scrh;
+      E.aR = -al * scrh;
+      E.dR = -al * ar * scrh;
+      E.dL = E.dR;
+    }
+    if constexpr (CO > 1) {
+      E.Et = -st * (ar * vL[Xt] - al * vR[Xt]) * scrh;
+      if constexpr (CO >= 3) E.Eb = -sb * (ar * vL[Xb] - al * vR[Xb]) * scrh;
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // HlldMHD, src/fluid/RiemannSolver/MHDsolvers/hlldMHD.hpp:147-589 (HAVE_ENERGY branch)
 // followed by the face-EMF stores of storeFlux.hpp:14-208.
@@ -621,81 +710,247 @@ __device__ __forceinline__ void hlld_mhd(const T *vL, const T *vR, T *flux, T &c
     }
   }
 
-  // face-EMF stores, hlldMHD.hpp:579-588 and storeFlux.hpp
-  T st, sb = T(1);  // hlldMHD.hpp:79-142
-  if constexpr (DIR == IDIR) { st = T(-1); sb = T(1); }
-  if constexpr (DIR == JDIR) { st = T(1); sb = (C::DIM == 3) ? T(-1) : T(1); }
-  if constexpr (DIR == KDIR) { st = T(-1); sb = T(1); }
-  if (emf <= EMF_UCT_CONTACT) {
-    // K_StoreEMF (arithmetic, uct0) and K_StoreContact
-    if constexpr (CO > 1) {
-      E.Et = st * flux[BXt];
-      if constexpr (C::DIM == 3) E.Eb = sb * flux[BXb];
-    }
-    if (emf == EMF_UCT_CONTACT) {
-      T s = T(0.5);
-      if (flux[C::RHO] > T(1.0e-6)) s = T(1);
-      if (flux[C::RHO] < -T(1.0e-6)) s = T(0);
-      E.sv = s;
-    }
-  } else if (emf == EMF_UCT_HLL) {
-    const T ar = fmx(T(0), sr);
-    const T al = fmn(T(0), sl);
-    const T scrh = T(1) / (ar - al);
-    if constexpr (CO > 1) {
-      E.Et = -st * (ar * vL[Xt] - al * vR[Xt]) * scrh;
-      if constexpr (CO >= 3) E.Eb = -sb * (ar * vL[Xb] - al * vR[Xb]) * scrh;
-    }
-    E.aL = ar * scrh;
-    E.aR = -al * scrh;
-    E.dR = -al * ar * scrh;
-    E.dL = E.dR;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, ptL, ptR, E, emf);
+}
+
+// ------------------------------------------------------------------------------------
+// RoeMHD, src/fluid/RiemannSolver/MHDsolvers/roeMHD.hpp:175-651 (HAVE_ENERGY,
+// ROE_AVERAGE 0; waves fast-, fast+, div B, entropy, slow-, slow+, Alfven-, Alfven+).
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void roe_mhd(const T *vL, const T *vR, T *flux, T &cmax,
+                                        FaceEmf<T> &E, int emf, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  constexpr int BXn = C::template BXn<DIR>();
+  constexpr int BXt = C::template BXt<DIR>();
+  constexpr int BXb = C::template BXb<DIR>();
+  constexpr int KFASTM = 0, KFASTP = 1, KDIVB = 2, KENTRP = 3, KSLOWM = 4, KSLOWP = 5,
+                KALFVM = 6, KALFVP = 7;
+  const T delta = T(1.e-6);
+  T dV[NV], uL[NV], uR[NV], dU[NV], fluxL[NV], fluxR[NV], Rc[NV][NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) dV[n] = vR[n] - vL[n];
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+#pragma unroll
+  for (int n = 0; n < NV; n++) dU[n] = uR[n] - uL[n];
+  kflux<C, DIR>(fluxL, vL, uL);
+  kflux<C, DIR>(fluxR, vR, uR);
+#pragma unroll
+  for (int p = 0; p < NV; p++)
+#pragma unroll
+    for (int q = 0; q < NV; q++) Rc[p][q] = 0;
+  const T sqr_rho_L = sqrt_(vL[C::RHO]);
+  const T sqr_rho_R = sqrt_(vR[C::RHO]);
+  T sl = sqr_rho_L / (sqr_rho_L + sqr_rho_R);
+  T sr = sqr_rho_R / (sqr_rho_L + sqr_rho_R);
+  const T rho = sr * vL[C::RHO] + sl * vR[C::RHO];
+  const T sqrt_rho = sqrt_(rho);
+  T u, v = 0, w = 0, Bx, By = 0, Bz = 0, bx, by = 0, bz = 0;
+  u = sl * vL[Xn] + sr * vR[Xn];
+  if constexpr (CO >= 2) v = sl * vL[Xt] + sr * vR[Xt];
+  if constexpr (CO >= 3) w = sl * vL[Xb] + sr * vR[Xb];
+  Bx = sr * vL[BXn] + sl * vR[BXn];
+  if constexpr (CO >= 2) By = sr * vL[BXt] + sl * vR[BXt];
+  if constexpr (CO >= 3) Bz = sr * vL[BXb] + sl * vR[BXb];
+  const T sBx = (Bx >= T(0) ? T(1) : T(-1));
+  bx = Bx / sqrt_rho;
+  if constexpr (CO >= 2) by = By / sqrt_rho;
+  if constexpr (CO >= 3) bz = Bz / sqrt_rho;
+  T bt2 = T(0);
+  if constexpr (CO >= 2) bt2 = bt2 + by * by;
+  if constexpr (CO >= 3) bt2 = bt2 + bz * bz;
+  const T b2 = bx * bx + bt2;
+  const T Btmag = sqrt_(bt2 * rho);
+  T X = dV[BXn] * dV[BXn];
+  if constexpr (CO >= 2) X = X + dV[BXt] * dV[BXt];
+  if constexpr (CO >= 3) X = X + dV[BXb] * dV[BXb];
+  X /= (sqr_rho_L + sqr_rho_R) * (sqr_rho_L + sqr_rho_R) * T(2);
+  T Bmag2L = vL[C::BX1] * vL[C::BX1];
+  if constexpr (CO >= 2) Bmag2L = Bmag2L + vL[C::BX2] * vL[C::BX2];
+  if constexpr (CO >= 3) Bmag2L = Bmag2L + vL[C::BX3] * vL[C::BX3];
+  T Bmag2R = vR[C::BX1] * vR[C::BX1];
+  if constexpr (CO >= 2) Bmag2R = Bmag2R + vR[C::BX2] * vR[C::BX2];
+  if constexpr (CO >= 3) Bmag2R = Bmag2R + vR[C::BX3] * vR[C::BX3];
+  const T pL = vL[C::ENG] + T(0.5) * Bmag2L;
+  const T pR = vR[C::ENG] + T(0.5) * Bmag2R;
+  T vdm = u * dU[Xn];
+  if constexpr (CO >= 2) vdm = vdm + v * dU[Xt];
+  if constexpr (CO >= 3) vdm = vdm + w * dU[Xb];
+  T BdB = Bx * dU[BXn];
+  if constexpr (CO >= 2) BdB = BdB + By * dU[BXt];
+  if constexpr (CO >= 3) BdB = BdB + Bz * dU[BXb];
+  T vel2 = u * u;
+  if constexpr (CO >= 2) vel2 = vel2 + v * v;
+  if constexpr (CO >= 3) vel2 = vel2 + w * w;
+  dV[C::ENG] = (gamma - T(1)) * ((T(0.5) * vel2 - X) * dV[C::RHO] - vdm + dU[C::ENG] - BdB);
+  const T HL = (uL[C::ENG] + pL) / vL[C::RHO];
+  const T HR = (uR[C::ENG] + pR) / vR[C::RHO];
+  const T H = sl * HL + sr * HR;
+  const T Hgas = H - b2;
+  const T a2 = (T(2) - gamma) * X + (gamma - T(1)) * (Hgas - T(0.5) * vel2);
+  T scrh = a2 - b2;
+  const T ca2 = bx * bx;
+  scrh = scrh * scrh + T(4) * bt2 * a2;
+  scrh = sqrt_(scrh);
+  const T cf2 = T(0.5) * (a2 + b2 + scrh);
+  const T cs2 = a2 * ca2 / cf2;
+  const T cf = sqrt_(cf2);
+  const T cs = sqrt_(cs2);
+  const T ca = sqrt_(ca2);
+  const T a = sqrt_(a2);
+  (void)ca;
+  T alpha_f, alpha_s;
+  if (cf == cs) {
+    alpha_f = T(1);
+    alpha_s = T(0);
+  } else if (a <= cs) {
+    alpha_f = T(0);
+    alpha_s = T(1);
+  } else if (cf <= a) {
+    alpha_f = T(1);
+    alpha_s = T(0);
   } else {
-    // K_StoreHLLD, storeFlux.hpp:77-208 (HAVE_ENERGY branch)
-    bool revert_to_hll = false;
-    const T Bn = (sr * vR[BXn] - sl * vL[BXn]) / (sr - sl);
-    const T duL = sl - vL[Xn];
-    const T duR = sr - vR[Xn];
-    T scrh = T(1) / (duR * uR[C::RHO] - duL * uL[C::RHO]);
-    const T Sc = (duR * uR[Xn] - duL * uL[Xn] - ptR + ptL) * scrh;
-    const T usLRHO = uL[C::RHO] * duL / (sl - Sc);
-    const T usRRHO = uR[C::RHO] * duR / (sr - Sc);
-    const T sqrL = sqrt_(usLRHO);
-    const T sqrR = sqrt_(usRRHO);
-    const T SaL = Sc - fab(Bn) / sqrL;
-    const T SaR = Sc + fab(Bn) / sqrR;
-    if ((SaL - sl) < T(1e-4) * (Sc - sl)) revert_to_hll = true;
-    if ((SaR - sr) > -T(1e-4) * (sr - Sc)) revert_to_hll = true;
-    const T chiL = (vL[Xn] - Sc) * (sl - Sc) / (SaL + sl - T(2) * Sc);
-    const T chiR = (vR[Xn] - Sc) * (sr - Sc) / (SaR + sr - T(2) * Sc);
-    const T nuL = (SaL + sl) / (fab(SaL) + fab(sl));
-    const T nuR = (SaR + sr) / (fab(SaR) + fab(sr));
-    const T nuLR = (SaL + SaR) / (fab(SaL) + fab(SaR));
-    if (fab(SaR - SaL) > T(1.e-9) * fab(sr - sl)) {
-      E.dL = T(0.5) * (chiL * nuL - chiL * nuLR) + T(0.5) * (fab(SaL) - nuLR * SaL);
-      E.dR = T(0.5) * (chiR * nuR - chiR * nuLR) + T(0.5) * (fab(SaR) - nuLR * SaR);
-      E.aL = T(0.5) * (T(1) + nuLR);
-      E.aR = T(0.5) * (T(1) - nuLR);
-    } else {
-      E.dL = T(0.5) * chiL * nuL + T(0.5) * fab(SaL);
-      E.dR = T(0.5) * chiR * nuR + T(0.5) * fab(SaR);
-      E.aL = T(0.5);
-      E.aR = T(0.5);
-    }
-    const T ar = fmx(T(0), sr);
-    const T al = fmn(T(0), sl);
-    scrh = T(1) / (ar - al);
-    if (revert_to_hll) {
-      E.aL = ar * scrh;
-      E.aR = -al * scrh;
-      E.dR = -al * ar * scrh;
-      E.dL = E.dR;
-    }
-    if constexpr (CO > 1) {
-      E.Et = -st * (ar * vL[Xt] - al * vR[Xt]) * scrh;
-      if constexpr (CO >= 3) E.Eb = -sb * (ar * vL[Xb] - al * vR[Xb]) * scrh;
-    }
+    scrh = T(1) / (cf2 - cs2);
+    alpha_f = (a2 - cs2) * scrh;
+    alpha_s = (cf2 - a2) * scrh;
+    alpha_f = fmx(T(0), alpha_f);
+    alpha_s = fmx(T(0), alpha_s);
+    alpha_f = sqrt_(alpha_f);
+    alpha_s = sqrt_(alpha_s);
   }
+  T beta_y = 0, beta_z = 0;
+  if (Btmag > T(1.e-9)) {
+    if constexpr (CO == 2) beta_y = (By >= T(0) ? T(1) : T(-1));
+    if constexpr (CO == 3) {
+      beta_y = By / Btmag;
+      beta_z = Bz / Btmag;
+    }
+  } else {
+    if constexpr (CO == 2) beta_y = T(1);
+    if constexpr (CO == 3) beta_z = beta_y = T(1);
+  }
+  T lambda[NV], eta[NV];
+  // fast wave u - c_f
+  lambda[KFASTM] = u - cf;
+  scrh = alpha_s * cs * sBx;
+  T beta_dv = T(0), beta_dB = T(0), beta_v = T(0);
+  if constexpr (CO >= 2) {
+    beta_dv = beta_dv + beta_y * dV[Xt];
+    beta_dB = beta_dB + beta_y * dV[BXt];
+    beta_v = beta_v + beta_y * v;
+  }
+  if constexpr (CO >= 3) {
+    beta_dv = beta_dv + beta_z * dV[Xb];
+    beta_dB = beta_dB + beta_z * dV[BXb];
+    beta_v = beta_v + beta_z * w;
+  }
+  Rc[C::RHO][KFASTM] = alpha_f;
+  Rc[Xn][KFASTM] = alpha_f * lambda[KFASTM];
+  if constexpr (CO >= 2) Rc[Xt][KFASTM] = alpha_f * v + scrh * beta_y;
+  if constexpr (CO >= 3) Rc[Xb][KFASTM] = alpha_f * w + scrh * beta_z;
+  if constexpr (CO >= 2) Rc[BXt][KFASTM] = alpha_s * a * beta_y / sqrt_rho;
+  if constexpr (CO >= 3) Rc[BXb][KFASTM] = alpha_s * a * beta_z / sqrt_rho;
+  Rc[C::ENG][KFASTM] = alpha_f * (Hgas - u * cf) + scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
+  eta[KFASTM] = alpha_f * (X * dV[C::RHO] + dV[C::ENG]);
+  eta[KFASTM] += rho * scrh * beta_dv - rho * alpha_f * cf * dV[Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  eta[KFASTM] *= T(0.5) / a2;
+  // fast wave u + c_f
+  lambda[KFASTP] = u + cf;
+  Rc[C::RHO][KFASTP] = alpha_f;
+  Rc[Xn][KFASTP] = alpha_f * lambda[KFASTP];
+  if constexpr (CO >= 2) Rc[Xt][KFASTP] = alpha_f * v - scrh * beta_y;
+  if constexpr (CO >= 3) Rc[Xb][KFASTP] = alpha_f * w - scrh * beta_z;
+  if constexpr (CO >= 2) Rc[BXt][KFASTP] = Rc[BXt][KFASTM];
+  if constexpr (CO >= 3) Rc[BXb][KFASTP] = Rc[BXb][KFASTM];
+  Rc[C::ENG][KFASTP] = alpha_f * (Hgas + u * cf) - scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
+  eta[KFASTP] = alpha_f * (X * dV[C::RHO] + dV[C::ENG]) - rho * scrh * beta_dv +
+                rho * alpha_f * cf * dV[Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  eta[KFASTP] *= T(0.5) / a2;
+  // entropy wave u
+  lambda[KENTRP] = u;
+  Rc[C::RHO][KENTRP] = T(1);
+  Rc[Xn][KENTRP] = u;
+  if constexpr (CO >= 2) Rc[Xt][KENTRP] = v;
+  if constexpr (CO >= 3) Rc[Xb][KENTRP] = w;
+  Rc[C::ENG][KENTRP] = T(0.5) * vel2 + (gamma - T(2)) / (gamma - T(1)) * X;
+  eta[KENTRP] = ((a2 - X) * dV[C::RHO] - dV[C::ENG]) / a2;
+  // div B wave u (zero jump)
+  lambda[KDIVB] = u;
+  eta[KDIVB] = T(0);
+  if constexpr (CO > 1) {
+    scrh = alpha_f * cf * sBx;
+    lambda[KSLOWM] = u - cs;
+    Rc[C::RHO][KSLOWM] = alpha_s;
+    Rc[Xn][KSLOWM] = alpha_s * lambda[KSLOWM];
+    Rc[Xt][KSLOWM] = alpha_s * v - scrh * beta_y;
+    if constexpr (CO >= 3) Rc[Xb][KSLOWM] = alpha_s * w - scrh * beta_z;
+    Rc[BXt][KSLOWM] = -alpha_f * a * beta_y / sqrt_rho;
+    if constexpr (CO >= 3) Rc[BXb][KSLOWM] = -alpha_f * a * beta_z / sqrt_rho;
+    Rc[C::ENG][KSLOWM] = alpha_s * (Hgas - u * cs) - scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
+    eta[KSLOWM] = alpha_s * (X * dV[C::RHO] + dV[C::ENG]) - rho * scrh * beta_dv -
+                  rho * alpha_s * cs * dV[Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    eta[KSLOWM] *= T(0.5) / a2;
+    lambda[KSLOWP] = u + cs;
+    Rc[C::RHO][KSLOWP] = alpha_s;
+    Rc[Xn][KSLOWP] = alpha_s * lambda[KSLOWP];
+    Rc[Xt][KSLOWP] = alpha_s * v + scrh * beta_y;
+    if constexpr (CO >= 3) Rc[Xb][KSLOWP] = alpha_s * w + scrh * beta_z;
+    Rc[BXt][KSLOWP] = Rc[BXt][KSLOWM];
+    if constexpr (CO >= 3) Rc[BXb][KSLOWP] = Rc[BXb][KSLOWM];
+    Rc[C::ENG][KSLOWP] = alpha_s * (Hgas + u * cs) + scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
+    eta[KSLOWP] = alpha_s * (X * dV[C::RHO] + dV[C::ENG]) + rho * scrh * beta_dv +
+                  rho * alpha_s * cs * dV[Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    eta[KSLOWP] *= T(0.5) / a2;
+  }
+  if constexpr (CO == 3) {
+    lambda[KALFVM] = u - ca;
+    Rc[Xt][KALFVM] = -rho * beta_z;
+    Rc[Xb][KALFVM] = +rho * beta_y;
+    Rc[BXt][KALFVM] = -sBx * sqrt_rho * beta_z;
+    Rc[BXb][KALFVM] = sBx * sqrt_rho * beta_y;
+    Rc[C::ENG][KALFVM] = -rho * (v * beta_z - w * beta_y);
+    eta[KALFVM] = +beta_y * dV[Xb] - beta_z * dV[Xt] +
+                  sBx / sqrt_rho * (beta_y * dV[BXb] - beta_z * dV[BXt]);
+    eta[KALFVM] *= T(0.5);
+    lambda[KALFVP] = u + ca;
+    Rc[Xt][KALFVP] = -Rc[Xt][KALFVM];
+    Rc[Xb][KALFVP] = -Rc[Xb][KALFVM];
+    Rc[BXt][KALFVP] = Rc[BXt][KALFVM];
+    Rc[BXb][KALFVP] = Rc[BXb][KALFVM];
+    Rc[C::ENG][KALFVP] = -Rc[C::ENG][KALFVM];
+    eta[KALFVP] = -beta_y * dV[Xb] + beta_z * dV[Xt] +
+                  sBx / sqrt_rho * (beta_y * dV[BXb] - beta_z * dV[BXt]);
+    eta[KALFVP] *= T(0.5);
+  }
+  cmax = fab(u) + cf;
+  sl = lambda[KFASTM];
+  sr = lambda[KFASTP];
+  T alambda[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) alambda[n] = fab(lambda[n]);
+  if (alambda[KFASTM] < T(0.5) * delta)
+    alambda[KFASTM] = lambda[KFASTM] * lambda[KFASTM] / delta + T(0.25) * delta;
+  if (alambda[KFASTP] < T(0.5) * delta)
+    alambda[KFASTP] = lambda[KFASTP] * lambda[KFASTP] / delta + T(0.25) * delta;
+  if constexpr (CO > 1) {
+    if (alambda[KSLOWM] < T(0.5) * delta)
+      alambda[KSLOWM] = lambda[KSLOWM] * lambda[KSLOWM] / delta + T(0.25) * delta;
+    if (alambda[KSLOWP] < T(0.5) * delta)
+      alambda[KSLOWP] = lambda[KSLOWP] * lambda[KSLOWP] / delta + T(0.25) * delta;
+  }
+#pragma unroll
+  for (int n1 = 0; n1 < NV; n1++) {
+    T sc = T(0);
+#pragma unroll
+    for (int n2 = 0; n2 < NV; n2++) sc += alambda[n2] * eta[n2] * Rc[n1][n2];
+    flux[n1] = T(0.5) * (fluxL[n1] + fluxR[n1] - sc);
+  }
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, pL, pR, E, emf);
 }
 
 // Dispatch to the configured Riemann solver.  Returns false only in the 1-D Roe case
@@ -704,8 +959,12 @@ template <class C, int DIR, typename T>
 __device__ __forceinline__ bool riemann(const T *vL, const T *vR, T *flux, T &cmax,
                                         FaceEmf<T> &E, int emf, T gamma) {
   if constexpr (C::MHD) {
-    static_assert(C::SOLVER == SOLVER_HLLD, "MHD solver not built");
-    hlld_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+    if constexpr (C::SOLVER == SOLVER_ROE) {
+      roe_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+    } else {
+      static_assert(C::SOLVER == SOLVER_HLLD, "MHD solver not built");
+      hlld_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+    }
     return true;
   } else if constexpr (C::SOLVER == SOLVER_HLLC) {
     hllc_hd<C, DIR>(vL, vR, flux, cmax, gamma);

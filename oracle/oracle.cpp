@@ -27,7 +27,7 @@
 //   HLLD + CT (all EMF schemes) ........ div B at round-off, conservation, oblique linear fast
 //                                        wave convergence (test_oracle_mhd)
 //   dt reduction ....................... brute-force min (test_oracle_pointwise)
-//   Roe MHD ............................ not restated (parity unpinned; the GPU path refuses it)
+//   Roe MHD ............................ consistency, mirror symmetry, linear fast wave (test_oracle_*)
 #include "oracle.h"
 
 #include <cmath>
@@ -401,6 +401,10 @@ bool roe_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, con
   return true;
 }
 
+void store_face_emf(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
+                    const double *vR, const double *uL, const double *uR, const double *flux,
+                    double sl, double sr, double ptL, double ptR, FaceEmf &E);
+
 // --------------------------------------------------------------------------------------
 // HlldMHD: src/fluid/RiemannSolver/MHDsolvers/hlldMHD.hpp:147-589 (HAVE_ENERGY branch),
 // with the face-EMF stores of storeFlux.hpp:14-208.
@@ -620,7 +624,16 @@ void hlld_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doub
     }
   }
 
-  // 7-- face EMF stores, hlldMHD.hpp:579-588
+  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, ptL, ptR, E);
+
+}
+
+// Face EMF stores (storeFlux.hpp:14-208) with the signs of hlldMHD.hpp:79-142 /
+// roeMHD.hpp:106-170; shared by HLLD and Roe MHD.
+void store_face_emf(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
+                    const double *vR, const double *uL, const double *uR, const double *flux,
+                    double sl, double sr, double ptL, double ptR, FaceEmf &E) {
+  const int comps = c.comps;
   double st = 1.0, sb = 1.0;  // hlldMHD.hpp:79-142
   if (dir == IDIR) { st = -1.0; if (c.dim == 3) sb = +1.0; }
   if (dir == JDIR) { st = +1.0; if (c.dim == 3) sb = -1.0; }
@@ -713,7 +726,248 @@ void hlld_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doub
         E.hasEb = true;
       }
     }
+  }}
+
+// --------------------------------------------------------------------------------------
+// RoeMHD: src/fluid/RiemannSolver/MHDsolvers/roeMHD.hpp:175-651 (HAVE_ENERGY,
+// ROE_AVERAGE 0, 8 waves: fast-, fast+, div B, entropy, slow-, slow+, Alfven-, Alfven+).
+// --------------------------------------------------------------------------------------
+void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
+             const double *vR, double *flux, double &cmax, FaceEmf &E) {
+  const int nv_ = V.nvar;
+  const int comps = c.comps;
+  const int KFASTM = 0, KFASTP = 1, KDIVB = 2, KENTRP = 3, KSLOWM = 4, KSLOWP = 5,
+            KALFVM = 6, KALFVP = 7;
+  const double delta = 1.e-6;
+  auto DSIGN = [](double x) { return x >= 0.0 ? 1.0 : -1.0; };
+  double dV[8], uL[8], uR[8], dU[8], fluxL[8], fluxR[8], Rc[8][8];
+  for (int n = 0; n < nv_; n++) dV[n] = vR[n] - vL[n];
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  const double a2L = 1.0, a2R = 1.0;  // unused with an energy equation
+  (void)a2R;
+  const double gamma = c.gamma;
+  for (int n = 0; n < nv_; n++) {
+    fluxL[n] = uL[n];
+    fluxR[n] = uR[n];
+    dU[n] = uR[n] - uL[n];
   }
+  kflux(c, V, D, fluxL, vL, fluxL);
+  kflux(c, V, D, fluxR, vR, fluxR);
+  for (int p = 0; p < 8; p++)
+    for (int q = 0; q < 8; q++) Rc[p][q] = 0;
+
+  const double sqr_rho_L = std::sqrt(vL[V.RHO]);
+  const double sqr_rho_R = std::sqrt(vR[V.RHO]);
+  double sl = sqr_rho_L / (sqr_rho_L + sqr_rho_R);
+  double sr = sqr_rho_R / (sqr_rho_L + sqr_rho_R);
+  const double rho = sr * vL[V.RHO] + sl * vR[V.RHO];
+  const double sqrt_rho = std::sqrt(rho);
+  double u = 0, v = 0, w = 0, Bx = 0, By = 0, Bz = 0, bx = 0, by = 0, bz = 0;
+  u = sl * vL[D.Xn] + sr * vR[D.Xn];
+  if (comps >= 2) v = sl * vL[D.Xt] + sr * vR[D.Xt];
+  if (comps >= 3) w = sl * vL[D.Xb] + sr * vR[D.Xb];
+  Bx = sr * vL[D.BXn] + sl * vR[D.BXn];
+  if (comps >= 2) By = sr * vL[D.BXt] + sl * vR[D.BXt];
+  if (comps >= 3) Bz = sr * vL[D.BXb] + sl * vR[D.BXb];
+  const double sBx = (Bx >= 0.0 ? 1.0 : -1.0);
+  bx = Bx / sqrt_rho;
+  if (comps >= 2) by = By / sqrt_rho;
+  if (comps >= 3) bz = Bz / sqrt_rho;
+  double bt2 = 0.0;
+  if (comps >= 2) bt2 = bt2 + by * by;
+  if (comps >= 3) bt2 = bt2 + bz * bz;
+  const double b2 = bx * bx + bt2;
+  const double Btmag = std::sqrt(bt2 * rho);
+  double X = dV[D.BXn] * dV[D.BXn];
+  if (comps >= 2) X = X + dV[D.BXt] * dV[D.BXt];
+  if (comps >= 3) X = X + dV[D.BXb] * dV[D.BXb];
+  X /= (sqr_rho_L + sqr_rho_R) * (sqr_rho_L + sqr_rho_R) * 2.0;
+  double Bmag2L = vL[V.BX1] * vL[V.BX1];
+  if (comps >= 2) Bmag2L = Bmag2L + vL[V.BX2] * vL[V.BX2];
+  if (comps >= 3) Bmag2L = Bmag2L + vL[V.BX3] * vL[V.BX3];
+  double Bmag2R = vR[V.BX1] * vR[V.BX1];
+  if (comps >= 2) Bmag2R = Bmag2R + vR[V.BX2] * vR[V.BX2];
+  if (comps >= 3) Bmag2R = Bmag2R + vR[V.BX3] * vR[V.BX3];
+  const double pL = vL[V.ENG] + 0.5 * Bmag2L;
+  const double pR = vR[V.ENG] + 0.5 * Bmag2R;
+
+  double vdm = u * dU[D.Xn];
+  if (comps >= 2) vdm = vdm + v * dU[D.Xt];
+  if (comps >= 3) vdm = vdm + w * dU[D.Xb];
+  double BdB = Bx * dU[D.BXn];
+  if (comps >= 2) BdB = BdB + By * dU[D.BXt];
+  if (comps >= 3) BdB = BdB + Bz * dU[D.BXb];
+  double vel2 = u * u;
+  if (comps >= 2) vel2 = vel2 + v * v;
+  if (comps >= 3) vel2 = vel2 + w * w;
+  dV[V.ENG] = (gamma - 1.0) * ((0.5 * vel2 - X) * dV[V.RHO] - vdm + dU[V.ENG] - BdB);
+  const double HL = (uL[V.ENG] + pL) / vL[V.RHO];
+  const double HR = (uR[V.ENG] + pR) / vR[V.RHO];
+  const double H = sl * HL + sr * HR;
+  const double Hgas = H - b2;
+  const double a2 = (2.0 - gamma) * X + (gamma - 1.0) * (Hgas - 0.5 * vel2);
+
+  double scrh = a2 - b2;
+  const double ca2 = bx * bx;
+  scrh = scrh * scrh + 4.0 * bt2 * a2;
+  scrh = std::sqrt(scrh);
+  const double cf2 = 0.5 * (a2 + b2 + scrh);
+  const double cs2 = a2 * ca2 / cf2;
+  const double cf = std::sqrt(cf2);
+  const double cs = std::sqrt(cs2);
+  const double ca = std::sqrt(ca2);
+  const double a = std::sqrt(a2);
+  double alpha_f, alpha_s;
+  if (cf == cs) {
+    alpha_f = 1.0;
+    alpha_s = 0.0;
+  } else if (a <= cs) {
+    alpha_f = 0.0;
+    alpha_s = 1.0;
+  } else if (cf <= a) {
+    alpha_f = 1.0;
+    alpha_s = 0.0;
+  } else {
+    scrh = 1.0 / (cf2 - cs2);
+    alpha_f = (a2 - cs2) * scrh;
+    alpha_s = (cf2 - a2) * scrh;
+    alpha_f = FMAX(0.0, alpha_f);
+    alpha_s = FMAX(0.0, alpha_s);
+    alpha_f = std::sqrt(alpha_f);
+    alpha_s = std::sqrt(alpha_s);
+  }
+  double beta_y = 0.0, beta_z = 0.0;
+  if (Btmag > 1.e-9) {
+    if (comps == 2) beta_y = DSIGN(By);
+    if (comps == 3) {
+      beta_y = By / Btmag;
+      beta_z = Bz / Btmag;
+    }
+  } else {
+    if (comps == 2) beta_y = 1.0;
+    if (comps == 3) beta_z = beta_y = 1.0;
+  }
+
+  double lambda[8], alambda[8], eta[8];
+  for (int n = 0; n < 8; n++) { lambda[n] = 0; eta[n] = 0; }
+  // fast wave u - c_f
+  int kk = KFASTM;
+  lambda[kk] = u - cf;
+  scrh = alpha_s * cs * sBx;
+  double beta_dv = 0.0, beta_dB = 0.0, beta_v = 0.0;
+  if (comps >= 2) {
+    beta_dv = beta_dv + beta_y * dV[D.Xt];
+    beta_dB = beta_dB + beta_y * dV[D.BXt];
+    beta_v = beta_v + beta_y * v;
+  }
+  if (comps >= 3) {
+    beta_dv = beta_dv + beta_z * dV[D.Xb];
+    beta_dB = beta_dB + beta_z * dV[D.BXb];
+    beta_v = beta_v + beta_z * w;
+  }
+  Rc[V.RHO][kk] = alpha_f;
+  Rc[D.Xn][kk] = alpha_f * lambda[kk];
+  if (comps >= 2) Rc[D.Xt][kk] = alpha_f * v + scrh * beta_y;
+  if (comps >= 3) Rc[D.Xb][kk] = alpha_f * w + scrh * beta_z;
+  if (comps >= 2) Rc[D.BXt][kk] = alpha_s * a * beta_y / sqrt_rho;
+  if (comps >= 3) Rc[D.BXb][kk] = alpha_s * a * beta_z / sqrt_rho;
+  Rc[V.ENG][kk] = alpha_f * (Hgas - u * cf) + scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
+  eta[kk] = alpha_f * (X * dV[V.RHO] + dV[V.ENG]);
+  eta[kk] += rho * scrh * beta_dv - rho * alpha_f * cf * dV[D.Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  eta[kk] *= 0.5 / a2;
+  // fast wave u + c_f
+  kk = KFASTP;
+  lambda[kk] = u + cf;
+  Rc[V.RHO][kk] = alpha_f;
+  Rc[D.Xn][kk] = alpha_f * lambda[kk];
+  if (comps >= 2) Rc[D.Xt][kk] = alpha_f * v - scrh * beta_y;
+  if (comps >= 3) Rc[D.Xb][kk] = alpha_f * w - scrh * beta_z;
+  if (comps >= 2) Rc[D.BXt][kk] = Rc[D.BXt][KFASTM];
+  if (comps >= 3) Rc[D.BXb][kk] = Rc[D.BXb][KFASTM];
+  Rc[V.ENG][kk] = alpha_f * (Hgas + u * cf) - scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
+  eta[kk] = alpha_f * (X * dV[V.RHO] + dV[V.ENG]) - rho * scrh * beta_dv +
+            rho * alpha_f * cf * dV[D.Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  eta[kk] *= 0.5 / a2;
+  // entropy wave u
+  kk = KENTRP;
+  lambda[kk] = u;
+  Rc[V.RHO][kk] = 1.0;
+  Rc[D.Xn][kk] = u;
+  if (comps >= 2) Rc[D.Xt][kk] = v;
+  if (comps >= 3) Rc[D.Xb][kk] = w;
+  Rc[V.ENG][kk] = 0.5 * vel2 + (gamma - 2.0) / (gamma - 1.0) * X;
+  eta[kk] = ((a2 - X) * dV[V.RHO] - dV[V.ENG]) / a2;
+  // div B wave u (zero jump)
+  kk = KDIVB;
+  lambda[kk] = u;
+  eta[kk] = 0.0;
+  if (comps > 1) {
+    scrh = alpha_f * cf * sBx;
+    kk = KSLOWM;
+    lambda[kk] = u - cs;
+    Rc[V.RHO][kk] = alpha_s;
+    Rc[D.Xn][kk] = alpha_s * lambda[kk];
+    Rc[D.Xt][kk] = alpha_s * v - scrh * beta_y;
+    if (comps >= 3) Rc[D.Xb][kk] = alpha_s * w - scrh * beta_z;
+    Rc[D.BXt][kk] = -alpha_f * a * beta_y / sqrt_rho;
+    if (comps >= 3) Rc[D.BXb][kk] = -alpha_f * a * beta_z / sqrt_rho;
+    Rc[V.ENG][kk] = alpha_s * (Hgas - u * cs) - scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
+    eta[kk] = alpha_s * (X * dV[V.RHO] + dV[V.ENG]) - rho * scrh * beta_dv -
+              rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    eta[kk] *= 0.5 / a2;
+    kk = KSLOWP;
+    lambda[kk] = u + cs;
+    Rc[V.RHO][kk] = alpha_s;
+    Rc[D.Xn][kk] = alpha_s * lambda[kk];
+    Rc[D.Xt][kk] = alpha_s * v + scrh * beta_y;
+    if (comps >= 3) Rc[D.Xb][kk] = alpha_s * w + scrh * beta_z;
+    Rc[D.BXt][kk] = Rc[D.BXt][KSLOWM];
+    if (comps >= 3) Rc[D.BXb][kk] = Rc[D.BXb][KSLOWM];
+    Rc[V.ENG][kk] = alpha_s * (Hgas + u * cs) + scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
+    eta[kk] = alpha_s * (X * dV[V.RHO] + dV[V.ENG]) + rho * scrh * beta_dv +
+              rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    eta[kk] *= 0.5 / a2;
+  }
+  if (comps == 3) {
+    kk = KALFVM;
+    lambda[kk] = u - ca;
+    Rc[D.Xt][kk] = -rho * beta_z;
+    Rc[D.Xb][kk] = +rho * beta_y;
+    Rc[D.BXt][kk] = -sBx * sqrt_rho * beta_z;
+    Rc[D.BXb][kk] = sBx * sqrt_rho * beta_y;
+    Rc[V.ENG][kk] = -rho * (v * beta_z - w * beta_y);
+    eta[kk] = +beta_y * dV[D.Xb] - beta_z * dV[D.Xt] +
+              sBx / sqrt_rho * (beta_y * dV[D.BXb] - beta_z * dV[D.BXt]);
+    eta[kk] *= 0.5;
+    kk = KALFVP;
+    lambda[kk] = u + ca;
+    Rc[D.Xt][kk] = -Rc[D.Xt][KALFVM];
+    Rc[D.Xb][kk] = -Rc[D.Xb][KALFVM];
+    Rc[D.BXt][kk] = Rc[D.BXt][KALFVM];
+    Rc[D.BXb][kk] = Rc[D.BXb][KALFVM];
+    Rc[V.ENG][kk] = -Rc[V.ENG][KALFVM];
+    eta[kk] = -beta_y * dV[D.Xb] + beta_z * dV[D.Xt] +
+              sBx / sqrt_rho * (beta_y * dV[D.BXb] - beta_z * dV[D.BXt]);
+    eta[kk] *= 0.5;
+  }
+  cmax = std::fabs(u) + cf;
+  sl = lambda[KFASTM];
+  sr = lambda[KFASTP];
+  for (int n = 0; n < nv_; n++) alambda[n] = std::fabs(lambda[n]);
+  if (alambda[KFASTM] < 0.5 * delta) alambda[KFASTM] = lambda[KFASTM] * lambda[KFASTM] / delta + 0.25 * delta;
+  if (alambda[KFASTP] < 0.5 * delta) alambda[KFASTP] = lambda[KFASTP] * lambda[KFASTP] / delta + 0.25 * delta;
+  if (comps > 1) {
+    if (alambda[KSLOWM] < 0.5 * delta) alambda[KSLOWM] = lambda[KSLOWM] * lambda[KSLOWM] / delta + 0.25 * delta;
+    if (alambda[KSLOWP] < 0.5 * delta) alambda[KSLOWP] = lambda[KSLOWP] * lambda[KSLOWP] / delta + 0.25 * delta;
+  }
+  for (int n1 = 0; n1 < nv_; n1++) {
+    double sc = 0.0;
+    for (int n2 = 0; n2 < nv_; n2++) sc += alambda[n2] * eta[n2] * Rc[n1][n2];
+    flux[n1] = 0.5 * (fluxL[n1] + fluxR[n1] - sc);
+  }
+  (void)a2L;
+  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, pL, pR, E);
 }
 
 // --------------------------------------------------------------------------------------
@@ -1097,8 +1351,10 @@ struct Oracle {
     int ext[3] = {0, 0, 0};
     if (c.mhd) {
       // hlldMHD.hpp:32-49 (roeMHD.hpp has the same extension)
+      // RoeMHD always extends by one cell (roeMHD.hpp:62-73)
       int perp = 1;
-      if (c.emf == ORC_EMF_UCT_HLL || c.emf == ORC_EMF_UCT_HLLD) perp = nghost[dir];
+      if ((c.emf == ORC_EMF_UCT_HLL || c.emf == ORC_EMF_UCT_HLLD) && c.solver != ORC_SOLVER_ROE)
+        perp = nghost[dir];
       for (int d = 0; d < 3; d++) ext[d] = (d == dir || d >= c.dim) ? 0 : perp;
     }
     const Dir D(V, dir);
@@ -1116,7 +1372,8 @@ struct Oracle {
           if (c.mhd) {
             vL[D.BXn] = vs(dir, k, j, i);
             vR[D.BXn] = vL[D.BXn];
-            hlld_mhd(c, V, D, dir, vL, vR, F, cmax, E);
+            if (c.solver == ORC_SOLVER_ROE) roe_mhd(c, V, D, dir, vL, vR, F, cmax, E);
+            else hlld_mhd(c, V, D, dir, vL, vR, F, cmax, E);
           } else if (c.solver == ORC_SOLVER_HLLC) {
             hllc_hd(c, V, D, vL, vR, F, cmax);
           } else {
@@ -1519,7 +1776,6 @@ void *orc_new(const orc_cfg *cfg) {
   if (cfg->dim < 1 || cfg->dim > 3 || cfg->comps < cfg->dim || cfg->comps > 3) return nullptr;
   if (cfg->mhd && cfg->solver == ORC_SOLVER_HLLC) return nullptr;
   if (!cfg->mhd && cfg->solver == ORC_SOLVER_HLLD) return nullptr;
-  if (cfg->mhd && cfg->solver == ORC_SOLVER_ROE) return nullptr;  // Roe MHD not restated
   return new Oracle(*cfg);
 }
 void orc_free(void *o) { delete static_cast<Oracle *>(o); }
@@ -1630,7 +1886,8 @@ int orc_face_flux(const orc_cfg *cfg, int dir, const double *vL_in, const double
   E.Et = E.Eb = E.sv = E.aL = E.aR = E.dL = E.dR = 0.0;
   bool wrote = true;
   if (cfg->mhd) {
-    hlld_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
+    if (cfg->solver == ORC_SOLVER_ROE) roe_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
+    else hlld_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
   } else if (cfg->solver == ORC_SOLVER_HLLC) {
     hllc_hd(*cfg, V, D, vL, vR, flux, *cmax);
   } else {

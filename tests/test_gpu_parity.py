@@ -34,14 +34,25 @@ def test_orszag_tang_cycles(emf, recon):
     compare_state(o, g, what="OT %s %s" % (emf, recon))
 
 
+@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact", "arithmetic"])
+@pytest.mark.parametrize("recon", ["plm", "ppm"])
+def test_orszag_tang_roe_cycles(emf, recon):
+    kw = cfg_kwargs(2, (48, 40), emf=emf, recon=recon, solver="roe", cfl=0.5)
+    o, g = make_pair(kw, problems.orszag_tang)
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="OT roe %s %s" % (emf, recon))
+
+
+@pytest.mark.parametrize("solver", ["hlld", "roe"])
 @pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact"])
 @pytest.mark.parametrize("recon", ["plm", "ppm"])
-def test_mhd3d_cycles(emf, recon):
-    kw = cfg_kwargs(3, (16, 12, 10), emf=emf, recon=recon, cfl=0.4, nstages=3)
+def test_mhd3d_cycles(emf, recon, solver):
+    kw = cfg_kwargs(3, (16, 12, 10), emf=emf, recon=recon, cfl=0.4, nstages=3, solver=solver)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=3, amp=0.2))
     o.set_boundaries()
     _cycle_both(o, g, 3)
-    compare_state(o, g, what="3D %s %s" % (emf, recon))
+    compare_state(o, g, what="3D %s %s %s" % (solver, emf, recon))
 
 
 @pytest.mark.parametrize("solver", ["hllc", "roe"])

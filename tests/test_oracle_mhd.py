@@ -46,9 +46,9 @@ def test_ot_divb_and_conservation(emf, recon):
     assert np.all(np.abs(t1 - t0) <= 1e-12 * np.abs(scale) + 1e-12), (t1 - t0)
 
 
-def _wave_error(n, emf, recon="plm"):
+def _wave_error(n, emf, recon="plm", solver="hlld"):
     Lx, Ly = 2.0, 1.0
-    cfg = O.make_cfg(2, mhd=True, recon=recon, solver="hlld", emf=emf, nx=(2 * n, n),
+    cfg = O.make_cfg(2, mhd=True, recon=recon, solver=solver, emf=emf, nx=(2 * n, n),
                      xend=(Lx, Ly, 1.0), cfl=0.6, first_dt=1e-4)
     o = O.Oracle(cfg)
     Vc, Vs, period = problems.linear_fast_wave_2d(O.OracleGrid(o), eps=1e-6, Lx=Lx, Ly=Ly)
@@ -62,13 +62,16 @@ def _wave_error(n, emf, recon="plm"):
     return np.mean(np.abs(v1[0] - v0[0])) / 1e-6, o.check_divb()
 
 
-@pytest.mark.parametrize("emf,recon,n", [("uct_hlld", "plm", 32), ("uct_contact", "plm", 32),
-                                         ("uct_hlld", "ppm", 16)])
-def test_linear_fast_wave_second_order(emf, recon, n):
+@pytest.mark.parametrize("emf,recon,n,solver", [("uct_hlld", "plm", 32, "hlld"),
+                                                ("uct_contact", "plm", 32, "hlld"),
+                                                ("uct_hlld", "ppm", 16, "hlld"),
+                                                ("uct_contact", "plm", 32, "roe"),
+                                                ("arithmetic", "ppm", 16, "roe")])
+def test_linear_fast_wave_second_order(emf, recon, n, solver):
     """After one period the wave returns to its initial state; the L1 error must fall by
     more than 3x per halving of the cell size (observed: 3.5 for PLM, 3.5 for PPM)."""
-    e1, _ = _wave_error(n, emf, recon)
-    e2, d2 = _wave_error(2 * n, emf, recon)
+    e1, _ = _wave_error(n, emf, recon, solver)
+    e2, d2 = _wave_error(2 * n, emf, recon, solver)
     assert d2 < 1e-12
     assert e2 < 0.02, e2  # relative to the perturbation amplitude
     assert e1 / e2 > 3.0, (e1, e2)

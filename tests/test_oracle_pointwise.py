@@ -155,10 +155,11 @@ def test_hd_flux_consistency(solver, d):
         assert cmax == pytest.approx(abs(v[1 + d]) + a, rel=1e-12)
 
 
+@pytest.mark.parametrize("solver", ["hlld", "roe"])
 @pytest.mark.parametrize("emf", ["uct_contact", "uct_hlld"])
 @pytest.mark.parametrize("d", [0, 1, 2])
-def test_hlld_flux_consistency(emf, d):
-    cfg = _cfg(3, True, "hlld", comps=3, emf=emf)
+def test_mhd_flux_consistency(solver, emf, d):
+    cfg = _cfg(3, True, solver, comps=3, emf=emf)
     for _ in range(200):
         v = _rand_mhd()
         F, cmax, _, _ = O.face_flux(cfg, d, v, v)
@@ -191,9 +192,10 @@ def test_hllc_mirror_symmetry(d):
         assert G[4] == pytest.approx(-F[4], rel=1e-12, abs=1e-13)
 
 
+@pytest.mark.parametrize("solver", ["hlld", "roe"])
 @pytest.mark.parametrize("d", [0, 1, 2])
-def test_hlld_mirror_symmetry(d):
-    cfg = _cfg(3, True, "hlld", comps=3)
+def test_mhd_mirror_symmetry(solver, d):
+    cfg = _cfg(3, True, solver, comps=3)
     for _ in range(300):
         vL, vR = _rand_mhd(), _rand_mhd()
         vR[5 + d] = vL[5 + d]  # one normal field on the face
@@ -256,3 +258,18 @@ def test_compute_timestep_is_brute_force_min():
     inv = o.array("InvDt")
     inv[...] = rng.uniform(0.5, 5.0, inv.shape)
     assert o.compute_timestep() == np.min(1.0 / o.interior(inv))
+
+
+def test_roe_mhd_2d_consistency():
+    """COMPONENTS = 2: the 6-wave Roe flux of a uniform state is the physical flux."""
+    cfg = _cfg(2, True, "roe", comps=2)
+    for _ in range(200):
+        v3 = _rand_mhd()
+        v3[3] = v3[7] = 0.0  # no third components
+        v = np.zeros(8)
+        v[0:3], v[3], v[4:6] = v3[0:3], v3[4], v3[5:7]
+        for d in range(2):
+            F, _, _, _ = O.face_flux(cfg, d, v, v)
+            ref = mhd_flux(v3, d, 5.0 / 3.0)
+            assert np.allclose(F[:6], [ref[0], ref[1], ref[2], ref[4], ref[5], ref[6]],
+                               rtol=1e-11, atol=1e-11)
