@@ -67,6 +67,8 @@ struct idfx_ctx {
   hipEvent_t ev_dt = nullptr;
   int target_waves = 8192;
   bool fp32 = false;
+  bool uniform = false;  // every direction has one cell width
+  double dx0[3] = {1.0, 1.0, 1.0};
   const void *emf() const { return fp32 ? (const void *)&e32 : (const void *)&e64; }
 };
 
@@ -150,6 +152,17 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
   a.flags = flags;
   a.dtmax = c->dtmax;
   for (int d = 0; d < 3; d++) a.ext[d] = 0;
+  // uniform grid: every cell has the volume of cell (beg); dt/dV computed once here with
+  // the device's association (dx1 * dx2) * dx3
+  a.uniform = c->uniform ? 1 : 0;
+  if (c->uniform) {
+    double v = c->dx0[0];
+    if (c->d.dim >= 2) v = v * c->dx0[1];
+    if (c->d.dim >= 3) v = v * c->dx0[2];
+    a.dtdV = dt / v;
+  } else {
+    a.dtdV = 0.0;
+  }
   if (c->d.mhd) {
     // face range extension across DIR required by CT (hlldMHD.hpp:32-49); RoeMHD
     // always extends by one cell (roeMHD.hpp:62-73)
@@ -253,6 +266,11 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
     if (st != IDFX_OK) break;
     std::vector<double> h(c->np_tot[q], 1.0);
     if (dxh[q]) std::memcpy(h.data(), dxh[q], sizeof(double) * c->np_tot[q]);
+    c->dx0[q] = h[0];
+    bool same = true;
+    for (double v : h) same = same && (v == h[0]);
+    if (q == 0) c->uniform = same;
+    else c->uniform = c->uniform && same;
     if (hipMemcpy(p, h.data(), sizeof(double) * c->np_tot[q], hipMemcpyHostToDevice) != hipSuccess)
       st = IDFX_E_LAUNCH;
     c->g.dx[q] = p;

@@ -62,6 +62,8 @@ struct SweepArgs {
   int flags;
   int ext[3];     // transverse extension of the face range (CT needs)
   unsigned long long *dtmax;  // bit pattern of max(InvDt) (non-negative doubles)
+  int uniform;    // all cells share one volume: dtdV below equals dt/dV(k,j,i)
+  double dtdV;
 };
 
 __device__ __forceinline__ long long idx3(const GridDesc &g, int k, int j, int i) {
@@ -203,7 +205,7 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
                                               int j, int i, const T *Fm, const T *Fp,
                                               T cmaxm, T cmaxp) {
   const long long p = idx3(g, k, j, i);
-  const double dtdV = a.dt / cell_dV<C::DIM>(g, k, j, i);
+  const double dtdV = a.uniform ? a.dtdV : a.dt / cell_dV<C::DIM>(g, k, j, i);
   const double A = face_A<C::DIM, DIR>(g, k, j, i);
   double u[C::NVAR];
   if (a.flags & (SW_INIT_UC | SW_SAVE_UC0)) {
@@ -250,13 +252,23 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
 }
 
 // ------------------------------------------------------------------------------------
-// x sweep: one face per lane (64 faces / 63 cells per wave)
+// x sweep: one cell per lane.  Lane l reconstructs cell c = i0 - 1 + l once; face c
+// takes its left state from lane l-1 (a cross-lane shuffle) and its right state from the
+// lane itself, and cell c takes the flux of face c+1 from lane l+1.  A wave therefore
+// solves 63 faces and updates the 62 cells between them.
 // ------------------------------------------------------------------------------------
+// occupancy target of the x sweep: 4 waves per SIMD (<= 128 VGPRs) in 2-D, 3 in 3-D
+template <class C>
+constexpr int x_waves_per_eu() {
+  return C::DIM == 3 ? 3 : 4;
+}
 template <class C, typename T, bool FUSED>
-__global__ __launch_bounds__(256) void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(x_waves_per_eu<C>())))
+void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                  int nseg) {
   constexpr int NV = C::NVAR;
-  constexpr int H = window<C>() / 2;
+  constexpr int NW = window<C>();
+  constexpr int H = NW / 2;
   const int lane = threadIdx.x & 63;
   const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int nj = g.end[1] - g.beg[1] + 2 * a.ext[1];
@@ -268,33 +280,35 @@ __global__ __launch_bounds__(256) void k_sweep_x(GridDesc g, SweepArgs a, EmfArr
     const int r = w / nseg;
     const int j = g.beg[1] - a.ext[1] + r % nj;
     const int k = g.beg[2] - a.ext[2] + r / nj;
-    const int i = g.beg[0] + seg * 63 + lane;  // face i-1/2
-    const bool face_ok = i <= g.end[0];
-    const int ic = face_ok ? i : g.end[0];
+    const int c = g.beg[0] - 1 + seg * 62 + lane;  // this lane's cell; its left face is c-1/2
+    const int cc = c <= g.end[0] ? c : g.end[0];     // clamped for the loads
     const long long row = idx3(g, k, j, 0);
 
+    // reconstruct the lane's cell: vl = right state of face c, vr = left state of face c+1
+    T vl[NV], vr[NV];
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      T wv[NW];
+#pragma unroll
+      for (int s = 0; s < NW; s++) wv[s] = (T)a.Vc[n * g.sv + row + cc - H + s];
+      recon_cell<C>(wv, n, vl[n], vr[n]);
+    }
     T vL[NV], vR[NV];
 #pragma unroll
     for (int n = 0; n < NV; n++) {
-      T wl[window<C>()], wr[window<C>()];
-#pragma unroll
-      for (int s = 0; s < window<C>(); s++) {
-        wl[s] = (T)a.Vc[n * g.sv + row + ic - 1 - H + s];
-        wr[s] = (T)a.Vc[n * g.sv + row + ic - H + s];
-      }
-      T dummy;
-      recon_cell<C>(wl, n, dummy, vL[n]);
-      recon_cell<C>(wr, n, vR[n], dummy);
+      vL[n] = __shfl_up(vr[n], 1);
+      vR[n] = vl[n];
     }
     if constexpr (C::MHD) {
-      vL[C::BX1] = (T)a.Vs[idxs(g, 0, k, j, ic)];
+      vL[C::BX1] = (T)a.Vs[idxs(g, 0, k, j, cc)];
       vR[C::BX1] = vL[C::BX1];
     }
     T F[NV], cmax;
     FaceEmf<T> E;
     const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
-    const bool own_face = face_ok && (lane < 63 || i == g.end[0]);
-    const long long p = row + i;
+    const bool face_ok = lane >= 1 && c <= g.end[0];
+    const bool own_face = face_ok && (lane <= 62 || c == g.end[0]);
+    const long long p = row + c;
     if (own_face) store_face_emf<C, IDIR>(e, p, a.emf, E);
     if constexpr (!FUSED) {
       if (own_face) {
@@ -309,10 +323,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(GridDesc g, SweepArgs a, EmfArr
 #pragma unroll
       for (int n = 0; n < NV; n++) Fp[n] = __shfl_down(F[n], 1);
       const T cmaxp = __shfl_down(cmax, 1);
-      const bool cell_ok = lane < 63 && i < g.end[0] && j >= g.beg[1] && j < g.end[1] &&
-                           k >= g.beg[2] && k < g.end[2];
+      const bool cell_ok = lane >= 1 && lane <= 62 && c < g.end[0] && j >= g.beg[1] &&
+                           j < g.end[1] && k >= g.beg[2] && k < g.end[2];
       if (cell_ok) {
-        const double inv = cell_update<C, IDIR>(g, a, k, j, i, F, Fp, cmax, cmaxp);
+        const double inv = cell_update<C, IDIR>(g, a, k, j, c, F, Fp, cmax, cmaxp);
         red = __double_as_longlong(inv);
       }
     }
@@ -333,8 +347,12 @@ struct MarchGeom {
   int nxb;        // 64-wide x blocks
 };
 
+#ifndef IDFX_MARCH_WAVES_PER_EU
+#define IDFX_MARCH_WAVES_PER_EU 2
+#endif
 template <class C, typename T, int DIR, bool FUSED>
-__global__ __launch_bounds__(256) void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IDFX_MARCH_WAVES_PER_EU)))
+void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                      MarchGeom m) {
   static_assert(DIR == JDIR || DIR == KDIR, "marching sweep is for y and z");
   constexpr int NV = C::NVAR;

@@ -47,7 +47,7 @@ hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArra
                             int target_waves, hipStream_t s) {
   if constexpr (DIR == IDIR) {
     const int nint = g.end[0] - g.beg[0];
-    const int nseg = (nint + 62) / 63;
+    const int nseg = (nint + 61) / 62;
     const long long nj = g.end[1] - g.beg[1] + 2 * a.ext[1];
     const long long nk = g.end[2] - g.beg[2] + 2 * a.ext[2];
     const long long waves = (long long)nseg * nj * nk;

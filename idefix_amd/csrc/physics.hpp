@@ -397,12 +397,36 @@ __device__ __forceinline__ bool roe_hd(const T *vL, const T *vR, T *flux, T &cma
   return true;
 }
 
+// Star-region quantities that HllD and K_StoreHLLD both compute from the same inputs
+// with the same expressions (hlldMHD.hpp:292-306, storeFlux.hpp:121-135): computed once.
+template <typename T>
+struct StarPre {
+  bool valid;
+  T duL, duR, scrh, Sc, usLRHO, usRRHO, sqrL, sqrR;
+};
+
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void star_pre(const T *vL, const T *vR, const T *uL, const T *uR,
+                                         T sl, T sr, T ptL, T ptR, StarPre<T> &P) {
+  constexpr int Xn = C::template Xn<DIR>();
+  P.valid = true;
+  P.duL = sl - vL[Xn];
+  P.duR = sr - vR[Xn];
+  P.scrh = T(1) / (P.duR * uR[C::RHO] - P.duL * uL[C::RHO]);
+  P.Sc = (P.duR * uR[Xn] - P.duL * uL[Xn] - ptR + ptL) * P.scrh;
+  P.usLRHO = uL[C::RHO] * P.duL / (sl - P.Sc);
+  P.usRRHO = uR[C::RHO] * P.duR / (sr - P.Sc);
+  P.sqrL = sqrt_(P.usLRHO);
+  P.sqrR = sqrt_(P.usRRHO);
+}
+
 // Face data for constrained transport (storeFlux.hpp:14-208) with the signs of
 // hlldMHD.hpp:79-142 / roeMHD.hpp:106-170; shared by HLLD and Roe MHD.
 template <class C, int DIR, typename T>
 __device__ __forceinline__ void store_face_emf(const T *vL, const T *vR, const T *uL,
                                                const T *uR, const T *flux, T sl, T sr, T ptL,
-                                               T ptR, FaceEmf<T> &E, int emf) {
+                                               T ptR, FaceEmf<T> &E, int emf,
+                                               const StarPre<T> &pre) {
   constexpr int CO = C::COMP;
   constexpr int Xn = C::template Xn<DIR>();
   constexpr int Xt = C::template Xt<DIR>();
@@ -442,14 +466,12 @@ __device__ __forceinline__ void store_face_emf(const T *vL, const T *vR, const T
     // K_StoreHLLD, storeFlux.hpp:77-208 (HAVE_ENERGY branch)
     bool revert_to_hll = false;
     const T Bn = (sr * vR[BXn] - sl * vL[BXn]) / (sr - sl);
-    const T duL = sl - vL[Xn];
-    const T duR = sr - vR[Xn];
-    T scrh = T(1) / (duR * uR[C::RHO] - duL * uL[C::RHO]);
-    const T Sc = (duR * uR[Xn] - duL * uL[Xn] - ptR + ptL) * scrh;
-    const T usLRHO = uL[C::RHO] * duL / (sl - Sc);
-    const T usRRHO = uR[C::RHO] * duR / (sr - Sc);
-    const T sqrL = sqrt_(usLRHO);
-    const T sqrR = sqrt_(usRRHO);
+    StarPre<T> P = pre;
+    if (!P.valid) star_pre<C, DIR>(vL, vR, uL, uR, sl, sr, ptL, ptR, P);
+    T scrh;
+    const T Sc = P.Sc;
+    const T sqrL = P.sqrL;
+    const T sqrR = P.sqrR;
     const T SaL = Sc - fab(Bn) / sqrL;
     const T SaR = Sc + fab(Bn) / sqrR;
     if ((SaL - sl) < T(1e-4) * (Sc - sl)) revert_to_hll = true;
@@ -546,6 +568,9 @@ __device__ __forceinline__ void hlld_mhd(const T *vL, const T *vR, T *flux, T &c
   if constexpr (CO >= 3) b2R = b2R + vR[C::BX3] * vR[C::BX3];
   const T ptL = vL[C::ENG] + T(0.5) * b2L;
   const T ptR = vR[C::ENG] + T(0.5) * b2R;
+  StarPre<T> pre;
+  pre.valid = false;
+  if (emf == EMF_UCT_HLLD) star_pre<C, DIR>(vL, vR, uL, uR, sl, sr, ptL, ptR, pre);
 
   if (sl > T(0)) {
 #pragma unroll
@@ -561,17 +586,19 @@ __device__ __forceinline__ void hlld_mhd(const T *vL, const T *vR, T *flux, T &c
     scrh = T(1) / (sr - sl);
     Bx = (sr * vR[BXn] - sl * vL[BXn]) * scrh;
     sBx = (Bx > T(0) ? T(1) : T(-1));
-    duL = sl - vL[Xn];
-    duR = sr - vR[Xn];
-    scrh = T(1) / (duR * uR[C::RHO] - duL * uL[C::RHO]);
-    SM = (duR * uR[Xn] - duL * uL[Xn] - ptR + ptL) * scrh;
+    StarPre<T> P = pre;
+    if (!P.valid) star_pre<C, DIR>(vL, vR, uL, uR, sl, sr, ptL, ptR, P);
+    duL = P.duL;
+    duR = P.duR;
+    scrh = P.scrh;
+    SM = P.Sc;
     pts = duR * uR[C::RHO] * ptL - duL * uL[C::RHO] * ptR +
           vL[C::RHO] * vR[C::RHO] * duR * duL * (vR[Xn] - vL[Xn]);
     pts *= scrh;
-    usL[C::RHO] = uL[C::RHO] * duL / (sl - SM);
-    usR[C::RHO] = uR[C::RHO] * duR / (sr - SM);
-    sqrL = sqrt_(usL[C::RHO]);
-    sqrR = sqrt_(usR[C::RHO]);
+    usL[C::RHO] = P.usLRHO;
+    usR[C::RHO] = P.usRRHO;
+    sqrL = P.sqrL;
+    sqrR = P.sqrR;
     S1L = SM - fab(Bx) / sqrL;
     S1R = SM + fab(Bx) / sqrR;
     if ((S1L - sl) < T(1.e-4) * (SM - sl)) revert_to_hllc = true;
@@ -710,7 +737,7 @@ __device__ __forceinline__ void hlld_mhd(const T *vL, const T *vR, T *flux, T &c
     }
   }
 
-  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, ptL, ptR, E, emf);
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, ptL, ptR, E, emf, pre);
 }
 
 // ------------------------------------------------------------------------------------
@@ -950,7 +977,9 @@ __device__ __forceinline__ void roe_mhd(const T *vL, const T *vR, T *flux, T &cm
     for (int n2 = 0; n2 < NV; n2++) sc += alambda[n2] * eta[n2] * Rc[n1][n2];
     flux[n1] = T(0.5) * (fluxL[n1] + fluxR[n1] - sc);
   }
-  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, pL, pR, E, emf);
+  StarPre<T> pre;
+  pre.valid = false;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, pL, pR, E, emf, pre);
 }
 
 // Dispatch to the configured Riemann solver.  Returns false only in the 1-D Roe case
