@@ -28,7 +28,7 @@ EXPORTS = [
     "idfx_reconstruct_vc_field", "idfx_reset_stage", "idfx_calc_flux", "idfx_calc_rhs",
     "idfx_sweep", "idfx_corner_emf", "idfx_evolve_mag_field", "idfx_evolve_stage",
     "idfx_compute_timestep", "idfx_add_and_store", "idfx_check_divb", "idfx_stage",
-    "idfx_cycle",
+    "idfx_cycle", "idfx_ctx_set_fusion",
 ]
 
 
@@ -81,6 +81,7 @@ def load():
         "idfx_check_divb": [vp, vp, dp, vp],
         "idfx_stage": [vp, i, i, d, d, vp, vp, vp, vp, i, i, dp, vp],
         "idfx_cycle": [vp, i, d, d, ctypes.c_long, dp, dp, vp, vp, vp, vp, vp],
+        "idfx_ctx_set_fusion": [vp, i],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -69,6 +69,12 @@ struct idfx_ctx {
   bool fp32 = false;
   bool uniform = false;  // every direction has one cell width
   double dx0[3] = {1.0, 1.0, 1.0};
+  // fused 2-D MHD stage: the stage reads (Vc, Vs) and writes the new state into the other
+  // buffer of a ping-pong pair, so idfx_cycle alternates between the caller's arrays and
+  // these
+  bool fused2d_ok = false;   // configuration supports it
+  bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
+  double *Vc2 = nullptr, *Vs2 = nullptr;
   const void *emf() const { return fp32 ? (const void *)&e32 : (const void *)&e64; }
 };
 
@@ -283,6 +289,18 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   if (st == IDFX_OK && needs_unfused(c)) st = alloc_n(c, &c->Flux, c->n3 * c->nvar);
   if (st == IDFX_OK) st = alloc_n(c, &c->dtmax, 1);
   if (st == IDFX_OK) st = alloc_n(c, &c->divb, 1);
+  if (st == IDFX_OK && d.mhd && d.dim == 2 && ops->fused2d_y && !needs_unfused(c)) {
+    bool user = false;
+    for (int q = 0; q < 2; q++)
+      user = user || d.lbound[q] == IDFX_BC_USERDEF || d.rbound[q] == IDFX_BC_USERDEF;
+    // user-defined ghost zones are written by the caller into its own arrays, which the
+    // ping-pong pair would bypass
+    if (!user) {
+      st = alloc_n(c, &c->Vc2, c->n3 * c->nvar);
+      if (st == IDFX_OK) st = alloc_n(c, &c->Vs2, c->nvs);
+      c->fused2d_ok = c->fused2d_on = st == IDFX_OK;
+    }
+  }
   if (st == IDFX_OK) {
     void *h = nullptr;
     if (hipHostMalloc(&h, 64, hipHostMallocDefault) != hipSuccess) st = IDFX_E_ALLOC;
@@ -534,6 +552,50 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
   return IDFX_OK;
 }
 
+// One stage of a 2-D MHD run with the fused second half: boundaries on the input pair, the
+// x sweep, then k_fused2d_y (y sweep, corner EMF, face-B update, AddAndStore, ConsToPrim),
+// which writes the interior of the output pair.  Same arithmetic as stage_enqueue.
+static int stage_fused2d(idfx_ctx *c, int stage, int nstages, double dt, double *Vc_in,
+                         double *Vs_in, double *Vc_out, double *Vs_out, double *Uc,
+                         double *InvDt, int write_uc, bool want_dt, hipStream_t s) {
+  static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
+  static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
+  IDFX_TRY(idfx_set_boundaries(c, Vc_in, Vs_in, s));
+  if (want_dt) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, c->dtmax);
+  int flags = SW_INIT_UC;
+  if (stage == 0 && nstages > 1) flags |= SW_SAVE_UC0;
+  if (stage == 0) flags |= SW_INVDT_SET;
+  SweepArgs ax = sweep_args(c, 0, dt, Vc_in, Vs_in, Uc, InvDt, flags);
+  IDFX_TRY(check(c->ops->sweep(0, true, c->g, ax, c->emf(), c->target_waves, s)));
+  flags = 0;
+  if (stage == 0) flags |= SW_INVDT_ADD | (want_dt ? SW_DT_REDUCE : 0);
+  SweepArgs ay = sweep_args(c, 1, dt, Vc_in, Vs_in, Uc, InvDt, flags);
+  Fused2dArgs f{};
+  f.Vs_in = Vs_in;
+  f.Vc_out = Vc_out;
+  f.Vs_out = Vs_out;
+  f.Vs0 = c->Vs0;
+  f.rk = nstages > 1 ? (stage == 0 ? 1 : 2) : 0;
+  f.wc = stage > 0 ? WC[nstages - 2][stage - 1] : 0.0;
+  f.w0 = stage > 0 ? W0[nstages - 2][stage - 1] : 0.0;
+  f.write_uc = write_uc;
+  f.ext0 = ay.ext[0];
+  IDFX_TRY(check(c->ops->fused2d_y(c->g, ay, c->emf(), f, c->target_waves, s)));
+  if (want_dt) {
+    IDFX_TRY(check(hipMemcpyAsync(c->host, c->dtmax, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s)));
+    IDFX_TRY(check(hipEventRecord(c->ev_dt, s)));
+  }
+  return IDFX_OK;
+}
+
+int idfx_ctx_set_fusion(idfx_ctx *c, int level) {
+  if (!c || level < 0 || level > 1) return IDFX_E_ARGS;
+  if (level == 1 && !c->fused2d_ok) return IDFX_E_ARGS;
+  c->fused2d_on = level == 1;
+  return IDFX_OK;
+}
+
 int idfx_stage(idfx_ctx *c, int stage, int nstages, double dt, double cfl, double *Vc, double *Vs,
                double *Uc, double *InvDt, int skip_boundaries, int write_uc, double *newdt_host,
                void *stream) {
@@ -556,9 +618,20 @@ int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long nc
   if (!c || !t || !dt || nstages < 1 || nstages > 3) return IDFX_E_ARGS;
   hipStream_t s = (hipStream_t)stream;
   const double t0 = *t;
+  // fused 2-D stages ping-pong between (Vc, Vs) and the context's pair; an odd stage count
+  // runs its first stage in place so that the last stage writes the caller's arrays
+  const bool fused = c->fused2d_on && nstages >= 2 && Vs;
+  const int first_fused = (nstages % 2 == 1) ? 1 : 0;
   for (int stage = 0; stage < nstages; stage++) {
-    IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0, stage == nstages - 1,
-                           stage == 0, s));
+    if (!fused || stage < first_fused) {
+      IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0,
+                             stage == nstages - 1, stage == 0, s));
+    } else {
+      const bool fwd = (stage - first_fused) % 2 == 0;
+      IDFX_TRY(stage_fused2d(c, stage, nstages, *dt, fwd ? Vc : c->Vc2, fwd ? Vs : c->Vs2,
+                             fwd ? c->Vc2 : Vc, fwd ? c->Vs2 : Vs, Uc, InvDt,
+                             stage == nstages - 1, stage == 0, s));
+    }
   }
   // the dt readback of stage 0 completes while the later stages still run
   IDFX_TRY(check(hipEventSynchronize(c->ev_dt)));

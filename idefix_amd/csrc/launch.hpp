@@ -6,6 +6,7 @@
 
 #include <algorithm>
 
+#include "fused2d.hpp"
 #include "kernels.hpp"
 
 namespace idfx {
@@ -36,6 +37,9 @@ struct Ops {
                     double *Uc, double *InvDt, double dt, hipStream_t s) = nullptr;
   hipError_t (*divb)(const GridDesc &g, const double *Vs, unsigned long long *out,
                      hipStream_t s) = nullptr;
+  // 2-D MHD only: y sweep + corner EMF + face-B update + ConsToPrim in one kernel
+  hipError_t (*fused2d_y)(const GridDesc &g, const SweepArgs &a, const void *emf,
+                          Fused2dArgs f, int target_waves, hipStream_t s) = nullptr;
 };
 
 inline unsigned blocks_for(long long n, int bs = 256) {
@@ -234,6 +238,43 @@ hipError_t launch_divb(const GridDesc &g, const double *Vs, unsigned long long *
   return hipSuccess;
 }
 
+// Lanes of halo a corner EMF at column x draws y-face data from: columns x-L .. x+R.
+inline void fused2d_halo(int order, int emf, int &L, int &R) {
+  if (emf == EMF_UCT_HLL || emf == EMF_UCT_HLLD) {
+    L = order == RECON_PPM ? 3 : 2;
+    R = order == RECON_PPM ? 2 : 1;
+  } else {
+    L = 1;
+    R = 0;
+  }
+}
+
+template <class C, typename T>
+hipError_t launch_fused2d_y(const GridDesc &g, const SweepArgs &a, const void *emf,
+                            Fused2dArgs f, int target_waves, hipStream_t s) {
+  if constexpr (C::MHD && C::DIM == 2) {
+    int L, R;
+    fused2d_halo(C::ORDER, a.emf, L, R);
+    // output columns hl .. hl+w-1 of each wave; corners at hl .. hl+w; y faces at
+    // hl-L .. hl+w+R, all inside the 64 lanes
+    f.hl = L;
+    f.w = 63 - L - R;
+    MarchGeom m{};
+    const int ncols = g.end[0] + 1 - g.beg[0];  // face columns beg .. end
+    m.nxb = (ncols + f.w - 1) / f.w;
+    const int nd = g.end[1] - g.beg[1];
+    int ts = 64;
+    while (ts > 8 && (long long)m.nxb * ((nd + ts - 1) / ts) < target_waves) ts /= 2;
+    m.ts = std::min(ts, nd);
+    m.nseg = (nd + m.ts - 1) / m.ts;
+    const long long waves = (long long)m.nxb * m.nseg;
+    hipLaunchKernelGGL((k_fused2d_y<C, T>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g, a,
+                       *static_cast<const EmfArrays<T> *>(emf), f, m);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
 template <class C, typename T>
 Ops make_ops() {
   Ops o;
@@ -251,6 +292,7 @@ Ops make_ops() {
   o.correct_flux = &launch_correct_flux<C>;
   o.rhs = &launch_rhs<C>;
   o.divb = &launch_divb<C>;
+  if constexpr (C::MHD && C::DIM == 2) o.fused2d_y = &launch_fused2d_y<C, T>;
   return o;
 }
 

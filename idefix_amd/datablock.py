@@ -206,6 +206,11 @@ class DataBlock:
                    "stage")
         return newdt.value
 
+    def set_fusion(self, level):
+        """Kernel schedule of cycle(): 0 = one kernel per reference step, 1 = fused 2-D MHD
+        second half (default where supported).  Results are identical."""
+        capi.check(self._L.idfx_ctx_set_fusion(self.ctx, int(level)), "set_fusion")
+
     def cycle(self):
         """TimeIntegrator::Cycle on the fused path (timeIntegrator.cpp:246-414)."""
         t = ctypes.c_double(self.t)

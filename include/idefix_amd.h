@@ -156,10 +156,22 @@ int idfx_stage(idfx_ctx *ctx, int stage, int nstages, double dt, double cfl, dou
 
 /* TimeIntegrator::Cycle (timeIntegrator.cpp:246-414) on the fused path: all stages and
  * the dt update (CFL_max_var, first-cycle check, dt < 1e-15).  *t and *dt are updated;
- * ncycles is the index of this cycle. */
+ * ncycles is the index of this cycle.  On return the interior cells and interior faces of
+ * Vc/Vs hold the new state; with the fused 2-D MHD schedule (idfx_ctx_set_fusion) the
+ * ghost zones hold the boundary values of an earlier stage, which the next stage's
+ * boundary fill rewrites before any use, and the corner/y-face EMF scratch arrays are not
+ * written. */
 int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long ncycles,
                double *t, double *dt, double *Vc, double *Vs, double *Uc, double *InvDt,
                void *stream);
+
+/* No counterpart in the reference: selects the kernel schedule idfx_cycle uses.
+ * level 0 runs each reference step as its own kernel (sweeps, CalcCornerEMF,
+ * EvolveMagField, ConvertConsToPrim); level 1 (the default where supported: 2-D MHD
+ * without user-defined boundaries) fuses the y sweep, corner EMF, face-field update,
+ * AddAndStore and ConvertConsToPrim into one kernel.  Both give identical results.
+ * Returns IDFX_E_ARGS for level 1 where the configuration does not support it. */
+int idfx_ctx_set_fusion(idfx_ctx *ctx, int level);
 
 #ifdef __cplusplus
 }

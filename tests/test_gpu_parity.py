@@ -179,3 +179,58 @@ def test_fp32_stencil_close_to_fp64():
         a = o.interior(Vc_g)[n]
         b = o.interior(o.Vc)[n]
         assert np.max(np.abs(a - b)) <= 1e-4 * max(1.0, np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact", "uct0"])
+@pytest.mark.parametrize("recon", ["plm", "ppm"])
+def test_orszag_tang_per_kernel_schedule(emf, recon):
+    """idfx_cycle with one kernel per reference step (fusion level 0)."""
+    kw = cfg_kwargs(2, (48, 40), emf=emf, recon=recon, cfl=0.6)
+    o, g = make_pair(kw, problems.orszag_tang)
+    g.set_fusion(0)
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="OT per-kernel %s %s" % (emf, recon))
+
+
+@pytest.mark.parametrize("nstages", [1, 3])
+@pytest.mark.parametrize("emf", ["uct_hlld", "arithmetic"])
+def test_mhd2d_stage_counts(nstages, emf):
+    """Odd stage counts: RK3 runs its first stage in place, then two fused stages; the
+    single-stage integrator runs per kernel."""
+    kw = cfg_kwargs(2, (40, 36), emf=emf, recon="ppm", cfl=0.4, nstages=nstages)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=13, amp=0.2))
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="2D MHD %d stages %s" % (nstages, emf))
+
+
+@pytest.mark.parametrize("emf,recon,bc", [
+    ("uct_hlld", "plm", "periodic"),
+    ("uct_hll", "ppm", ["outflow", "periodic"]),
+    ("uct_contact", "ppm", ["periodic", "reflective"]),
+])
+def test_fused_schedule_bitwise_equals_per_kernel(emf, recon, bc):
+    """At a size the oracle does not reach in seconds: the fused 2-D schedule and the
+    per-kernel schedule give bit-identical interiors and the same dt sequence.  The x
+    extent leaves a partial last column block for every halo width."""
+    import torch
+    from idefix_amd.datablock import DataBlock
+    nx = (301, 263)
+    blocks = []
+    for level in (1, 0):
+        g = DataBlock(2, nx, emf=emf, recon=recon, bc=bc, cfl=0.5)
+        g.set_fusion(level)
+        Vc, Vs = problems.orszag_tang(g)
+        g.upload(Vc, Vs)
+        blocks.append(g)
+    for _ in range(5):
+        for g in blocks:
+            g.cycle()
+        assert blocks[0].dt == blocks[1].dt
+    (a_c, a_s), (b_c, b_s) = blocks[0].download(), blocks[1].download()
+    b, e = blocks[0].beg, blocks[0].end
+    assert np.array_equal(a_c[:, :, b[1]:e[1], b[0]:e[0]], b_c[:, :, b[1]:e[1], b[0]:e[0]])
+    assert np.array_equal(a_s[0][:, b[1]:e[1], b[0]:e[0] + 1], b_s[0][:, b[1]:e[1], b[0]:e[0] + 1])
+    assert np.array_equal(a_s[1][:, b[1]:e[1] + 1, b[0]:e[0]], b_s[1][:, b[1]:e[1] + 1, b[0]:e[0]])
+    torch.cuda.synchronize()
