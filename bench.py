@@ -77,56 +77,38 @@ def time_cycles(g, steps, warmup):
     return time.perf_counter() - t0
 
 
-def kernel_times(g, reps):
-    """Average duration of each kernel of a stage, HIP events on the launch stream."""
-    import torch
-    from idefix_amd import capi
-    s = torch.cuda.current_stream()
-    dt = g.dt
-    flux_flags = capi.SWEEP_INVDT_ADD
-
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(s)
-        for _ in range(reps):
-            fn()
-        b.record(s)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps
-
-    g.set_boundaries()
-    g.prim_to_cons()
-    out = {
-        "boundaries": timed(g.set_boundaries),
-        "sweep_x": timed(lambda: g.sweep(0, dt, flux_flags)),
-        "sweep_y": timed(lambda: g.sweep(1, dt, flux_flags)),
-        "corner_emf": timed(g.calc_corner_emf),
-        "evolve_b": timed(lambda: g.evolve_mag_field(0.0)),
-        "cons_to_prim": timed(g.cons_to_prim),
-    }
-    return out  # ms per launch
+def kernel_times(g, cycles):
+    """Average duration per launch of each region of the cycle, from the library's region
+    profiler: HIP event pairs recorded on the stream the kernels run on."""
+    g.set_profiling(True)
+    for _ in range(cycles):
+        g.cycle()
+    rt = g.region_times()
+    g.set_profiling(False)
+    return {k: ms / n for k, (ms, n) in rt.items()}, {k: ms / cycles for k, (ms, n) in rt.items()}
 
 
 def algorithmic_bytes(nx):
-    """Compulsory HBM bytes per launch of each stage kernel (each input read once, each
-    output written once; 2-D MHD, nvar 6, UCT-HLLD face data 5 words per face)."""
+    """Compulsory HBM bytes per launch of the single-kernel regions of a 2-D MHD RK2 stage
+    (each input read once, each output written once, 8-byte words per interior cell),
+    averaged over the two stages of the cycle.
+      sweep_x   stage 0: Vc 6 + BX1s 1 -> Uc 4 + Uc0 4 + InvDt 1 + face data 5 = 21
+                stage 1: Vc 6 + BX1s 1 -> Uc 4 + face data 5                  = 16
+      fused2d_y stage 0: Vc 6 + Vs 2 + Uc 4 + x-face data 5 + InvDt 1
+                         -> Vc 6 + Vs 2 + Vs0 2 + InvDt 1                    = 29
+                stage 1: Vc 6 + Vs 2 + Uc 4 + Uc0 4 + Vs0 2 + x-face data 5
+                         -> Vc 6 + Vs 2 + Uc 6                                = 37
+    """
     n = nx * nx
     w = 8
-    return {
-        # Vc (6) + normal Vs (1) + Uc r/w (4+4) + face EMF data (5) + InvDt r/w (2)
-        "sweep_x": n * w * (6 + 1 + 8 + 5 + 2),
-        "sweep_y": n * w * (6 + 1 + 8 + 5 + 2),
-        # face data 2x5 + Vs 2 -> corner EMF 1
-        "corner_emf": n * w * (10 + 2 + 1),
-        # corner EMF 1 + Vs r/w 2x2
-        "evolve_b": n * w * (1 + 4),
-        # Uc (6 incl. B) + Vs 2 -> Vc 6 + Uc 6
-        "cons_to_prim": n * w * (4 + 2 + 6 + 6),
-        "boundaries": 0,
-    }
+    return {"sweep_x": n * w * (21 + 16) / 2, "fused2d_y": n * w * (29 + 37) / 2}
+
+
+# HBM traffic per launch of each kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE on this workload and scaled by the plain-copy calibration (reads x2.0,
+# writes x1.0); the summary and the commands are in
+# profiles/ot2048_hlld_uct_hlld_fp64_kernels.md.  None where not measured.
+MEASURED_TRAFFIC = {2048: {}}
 
 
 def cpu_baseline(nx, seconds):
@@ -175,11 +157,12 @@ def main():
     el = time_cycles(g, args.steps, args.warmup)
     zu = nx * nx * args.steps / el
     ms = 1e3 * el / args.steps
-    kt = kernel_times(g, args.kernel_reps)
+    kt, per_cycle = kernel_times(g, args.kernel_reps)
     ab = algorithmic_bytes(nx)
-    # per-cycle share: two stages, each with one launch of every kernel
-    dom = max((k for k in kt if ab[k] > 0), key=lambda k: kt[k])
+    # the single-kernel region with the largest share of the cycle
+    dom = max((k for k in per_cycle if k in ab), key=lambda k: per_cycle[k])
     achieved = ab[dom] / (kt[dom] * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(nx, {}).get(dom)
     del g
     torch.cuda.empty_cache()
     fp32 = None
@@ -210,10 +193,12 @@ def main():
                    "parallelism": "1 GPU"},
         "fp32_stencil_value": fp32,
         "kernel_ms": kt,
+        "kernel_ms_per_cycle": per_cycle,
         "cycle_hbm_frac": cycle_bytes * zu / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "algorithmic_bytes": ab[dom], "ms_per_launch": kt[dom],
+                     "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -28,8 +28,11 @@ EXPORTS = [
     "idfx_reconstruct_vc_field", "idfx_reset_stage", "idfx_calc_flux", "idfx_calc_rhs",
     "idfx_sweep", "idfx_corner_emf", "idfx_evolve_mag_field", "idfx_evolve_stage",
     "idfx_compute_timestep", "idfx_add_and_store", "idfx_check_divb", "idfx_stage",
-    "idfx_cycle", "idfx_ctx_set_fusion",
+    "idfx_cycle", "idfx_ctx_set_fusion", "idfx_ctx_set_profiling", "idfx_ctx_region_times",
 ]
+
+REGIONS = ["boundaries", "sweep_x", "sweep_y", "sweep_z", "corner_emf", "evolve_b",
+           "cons_to_prim", "fused2d_y"]
 
 
 class IdfxDesc(ctypes.Structure):
@@ -82,6 +85,8 @@ def load():
         "idfx_stage": [vp, i, i, d, d, vp, vp, vp, vp, i, i, dp, vp],
         "idfx_cycle": [vp, i, d, d, ctypes.c_long, dp, dp, vp, vp, vp, vp, vp],
         "idfx_ctx_set_fusion": [vp, i],
+        "idfx_ctx_set_profiling": [vp, i],
+        "idfx_ctx_region_times": [vp, dp, ctypes.POINTER(ctypes.c_long), i],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

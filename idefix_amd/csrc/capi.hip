@@ -72,6 +72,13 @@ struct idfx_ctx {
   // fused 2-D MHD stage: the stage reads (Vc, Vs) and writes the new state into the other
   // buffer of a ping-pong pair, so idfx_cycle alternates between the caller's arrays and
   // these
+  // region timer (idfx_ctx_set_profiling): HIP event pairs recorded on the launch stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  std::vector<int> ev_region;
+  size_t ev_used = 0;
+  double region_ms[IDFX_NREGIONS] = {};
+  long region_n[IDFX_NREGIONS] = {};
   bool fused2d_ok = false;   // configuration supports it
   bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
   double *Vc2 = nullptr, *Vs2 = nullptr;
@@ -209,6 +216,52 @@ double bits_to_double(unsigned long long b) {
   return d;
 }
 
+// Fold every recorded event pair into the per-region totals (blocks until they complete).
+int resolve_regions(idfx_ctx *c) {
+  for (size_t q = 0; q < c->ev_used; q++) {
+    IDFX_TRY(check(hipEventSynchronize(c->ev_stop[q])));
+    float ms = 0.0f;
+    IDFX_TRY(check(hipEventElapsedTime(&ms, c->ev_start[q], c->ev_stop[q])));
+    c->region_ms[c->ev_region[q]] += ms;
+    c->region_n[c->ev_region[q]] += 1;
+  }
+  c->ev_used = 0;
+  return IDFX_OK;
+}
+
+// Times the launches issued in its scope on stream s as one region (pushRegion/popRegion
+// with the profiler enabled, global.cpp:66-95, without the fences: the events sit in the
+// stream).
+struct Region {
+  idfx_ctx *c;
+  hipStream_t s;
+  long slot = -1;
+  Region(idfx_ctx *ctx, int id, hipStream_t st) : c(ctx), s(st) {
+    if (!c->prof) return;
+    if (c->ev_used == c->ev_start.size()) {
+      if (c->ev_start.size() >= 512) {
+        if (resolve_regions(c) != IDFX_OK) return;
+      } else {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return;
+        if (hipEventCreate(&b) != hipSuccess) {
+          hipEventDestroy(a);
+          return;
+        }
+        c->ev_start.push_back(a);
+        c->ev_stop.push_back(b);
+        c->ev_region.push_back(0);
+      }
+    }
+    slot = (long)c->ev_used++;
+    c->ev_region[slot] = id;
+    hipEventRecord(c->ev_start[slot], s);
+  }
+  ~Region() {
+    if (slot >= 0) hipEventRecord(c->ev_stop[slot], s);
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -261,6 +314,11 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   }
   c->nvar = ops->nvar;
   c->n3 = (long long)c->np_tot[0] * c->np_tot[1] * c->np_tot[2];
+  // kernels address each variable (and each Vs component) with 32-bit byte offsets
+  if ((long long)c->vsn[0] * c->vsn[1] * c->vsn[2] >= (1LL << 29)) {
+    delete c;
+    return IDFX_E_ARGS;
+  }
   c->nvs = d.mhd ? (long long)d.dim * c->vsn[0] * c->vsn[1] * c->vsn[2] : 0;
   c->g.sv = c->n3;
   c->g.svs = (long long)c->vsn[0] * c->vsn[1] * c->vsn[2];
@@ -330,6 +388,8 @@ int idfx_ctx_destroy(idfx_ctx *c) {
   for (void *p : c->allocs) hipFree(p);
   if (c->host) hipHostFree(c->host);
   if (c->ev_dt) hipEventDestroy(c->ev_dt);
+  for (hipEvent_t e : c->ev_start) hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_stop) hipEventDestroy(e);
   delete c;
   return IDFX_OK;
 }
@@ -495,7 +555,10 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
                          bool want_dt, hipStream_t s) {
   static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
   static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
-  if (!skip_boundaries) IDFX_TRY(idfx_set_boundaries(c, Vc, Vs, s));
+  if (!skip_boundaries) {
+    Region r(c, IDFX_REGION_BOUNDARIES, s);
+    IDFX_TRY(idfx_set_boundaries(c, Vc, Vs, s));
+  }
   double wc = 0.0, w0 = 0.0;
   if (stage > 0) {
     wc = WC[nstages - 2][stage - 1];
@@ -513,6 +576,7 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
       flags |= (dir == 0) ? SW_INVDT_SET : SW_INVDT_ADD;
       if (dir == dim - 1 && want_dt) flags |= SW_DT_REDUCE;
     }
+    Region r(c, IDFX_REGION_SWEEP_X + dir, s);
     if (needs_unfused(c)) {
       // reference sequence: PrimToCons, (begin copy), CalcFlux, CalcRightHandSide
       if (dir == 0) {
@@ -534,8 +598,12 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
     }
   }
   if (c->d.mhd && dim >= 2) {
-    IDFX_TRY(check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, s)));
+    {
+      Region r(c, IDFX_REGION_CORNER_EMF, s);
+      IDFX_TRY(check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, s)));
+    }
     const int mode = nstages > 1 ? (stage == 0 ? 1 : 2) : 0;
+    Region r(c, IDFX_REGION_EVOLVE_B, s);
     IDFX_TRY(check(c->ops->evolve_b(c->g, Vs, c->Vs0, c->emf(), dt, mode, wc, w0, s)));
   } else if (c->d.mhd && stage > 0) {
     // 1-D MHD keeps Vs constant; AddAndStore still applies (a no-op on equal values)
@@ -548,6 +616,7 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
   const int lo[3] = {c->g.beg[0], c->g.beg[1], c->g.beg[2]};
   const int hi[3] = {c->g.end[0], c->g.end[1], c->g.end[2]};
   const int flags = (stage > 0 ? 1 : 0) | (write_uc ? 2 : 0);
+  Region r(c, IDFX_REGION_CONS_TO_PRIM, s);
   IDFX_TRY(check(c->ops->cons_to_prim(c->g, Uc, c->Uc0, Vc, Vs, lo, hi, flags, wc, w0, s)));
   return IDFX_OK;
 }
@@ -560,13 +629,19 @@ static int stage_fused2d(idfx_ctx *c, int stage, int nstages, double dt, double 
                          double *InvDt, int write_uc, bool want_dt, hipStream_t s) {
   static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
   static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
-  IDFX_TRY(idfx_set_boundaries(c, Vc_in, Vs_in, s));
+  {
+    Region r(c, IDFX_REGION_BOUNDARIES, s);
+    IDFX_TRY(idfx_set_boundaries(c, Vc_in, Vs_in, s));
+  }
   if (want_dt) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, c->dtmax);
   int flags = SW_INIT_UC;
   if (stage == 0 && nstages > 1) flags |= SW_SAVE_UC0;
   if (stage == 0) flags |= SW_INVDT_SET;
   SweepArgs ax = sweep_args(c, 0, dt, Vc_in, Vs_in, Uc, InvDt, flags);
-  IDFX_TRY(check(c->ops->sweep(0, true, c->g, ax, c->emf(), c->target_waves, s)));
+  {
+    Region r(c, IDFX_REGION_SWEEP_X, s);
+    IDFX_TRY(check(c->ops->sweep(0, true, c->g, ax, c->emf(), c->target_waves, s)));
+  }
   flags = 0;
   if (stage == 0) flags |= SW_INVDT_ADD | (want_dt ? SW_DT_REDUCE : 0);
   SweepArgs ay = sweep_args(c, 1, dt, Vc_in, Vs_in, Uc, InvDt, flags);
@@ -580,11 +655,35 @@ static int stage_fused2d(idfx_ctx *c, int stage, int nstages, double dt, double 
   f.w0 = stage > 0 ? W0[nstages - 2][stage - 1] : 0.0;
   f.write_uc = write_uc;
   f.ext0 = ay.ext[0];
-  IDFX_TRY(check(c->ops->fused2d_y(c->g, ay, c->emf(), f, c->target_waves, s)));
+  {
+    Region r(c, IDFX_REGION_FUSED2D_Y, s);
+    IDFX_TRY(check(c->ops->fused2d_y(c->g, ay, c->emf(), f, c->target_waves, s)));
+  }
   if (want_dt) {
     IDFX_TRY(check(hipMemcpyAsync(c->host, c->dtmax, sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s)));
     IDFX_TRY(check(hipEventRecord(c->ev_dt, s)));
+  }
+  return IDFX_OK;
+}
+
+int idfx_ctx_set_profiling(idfx_ctx *c, int on) {
+  if (!c) return IDFX_E_ARGS;
+  if (!on) IDFX_TRY(resolve_regions(c));
+  c->prof = on != 0;
+  return IDFX_OK;
+}
+
+int idfx_ctx_region_times(idfx_ctx *c, double *ms, long *count, int n) {
+  if (!c || n < 0 || n > IDFX_NREGIONS) return IDFX_E_ARGS;
+  IDFX_TRY(resolve_regions(c));
+  for (int q = 0; q < n; q++) {
+    if (ms) ms[q] = c->region_ms[q];
+    if (count) count[q] = c->region_n[q];
+  }
+  for (int q = 0; q < IDFX_NREGIONS; q++) {
+    c->region_ms[q] = 0.0;
+    c->region_n[q] = 0;
   }
   return IDFX_OK;
 }

@@ -39,8 +39,8 @@ __device__ __forceinline__ T lane_at(T v, int lane, int off) {
   return __shfl(v, lane + off);
 }
 
-template <class C, typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+template <class C, typename T, int EMF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGeom m) {
   static_assert(C::DIM == 2 && C::MHD, "2-D MHD only");
   constexpr int NV = C::NVAR;
@@ -61,50 +61,56 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
     const bool yface_col = c >= g.beg[0] - s.ext0 && c < g.end[0] + s.ext0;
     const int c0 = g.beg[1] + seg * m.ts;
     const int c1 = min(c0 + m.ts, g.end[1]);
-    const long long sd = g.nt[0];
-    const long long col = cc;                 // idx3(g, 0, 0, cc)
-    const long long vsd = g.vsn[0];
-    const long long vs1 = g.svs;              // offset of the BX2s component
+    // 32-bit element offsets from uniform bases (arrays < 2^29 elements)
+    const unsigned sd = g.nt[0];
+    const unsigned col = cc;                  // idx3(g, 0, 0, cc)
+    const unsigned vsd = g.vsn[0];
+    const double *Vs1_in = s.Vs_in + g.svs;   // BX2s component
+    double *Vs1_out = s.Vs_out + g.svs;
+    double *Vs1_0 = s.Vs0 + g.svs;
     const double dtdx1 = a.dt / g.dx[0][cc];  // evolveMagField.hpp:67 (dt/dx1(i))
-    const int emf = a.emf;
+    constexpr int emf = EMF;  // the corner-EMF scheme is a compile-time choice here
 
-    T win[NW][NV];
-#pragma unroll
-    for (int q = 0; q < NW; q++)
-#pragma unroll
-      for (int n = 0; n < NV; n++)
-        win[q][n] = (T)a.Vc[n * g.sv + col + (long long)(c0 - 1 - H + q) * sd];
     T carry[NV], Fprev[NV];
     T cprev = 0, ezprev = 0;
     double b2prev = 0.0;
+    // right-face value of cell c0-1: the left state of the first face
 #pragma unroll
     for (int n = 0; n < NV; n++) {
+      Fprev[n] = 0;
+      carry[n] = 0;
+      if (n == BX2) continue;  // the normal field comes from Vs
       T wv[NW];
 #pragma unroll
-      for (int q = 0; q < NW; q++) wv[q] = win[q][n];
+      for (int q = 0; q < NW; q++)
+        wv[q] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(c0 - 1 - H + q) * sd);
       T dummy;
       recon_cell<C>(wv, n, dummy, carry[n]);
-      Fprev[n] = 0;
     }
     for (int f = c0; f <= c1; f++) {
       // ---- 1. y face f -------------------------------------------------------------
+      // stencil rows f-H .. f+H reloaded each step (cache hits) instead of carried in
+      // registers: the register budget goes to occupancy
+      T win[NW][NV];
 #pragma unroll
-      for (int q = 0; q < NW - 1; q++)
+      for (int q = 0; q < NW; q++)
 #pragma unroll
-        for (int n = 0; n < NV; n++) win[q][n] = win[q + 1][n];
-#pragma unroll
-      for (int n = 0; n < NV; n++)
-        win[NW - 1][n] = (T)a.Vc[n * g.sv + col + (long long)(f + H) * sd];
+        for (int n = 0; n < NV; n++)
+          win[q][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f - H + q) * sd);
       T vL[NV], vR[NV], vr_next[NV];
 #pragma unroll
       for (int n = 0; n < NV; n++) {
+        vL[n] = carry[n];
+        if (n == BX2) {
+          vR[n] = vr_next[n] = 0;
+          continue;
+        }
         T wv[NW];
 #pragma unroll
         for (int q = 0; q < NW; q++) wv[q] = win[q][n];
         recon_cell<C>(wv, n, vR[n], vr_next[n]);
-        vL[n] = carry[n];
       }
-      const double b2old = s.Vs_in[vs1 + (long long)f * vsd + cc];
+      const double b2old = ld32(Vs1_in, (unsigned)f * vsd + col);
       vL[BX2] = (T)b2old;
       vR[BX2] = vL[BX2];
       T F[NV], cmax;
@@ -118,8 +124,8 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
 
       // ---- 2. corner EMF ez(c, f) ----------------------------------------------------
       T ez;
-      auto xf = [&](const T *arr, int row) { return arr[(long long)row * sd + cc]; };
-      if (emf == EMF_UCT_HLL || emf == EMF_UCT_HLLD) {
+      auto xf = [&](const T *arr, int row) { return ld32(arr, (unsigned)row * sd + col); };
+      if constexpr (emf == EMF_UCT_HLL || emf == EMF_UCT_HLLD) {
         constexpr int SLO = C::ORDER == RECON_PPM ? 1 : 2;
         constexpr int SHI = C::ORDER == RECON_PPM ? 6 : 5;
         T b[7], v[7];
@@ -138,7 +144,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         // y faces: B = BX1s of x face c and v = ezi along y (rows f-4+s)
 #pragma unroll
         for (int q = SLO; q <= SHI; q++) {
-          b[q] = (T)s.Vs_in[(long long)(f - 4 + q) * vsd + cc];
+          b[q] = (T)ld32(s.Vs_in, (unsigned)(f - 4 + q) * vsd + col);
           v[q] = xf(e.ezi, f - 4 + q);
         }
         aL = T(0.5) * (lane_at(E.aL, lane, -1) + E.aL);
@@ -151,7 +157,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         const T ezj_m = lane_at(E.Et, lane, -1);
         const T ezi_f = xf(e.ezi, f);
         const T ezi_m = xf(e.ezi, f - 1);
-        if (emf == EMF_ARITHMETIC) {
+        if constexpr (emf == EMF_ARITHMETIC) {
           ez = T(0.25) * (ezi_f + ezi_m + ezj_c + ezj_m);
         } else {
           // cell-centred E3 = vx2 Bx1 - vx1 Bx2 of cells (c, f-1), (c, f) and column c-1
@@ -162,7 +168,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
           const T E3_c = e3(H);       // cell (c, f)
           const T E3_mm = lane_at(E3_cm, lane, -1);
           const T E3_m = lane_at(E3_c, lane, -1);
-          if (emf == EMF_UCT0) {
+          if constexpr (emf == EMF_UCT0) {
             const T a1 = T(2) * ezi_f - T(0.5) * (E3_m + E3_c);
             const T a2 = T(2) * ezi_m - T(0.5) * (E3_mm + E3_cm);
             const T a3 = T(2) * ezj_c - T(0.5) * (E3_cm + E3_c);
@@ -183,26 +189,26 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       // ---- 3. face fields ------------------------------------------------------------
       const T ezp = lane_at(ez, lane, 1);
       double b2new = b2old + dtdx1 * ((double)ezp - (double)ez);
-      if (s.rk == 2) b2new = s.wc * b2new + s.w0 * s.Vs0[vs1 + (long long)f * vsd + cc];
+      if (s.rk == 2) b2new = s.wc * b2new + s.w0 * ld32((const double *)Vs1_0, (unsigned)f * vsd + col);
       if (cell_col && (f < c1 || c1 == g.end[1])) {
-        const long long q = vs1 + (long long)f * vsd + c;
-        if (s.rk == 1) s.Vs0[q] = b2old;
-        s.Vs_out[q] = b2new;
+        const unsigned q = (unsigned)f * vsd + col;
+        if (s.rk == 1) st32(Vs1_0, q, b2old);
+        st32(Vs1_out, q, b2new);
       }
       if (f > c0) {
         const int fr = f - 1;
-        const long long q1 = (long long)fr * vsd + cc;
-        const double b1old = s.Vs_in[q1];
+        const unsigned q1 = (unsigned)fr * vsd + col;
+        const double b1old = ld32(s.Vs_in, q1);
         double b1new = b1old + (0.0 - a.dt / g.dx[1][fr] * ((double)ez - (double)ezprev));
-        if (s.rk == 2) b1new = s.wc * b1new + s.w0 * s.Vs0[q1];
+        if (s.rk == 2) b1new = s.wc * b1new + s.w0 * ld32((const double *)s.Vs0, q1);
         const double b1next = __shfl(b1new, lane + 1);
         if (out_col) {
-          if (s.rk == 1) s.Vs0[q1] = b1old;
-          s.Vs_out[q1] = b1new;
+          if (s.rk == 1) st32(s.Vs0, q1, b1old);
+          st32(s.Vs_out, q1, b1new);
         }
         // ---- 4. cell (c, f-1) ---------------------------------------------------------
         if (cell_col) {
-          const long long p = (long long)fr * sd + c;
+          const unsigned p = (unsigned)fr * sd + col;
           const double dtdV = a.uniform ? a.dtdV : a.dt / cell_dV<2>(g, 0, fr, c);
           const double A = face_A<2, JDIR>(g, 0, fr, c);
           double u[NV], v[NV];
@@ -210,28 +216,29 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
           for (int n = 0; n < NV; n++) {
             if (C::is_ct_field(n)) continue;
             const double rhs = -dtdV * ((double)F[n] * A - (double)Fprev[n] * A);
-            u[n] = a.Uc[n * g.sv + p] + rhs;
+            u[n] = ld32((const double *)a.Uc + n * g.sv, p) + rhs;
           }
           const double term = 0.5 * ((double)cmax + (double)cprev) / (g.dx[1][fr]);
           if (a.flags & SW_INVDT_ADD) {
-            const double inv = a.InvDt[p] + term;
-            a.InvDt[p] = inv;
+            const double inv = ld32((const double *)a.InvDt, p) + term;
+            st32(a.InvDt, p, inv);
             const unsigned long long bb = __double_as_longlong(inv);
             red = bb > red ? bb : red;
           }
           if (s.rk == 2) {
 #pragma unroll
             for (int n = 0; n < NV; n++)
-              if (!C::is_ct_field(n)) u[n] = s.wc * u[n] + s.w0 * a.Uc0[n * g.sv + p];
+              if (!C::is_ct_field(n))
+                u[n] = s.wc * u[n] + s.w0 * ld32((const double *)a.Uc0 + n * g.sv, p);
           }
           u[C::BX1] = 0.5 * (b1new + b1next);
           u[C::BX2] = 0.5 * (b2prev + b2new);
           cons2prim<C>(u, v, g.gamma);
 #pragma unroll
-          for (int n = 0; n < NV; n++) s.Vc_out[n * g.sv + p] = v[n];
+          for (int n = 0; n < NV; n++) st32(s.Vc_out + n * g.sv, p, v[n]);
           if (s.write_uc) {
 #pragma unroll
-            for (int n = 0; n < NV; n++) a.Uc[n * g.sv + p] = u[n];
+            for (int n = 0; n < NV; n++) st32(a.Uc + n * g.sv, p, u[n]);
           }
         }
       }

@@ -66,6 +66,20 @@ struct SweepArgs {
   double dtdV;
 };
 
+// Element i of an array whose base is uniform across the wave, addressed as base + 32-bit
+// byte offset: the compiler keeps the base in scalar registers and one 32-bit VGPR per
+// access (global_load ... saddr), instead of a 64-bit VGPR address pair.  Arrays handled
+// this way hold fewer than 2^29 elements (checked at context creation).
+template <typename T>
+__device__ __forceinline__ T ld32(const T *base, unsigned i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      (unsigned)(i * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T *base, unsigned i, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (unsigned)(i * (unsigned)sizeof(T))) = v;
+}
+
 __device__ __forceinline__ long long idx3(const GridDesc &g, int k, int j, int i) {
   return ((long long)k * g.nt[1] + j) * g.nt[0] + i;
 }
@@ -111,24 +125,24 @@ __device__ __forceinline__ T *eb_of(const EmfArrays<T> &e) {
 
 // Store the face-EMF data of one face (the K_Store* of storeFlux.hpp).
 template <class C, int DIR, typename T>
-__device__ __forceinline__ void store_face_emf(const EmfArrays<T> &e, long long p, int emf,
+__device__ __forceinline__ void store_face_emf(const EmfArrays<T> &e, unsigned p, int emf,
                                                const FaceEmf<T> &E) {
   if constexpr (C::MHD) {
     if (emf <= EMF_UCT_CONTACT) {
       if constexpr (C::COMP > 1) {
-        et_of<T, DIR>(e)[p] = E.Et;
-        if constexpr (C::DIM == 3) eb_of<T, DIR>(e)[p] = E.Eb;
+        st32(et_of<T, DIR>(e), p, E.Et);
+        if constexpr (C::DIM == 3) st32(eb_of<T, DIR>(e), p, E.Eb);
       }
-      if (emf == EMF_UCT_CONTACT) (DIR == 0 ? e.svx : DIR == 1 ? e.svy : e.svz)[p] = E.sv;
+      if (emf == EMF_UCT_CONTACT) st32(DIR == 0 ? e.svx : DIR == 1 ? e.svy : e.svz, p, E.sv);
     } else {
       if constexpr (C::COMP > 1) {
-        et_of<T, DIR>(e)[p] = E.Et;
-        if constexpr (C::COMP >= 3) eb_of<T, DIR>(e)[p] = E.Eb;
+        st32(et_of<T, DIR>(e), p, E.Et);
+        if constexpr (C::COMP >= 3) st32(eb_of<T, DIR>(e), p, E.Eb);
       }
-      (DIR == 0 ? e.axL : DIR == 1 ? e.ayL : e.azL)[p] = E.aL;
-      (DIR == 0 ? e.axR : DIR == 1 ? e.ayR : e.azR)[p] = E.aR;
-      (DIR == 0 ? e.dxL : DIR == 1 ? e.dyL : e.dzL)[p] = E.dL;
-      (DIR == 0 ? e.dxR : DIR == 1 ? e.dyR : e.dzR)[p] = E.dR;
+      st32(DIR == 0 ? e.axL : DIR == 1 ? e.ayL : e.azL, p, E.aL);
+      st32(DIR == 0 ? e.axR : DIR == 1 ? e.ayR : e.azR, p, E.aR);
+      st32(DIR == 0 ? e.dxL : DIR == 1 ? e.dyL : e.dzL, p, E.dL);
+      st32(DIR == 0 ? e.dxR : DIR == 1 ? e.dyR : e.dzR, p, E.dR);
     }
   }
 }
@@ -172,11 +186,11 @@ constexpr int window() {
 
 // Conserved state at the start of the sweep for one cell: PrimToCons(Vc) (FP64).
 template <class C>
-__device__ __forceinline__ void cell_p2c(const GridDesc &g, const double *Vc, long long p,
+__device__ __forceinline__ void cell_p2c(const GridDesc &g, const double *Vc, unsigned p,
                                          double *u) {
   double v[C::NVAR];
 #pragma unroll
-  for (int n = 0; n < C::NVAR; n++) v[n] = Vc[n * g.sv + p];
+  for (int n = 0; n < C::NVAR; n++) v[n] = ld32(Vc + n * g.sv, p);
   prim2cons<C>(v, u, g.gamma);
 }
 
@@ -204,7 +218,7 @@ template <class C, int DIR, typename T>
 __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs &a, int k,
                                               int j, int i, const T *Fm, const T *Fp,
                                               T cmaxm, T cmaxp) {
-  const long long p = idx3(g, k, j, i);
+  const unsigned p = (unsigned)idx3(g, k, j, i);
   const double dtdV = a.uniform ? a.dtdV : a.dt / cell_dV<C::DIM>(g, k, j, i);
   const double A = face_A<C::DIM, DIR>(g, k, j, i);
   double u[C::NVAR];
@@ -214,7 +228,7 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     if (a.flags & SW_SAVE_UC0) {
 #pragma unroll
       for (int n = 0; n < C::NVAR; n++)
-        if (!C::is_ct_field(n)) a.Uc0[n * g.sv + p] = u0[n];
+        if (!C::is_ct_field(n)) st32(a.Uc0 + n * g.sv, p, u0[n]);
     }
     if (a.flags & SW_INIT_UC) {
 #pragma unroll
@@ -222,12 +236,12 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     } else {
 #pragma unroll
       for (int n = 0; n < C::NVAR; n++)
-        if (!C::is_ct_field(n)) u[n] = a.Uc[n * g.sv + p];
+        if (!C::is_ct_field(n)) u[n] = ld32((const double *)a.Uc + n * g.sv, p);
     }
   } else {
 #pragma unroll
     for (int n = 0; n < C::NVAR; n++)
-      if (!C::is_ct_field(n)) u[n] = a.Uc[n * g.sv + p];
+      if (!C::is_ct_field(n)) u[n] = ld32((const double *)a.Uc + n * g.sv, p);
   }
 #pragma unroll
   for (int n = 0; n < C::NVAR; n++) {
@@ -235,7 +249,7 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     const double fp = (double)Fp[n] * A;
     const double fm = (double)Fm[n] * A;
     const double rhs = -dtdV * (fp - fm);
-    a.Uc[n * g.sv + p] = u[n] + rhs;
+    st32(a.Uc + n * g.sv, p, u[n] + rhs);
   }
   const int ig = DIR == 0 ? i : DIR == 1 ? j : k;
   const double dl = g.dx[DIR][ig];
@@ -243,10 +257,10 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
   double inv = 0.0;
   if (a.flags & SW_INVDT_SET) {
     inv = term;
-    if (!(a.flags & SW_NO_INVDT_STORE)) a.InvDt[p] = inv;
+    if (!(a.flags & SW_NO_INVDT_STORE)) st32(a.InvDt, p, inv);
   } else if (a.flags & SW_INVDT_ADD) {
-    inv = a.InvDt[p] + term;
-    a.InvDt[p] = inv;
+    inv = ld32((const double *)a.InvDt, p) + term;
+    st32(a.InvDt, p, inv);
   }
   return inv;
 }
@@ -282,7 +296,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
     const int k = g.beg[2] - a.ext[2] + r / nj;
     const int c = g.beg[0] - 1 + seg * 62 + lane;  // this lane's cell; its left face is c-1/2
     const int cc = c <= g.end[0] ? c : g.end[0];     // clamped for the loads
-    const long long row = idx3(g, k, j, 0);
+    const unsigned row = (unsigned)idx3(g, k, j, 0);
 
     // reconstruct the lane's cell: vl = right state of face c, vr = left state of face c+1
     T vl[NV], vr[NV];
@@ -290,7 +304,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
     for (int n = 0; n < NV; n++) {
       T wv[NW];
 #pragma unroll
-      for (int s = 0; s < NW; s++) wv[s] = (T)a.Vc[n * g.sv + row + cc - H + s];
+      for (int s = 0; s < NW; s++) wv[s] = (T)ld32(a.Vc + n * g.sv, row + (unsigned)(cc - H + s));
       recon_cell<C>(wv, n, vl[n], vr[n]);
     }
     T vL[NV], vR[NV];
@@ -300,7 +314,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
       vR[n] = vl[n];
     }
     if constexpr (C::MHD) {
-      vL[C::BX1] = (T)a.Vs[idxs(g, 0, k, j, cc)];
+      vL[C::BX1] = (T)ld32(a.Vs, (unsigned)idxs(g, 0, k, j, cc));
       vR[C::BX1] = vL[C::BX1];
     }
     T F[NV], cmax;
@@ -308,15 +322,15 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
     const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
     const bool face_ok = lane >= 1 && c <= g.end[0];
     const bool own_face = face_ok && (lane <= 62 || c == g.end[0]);
-    const long long p = row + c;
+    const unsigned p = row + (unsigned)c;
     if (own_face) store_face_emf<C, IDIR>(e, p, a.emf, E);
     if constexpr (!FUSED) {
       if (own_face) {
         if (wrote) {
 #pragma unroll
-          for (int n = 0; n < NV; n++) a.Flux[n * g.sv + p] = (double)F[n];
+          for (int n = 0; n < NV; n++) st32(a.Flux + n * g.sv, p, (double)F[n]);
         }
-        a.cMax[p] = (double)cmax;
+        st32(a.cMax, p, (double)cmax);
       }
     } else {
       T Fp[NV];
@@ -373,23 +387,24 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
     const int c0 = g.beg[DIR] + seg * m.ts;
     const int c1 = min(c0 + m.ts, g.end[DIR]);
     // stride along DIR and base offset of the column
-    const long long sd = (DIR == JDIR) ? g.nt[0] : (long long)g.nt[0] * g.nt[1];
+    const unsigned sd = (DIR == JDIR) ? g.nt[0] : (unsigned)g.nt[0] * g.nt[1];
     const int jj = (DIR == JDIR) ? 0 : tr;  // j for the z sweep
     const int kk = (DIR == JDIR) ? tr : 0;  // k for the y sweep
-    const long long col = idx3(g, kk, jj, ic);  // index with DIR coordinate = 0
+    const unsigned col = (unsigned)idx3(g, kk, jj, ic);  // index with DIR coordinate = 0
     const bool rhs_lane = lane_ok && i >= g.beg[0] && i < g.end[0] &&
                           ((DIR == JDIR) ? (tr >= g.beg[2] && tr < g.end[2])
                                          : (tr >= g.beg[1] && tr < g.end[1]));
     // Vs normal component: stride along DIR in the Vs array
-    const long long vsd = (DIR == JDIR) ? g.vsn[0] : (long long)g.vsn[0] * g.vsn[1];
-    const long long vcol = idxs(g, DIR, kk, jj, ic);
+    const unsigned vsd = (DIR == JDIR) ? g.vsn[0] : (unsigned)g.vsn[0] * g.vsn[1];
+    const unsigned vcol = (unsigned)idxs(g, 0, kk, jj, ic);
+    const double *VsDir = a.Vs + DIR * g.svs;  // the normal component
 
     T win[NW][NV];
 #pragma unroll
     for (int s = 0; s < NW; s++)
 #pragma unroll
       for (int n = 0; n < NV; n++)
-        win[s][n] = (T)a.Vc[n * g.sv + col + (long long)(c0 - 1 - H + s) * sd];
+        win[s][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(c0 - 1 - H + s) * sd);
     // right-face value of cell c0-1: the left state of the first face
     T carry[NV], Fprev[NV], cprev = 0;
 #pragma unroll
@@ -410,7 +425,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         for (int n = 0; n < NV; n++) win[s][n] = win[s + 1][n];
 #pragma unroll
       for (int n = 0; n < NV; n++)
-        win[NW - 1][n] = (T)a.Vc[n * g.sv + col + (long long)(f + H) * sd];
+        win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
       T vL[NV], vR[NV], vr_next[NV];
 #pragma unroll
       for (int n = 0; n < NV; n++) {
@@ -422,22 +437,22 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
       }
       if constexpr (C::MHD) {
         constexpr int BXn = C::template BXn<DIR>();
-        vL[BXn] = (T)a.Vs[vcol + (long long)f * vsd];
+        vL[BXn] = (T)ld32(VsDir, vcol + (unsigned)f * vsd);
         vR[BXn] = vL[BXn];
       }
       T F[NV], cmax;
       FaceEmf<T> E;
       const bool wrote = riemann<C, DIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
-      const long long p = col + (long long)f * sd;
+      const unsigned p = col + (unsigned)f * sd;
       const bool own_face = lane_ok && (f < c1 || c1 == g.end[DIR]);
       if (own_face) store_face_emf<C, DIR>(e, p, a.emf, E);
       if constexpr (!FUSED) {
         if (own_face) {
           if (wrote) {
 #pragma unroll
-            for (int n = 0; n < NV; n++) a.Flux[n * g.sv + p] = (double)F[n];
+            for (int n = 0; n < NV; n++) st32(a.Flux + n * g.sv, p, (double)F[n]);
           }
-          a.cMax[p] = (double)cmax;
+          st32(a.cMax, p, (double)cmax);
         }
       } else {
         if (f > c0 && rhs_lane) {

@@ -46,6 +46,33 @@ inline unsigned blocks_for(long long n, int bs = 256) {
   return (unsigned)std::max<long long>(1, (n + bs - 1) / bs);
 }
 
+// Resident waves of a 256-thread kernel over the device (CUs x blocks per CU x 4), queried
+// once per kernel.  target_waves carries 32 waves per CU.
+template <typename K>
+long long resident_waves(K kernel, int target_waves) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1)
+    nb = 1;
+  return (long long)nb * 4 * std::max(1, target_waves / 32);
+}
+
+// Segment length of a marching kernel: minimises the modelled duration
+// ceil(waves / slots) x (ts + 2), waves = columns x ceil(nd / ts), where ts + 2 counts the
+// ts cells, the extra face and the window priming of one segment.
+inline int march_segment(long long columns, int nd, long long slots) {
+  double best = 1e300;
+  int bt = nd;
+  for (int ts = std::min(nd, 4); ts <= nd; ts++) {
+    const long long waves = columns * ((nd + ts - 1) / ts);
+    const double cost = (double)((waves + slots - 1) / slots) * (ts + 2);
+    if (cost < best) {
+      best = cost;
+      bt = ts;
+    }
+  }
+  return bt;
+}
+
 template <class C, typename T, int DIR, bool FUSED>
 hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
                             int target_waves, hipStream_t s) {
@@ -67,11 +94,8 @@ hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArra
     m.nxb = (m.ihi - m.ilo + 63) / 64;
     const int nd = g.end[DIR] - g.beg[DIR];
     const long long columns = (long long)m.nxb * (m.thi - m.tlo);
-    // segment length: enough waves to fill the chip, long enough to amortise the
-    // extra face and the window priming of each segment
-    int ts = 64;
-    while (ts > 8 && columns * ((nd + ts - 1) / ts) < target_waves) ts /= 2;
-    m.ts = std::min(ts, nd);
+    static const long long slots = resident_waves(k_sweep_march<C, T, DIR, FUSED>, target_waves);
+    m.ts = march_segment(columns, nd, slots);
     m.nseg = (nd + m.ts - 1) / m.ts;
     const long long waves = columns * m.nseg;
     hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED>), dim3(blocks_for(waves, 4)), dim3(256),
@@ -249,28 +273,40 @@ inline void fused2d_halo(int order, int emf, int &L, int &R) {
   }
 }
 
+template <class C, typename T, int EMF>
+hipError_t launch_fused2d_y_emf(const GridDesc &g, const SweepArgs &a, const void *emf,
+                                Fused2dArgs f, int target_waves, hipStream_t s) {
+  int L, R;
+  fused2d_halo(C::ORDER, EMF, L, R);
+  // output columns hl .. hl+w-1 of each wave; corners at hl .. hl+w; y faces at
+  // hl-L .. hl+w+R, all inside the 64 lanes
+  f.hl = L;
+  f.w = 63 - L - R;
+  MarchGeom m{};
+  const int ncols = g.end[0] + 1 - g.beg[0];  // face columns beg .. end
+  m.nxb = (ncols + f.w - 1) / f.w;
+  const int nd = g.end[1] - g.beg[1];
+  static const long long slots = resident_waves(k_fused2d_y<C, T, EMF>, target_waves);
+  m.ts = march_segment(m.nxb, nd, slots);
+  m.nseg = (nd + m.ts - 1) / m.ts;
+  const long long waves = (long long)m.nxb * m.nseg;
+  hipLaunchKernelGGL((k_fused2d_y<C, T, EMF>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g, a,
+                     *static_cast<const EmfArrays<T> *>(emf), f, m);
+  return hipGetLastError();
+}
+
 template <class C, typename T>
 hipError_t launch_fused2d_y(const GridDesc &g, const SweepArgs &a, const void *emf,
                             Fused2dArgs f, int target_waves, hipStream_t s) {
   if constexpr (C::MHD && C::DIM == 2) {
-    int L, R;
-    fused2d_halo(C::ORDER, a.emf, L, R);
-    // output columns hl .. hl+w-1 of each wave; corners at hl .. hl+w; y faces at
-    // hl-L .. hl+w+R, all inside the 64 lanes
-    f.hl = L;
-    f.w = 63 - L - R;
-    MarchGeom m{};
-    const int ncols = g.end[0] + 1 - g.beg[0];  // face columns beg .. end
-    m.nxb = (ncols + f.w - 1) / f.w;
-    const int nd = g.end[1] - g.beg[1];
-    int ts = 64;
-    while (ts > 8 && (long long)m.nxb * ((nd + ts - 1) / ts) < target_waves) ts /= 2;
-    m.ts = std::min(ts, nd);
-    m.nseg = (nd + m.ts - 1) / m.ts;
-    const long long waves = (long long)m.nxb * m.nseg;
-    hipLaunchKernelGGL((k_fused2d_y<C, T>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g, a,
-                       *static_cast<const EmfArrays<T> *>(emf), f, m);
-    return hipGetLastError();
+    switch (a.emf) {
+      case EMF_ARITHMETIC: return launch_fused2d_y_emf<C, T, EMF_ARITHMETIC>(g, a, emf, f, target_waves, s);
+      case EMF_UCT0: return launch_fused2d_y_emf<C, T, EMF_UCT0>(g, a, emf, f, target_waves, s);
+      case EMF_UCT_CONTACT: return launch_fused2d_y_emf<C, T, EMF_UCT_CONTACT>(g, a, emf, f, target_waves, s);
+      case EMF_UCT_HLL: return launch_fused2d_y_emf<C, T, EMF_UCT_HLL>(g, a, emf, f, target_waves, s);
+      case EMF_UCT_HLLD: return launch_fused2d_y_emf<C, T, EMF_UCT_HLLD>(g, a, emf, f, target_waves, s);
+      default: break;
+    }
   }
   return hipErrorInvalidValue;
 }

@@ -211,6 +211,19 @@ class DataBlock:
         second half (default where supported).  Results are identical."""
         capi.check(self._L.idfx_ctx_set_fusion(self.ctx, int(level)), "set_fusion")
 
+    def set_profiling(self, on):
+        """Region profiler (idfx::pushRegion/popRegion with profiling on, global.cpp:66-95):
+        HIP event pairs around each region of cycle() and stage()."""
+        capi.check(self._L.idfx_ctx_set_profiling(self.ctx, int(bool(on))), "set_profiling")
+
+    def region_times(self):
+        """{region: (total ms, launches)} since the last call; resets the totals."""
+        n = len(capi.REGIONS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_long * n)()
+        capi.check(self._L.idfx_ctx_region_times(self.ctx, ms, cnt, n), "region_times")
+        return {name: (ms[q], cnt[q]) for q, name in enumerate(capi.REGIONS) if cnt[q] > 0}
+
     def cycle(self):
         """TimeIntegrator::Cycle on the fused path (timeIntegrator.cpp:246-414)."""
         t = ctypes.c_double(self.t)

@@ -79,7 +79,10 @@ typedef struct {
 
 /* Context: replaces the Fluid<Phys>/RiemannSolver/ConstrainedTransport constructors
  * (fluid.hpp:263-755, riemannSolver.hpp:90-169, constrainedTransport.hpp:152-263).
- * dx1..dx3 are host arrays of np_tot[d] cell widths (DataBlock::dx, gridHost.cpp:79-86). */
+ * dx1..dx3 are host arrays of np_tot[d] cell widths (DataBlock::dx, gridHost.cpp:79-86).
+ * A block holds fewer than 2^29 cells including ghosts and face padding
+ * ((np_tot+1) product; e.g. up to 805^3 interior cells with PLM): the kernels address each
+ * variable with 32-bit byte offsets.  Larger blocks return IDFX_E_ARGS. */
 int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
                     const double *dx3, idfx_ctx **out);
 int idfx_ctx_destroy(idfx_ctx *ctx);
@@ -172,6 +175,26 @@ int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long 
  * AddAndStore and ConvertConsToPrim into one kernel.  Both give identical results.
  * Returns IDFX_E_ARGS for level 1 where the configuration does not support it. */
 int idfx_ctx_set_fusion(idfx_ctx *ctx, int level);
+
+/* Regions of idfx_cycle / idfx_stage timed by the region profiler. */
+enum {
+  IDFX_REGION_BOUNDARIES = 0,   /* SetBoundaries (ghost fill, normal field, cell B) */
+  IDFX_REGION_SWEEP_X = 1,      /* CalcFlux + CalcRightHandSide along x (y: 2, z: 3) */
+  IDFX_REGION_SWEEP_Y = 2,
+  IDFX_REGION_SWEEP_Z = 3,
+  IDFX_REGION_CORNER_EMF = 4,   /* CalcCornerEMF */
+  IDFX_REGION_EVOLVE_B = 5,     /* EvolveMagField (+ AddAndStore of Vs) */
+  IDFX_REGION_CONS_TO_PRIM = 6, /* AddAndStore of Uc + ConvertConsToPrim */
+  IDFX_REGION_FUSED2D_Y = 7,    /* fused 2-D second half (fusion level 1) */
+  IDFX_NREGIONS = 8
+};
+
+/* idfx::pushRegion / popRegion with the profiler enabled (global.cpp:66-95, profiler.cpp):
+ * when on, every region of idfx_cycle / idfx_stage is bracketed by a HIP event pair on the
+ * launch stream.  idfx_ctx_region_times waits for the recorded regions, returns the total
+ * milliseconds and launch count of the first n regions, and resets the totals. */
+int idfx_ctx_set_profiling(idfx_ctx *ctx, int on);
+int idfx_ctx_region_times(idfx_ctx *ctx, double *ms, long *count, int n);
 
 #ifdef __cplusplus
 }

@@ -234,3 +234,24 @@ def test_fused_schedule_bitwise_equals_per_kernel(emf, recon, bc):
     assert np.array_equal(a_s[0][:, b[1]:e[1], b[0]:e[0] + 1], b_s[0][:, b[1]:e[1], b[0]:e[0] + 1])
     assert np.array_equal(a_s[1][:, b[1]:e[1] + 1, b[0]:e[0]], b_s[1][:, b[1]:e[1] + 1, b[0]:e[0]])
     torch.cuda.synchronize()
+
+
+def test_region_profiler_counts():
+    """Region profiler: one event pair per region per stage, positive durations."""
+    kw = cfg_kwargs(2, (64, 64), emf="uct_hlld")
+    _, g = make_pair(kw, problems.orszag_tang)
+    g.set_profiling(True)
+    for _ in range(3):
+        g.cycle()
+    rt = g.region_times()
+    g.set_profiling(False)
+    assert set(rt) == {"boundaries", "sweep_x", "fused2d_y"}
+    for name, (ms, n) in rt.items():
+        assert n == 6 and ms > 0.0, (name, ms, n)
+    g.set_fusion(0)
+    g.set_profiling(True)
+    g.cycle()
+    rt = g.region_times()
+    assert set(rt) == {"boundaries", "sweep_x", "sweep_y", "corner_emf", "evolve_b",
+                       "cons_to_prim"}
+    assert all(n == 2 for _, n in rt.values())
