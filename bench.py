@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--kernel-reps", type=int, default=10)
+    p.add_argument("--no-headline", action="store_true",
+                   help="skip the reference's own 3-D benchmark configuration")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB device copy 10 times (counter calibration)")
     return p.parse_args()
@@ -111,6 +113,35 @@ def algorithmic_bytes(nx):
 MEASURED_TRAFFIC = {2048: {}}
 
 
+# The reference's published benchmark (doc/source/performances.rst:5-10, bench.json): 3-D
+# Orszag-Tang, 128^3 cells per GPU, PLM, HLLD, uct_contact, RK2; cell-updates/s.
+REFERENCE_PUBLISHED = {"NVIDIA H100": 3.079643e8, "NVIDIA A100": 2.044728e8,
+                       "NVIDIA V100": 1.193720e8, "AMD MI250X (one GCD)": 1.436580e8}
+
+
+def reference_headline(steps, warmup, n=128):
+    """The reference's own benchmark configuration timed on this GPU (FP64)."""
+    import torch
+    from idefix_amd.datablock import DataBlock
+    from idefix_amd import problems
+    g = DataBlock(3, (n, n, n), mhd=True, recon="plm", solver="hlld", emf="uct_contact",
+                  precision="fp64", bc="periodic", gamma=5.0 / 3.0, nstages=2, cfl=0.9,
+                  first_dt=1e-4)
+    Vc, Vs = problems.orszag_tang_3d(g)
+    g.upload(Vc, Vs)
+    el = time_cycles(g, steps, warmup)
+    zu = n ** 3 * steps / el
+    kt, _ = kernel_times(g, 3)
+    del g
+    torch.cuda.empty_cache()
+    return {"workload": "3D Orszag-Tang %d^3, PLM, HLLD, CT uct_contact, RK2, CFL 0.9, "
+                        "periodic, FP64 (the reference's published benchmark)" % n,
+            "value": zu, "unit": "cell-updates/s", "ms_per_step": 1e3 * el / steps,
+            "kernel_ms": kt,
+            "published": REFERENCE_PUBLISHED,
+            "ratio_vs_published": {k: zu / v for k, v in REFERENCE_PUBLISHED.items()}}
+
+
 def cpu_baseline(nx, seconds):
     """CPU oracle on the host cores: Orszag-Tang at the same resolution, as many cycles as
     fit in about `seconds`."""
@@ -172,6 +203,7 @@ def main():
         fp32 = nx * nx * args.steps / el32
         del g32
         torch.cuda.empty_cache()
+    headline = None if args.no_headline else reference_headline(args.steps, args.warmup)
     cpu = None if args.no_cpu_baseline else cpu_baseline(nx, args.cpu_seconds)
     cycle_bytes = 5 * 6 * 8  # RK2: 5 state transfers of S = 6 words per zone-update
     line = {
@@ -200,6 +232,7 @@ def main():
                      "algorithmic_bytes": ab[dom], "ms_per_launch": kt[dom],
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "reference_benchmark": headline,
     }
     print(json.dumps(line))
 

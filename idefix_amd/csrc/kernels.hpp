@@ -276,7 +276,7 @@ template <class C>
 constexpr int x_waves_per_eu() {
   return C::DIM == 3 ? 3 : 4;
 }
-template <class C, typename T, bool FUSED>
+template <class C, typename T, bool FUSED, int EMF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(x_waves_per_eu<C>())))
 void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                  int nseg) {
@@ -319,11 +319,11 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
     }
     T F[NV], cmax;
     FaceEmf<T> E;
-    const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
+    const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, EMF, (T)g.gamma);
     const bool face_ok = lane >= 1 && c <= g.end[0];
     const bool own_face = face_ok && (lane <= 62 || c == g.end[0]);
     const unsigned p = row + (unsigned)c;
-    if (own_face) store_face_emf<C, IDIR>(e, p, a.emf, E);
+    if (own_face) store_face_emf<C, IDIR>(e, p, EMF, E);
     if constexpr (!FUSED) {
       if (own_face) {
         if (wrote) {
@@ -364,7 +364,7 @@ struct MarchGeom {
 #ifndef IDFX_MARCH_WAVES_PER_EU
 #define IDFX_MARCH_WAVES_PER_EU 2
 #endif
-template <class C, typename T, int DIR, bool FUSED>
+template <class C, typename T, int DIR, bool FUSED, int EMF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IDFX_MARCH_WAVES_PER_EU)))
 void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                      MarchGeom m) {
@@ -442,10 +442,10 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
       }
       T F[NV], cmax;
       FaceEmf<T> E;
-      const bool wrote = riemann<C, DIR>(vL, vR, F, cmax, E, a.emf, (T)g.gamma);
+      const bool wrote = riemann<C, DIR>(vL, vR, F, cmax, E, EMF, (T)g.gamma);
       const unsigned p = col + (unsigned)f * sd;
       const bool own_face = lane_ok && (f < c1 || c1 == g.end[DIR]);
-      if (own_face) store_face_emf<C, DIR>(e, p, a.emf, E);
+      if (own_face) store_face_emf<C, DIR>(e, p, EMF, E);
       if constexpr (!FUSED) {
         if (own_face) {
           if (wrote) {

@@ -73,7 +73,7 @@ inline int march_segment(long long columns, int nd, long long slots) {
   return bt;
 }
 
-template <class C, typename T, int DIR, bool FUSED>
+template <class C, typename T, int DIR, bool FUSED, int EMF>
 hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
                             int target_waves, hipStream_t s) {
   if constexpr (DIR == IDIR) {
@@ -82,7 +82,7 @@ hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArra
     const long long nj = g.end[1] - g.beg[1] + 2 * a.ext[1];
     const long long nk = g.end[2] - g.beg[2] + 2 * a.ext[2];
     const long long waves = (long long)nseg * nj * nk;
-    hipLaunchKernelGGL((k_sweep_x<C, T, FUSED>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g,
+    hipLaunchKernelGGL((k_sweep_x<C, T, FUSED, EMF>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g,
                        a, e, nseg);
   } else {
     MarchGeom m;
@@ -94,14 +94,38 @@ hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArra
     m.nxb = (m.ihi - m.ilo + 63) / 64;
     const int nd = g.end[DIR] - g.beg[DIR];
     const long long columns = (long long)m.nxb * (m.thi - m.tlo);
-    static const long long slots = resident_waves(k_sweep_march<C, T, DIR, FUSED>, target_waves);
+    static const long long slots =
+        resident_waves(k_sweep_march<C, T, DIR, FUSED, EMF>, target_waves);
     m.ts = march_segment(columns, nd, slots);
     m.nseg = (nd + m.ts - 1) / m.ts;
     const long long waves = columns * m.nseg;
-    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED>), dim3(blocks_for(waves, 4)), dim3(256),
+    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED, EMF>), dim3(blocks_for(waves, 4)), dim3(256),
                        0, s, g, a, e, m);
   }
   return hipGetLastError();
+}
+
+// One direction of the sweep with the corner-EMF scheme as a compile-time parameter (the
+// face data a sweep stores depend on it); hydro configurations have a single variant.
+template <class C, typename T, int DIR, bool FUSED>
+hipError_t launch_sweep_emf(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
+                            int target_waves, hipStream_t s) {
+  if constexpr (C::MHD) {
+    switch (a.emf) {
+      case EMF_ARITHMETIC:
+        return launch_sweep_dir<C, T, DIR, FUSED, EMF_ARITHMETIC>(g, a, e, target_waves, s);
+      case EMF_UCT0: return launch_sweep_dir<C, T, DIR, FUSED, EMF_UCT0>(g, a, e, target_waves, s);
+      case EMF_UCT_CONTACT:
+        return launch_sweep_dir<C, T, DIR, FUSED, EMF_UCT_CONTACT>(g, a, e, target_waves, s);
+      case EMF_UCT_HLL:
+        return launch_sweep_dir<C, T, DIR, FUSED, EMF_UCT_HLL>(g, a, e, target_waves, s);
+      case EMF_UCT_HLLD:
+        return launch_sweep_dir<C, T, DIR, FUSED, EMF_UCT_HLLD>(g, a, e, target_waves, s);
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    return launch_sweep_dir<C, T, DIR, FUSED, EMF_ARITHMETIC>(g, a, e, target_waves, s);
+  }
 }
 
 template <class C, typename T>
@@ -109,22 +133,22 @@ hipError_t launch_sweep(int dir, bool fused, const GridDesc &g, const SweepArgs 
                         const void *emf, int target_waves, hipStream_t s) {
   const EmfArrays<T> &e = *static_cast<const EmfArrays<T> *>(emf);
   if (dir == 0) {
-    if (fused) return launch_sweep_dir<C, T, 0, true>(g, a, e, target_waves, s);
+    if (fused) return launch_sweep_emf<C, T, 0, true>(g, a, e, target_waves, s);
     if constexpr (std::is_same<T, double>::value)
-      return launch_sweep_dir<C, T, 0, false>(g, a, e, target_waves, s);
+      return launch_sweep_emf<C, T, 0, false>(g, a, e, target_waves, s);
   }
   if constexpr (C::DIM >= 2) {
     if (dir == 1) {
-      if (fused) return launch_sweep_dir<C, T, 1, true>(g, a, e, target_waves, s);
+      if (fused) return launch_sweep_emf<C, T, 1, true>(g, a, e, target_waves, s);
       if constexpr (std::is_same<T, double>::value)
-        return launch_sweep_dir<C, T, 1, false>(g, a, e, target_waves, s);
+        return launch_sweep_emf<C, T, 1, false>(g, a, e, target_waves, s);
     }
   }
   if constexpr (C::DIM >= 3) {
     if (dir == 2) {
-      if (fused) return launch_sweep_dir<C, T, 2, true>(g, a, e, target_waves, s);
+      if (fused) return launch_sweep_emf<C, T, 2, true>(g, a, e, target_waves, s);
       if constexpr (std::is_same<T, double>::value)
-        return launch_sweep_dir<C, T, 2, false>(g, a, e, target_waves, s);
+        return launch_sweep_emf<C, T, 2, false>(g, a, e, target_waves, s);
     }
   }
   return hipErrorInvalidValue;

@@ -51,6 +51,58 @@ def orszag_tang(g):
     return Vc, Vs
 
 
+def orszag_tang_3d(g):
+    """3-D Orszag-Tang vortex, test/MHD/OrszagTang3D/setup.cpp:213-263: the velocity field
+    on cell centres and B = curl A from the edge vector potential
+    A = B0/(2 pi) (cos 2 pi y, sin 2 pi x, cos 2 pi y + cos(4 pi x)/2) taken on the
+    interior faces as DataBlockHost::MakeVsFromAmag does (dataBlockHost.cpp:186-214)."""
+    Vc, Vs = _empty(g)
+    x, y, z = _coords(g)
+    xl, yl, _ = _coords(g, "xl")
+    nx, ny, nz = g.np_tot
+    X = x[None, None, :]
+    Y = y[None, :, None]
+    Z = z[:, None, None]
+    B0 = 1.0 / np.sqrt(4.0 * np.pi)
+    Vc[0] = 25.0 / (36.0 * np.pi)
+    Vc[4] = 5.0 / (12.0 * np.pi)
+    Vc[1] = -np.sin(2.0 * np.pi * Y) + 0 * X + 0 * Z
+    Vc[2] = np.sin(2.0 * np.pi * X) + np.cos(2.0 * np.pi * Z) + 0 * Y
+    Vc[3] = np.cos(2.0 * np.pi * X) + 0 * Y + 0 * Z
+    # vector potential on cell corners (xl, yl, zl) of cells 0 .. np_tot-1
+    XL = xl[None, None, :]
+    YL = yl[None, :, None]
+    c = B0 / (2.0 * np.pi)
+    A = np.zeros((3, nz, ny, nx))
+    A[0] = c * np.cos(2.0 * np.pi * YL) + 0 * XL
+    A[1] = c * np.sin(2.0 * np.pi * XL) + 0 * YL
+    A[2] = c * (np.cos(2.0 * np.pi * YL) + np.cos(4.0 * np.pi * XL) / 2.0)
+    b, e = g.beg, g.end
+    k0, k1 = b[2], e[2] + 1
+    j0, j1 = b[1], e[1] + 1
+    i0, i1 = b[0], e[0] + 1
+    d1 = _widths(g, 0)[None, None, i0:i1]
+    d2 = _widths(g, 1)[None, j0:j1, None]
+    d3 = _widths(g, 2)[k0:k1, None, None]
+    Ai = A[:, k0:k1, j0:j1, i0:i1]
+    Aip = A[:, k0:k1, j0:j1, i0 + 1:i1 + 1]
+    Ajp = A[:, k0:k1, j0 + 1:j1 + 1, i0:i1]
+    Akp = A[:, k0 + 1:k1 + 1, j0:j1, i0:i1]
+    Vs[0, k0:k1, j0:j1, i0:i1] = (0.0 + 1 / d2 * (Ajp[2] - Ai[2])) - 1 / d3 * (Akp[1] - Ai[1])
+    Vs[1, k0:k1, j0:j1, i0:i1] = -1 / d1 * (Aip[2] - Ai[2]) + 1 / d3 * (Akp[0] - Ai[0])
+    Vs[2, k0:k1, j0:j1, i0:i1] = 1 / d1 * (Aip[1] - Ai[1]) - 1 / d2 * (Ajp[0] - Ai[0])
+    return Vc, Vs
+
+
+def _widths(g, d):
+    """Cell widths along direction d over np_tot cells."""
+    if hasattr(g, "dx"):
+        return np.asarray(g.dx[d], dtype=np.float64)
+    xl = np.asarray(_coords(g, "xl")[d], dtype=np.float64)
+    xr = np.asarray(_coords(g, "xr")[d], dtype=np.float64)
+    return xr - xl
+
+
 def sod(g, x0=0.5):
     """1-D Sod shock tube, test/HD/sod/setup.cpp:35-38 (interface at x0)."""
     Vc, _ = _empty(g)

@@ -249,6 +249,9 @@ class OracleGrid:
         self.comps = o.cfg.comps
         self.x = [o.coord(d, "x") for d in range(3)]
         self.xl = [o.coord(d, "xl") for d in range(3)]
+        self.dx = [o.coord(d, "dx") for d in range(3)]
+        self.beg = list(o.beg)
+        self.end = list(o.end)
 
 
 def set_threads(n):

@@ -255,3 +255,13 @@ def test_region_profiler_counts():
     assert set(rt) == {"boundaries", "sweep_x", "sweep_y", "corner_emf", "evolve_b",
                        "cons_to_prim"}
     assert all(n == 2 for _, n in rt.values())
+
+
+def test_orszag_tang_3d_reference_benchmark_config():
+    """The reference's published benchmark setup (3-D Orszag-Tang, PLM, HLLD, uct_contact,
+    RK2, CFL 0.9) at a small size."""
+    kw = cfg_kwargs(3, (16, 24, 12), emf="uct_contact", recon="plm", cfl=0.9, nstages=2)
+    o, g = make_pair(kw, problems.orszag_tang_3d)
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="OT3D")
