@@ -71,6 +71,24 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
     const double dtdx1 = a.dt / g.dx[0][cc];  // evolveMagField.hpp:67 (dt/dx1(i))
     constexpr int emf = EMF;  // the corner-EMF scheme is a compile-time choice here
 
+    // PLM: the stencil rows a marching step shares with the next ones are carried in LDS
+    // (slot q of thread t at lds[q][t]) instead of being fetched again: each step then
+    // reads every input row once.  PPM reloads them (its wider window does not fit).
+    constexpr bool CARRY = C::ORDER != RECON_PPM;
+    constexpr bool HLLE = emf == EMF_UCT_HLL || emf == EMF_UCT_HLLD;
+    constexpr bool NEED_E3 = emf == EMF_UCT0 || emf == EMF_UCT_CONTACT;
+    constexpr int RW = NW - 1;                  // window rows carried (PLM: f-1, f)
+    constexpr int S_BX = RW * NV;               // uct_hll(d): BX1s rows f-2 .. f (ring of 4)
+    constexpr int S_EZ = S_BX + 4;              //             ezi rows f-2 .. f (ring of 4)
+    constexpr int S_AD = S_EZ + 4;              //             axL axR dxL dxR of row f-1
+    constexpr int S_E1 = RW * NV;               // others: ezi of row f-1
+    constexpr int S_S1 = S_E1 + 1;              //         svx of row f-1
+    constexpr int NSLOT = CARRY ? (HLLE ? S_AD + 4 : S_S1 + 1) : 1;
+    __shared__ T lds[NSLOT][256];
+    const int tid = threadIdx.x;
+    auto wslot = [&](int row, int n) -> T & { return lds[(row & (RW - 1)) * NV + n][tid]; };
+    auto ld_row = [&](const T *arr, int row) { return ld32(arr, (unsigned)row * sd + col); };
+
     T carry[NV], Fprev[NV];
     T cprev = 0, ezprev = 0;
     double b2prev = 0.0;
@@ -79,24 +97,56 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
     for (int n = 0; n < NV; n++) {
       Fprev[n] = 0;
       carry[n] = 0;
-      if (n == BX2) continue;  // the normal field comes from Vs
+      const bool keep = n != BX2 || NEED_E3;
+      if (n == BX2 && !(CARRY && keep)) continue;  // the normal field comes from Vs
       T wv[NW];
 #pragma unroll
       for (int q = 0; q < NW; q++)
         wv[q] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(c0 - 1 - H + q) * sd);
+      if constexpr (CARRY) {
+        if (keep) {
+#pragma unroll
+          for (int q = 1; q < NW; q++) wslot(c0 - 1 - H + q, n) = wv[q];
+        }
+      }
+      if (n == BX2) continue;
       T dummy;
       recon_cell<C>(wv, n, dummy, carry[n]);
     }
+    if constexpr (CARRY) {
+      if constexpr (HLLE) {
+#pragma unroll
+        for (int r = c0 - 2; r <= c0; r++) {
+          lds[S_BX + (r & 3)][tid] = (T)ld32(s.Vs_in, (unsigned)r * vsd + col);
+          lds[S_EZ + (r & 3)][tid] = ld_row(e.ezi, r);
+        }
+        lds[S_AD + 0][tid] = ld_row(e.axL, c0 - 1);
+        lds[S_AD + 1][tid] = ld_row(e.axR, c0 - 1);
+        lds[S_AD + 2][tid] = ld_row(e.dxL, c0 - 1);
+        lds[S_AD + 3][tid] = ld_row(e.dxR, c0 - 1);
+      } else {
+        lds[S_E1][tid] = ld_row(e.ezi, c0 - 1);
+        if constexpr (emf == EMF_UCT_CONTACT) lds[S_S1][tid] = ld_row(e.svx, c0 - 1);
+      }
+    }
     for (int f = c0; f <= c1; f++) {
       // ---- 1. y face f -------------------------------------------------------------
-      // stencil rows f-H .. f+H reloaded each step (cache hits) instead of carried in
-      // registers: the register budget goes to occupancy
       T win[NW][NV];
 #pragma unroll
-      for (int q = 0; q < NW; q++)
+      for (int n = 0; n < NV; n++) {
+        const bool keep = n != BX2 || NEED_E3;
+        if (!keep) continue;
+        if constexpr (CARRY) {
 #pragma unroll
-        for (int n = 0; n < NV; n++)
-          win[q][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f - H + q) * sd);
+          for (int q = 0; q < NW - 1; q++) win[q][n] = wslot(f - H + q, n);
+          win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
+          wslot(f + H, n) = win[NW - 1][n];  // replaces row f-H, read above
+        } else {
+#pragma unroll
+          for (int q = 0; q < NW; q++)
+            win[q][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f - H + q) * sd);
+        }
+      }
       T vL[NV], vR[NV], vr_next[NV];
 #pragma unroll
       for (int n = 0; n < NV; n++) {
@@ -125,7 +175,8 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       // ---- 2. corner EMF ez(c, f) ----------------------------------------------------
       T ez;
       auto xf = [&](const T *arr, int row) { return ld32(arr, (unsigned)row * sd + col); };
-      if constexpr (emf == EMF_UCT_HLL || emf == EMF_UCT_HLLD) {
+      double b1old;  // BX1s of row f-1
+      if constexpr (HLLE) {
         constexpr int SLO = C::ORDER == RECON_PPM ? 1 : 2;
         constexpr int SHI = C::ORDER == RECON_PPM ? 6 : 5;
         T b[7], v[7];
@@ -136,27 +187,64 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
           b[q] = lane_at(b2T, lane, q - 4);
           v[q] = lane_at(E.Et, lane, q - 4);
         }
-        T aL = T(0.5) * (xf(e.axL, f - 1) + xf(e.axL, f));
-        T aR = T(0.5) * (xf(e.axR, f - 1) + xf(e.axR, f));
-        T dL = T(0.5) * (xf(e.dxL, f - 1) + xf(e.dxL, f));
-        T dR = T(0.5) * (xf(e.dxR, f - 1) + xf(e.dxR, f));
+        T aL, aR, dL, dR;
+        {
+          const T axL = xf(e.axL, f), axR = xf(e.axR, f), dxL = xf(e.dxL, f), dxR = xf(e.dxR, f);
+          if constexpr (CARRY) {
+            aL = T(0.5) * (lds[S_AD + 0][tid] + axL);
+            aR = T(0.5) * (lds[S_AD + 1][tid] + axR);
+            dL = T(0.5) * (lds[S_AD + 2][tid] + dxL);
+            dR = T(0.5) * (lds[S_AD + 3][tid] + dxR);
+            lds[S_AD + 0][tid] = axL;
+            lds[S_AD + 1][tid] = axR;
+            lds[S_AD + 2][tid] = dxL;
+            lds[S_AD + 3][tid] = dxR;
+          } else {
+            aL = T(0.5) * (xf(e.axL, f - 1) + axL);
+            aR = T(0.5) * (xf(e.axR, f - 1) + axR);
+            dL = T(0.5) * (xf(e.dxL, f - 1) + dxL);
+            dR = T(0.5) * (xf(e.dxR, f - 1) + dxR);
+          }
+        }
         ez = edge_term<C>(b, v, aL, aR, dL, dR, true);
         // y faces: B = BX1s of x face c and v = ezi along y (rows f-4+s)
+        if constexpr (CARRY) {
 #pragma unroll
-        for (int q = SLO; q <= SHI; q++) {
-          b[q] = (T)ld32(s.Vs_in, (unsigned)(f - 4 + q) * vsd + col);
-          v[q] = xf(e.ezi, f - 4 + q);
+          for (int q = SLO; q < SHI; q++) {
+            b[q] = lds[S_BX + ((f - 4 + q) & 3)][tid];
+            v[q] = lds[S_EZ + ((f - 4 + q) & 3)][tid];
+          }
+          b[SHI] = (T)ld32(s.Vs_in, (unsigned)(f - 4 + SHI) * vsd + col);
+          v[SHI] = xf(e.ezi, f - 4 + SHI);
+          lds[S_BX + ((f - 4 + SHI) & 3)][tid] = b[SHI];
+          lds[S_EZ + ((f - 4 + SHI) & 3)][tid] = v[SHI];
+        } else {
+#pragma unroll
+          for (int q = SLO; q <= SHI; q++) {
+            b[q] = (T)ld32(s.Vs_in, (unsigned)(f - 4 + q) * vsd + col);
+            v[q] = xf(e.ezi, f - 4 + q);
+          }
         }
         aL = T(0.5) * (lane_at(E.aL, lane, -1) + E.aL);
         aR = T(0.5) * (lane_at(E.aR, lane, -1) + E.aR);
         dL = T(0.5) * (lane_at(E.dL, lane, -1) + E.dL);
         dR = T(0.5) * (lane_at(E.dR, lane, -1) + E.dR);
         ez += edge_term<C>(b, v, aL, aR, dL, dR, false);
+        // BX1s of row f-1 is slot 3 of the stencil (exact when T is double)
+        if constexpr (CARRY && std::is_same<T, double>::value) b1old = b[3];
+        else b1old = ld32(s.Vs_in, (unsigned)(f - 1) * vsd + col);
       } else {
+        b1old = ld32(s.Vs_in, (unsigned)(f - 1) * vsd + col);
         const T ezj_c = E.Et;
         const T ezj_m = lane_at(E.Et, lane, -1);
         const T ezi_f = xf(e.ezi, f);
-        const T ezi_m = xf(e.ezi, f - 1);
+        T ezi_m;
+        if constexpr (CARRY) {
+          ezi_m = lds[S_E1][tid];
+          lds[S_E1][tid] = ezi_f;
+        } else {
+          ezi_m = xf(e.ezi, f - 1);
+        }
         if constexpr (emf == EMF_ARITHMETIC) {
           ez = T(0.25) * (ezi_f + ezi_m + ezj_c + ezj_m);
         } else {
@@ -175,7 +263,14 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
             const T a4 = T(2) * ezj_m - T(0.5) * (E3_mm + E3_m);
             ez = T(0.25) * (a1 + a2 + a3 + a4);
           } else {
-            const T svx_m = xf(e.svx, f - 1), svx_c = xf(e.svx, f);
+            const T svx_c = xf(e.svx, f);
+            T svx_m;
+            if constexpr (CARRY) {
+              svx_m = lds[S_S1][tid];
+              lds[S_S1][tid] = svx_c;
+            } else {
+              svx_m = xf(e.svx, f - 1);
+            }
             const T svy_c = E.sv, svy_m = lane_at(E.sv, lane, -1);
             const T ez_l2 = (1 - svx_m) * (ezj_c - E3_cm) + (svx_m) * (ezj_m - E3_mm);
             const T ez_r2 = (1 - svx_c) * (ezj_c - E3_c) + (svx_c) * (ezj_m - E3_m);
@@ -198,7 +293,6 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       if (f > c0) {
         const int fr = f - 1;
         const unsigned q1 = (unsigned)fr * vsd + col;
-        const double b1old = ld32(s.Vs_in, q1);
         double b1new = b1old + (0.0 - a.dt / g.dx[1][fr] * ((double)ez - (double)ezprev));
         if (s.rk == 2) b1new = s.wc * b1new + s.w0 * ld32((const double *)s.Vs0, q1);
         const double b1next = __shfl(b1new, lane + 1);
