@@ -22,6 +22,7 @@ void register_mhd3_plm_f64(Ops *t);
 void register_mhd3_ppm_f64(Ops *t);
 void register_mhd3_f32(Ops *t);
 void register_mhd_roe(Ops *t);
+void register_mhd2c3(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -35,6 +36,7 @@ Ops *ops_table() {
     register_mhd3_ppm_f64(t);
     register_mhd3_f32(t);
     register_mhd_roe(t);
+    register_mhd2c3(t);
     return t;
   }();
   return table;

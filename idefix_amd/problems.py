@@ -94,6 +94,144 @@ def orszag_tang_3d(g):
     return Vc, Vs
 
 
+class ParkMiller:
+    """The reference's portable generator idfx::randm (global.cpp:126-137): state <-
+    16807 * state mod (2^31 - 1) (exact in double), returns state / (2^31 - 1); rank 0
+    seeds it with 13763."""
+
+    def __init__(self, seed=13763):
+        self.state = int(seed)
+
+    def __call__(self):
+        self.state = (16807 * self.state) % 2147483647
+        return self.state / 2147483647.0
+
+
+def _half_space_modes(dim, kmin, kmax):
+    """Integer wave vectors n with kmin <= |n| <= kmax, one of each +-n pair, in a fixed
+    order (nz outermost, then ny, then nx)."""
+    out = []
+    r = range(-kmax, kmax + 1)
+    for nz in (range(0, kmax + 1) if dim == 3 else [0]):
+        for ny in (r if dim >= 2 else [0]):
+            for nx in r:
+                if nz < 0 or (nz == 0 and (ny < 0 or (ny == 0 and nx <= 0))):
+                    continue
+                k2 = nx * nx + ny * ny + nz * nz
+                if kmin * kmin <= k2 <= kmax * kmax:
+                    out.append((nx, ny, nz))
+    return out
+
+
+def _synth(n_int, dim, modes, coef, shift):
+    """Real field sum_n Re(coef_n exp(2 pi i n.(x + shift))) on the interior grid
+    x_j = j / N_d (per direction), through one inverse FFT; coef is complex per mode."""
+    shape = tuple(n_int[d] for d in range(dim))[::-1]  # (nz, ny, nx) order of the arrays
+    F = np.zeros(shape, dtype=np.complex128)
+    total = float(np.prod(shape))
+    for (nx, ny, nz), c in zip(modes, coef):
+        ph = np.exp(2j * np.pi * (nx * shift[0] + ny * shift[1] + nz * shift[2]))
+        idx = [nx % n_int[0]]
+        if dim >= 2:
+            idx.insert(0, ny % n_int[1])
+        if dim >= 3:
+            idx.insert(0, nz % n_int[2])
+        F[tuple(idx)] += total * c * ph
+    return np.real(np.fft.ifftn(F))
+
+
+def _wrap(g, field, dim):
+    """Periodic extension of an interior field ((nz,) ny, nx) to the index box
+    np_tot + 1 per direction (cells and faces): out[k, j, i] = field[(k - beg_z) mod nz,
+    (j - beg_y) mod ny, (i - beg_x) mod nx]."""
+    f3 = field.reshape((1,) * (3 - field.ndim) + field.shape)
+    idx = []
+    for d in (2, 1, 0):
+        n = f3.shape[2 - d]
+        idx.append((np.arange(g.np_tot[d] + 1) - g.beg[d]) % n)
+    return f3[np.ix_(idx[0], idx[1], idx[2])]
+
+
+def turbulence(g, seed=13763, kmin=1, kmax=8, v_rms=0.5, db_rms=0.1, b0=0.1, gamma=5.0 / 3.0):
+    """Decaying MHD turbulence on the periodic unit box (2-D or 3-D, three components):
+    rho = 1, p = 1/gamma, uniform Bx = b0, plus random-phase solenoidal velocity and
+    vector-potential modes with kmin <= |n| <= kmax and a shell spectrum E(k) ~ k^-2
+    (per-mode amplitude |n|^-(dim+1)/2 for v, one power more for A so that B = curl A
+    follows the same spectrum).  Draws from ParkMiller(seed), per mode in the order of
+    _half_space_modes: velocity direction (3 draws, centred), velocity phase, potential
+    direction (3 draws), potential phase.  The velocity is projected normal to n
+    (solenoidal); v is sampled at cell centres and A at cell corners, B = discrete curl
+    of A on the faces (MakeVsFromAmag, dataBlockHost.cpp:186-214), so div B = 0 to
+    round-off.  v and delta-B are scaled to the requested rms over the interior."""
+    dim = g.dim
+    assert _comps(g) == 3 and _is_mhd(g), "turbulence needs three-component MHD"
+    Vc, Vs = _empty(g)
+    n_int = [g.end[d] - g.beg[d] if d < dim else 1 for d in range(3)]
+    rng = ParkMiller(seed)
+    modes = _half_space_modes(dim, kmin, kmax)
+    cv = [[], [], []]
+    ca = [[], [], []]
+    for (nx, ny, nz) in modes:
+        k = np.array([nx, ny, nz], dtype=np.float64)
+        kk = np.sqrt(k @ k)
+        u = np.array([rng() - 0.5, rng() - 0.5, rng() - 0.5])
+        phi = 2.0 * np.pi * rng()
+        u = u - (u @ k) / (kk * kk) * k
+        amp = kk ** (-(dim + 1) / 2.0)
+        w = np.array([rng() - 0.5, rng() - 0.5, rng() - 0.5])
+        psi = 2.0 * np.pi * rng()
+        ampa = kk ** (-(dim + 1) / 2.0 - 1.0)
+        for c in range(3):
+            cv[c].append(amp * u[c] * np.exp(1j * phi))
+            ca[c].append(ampa * w[c] * np.exp(1j * psi))
+    h = [0.5 / n_int[d] if d < dim else 0.0 for d in range(3)]
+    v = [_synth(n_int, dim, modes, cv[c], h) for c in range(3)]
+    A = [_synth(n_int, dim, modes, ca[c], [0.0, 0.0, 0.0]) for c in range(3)]
+    vscale = v_rms / np.sqrt(np.mean(v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+    # discrete curl of corner A on the interior faces (periodic differences)
+    dxs = [1.0 / n_int[d] if d < dim else 1.0 for d in range(3)]
+    ax = {0: -1, 1: -2, 2: -3}  # numpy axis of direction d in (nz, ny, nx) / (ny, nx)
+
+    def dA(f, d):  # forward difference along direction d / width
+        if d >= dim:
+            return np.zeros_like(f)
+        return (np.roll(f, -1, axis=ax[d]) - f) / dxs[d]
+
+    bx = dA(A[2], 1) - dA(A[1], 2)
+    by = -dA(A[2], 0) + dA(A[0], 2)
+    if dim == 3:
+        bz = dA(A[1], 0) - dA(A[0], 1)
+    else:
+        # cell-centred Bz from the corner potential, averaged over the two edges per side
+        ay_x = dA(A[1], 0)
+        ax_y = dA(A[0], 1)
+        bz = 0.5 * (ay_x + np.roll(ay_x, -1, axis=-2)) - 0.5 * (ax_y + np.roll(ax_y, -1, axis=-1))
+    bscale = db_rms / np.sqrt(np.mean(bx ** 2 + by ** 2 + bz ** 2))
+    full = lambda f: _wrap(g, f, dim)[:g.np_tot[2], :g.np_tot[1], :g.np_tot[0]]
+    fullf = lambda f: _wrap(g, f, dim)
+    Vc[0] = 1.0
+    Vc[4] = 1.0 / gamma
+    for c in range(3):
+        Vc[1 + c] = vscale * full(v[c])
+    vz, vy, vx = Vs.shape[1:]
+    Vs[0] = b0 + bscale * fullf(bx)[:vz, :vy, :vx]
+    Vs[1] = bscale * fullf(by)[:vz, :vy, :vx]
+    if dim == 3:
+        Vs[2] = bscale * fullf(bz)[:vz, :vy, :vx]
+        Vc[7] = 0.0
+    else:
+        Vc[7] = bscale * full(bz)
+    # cell-centred B consistent with the faces (ReconstructVcField)
+    Vc[5] = 0.5 * (Vs[0][:g.np_tot[2], :g.np_tot[1], :g.np_tot[0]] +
+                   Vs[0][:g.np_tot[2], :g.np_tot[1], 1:g.np_tot[0] + 1])
+    Vc[6] = 0.5 * (Vs[1][:g.np_tot[2], :g.np_tot[1], :g.np_tot[0]] +
+                   Vs[1][:g.np_tot[2], 1:g.np_tot[1] + 1, :g.np_tot[0]])
+    if dim == 3:
+        Vc[7] = 0.5 * (Vs[2][:g.np_tot[2], :g.np_tot[1], :g.np_tot[0]] +
+                       Vs[2][1:g.np_tot[2] + 1, :g.np_tot[1], :g.np_tot[0]])
+    return Vc, Vs
+
+
 def _widths(g, d):
     """Cell widths along direction d over np_tot cells."""
     if hasattr(g, "dx"):

@@ -265,3 +265,24 @@ def test_orszag_tang_3d_reference_benchmark_config():
     o.set_boundaries()
     _cycle_both(o, g, 4)
     compare_state(o, g, what="OT3D")
+
+
+@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact"])
+@pytest.mark.parametrize("recon", ["plm", "ppm"])
+def test_turbulence_2d_three_components(emf, recon):
+    """2-D MHD with three velocity/field components (BX3 cell-centred), seeded turbulence,
+    fused 2-D schedule."""
+    kw = cfg_kwargs(2, (40, 36), emf=emf, recon=recon, cfl=0.5, comps=3)
+    o, g = make_pair(kw, problems.turbulence)
+    o.set_boundaries()
+    _cycle_both(o, g, 5)
+    compare_state(o, g, what="turb2d c3 %s %s" % (emf, recon))
+
+
+def test_turbulence_3d_ppm_rk3():
+    """3-D decaying turbulence, PPM, HLLD, RK3 (the 3-D turbulence configuration)."""
+    kw = cfg_kwargs(3, (16, 12, 14), emf="uct_contact", recon="ppm", cfl=0.5, nstages=3)
+    o, g = make_pair(kw, problems.turbulence)
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="turb3d ppm rk3")

@@ -1,0 +1,88 @@
+"""Initial-condition generators: the seeded turbulence field (Park-Miller draws, rms
+normalisation, solenoidal velocity, discrete div B) and the 3-D Orszag-Tang state."""
+import numpy as np
+import pytest
+
+from idefix_amd import problems
+from oracle import oracle as O
+
+
+def test_park_miller_published_check_value():
+    # Park & Miller (1988): from seed 1 the 10000th state of the minimal standard
+    # generator is 1043618065
+    r = problems.ParkMiller(1)
+    for _ in range(10000):
+        r()
+    assert r.state == 1043618065
+
+
+def test_park_miller_reference_seed_first_draw():
+    r = problems.ParkMiller()
+    x = r()
+    assert r.state == (16807 * 13763) % 2147483647
+    assert x == r.state / 2147483647.0
+
+
+def _grid(dim, nx, recon="plm"):
+    cfg = O.make_cfg(dim, 3, mhd=True, recon=recon, solver="hlld", emf="uct_contact",
+                     nstages=2, nx=nx, bc="periodic", gamma=5.0 / 3.0, cfl=0.4)
+    o = O.Oracle(cfg)
+    return o, O.OracleGrid(o)
+
+
+@pytest.mark.parametrize("dim,nx", [(2, (48, 40)), (3, (16, 12, 20))])
+def test_turbulence_normalisation_and_div_b(dim, nx):
+    o, g = _grid(dim, nx)
+    Vc, Vs = problems.turbulence(g)
+    o.Vc[...] = Vc
+    o.Vs[...] = Vs
+    o.set_boundaries()
+    I = o.interior(o.Vc)
+    assert np.sqrt(np.mean(I[1] ** 2 + I[2] ** 2 + I[3] ** 2)) == pytest.approx(0.5, rel=1e-12)
+    assert np.all(I[0] == 1.0) and np.all(I[4] == 1.0 / (5.0 / 3.0))
+    assert o.check_divb() < 1e-13
+    # delta-B rms from the interior faces, cell-averaged B mean carries the uniform Bx
+    assert np.mean(I[5]) == pytest.approx(0.1, abs=1e-12)
+
+
+def test_turbulence_velocity_is_solenoidal_spectrally():
+    # the velocity field's Fourier coefficients are normal to their wave vectors
+    o, g = _grid(3, (16, 16, 16))
+    Vc, _ = problems.turbulence(g, kmax=4)
+    I = o.interior(Vc)
+    V = [np.fft.fftn(I[1 + c]) for c in range(3)]
+    n = np.fft.fftfreq(16, 1.0 / 16)
+    NZ, NY, NX = np.meshgrid(n, n, n, indexing="ij")
+    # cell-centre shift multiplies every coefficient by a phase; k.V is unaffected
+    div = NX * V[0] + NY * V[1] + NZ * V[2]
+    assert np.max(np.abs(div)) < 1e-9 * np.max(np.abs(V[0]))
+
+
+def test_turbulence_spectrum_band():
+    # energy only in 1 <= |n| <= 8
+    o, g = _grid(2, (64, 64))
+    Vc, _ = problems.turbulence(g)
+    I = o.interior(Vc)[:, 0]
+    E = sum(np.abs(np.fft.fft2(I[1 + c])) ** 2 for c in range(3))
+    n = np.fft.fftfreq(64, 1.0 / 64)
+    NY, NX = np.meshgrid(n, n, indexing="ij")
+    k = np.sqrt(NX ** 2 + NY ** 2)
+    assert np.sum(E[(k < 1) | (k > 8)]) < 1e-20 * np.sum(E)
+
+
+def test_orszag_tang_3d_div_b_and_fields():
+    o, g = _grid(3, (16, 12, 10))
+    Vc, Vs = problems.orszag_tang_3d(g)
+    o.Vc[...] = Vc
+    o.Vs[...] = Vs
+    o.set_boundaries()
+    assert o.check_divb() < 1e-13
+    x, y, z = g.x
+    b, e = g.beg, g.end
+    # velocity: cell centres, test/MHD/OrszagTang3D/setup.cpp:238-240
+    I = o.interior(o.Vc)
+    Y = y[b[1]:e[1]][None, :, None]
+    X = x[b[0]:e[0]][None, None, :]
+    Z = z[b[2]:e[2]][:, None, None]
+    assert np.allclose(I[1], -np.sin(2 * np.pi * Y) + 0 * X + 0 * Z, rtol=0, atol=1e-15)
+    assert np.allclose(I[2], np.sin(2 * np.pi * X) + np.cos(2 * np.pi * Z) + 0 * Y, rtol=0, atol=1e-15)
