@@ -1,0 +1,173 @@
+"""CPU restatement of the learned reconstruction (test infrastructure: only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline may use it).
+
+The reference has no learned reconstruction; this module is the specification the HIP
+kernels are checked against.  One interface x_{i-1/2} along a sweep direction sees the
+6-cell stencil i-3 .. i+2 of the 8 primitives in the face frame
+(rho, v_n, v_t, v_b, p, B_n, B_t, B_b).
+
+Features (48): per variable, c = (V[i-1] + V[i]) / 2 and s = max_j |V_j - c| over the
+stencil; x_j = (V_j - c) / s (0 when s = 0); ordered [variable][stencil cell].
+
+Network: z1 = x W1 + b1 (48 -> 64), h1 = gelu(z1); z2 = h1 W2 + b2 (64 -> 64),
+h2 = gelu(z2); z3 = h2 W3 + b3 (64 -> 16), y = sigmoid(z3), with
+gelu(z) = z * sigmoid(1.702 z) (Hendrycks & Gimpel 2016, sigmoid form).
+
+Output: y[2v] = wL_v, y[2v+1] = wR_v in (0, 1) per variable v; the interface states are
+    vL_v = V_v[i-1] + wL_v (V_v[i] - V_v[i-1]),   vR_v = V_v[i] - wR_v (V_v[i] - V_v[i-1]),
+so both lie between the two adjacent cell values (monotone, and positive when the cells
+are).  Van Leer PLM is the member wL = dvm / (dvm + dvp) (monotone data) or 0 (extremum).
+
+Training target: the interface value t_v of a finer solution; loss per interface
+sum_v ((vL_v - t_v)/s_v)^2 + ((vR_v - t_v)/s_v)^2 (both terms 0 when s_v = 0), mean
+over the batch.
+
+Inference on the GPU runs with BF16 inputs, weights and hidden activations and FP32
+accumulation; `forward(..., bf16=True)` rounds at the same points.
+"""
+import numpy as np
+
+NF, NH, NO = 48, 64, 16
+GELU_K = 1.702
+
+
+def sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def gelu(z):
+    return z * sigmoid(GELU_K * z)
+
+
+def gelu_grad(z):
+    s = sigmoid(GELU_K * z)
+    return s + GELU_K * z * s * (1.0 - s)
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (round to nearest, ties to even), returned as
+    float32."""
+    a = np.asarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    rounding = ((u >> 16) & 1) + 0x7FFF
+    r = ((u + rounding) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32)
+    nan = np.isnan(a)
+    if np.any(nan):
+        out = out.copy()
+        out[nan] = np.nan
+    return out
+
+
+def init_params(seed=0):
+    """Glorot-uniform weights, zero biases (float64)."""
+    rng = np.random.default_rng(seed)
+
+    def glorot(n_in, n_out):
+        lim = np.sqrt(6.0 / (n_in + n_out))
+        return rng.uniform(-lim, lim, size=(n_in, n_out))
+
+    return {"W1": glorot(NF, NH), "b1": np.zeros(NH), "W2": glorot(NH, NH), "b2": np.zeros(NH),
+            "W3": glorot(NH, NO), "b3": np.zeros(NO)}
+
+
+def features(S):
+    """S: [..., 8, 6] stencil (variables x cells i-3..i+2).  Returns (x [..., 48], c, s)
+    with c, s of shape [..., 8]."""
+    S = np.asarray(S, dtype=np.float64)
+    c = 0.5 * (S[..., 2] + S[..., 3])
+    s = np.max(np.abs(S - c[..., None]), axis=-1)
+    safe = np.where(s > 0, s, 1.0)
+    x = np.where(s[..., None] > 0, (S - c[..., None]) / safe[..., None], 0.0)
+    return x.reshape(S.shape[:-2] + (NF,)), c, s
+
+
+def forward(P, x, bf16_mode=False):
+    """Returns (y, cache).  bf16_mode rounds x, the weights and h1, h2 to bfloat16 and
+    accumulates in float32, as the MFMA inference does."""
+    if bf16_mode:
+        f = np.float32
+        xb = bf16(x.astype(f))
+        W1, W2, W3 = (bf16(P[k].astype(f)) for k in ("W1", "W2", "W3"))
+        b1, b2, b3 = (P[k].astype(f) for k in ("b1", "b2", "b3"))
+        z1 = (xb.astype(np.float64) @ W1.astype(np.float64)).astype(f) + b1
+        h1 = bf16(gelu(z1).astype(f))
+        z2 = (h1.astype(np.float64) @ W2.astype(np.float64)).astype(f) + b2
+        h2 = bf16(gelu(z2).astype(f))
+        z3 = (h2.astype(np.float64) @ W3.astype(np.float64)).astype(f) + b3
+        y = sigmoid(z3.astype(f)).astype(f)
+        return y, None
+    z1 = x @ P["W1"] + P["b1"]
+    h1 = gelu(z1)
+    z2 = h1 @ P["W2"] + P["b2"]
+    h2 = gelu(z2)
+    z3 = h2 @ P["W3"] + P["b3"]
+    y = sigmoid(z3)
+    return y, (x, z1, h1, z2, h2, z3, y)
+
+
+def face_states(S, y):
+    """Interface states (vL, vR) [..., 8] from the stencil S [..., 8, 6] and outputs y."""
+    a = S[..., 2]
+    b = S[..., 3]
+    wl = y[..., 0::2]
+    wr = y[..., 1::2]
+    return a + wl * (b - a), b - wr * (b - a)
+
+
+def loss_and_grad(P, x, tau):
+    """Loss (mean over the batch) and parameter gradients.  x: [B, 48] features,
+    tau: [B, 8] normalised targets (t - c) / s (0 where s = 0)."""
+    B = x.shape[0]
+    y, (x, z1, h1, z2, h2, z3, _) = forward(P, x)
+    xv = x.reshape(B, 8, 6)
+    d = xv[:, :, 3] - xv[:, :, 2]
+    eL = xv[:, :, 2] + y[:, 0::2] * d - tau
+    eR = xv[:, :, 3] - y[:, 1::2] * d - tau
+    loss = np.sum(eL ** 2 + eR ** 2) / B
+    dy = np.zeros_like(y)
+    dy[:, 0::2] = 2.0 * eL * d / B
+    dy[:, 1::2] = -2.0 * eR * d / B
+    dz3 = dy * y * (1.0 - y)
+    g = {"W3": h2.T @ dz3, "b3": dz3.sum(0)}
+    dh2 = dz3 @ P["W3"].T
+    dz2 = dh2 * gelu_grad(z2)
+    g["W2"] = h1.T @ dz2
+    g["b2"] = dz2.sum(0)
+    dh1 = dz2 @ P["W2"].T
+    dz1 = dh1 * gelu_grad(z1)
+    g["W1"] = x.T @ dz1
+    g["b1"] = dz1.sum(0)
+    return loss, g
+
+
+def adam_step(P, g, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015) in place; t is the 1-based step count."""
+    for k in P:
+        m[k] = b1 * m[k] + (1 - b1) * g[k]
+        v[k] = b2 * v[k] + (1 - b2) * g[k] * g[k]
+        mh = m[k] / (1 - b1 ** t)
+        vh = v[k] / (1 - b2 ** t)
+        P[k] = P[k] - lr * mh / (np.sqrt(vh) + eps)
+    return P
+
+
+def van_leer_weights(S):
+    """The weights that reproduce van Leer PLM (slopeLimiter.hpp:54-57) for the two states
+    of the interface: wL from cell i-1's slope, wR from cell i's slope."""
+    a, b = S[..., 1], S[..., 2]
+    c, d = S[..., 3], S[..., 4]
+    jump = c - b
+
+    def vl(dp, dm):
+        return np.where(dp * dm > 0, 2.0 * dp * dm / np.where(dp + dm != 0, dp + dm, 1.0), 0.0)
+
+    slope_l = vl(c - b, b - a)   # cell i-1
+    slope_r = vl(d - c, c - b)   # cell i
+    safe = np.where(jump != 0, jump, 1.0)
+    wl = np.where(jump != 0, 0.5 * slope_l / safe, 0.0)
+    wr = np.where(jump != 0, 0.5 * slope_r / safe, 0.0)
+    y = np.empty(S.shape[:-2] + (16,))
+    y[..., 0::2] = wl
+    y[..., 1::2] = wr
+    return y
