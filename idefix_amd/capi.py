@@ -14,7 +14,7 @@ OK = 0
 E_ARGS, E_LAUNCH, E_NAN, E_DT_SMALL, E_FIRST_DT, E_ALLOC = 1, 2, 3, 4, 5, 6
 
 SOLVER = {"hllc": 1, "hlld": 2, "roe": 3}
-RECON = {"constant": 1, "plm": 2, "linear": 2, "ppm": 4, "parabolic": 4}
+RECON = {"constant": 1, "plm": 2, "linear": 2, "ppm": 4, "parabolic": 4, "mlp": 8}
 EMF = {"arithmetic": 0, "uct0": 1, "uct_contact": 2, "uct_hll": 3, "uct_hlld": 4}
 BC = {"periodic": 0, "outflow": 1, "reflective": 2, "userdef": 3}
 PREC = {"fp64": 0, "fp32": 1}
@@ -29,6 +29,9 @@ EXPORTS = [
     "idfx_sweep", "idfx_corner_emf", "idfx_evolve_mag_field", "idfx_evolve_stage",
     "idfx_compute_timestep", "idfx_add_and_store", "idfx_check_divb", "idfx_stage",
     "idfx_cycle", "idfx_ctx_set_fusion", "idfx_ctx_set_profiling", "idfx_ctx_region_times",
+    "idfx_mlp_param_count", "idfx_mlp_create", "idfx_mlp_destroy", "idfx_mlp_set_params",
+    "idfx_mlp_get_params", "idfx_mlp_loss_grad", "idfx_mlp_train_step", "idfx_mlp_loss",
+    "idfx_ctx_set_mlp_params", "idfx_mlp_faces",
 ]
 
 REGIONS = ["boundaries", "sweep_x", "sweep_y", "sweep_z", "corner_emf", "evolve_b",
@@ -87,6 +90,16 @@ def load():
         "idfx_ctx_set_fusion": [vp, i],
         "idfx_ctx_set_profiling": [vp, i],
         "idfx_ctx_region_times": [vp, dp, ctypes.POINTER(ctypes.c_long), i],
+        "idfx_mlp_param_count": [],
+        "idfx_mlp_create": [i, ctypes.POINTER(vp)],
+        "idfx_mlp_destroy": [vp],
+        "idfx_mlp_set_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "idfx_mlp_get_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "idfx_mlp_loss_grad": [vp, vp, vp, i, dp, ctypes.POINTER(ctypes.c_float), vp],
+        "idfx_mlp_train_step": [vp, vp, vp, i, ctypes.c_float, vp],
+        "idfx_mlp_loss": [vp, dp, vp],
+        "idfx_ctx_set_mlp_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "idfx_mlp_faces": [vp, i, vp, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

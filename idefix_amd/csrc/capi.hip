@@ -23,6 +23,7 @@ void register_mhd3_ppm_f64(Ops *t);
 void register_mhd3_f32(Ops *t);
 void register_mhd_roe(Ops *t);
 void register_mhd2c3(Ops *t);
+void register_mlp(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -37,6 +38,7 @@ Ops *ops_table() {
     register_mhd3_f32(t);
     register_mhd_roe(t);
     register_mhd2c3(t);
+    register_mlp(t);
     return t;
   }();
   return table;
@@ -81,6 +83,12 @@ struct idfx_ctx {
   size_t ev_used = 0;
   double region_ms[IDFX_NREGIONS] = {};
   long region_n[IDFX_NREGIONS] = {};
+  // learned limiter (IDFX_RECON_MLP): interface weights per direction and the BF16 network
+  bool mlp = false;
+  float *mlpW[3] = {nullptr, nullptr, nullptr};
+  short *mlp_wt = nullptr;   // w1t | w2t | w3t, transposed BF16
+  float *mlp_b = nullptr;    // b1 | b2 | b3
+  MlpNet net{};
   bool fused2d_ok = false;   // configuration supports it
   bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
   double *Vc2 = nullptr, *Vs2 = nullptr;
@@ -166,6 +174,7 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
   a.emf = c->d.emf;
   a.flags = flags;
   a.dtmax = c->dtmax;
+  a.mlpw = c->mlp ? c->mlpW[dir] : nullptr;
   for (int d = 0; d < 3; d++) a.ext[d] = 0;
   // uniform grid: every cell has the volume of cell (beg); dt/dV computed once here with
   // the device's association (dx1 * dx2) * dx3
@@ -190,6 +199,14 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
   return a;
 }
 
+// Learned limiter: interface weights of direction dir from Vc into the context's array
+// (or W when given), before a sweep of that direction.
+int mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, hipStream_t s) {
+  if (!c->mlp) return IDFX_OK;
+  return c->ops->mlp_infer(dir, c->g, Vc, c->net, W ? W : c->mlpW[dir], s) == hipSuccess
+             ? IDFX_OK : IDFX_E_LAUNCH;
+}
+
 bool needs_unfused(const idfx_ctx *c) {
   // the 1-D Roe flux keeps the previous flux at strong shocks (roeHD.hpp:315-328)
   return c->d.dim == 1 && !c->d.mhd && c->d.solver == IDFX_SOLVER_ROE;
@@ -199,6 +216,7 @@ int do_sweep_unfused(idfx_ctx *c, int dir, double dt, const double *Vc, const do
                      double *Uc, double *InvDt, double *Flux, hipStream_t s) {
   SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, 0);
   a.Flux = Flux;
+  IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
   IDFX_TRY(check(c->ops->sweep(dir, false, c->g, a, c->emf(), c->target_waves, s)));
   IDFX_TRY(check(c->ops->correct_flux(dir, c->g, Flux, s)));
   IDFX_TRY(check(c->ops->rhs(dir, c->g, Flux, c->cMax, Uc, a.InvDt, dt, s)));
@@ -287,7 +305,7 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   *out = nullptr;
   const idfx_desc &d = *desc;
   if (d.dim < 1 || d.dim > 3 || d.comps < d.dim || d.comps > 3) return IDFX_E_ARGS;
-  if (d.recon != 1 && d.recon != 2 && d.recon != 4) return IDFX_E_ARGS;
+  if (d.recon != 1 && d.recon != 2 && d.recon != 4 && d.recon != IDFX_RECON_MLP) return IDFX_E_ARGS;
   if (d.mhd && d.solver != IDFX_SOLVER_HLLD && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
   if (!d.mhd && d.solver != IDFX_SOLVER_HLLC && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
   if (d.emf < 0 || d.emf > 4) return IDFX_E_ARGS;
@@ -349,6 +367,19 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   if (st == IDFX_OK && needs_unfused(c)) st = alloc_n(c, &c->Flux, c->n3 * c->nvar);
   if (st == IDFX_OK) st = alloc_n(c, &c->dtmax, 1);
   if (st == IDFX_OK) st = alloc_n(c, &c->divb, 1);
+  if (st == IDFX_OK && d.recon == IDFX_RECON_MLP) {
+    c->mlp = true;
+    for (int q = 0; q < d.dim && st == IDFX_OK; q++) st = alloc_n(c, &c->mlpW[q], 16 * c->n3);
+    if (st == IDFX_OK) st = alloc_n(c, &c->mlp_wt, 64 * 48 + 64 * 64 + 16 * 64);
+    if (st == IDFX_OK) st = alloc_n(c, &c->mlp_b, 64 + 64 + 16);
+    c->net.w1t = c->mlp_wt;
+    c->net.w2t = c->mlp_wt + 64 * 48;
+    c->net.w3t = c->mlp_wt + 64 * 48 + 64 * 64;
+    c->net.b1 = c->mlp_b;
+    c->net.b2 = c->mlp_b + 64;
+    c->net.b3 = c->mlp_b + 128;
+    c->net.valid = 0;
+  }
   if (st == IDFX_OK && d.mhd && d.dim == 2 && ops->fused2d_y && !needs_unfused(c)) {
     bool user = false;
     for (int q = 0; q < 2; q++)
@@ -413,6 +444,10 @@ int idfx_ctx_array(idfx_ctx *c, int which, double **ptr) {
     *ptr = c->cMax;
     return IDFX_OK;
   }
+  if (which >= 25 && which <= 27) {  // learned-limiter interface weights (float [16][n3])
+    *ptr = reinterpret_cast<double *>(c->mlp ? c->mlpW[which - 25] : nullptr);
+    return *ptr ? IDFX_OK : IDFX_E_ARGS;
+  }
   double *const arr[24] = {c->e64.ex,  c->e64.ey,  c->e64.ez,  c->e64.ezi, c->e64.ezj, c->e64.exj,
                            c->e64.exk, c->e64.eyi, c->e64.eyk, c->e64.svx, c->e64.svy, c->e64.svz,
                            c->e64.axL, c->e64.axR, c->e64.ayL, c->e64.ayR, c->e64.azL, c->e64.azR,
@@ -467,6 +502,7 @@ int idfx_calc_flux(idfx_ctx *c, int dir, const double *Vc, const double *Vs, dou
     return IDFX_E_ARGS;
   SweepArgs a = sweep_args(c, dir, 0.0, Vc, Vs, nullptr, nullptr, 0);
   a.Flux = Flux;
+  IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, (hipStream_t)stream));
   return check(c->ops->sweep(dir, false, c->g, a, c->emf(), c->target_waves, (hipStream_t)stream));
 }
 
@@ -484,6 +520,7 @@ int idfx_sweep(idfx_ctx *c, int dir, double dt, const double *Vc, const double *
   if (needs_unfused(c)) return IDFX_E_ARGS;
   const int f = flags & (SW_INIT_UC | SW_SAVE_UC0 | SW_INVDT_SET | SW_INVDT_ADD);
   SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, f);
+  IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, (hipStream_t)stream));
   return check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, (hipStream_t)stream));
 }
 
@@ -507,6 +544,7 @@ int idfx_evolve_stage(idfx_ctx *c, double t, double dt, const double *Vc, double
       IDFX_TRY(do_sweep_unfused(c, dir, dt, Vc, Vs, Uc, InvDt, c->Flux, s));
     } else {
       SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, SW_INVDT_ADD);
+      IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
       IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
     }
   }
@@ -596,6 +634,7 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
                            InvDt ? InvDt : c->InvDt, c->dtmax);
     } else {
       SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, flags);
+      IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
       IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
     }
   }
@@ -688,6 +727,45 @@ int idfx_ctx_region_times(idfx_ctx *c, double *ms, long *count, int n) {
     c->region_n[q] = 0;
   }
   return IDFX_OK;
+}
+
+static unsigned short host_bf16(float f) {
+  unsigned u;
+  std::memcpy(&u, &f, sizeof(u));
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
+  const unsigned rounding = ((u >> 16) & 1u) + 0x7fffu;  // round to nearest even
+  return (unsigned short)((u + rounding) >> 16);
+}
+
+int idfx_ctx_set_mlp_params(idfx_ctx *c, const float *p) {
+  if (!c || !c->mlp) return IDFX_E_ARGS;
+  if (!p) {
+    c->net.valid = 0;
+    return IDFX_OK;
+  }
+  // packed [in][out] FP32 -> transposed [out][in] BF16
+  std::vector<unsigned short> wt(64 * 48 + 64 * 64 + 16 * 64);
+  std::vector<float> b(64 + 64 + 16);
+  const int W1 = 0, B1 = 3072, W2 = 3136, B2 = 7232, W3 = 7296, B3 = 8320;
+  for (int o = 0; o < 64; o++)
+    for (int i = 0; i < 48; i++) wt[o * 48 + i] = host_bf16(p[W1 + i * 64 + o]);
+  for (int o = 0; o < 64; o++)
+    for (int i = 0; i < 64; i++) wt[3072 + o * 64 + i] = host_bf16(p[W2 + i * 64 + o]);
+  for (int o = 0; o < 16; o++)
+    for (int i = 0; i < 64; i++) wt[3072 + 4096 + o * 64 + i] = host_bf16(p[W3 + i * 16 + o]);
+  for (int q = 0; q < 64; q++) b[q] = p[B1 + q];
+  for (int q = 0; q < 64; q++) b[64 + q] = p[B2 + q];
+  for (int q = 0; q < 16; q++) b[128 + q] = p[B3 + q];
+  if (hipMemcpy(c->mlp_wt, wt.data(), wt.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->mlp_b, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return IDFX_E_LAUNCH;
+  c->net.valid = 1;
+  return IDFX_OK;
+}
+
+int idfx_mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, void *stream) {
+  if (!c || !c->mlp || !Vc || dir < 0 || dir >= c->d.dim) return IDFX_E_ARGS;
+  return mlp_faces(c, dir, Vc, W, (hipStream_t)stream);
 }
 
 int idfx_ctx_set_fusion(idfx_ctx *c, int level) {

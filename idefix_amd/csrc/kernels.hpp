@@ -64,6 +64,7 @@ struct SweepArgs {
   unsigned long long *dtmax;  // bit pattern of max(InvDt) (non-negative doubles)
   int uniform;    // all cells share one volume: dtdV below equals dt/dV(k,j,i)
   double dtdV;
+  const float *mlpw;  // learned-limiter interface weights of this direction (RECON_MLP)
 };
 
 // Element i of an array whose base is uniform across the wave, addressed as base + 32-bit
@@ -181,7 +182,29 @@ __device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr) {
 }
 template <class C>
 constexpr int window() {
-  return C::ORDER == RECON_CONSTANT ? 1 : (C::ORDER == RECON_PLM ? 3 : 5);
+  return C::ORDER == RECON_CONSTANT ? 1 : ((C::ORDER == RECON_PLM || C::ORDER == RECON_MLP) ? 3 : 5);
+}
+
+// Learned-limiter reconstruction of one cell from the window V(c-1), V(c), V(c+1) and
+// the interface weights W[q][face] (float, SoA with stride sv): vl, the right state of
+// face c (index pc), from wR of that face; vr, the left state of face c+1 (index
+// pc + stride), from its wL.  q is the face-frame index of variable n.
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void recon_mlp(const T *w, int n, const float *W, long long sv,
+                                          unsigned pc, unsigned stride, T &vl, T &vr) {
+  int q = 0;
+  if (n == C::RHO) q = 0;
+  else if (n == C::template Xn<DIR>()) q = 1;
+  else if (n == C::template Xt<DIR>()) q = 2;
+  else if (n == C::template Xb<DIR>()) q = 3;
+  else if (n == C::PRS) q = 4;
+  else if (n == C::template BXn<DIR>()) q = 5;
+  else if (n == C::template BXt<DIR>()) q = 6;
+  else q = 7;
+  const T wr = (T)ld32(W + (2 * q + 1) * sv, pc);
+  const T wl = (T)ld32(W + (2 * q) * sv, pc + stride);
+  vl = w[1] - wr * (w[1] - w[0]);
+  vr = w[1] + wl * (w[2] - w[1]);
 }
 
 // Conserved state at the start of the sweep for one cell: PrimToCons(Vc) (FP64).
@@ -305,7 +328,12 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
       T wv[NW];
 #pragma unroll
       for (int s = 0; s < NW; s++) wv[s] = (T)ld32(a.Vc + n * g.sv, row + (unsigned)(cc - H + s));
-      recon_cell<C>(wv, n, vl[n], vr[n]);
+      if constexpr (C::ORDER == RECON_MLP) {
+        if (C::MHD && n == C::BX1) { vl[n] = vr[n] = wv[H]; continue; }  // normal field: Vs
+        recon_mlp<C, IDIR>(wv, n, a.mlpw, g.sv, row + (unsigned)cc, 1u, vl[n], vr[n]);
+      } else {
+        recon_cell<C>(wv, n, vl[n], vr[n]);
+      }
     }
     T vL[NV], vR[NV];
 #pragma unroll
@@ -415,7 +443,12 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
 #pragma unroll
       for (int s = 0; s < NW; s++) wv[s] = win[s][n];
       T dummy;
-      recon_cell<C>(wv, n, dummy, carry[n]);
+      if constexpr (C::ORDER == RECON_MLP) {
+        if (C::MHD && n == C::template BXn<DIR>()) { carry[n] = wv[H]; continue; }
+        recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)(c0 - 1) * sd, sd, dummy, carry[n]);
+      } else {
+        recon_cell<C>(wv, n, dummy, carry[n]);
+      }
     }
     for (int f = c0; f <= c1; f++) {
       // shift the window to cell f and load its leading row
@@ -432,7 +465,15 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         T wv[NW];
 #pragma unroll
         for (int s = 0; s < NW; s++) wv[s] = win[s][n];
-        recon_cell<C>(wv, n, vR[n], vr_next[n]);
+        if constexpr (C::ORDER == RECON_MLP) {
+          if (C::MHD && n == C::template BXn<DIR>()) {
+            vR[n] = vr_next[n] = wv[H];
+          } else {
+            recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)f * sd, sd, vR[n], vr_next[n]);
+          }
+        } else {
+          recon_cell<C>(wv, n, vR[n], vr_next[n]);
+        }
         vL[n] = carry[n];
       }
       if constexpr (C::MHD) {

@@ -8,6 +8,7 @@
 
 #include "fused2d.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 namespace idfx {
 
@@ -40,6 +41,9 @@ struct Ops {
   // 2-D MHD only: y sweep + corner EMF + face-B update + ConsToPrim in one kernel
   hipError_t (*fused2d_y)(const GridDesc &g, const SweepArgs &a, const void *emf,
                           Fused2dArgs f, int target_waves, hipStream_t s) = nullptr;
+  // learned limiter (RECON_MLP): interface weights of direction dir
+  hipError_t (*mlp_infer)(int dir, const GridDesc &g, const double *Vc, const MlpNet &net,
+                          float *W, hipStream_t s) = nullptr;
 };
 
 inline unsigned blocks_for(long long n, int bs = 256) {
@@ -335,6 +339,40 @@ hipError_t launch_fused2d_y(const GridDesc &g, const SweepArgs &a, const void *e
   return hipErrorInvalidValue;
 }
 
+// Interface weights for every face of direction DIR whose stencil lies in the array:
+// faces beg .. end along DIR, every transverse index (the sweeps' extended rows included).
+template <class C, int DIR>
+hipError_t launch_mlp_dir(const GridDesc &g, const double *Vc, const MlpNet &net, float *W,
+                          hipStream_t s) {
+  int lo[3], n[3];
+  for (int d = 0; d < 3; d++) {
+    if (d == DIR) {
+      lo[d] = g.beg[d];
+      n[d] = g.end[d] + 1 - g.beg[d];
+    } else {
+      lo[d] = 0;
+      n[d] = g.nt[d];
+    }
+  }
+  const long long waves = (long long)((n[0] + 63) / 64) * n[1] * n[2];
+  hipLaunchKernelGGL((k_mlp_infer<C, DIR>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g, Vc,
+                     net, W, lo[0], lo[1], lo[2], n[0], n[1], n[2]);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_mlp(int dir, const GridDesc &g, const double *Vc, const MlpNet &net,
+                      float *W, hipStream_t s) {
+  if constexpr (C::ORDER == RECON_MLP) {
+    if (dir == 0) return launch_mlp_dir<C, 0>(g, Vc, net, W, s);
+    if constexpr (C::DIM >= 2)
+      if (dir == 1) return launch_mlp_dir<C, 1>(g, Vc, net, W, s);
+    if constexpr (C::DIM >= 3)
+      if (dir == 2) return launch_mlp_dir<C, 2>(g, Vc, net, W, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 template <class C, typename T>
 Ops make_ops() {
   Ops o;
@@ -352,15 +390,17 @@ Ops make_ops() {
   o.correct_flux = &launch_correct_flux<C>;
   o.rhs = &launch_rhs<C>;
   o.divb = &launch_divb<C>;
-  if constexpr (C::MHD && C::DIM == 2) o.fused2d_y = &launch_fused2d_y<C, T>;
+  if constexpr (C::MHD && C::DIM == 2 && C::ORDER != RECON_MLP)
+    o.fused2d_y = &launch_fused2d_y<C, T>;
+  if constexpr (C::ORDER == RECON_MLP) o.mlp_infer = &launch_mlp<C>;
   return o;
 }
 
 // Registry key: one entry per built configuration and precision.
 inline int ops_key(int dim, int comps, int mhd, int order, int solver, int fp32) {
-  return ((((dim * 4 + comps) * 2 + mhd) * 8 + order) * 4 + solver) * 2 + fp32;
+  return ((((dim * 4 + comps) * 2 + mhd) * 16 + order) * 4 + solver) * 2 + fp32;
 }
-constexpr int kOpsTableSize = (((3 * 4 + 3) * 2 + 1) * 8 + 7) * 4 * 2 + 8;
+constexpr int kOpsTableSize = (((3 * 4 + 3) * 2 + 1) * 16 + 15) * 4 * 2 + 8;
 Ops *ops_table();  // defined in capi.hip
 
 }  // namespace idfx

@@ -15,7 +15,8 @@
 namespace idfx {
 
 constexpr int IDIR = 0, JDIR = 1, KDIR = 2;
-enum Recon { RECON_CONSTANT = 1, RECON_PLM = 2, RECON_PPM = 4 };
+// RECON_MLP: the learned limiter (no counterpart in the reference), 3 ghost cells
+enum Recon { RECON_CONSTANT = 1, RECON_PLM = 2, RECON_PPM = 4, RECON_MLP = 8 };
 enum Solver { SOLVER_HLLC = 1, SOLVER_HLLD = 2, SOLVER_ROE = 3 };
 enum EmfScheme { EMF_ARITHMETIC = 0, EMF_UCT0 = 1, EMF_UCT_CONTACT = 2, EMF_UCT_HLL = 3,
                  EMF_UCT_HLLD = 4 };

@@ -206,6 +206,25 @@ class DataBlock:
                    "stage")
         return newdt.value
 
+    def set_mlp_params(self, params):
+        """Loads packed FP32 learned-limiter parameters (numpy, idfx_mlp_param_count()
+        floats) for BF16 inference, or unloads them with None (van Leer weights)."""
+        if params is None:
+            capi.check(self._L.idfx_ctx_set_mlp_params(self.ctx, None), "set_mlp_params")
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        capi.check(self._L.idfx_ctx_set_mlp_params(
+            self.ctx, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "set_mlp_params")
+
+    def mlp_faces(self, d):
+        """Interface weights of direction d from the current Vc: torch float32
+        [16, nz, ny, nx] (face index = cell right of the face)."""
+        nxt, nyt, nzt = self.np_tot
+        W = self.torch.zeros((16, nzt, nyt, nxt), dtype=self.torch.float32, device=self.device)
+        capi.check(self._L.idfx_mlp_faces(self.ctx, d, self._p(self.Vc), ctypes.c_void_p(W.data_ptr()),
+                                          self._stream()), "mlp_faces")
+        return W
+
     def set_fusion(self, level):
         """Kernel schedule of cycle(): 0 = one kernel per reference step, 1 = fused 2-D MHD
         second half (default where supported).  Results are identical."""

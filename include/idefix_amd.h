@@ -40,6 +40,9 @@ extern "C" {
 #define IDFX_RECON_CONSTANT 1
 #define IDFX_RECON_PLM 2
 #define IDFX_RECON_PPM 4
+/* No counterpart in the reference: the learned limiter (per-interface MLP weights inside
+ * the flux form, van Leer PLM fallback), 3 ghost cells, three-component MHD only. */
+#define IDFX_RECON_MLP 8
 /* [Hydro] emf (constrainedTransport.hpp:156-176) */
 #define IDFX_EMF_ARITHMETIC 0
 #define IDFX_EMF_UCT0 1
@@ -91,7 +94,9 @@ int idfx_ctx_layout(const idfx_ctx *ctx, int *np_tot, int *nghost, int *nvar, in
 const char *idfx_strerror(int status);
 /* Device pointer of a context-owned array for user hooks: which = 0..23 in the order
  * ex ey ez ezi ezj exj exk eyi eyk svx svy svz axL axR ayL ayR azL azR dxL dxR dyL dyR
- * dzL dzR (constrainedTransport.hpp members), 24 = cMax (fluid.hpp cMax).  FP64 only. */
+ * dzL dzR (constrainedTransport.hpp members), 24 = cMax (fluid.hpp cMax).  FP64 only.
+ * 25..27: the learned limiter's interface weights of x, y, z (float [16][np_tot cells],
+ * returned through the double* slot; IDFX_RECON_MLP contexts only). */
 int idfx_ctx_array(idfx_ctx *ctx, int which, double **ptr);
 
 /* Boundary::SetBoundaries (boundary.hpp:239-282) for the built-in side types, with
@@ -195,6 +200,41 @@ enum {
  * milliseconds and launch count of the first n regions, and resets the totals. */
 int idfx_ctx_set_profiling(idfx_ctx *ctx, int on);
 int idfx_ctx_region_times(idfx_ctx *ctx, double *ms, long *count, int n);
+
+/* ---- learned limiter (no counterpart in the reference) ------------------------------
+ * An interface x_{i-1/2} sees the stencil i-3 .. i+2 of the 8 face-frame primitives
+ * (rho, v_n, v_t, v_b, p, B_n, B_t, B_b).  Features per variable: (V_j - c) / s with
+ * c = (V[i-1] + V[i]) / 2, s = max_j |V_j - c|.  Network 48 -> 64 -> 64 -> 16, hidden
+ * activation z * sigmoid(1.702 z), sigmoid outputs wL_v (2v) and wR_v (2v+1); states
+ * vL = V[i-1] + wL (V[i] - V[i-1]), vR = V[i] - wR (V[i] - V[i-1]).  Parameters are
+ * packed FP32, row-major [in][out]: W1 48x64, b1 64, W2 64x64, b2 64, W3 64x16, b3 16
+ * (idfx_mlp_param_count() floats).  Callers: setups that run IDFX_RECON_MLP and the
+ * training pipeline that fits the network to block-averaged high-resolution runs. */
+typedef struct idfx_mlp idfx_mlp;
+int idfx_mlp_param_count(void);
+/* Training state (FP32 parameters, Adam moments, activations for batches up to
+ * max_batch rows).  Allocates device memory. */
+int idfx_mlp_create(int max_batch, idfx_mlp **out);
+int idfx_mlp_destroy(idfx_mlp *h);
+/* Host arrays of idfx_mlp_param_count() floats; set_params also resets Adam. */
+int idfx_mlp_set_params(idfx_mlp *h, const float *params);
+int idfx_mlp_get_params(idfx_mlp *h, float *params);
+/* X: device [B][48] features, tau: device [B][8] normalised targets (t - c) / s.
+ * loss_grad: blocking; mean loss and the gradient of every parameter (host arrays).
+ * train_step: one Adam step (lr, beta 0.9 / 0.999, eps 1e-8), asynchronous;
+ * idfx_mlp_loss returns the loss of the last step's batch (blocking). */
+int idfx_mlp_loss_grad(idfx_mlp *h, const float *X, const float *tau, int B, double *loss,
+                       float *grad, void *stream);
+int idfx_mlp_train_step(idfx_mlp *h, const float *X, const float *tau, int B, float lr,
+                        void *stream);
+int idfx_mlp_loss(idfx_mlp *h, double *loss, void *stream);
+/* Loads packed FP32 parameters (host) into an IDFX_RECON_MLP context as BF16 for MFMA
+ * inference; params == NULL unloads them (every interface then takes van Leer weights). */
+int idfx_ctx_set_mlp_params(idfx_ctx *ctx, const float *params);
+/* Interface weights of direction dir for the faces beg..end along dir (every transverse
+ * index) from Vc, into W (device float [16][np_tot cells], or the context's own array
+ * when W is NULL).  The sweeps of an IDFX_RECON_MLP context call this themselves. */
+int idfx_mlp_faces(idfx_ctx *ctx, int dir, const double *Vc, float *W, void *stream);
 
 #ifdef __cplusplus
 }

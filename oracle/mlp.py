@@ -13,14 +13,20 @@ Network: z1 = x W1 + b1 (48 -> 64), h1 = gelu(z1); z2 = h1 W2 + b2 (64 -> 64),
 h2 = gelu(z2); z3 = h2 W3 + b3 (64 -> 16), y = sigmoid(z3), with
 gelu(z) = z * sigmoid(1.702 z) (Hendrycks & Gimpel 2016, sigmoid form).
 
-Output: y[2v] = wL_v, y[2v+1] = wR_v in (0, 1) per variable v; the interface states are
-    vL_v = V_v[i-1] + wL_v (V_v[i] - V_v[i-1]),   vR_v = V_v[i] - wR_v (V_v[i] - V_v[i-1]),
-so both lie between the two adjacent cell values (monotone, and positive when the cells
-are).  Van Leer PLM is the member wL = dvm / (dvm + dvp) (monotone data) or 0 (extremum).
+Output: y in (0, 1)^16 are the limiter weights w: w[2v] = wL_v, w[2v+1] = wR_v per
+variable v, and the interface states are
+    vL_v = V_v[i-1] + wL_v (V_v[i] - V_v[i-1]),   vR_v = V_v[i] - wR_v (V_v[i] - V_v[i-1]).
+Van Leer PLM is the member wL = dvm / (dvm + dvp) (monotone data) or 0 (extremum).  Each
+variable's pair of weights is checked against the TVD region of slope limiters (Sweby
+1984: limited slope between 0 and twice each one-sided difference): the left state's
+deviation wL (V[i] - V[i-1]) must have the sign of V[i-1] - V[i-2] and not exceed it in
+magnitude (or be 0), the right state's deviation wR (V[i] - V[i-1]) likewise against
+V[i+1] - V[i], and 0 <= w <= 1.  A variable that fails, or has non-finite weights, takes
+the van Leer weights.
 
 Training target: the interface value t_v of a finer solution; loss per interface
 sum_v ((vL_v - t_v)/s_v)^2 + ((vR_v - t_v)/s_v)^2 (both terms 0 when s_v = 0), mean
-over the batch.
+over the batch, with the unchecked weights.
 
 Inference on the GPU runs with BF16 inputs, weights and hidden activations and FP32
 accumulation; `forward(..., bf16=True)` rounds at the same points.
@@ -106,13 +112,41 @@ def forward(P, x, bf16_mode=False):
     return y, (x, z1, h1, z2, h2, z3, y)
 
 
-def face_states(S, y):
-    """Interface states (vL, vR) [..., 8] from the stencil S [..., 8, 6] and outputs y."""
+W_SCALE, W_SHIFT = 1.0, 0.0
+
+
+def weights(y):
+    """Limiter weights from the network outputs (the identity: w = y)."""
+    return W_SCALE * y + W_SHIFT
+
+
+def face_states(S, w):
+    """Interface states (vL, vR) [..., 8] from the stencil S [..., 8, 6] and weights w."""
     a = S[..., 2]
     b = S[..., 3]
-    wl = y[..., 0::2]
-    wr = y[..., 1::2]
+    wl = w[..., 0::2]
+    wr = w[..., 1::2]
     return a + wl * (b - a), b - wr * (b - a)
+
+
+def checked_weights(S, w):
+    """The weights the solver uses: per variable, van Leer where a weight leaves the TVD
+    region or is not finite."""
+    a, b, c, d = S[..., 1], S[..., 2], S[..., 3], S[..., 4]
+    jump = c - b
+    wl = w[..., 0::2]
+    wr = w[..., 1::2]
+    dl = wl * jump
+    dr = wr * jump
+    okl = (wl >= 0) & (wl <= 1) & ((dl == 0) | ((dl * (b - a) > 0) & (np.abs(dl) <= np.abs(b - a))))
+    okr = (wr >= 0) & (wr <= 1) & ((dr == 0) | ((dr * (d - c) > 0) & (np.abs(dr) <= np.abs(d - c))))
+    ok = (okl & okr) | (jump == 0)
+    ok &= np.isfinite(wl) & np.isfinite(wr)
+    vl = van_leer_weights(S)
+    out = np.array(w, dtype=np.float64, copy=True)
+    out[..., 0::2] = np.where(ok, wl, vl[..., 0::2])
+    out[..., 1::2] = np.where(ok, wr, vl[..., 1::2])
+    return out
 
 
 def loss_and_grad(P, x, tau):
@@ -122,12 +156,13 @@ def loss_and_grad(P, x, tau):
     y, (x, z1, h1, z2, h2, z3, _) = forward(P, x)
     xv = x.reshape(B, 8, 6)
     d = xv[:, :, 3] - xv[:, :, 2]
-    eL = xv[:, :, 2] + y[:, 0::2] * d - tau
-    eR = xv[:, :, 3] - y[:, 1::2] * d - tau
+    w = weights(y)
+    eL = xv[:, :, 2] + w[:, 0::2] * d - tau
+    eR = xv[:, :, 3] - w[:, 1::2] * d - tau
     loss = np.sum(eL ** 2 + eR ** 2) / B
     dy = np.zeros_like(y)
-    dy[:, 0::2] = 2.0 * eL * d / B
-    dy[:, 1::2] = -2.0 * eR * d / B
+    dy[:, 0::2] = W_SCALE * 2.0 * eL * d / B
+    dy[:, 1::2] = -W_SCALE * 2.0 * eR * d / B
     dz3 = dy * y * (1.0 - y)
     g = {"W3": h2.T @ dz3, "b3": dz3.sum(0)}
     dh2 = dz3 @ P["W3"].T
@@ -153,8 +188,8 @@ def adam_step(P, g, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
 
 
 def van_leer_weights(S):
-    """The weights that reproduce van Leer PLM (slopeLimiter.hpp:54-57) for the two states
-    of the interface: wL from cell i-1's slope, wR from cell i's slope."""
+    """The weights w that reproduce van Leer PLM (slopeLimiter.hpp:54-57) for the two
+    states of the interface: wL from cell i-1's slope, wR from cell i's slope."""
     a, b = S[..., 1], S[..., 2]
     c, d = S[..., 3], S[..., 4]
     jump = c - b

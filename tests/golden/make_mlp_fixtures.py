@@ -35,8 +35,9 @@ def main():
         y = torch.sigmoid(h2 @ T["W3"] + T["b3"])
         xv = xt.reshape(B, 8, 6)
         d = xv[:, :, 3] - xv[:, :, 2]
-        eL = xv[:, :, 2] + y[:, 0::2] * d - tt
-        eR = xv[:, :, 3] - y[:, 1::2] * d - tt
+        w = y
+        eL = xv[:, :, 2] + w[:, 0::2] * d - tt
+        eR = xv[:, :, 3] - w[:, 1::2] * d - tt
         loss = (eL ** 2 + eR ** 2).sum() / B
         loss.backward()
         for k, v in P.items():

@@ -63,14 +63,31 @@ def test_bf16_rounding_matches_torch():
     assert np.array_equal(mlp.bf16(x), ref)
 
 
-def test_interface_states_are_bounded():
+def test_checked_weights_are_tvd():
+    """Checked weights give slopes inside the TVD region: per cell, the deviation of each
+    face state from the cell value has the sign of, and is at most, each one-sided
+    difference."""
     rng = np.random.default_rng(7)
-    S = rng.normal(0, 1, (200, 8, 6))
-    y = rng.uniform(0, 1, (200, 16))
-    vL, vR = mlp.face_states(S, y)
-    lo = np.minimum(S[..., 2], S[..., 3])
-    hi = np.maximum(S[..., 2], S[..., 3])
-    assert np.all((vL >= lo) & (vL <= hi) & (vR >= lo) & (vR <= hi))
+    S = rng.normal(0, 1, (400, 8, 6))
+    w = mlp.weights(rng.uniform(0, 1, (400, 16)))
+    w[0, 3] = np.nan
+    wc = mlp.checked_weights(S, w)
+    vL, vR = mlp.face_states(S, wc)
+    a, b, c, d = S[..., 1], S[..., 2], S[..., 3], S[..., 4]
+    dl = vL - b   # deviation of cell i-1's right state
+    dr = c - vR   # deviation of cell i's left state
+    tol = 1e-12
+    assert np.all((np.abs(dl) <= tol) | ((dl * (b - a) > 0) & (np.abs(dl) <= np.abs(b - a) + tol)
+                                         & (dl * (c - b) > 0) & (np.abs(dl) <= np.abs(c - b) + tol)))
+    assert np.all((np.abs(dr) <= tol) | ((dr * (d - c) > 0) & (np.abs(dr) <= np.abs(d - c) + tol)
+                                         & (dr * (c - b) > 0) & (np.abs(dr) <= np.abs(c - b) + tol)))
+    # weights kept where they pass, van Leer where they fail
+    ok = np.isclose(wc, w)
+    assert 0.0 < np.mean(ok) < 1.0
+    vl = mlp.van_leer_weights(S)
+    assert np.allclose(wc[~ok], vl[~ok])
+    # the van Leer member passes its own check
+    assert np.allclose(mlp.checked_weights(S, vl), vl)
 
 
 def test_features_shift_and_scale_invariant():
@@ -89,8 +106,8 @@ def test_features_shift_and_scale_invariant():
 def test_van_leer_member_reproduces_plm():
     rng = np.random.default_rng(11)
     S = rng.normal(0, 1, (300, 8, 6))
-    y = mlp.van_leer_weights(S)
-    vL, vR = mlp.face_states(S, y)
+    w = mlp.van_leer_weights(S)
+    vL, vR = mlp.face_states(S, w)
     # PLM van Leer states of the interface between cells 2 and 3 (oracle's limiter)
     vl = np.vectorize(O.vanleer)
     slope_l = vl(S[..., 3] - S[..., 2], S[..., 2] - S[..., 1])
