@@ -1,0 +1,93 @@
+"""FP32-stencil mode against the plain FP64 method: runs one workload in both modes to the
+same time and reports the L1 difference (per variable, mean over the interior cells of
+|q_fp32 - q_fp64|, averaged over rho, velocity, pressure and magnetic components, face
+fields for the components held on faces) and the throughput of each run.
+
+    python -m idefix_amd.accuracy --config ot --n 512 --tend 0.5
+    python -m idefix_amd.accuracy --config turb3d --n 256 --tend 1.0
+    python -m idefix_amd.accuracy --config kh --n 128 --tend 2.0
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+CONFIGS = {
+    # 2-D Orszag-Tang: PLM, HLLD, UCT-HLLD, RK2, CFL 0.6
+    "ot": dict(dim=2, comps=2, recon="plm", emf="uct_hlld", nstages=2, cfl=0.6,
+               shape=lambda n: (n, n), xend=(1, 1, 1), ic="orszag_tang"),
+    # 3-D decaying turbulence: PPM, HLLD, CT, RK3
+    "turb3d": dict(dim=3, comps=3, recon="ppm", emf="uct_contact", nstages=3, cfl=0.4,
+                   shape=lambda n: (n, n, n), xend=(1, 1, 1), ic="turbulence"),
+    # 2-D decaying turbulence (three components): PPM, HLLD, CT, RK3
+    "turb2d": dict(dim=2, comps=3, recon="ppm", emf="uct_contact", nstages=3, cfl=0.4,
+                   shape=lambda n: (n, n), xend=(1, 1, 1), ic="turbulence"),
+    # 3-D MHD Kelvin-Helmholtz on [0,1] x [0,1] x [0,1/2]: PPM, HLLD, CT, RK2
+    "kh": dict(dim=3, comps=3, recon="ppm", emf="uct_contact", nstages=2, cfl=0.4,
+               shape=lambda n: (2 * n, 2 * n, n), xend=(1, 1, 0.5), ic="kelvin_helmholtz"),
+}
+
+
+def run(config, n, tend, precision, recon=None, params=None):
+    import torch
+    from .datablock import DataBlock
+    from . import problems
+    c = CONFIGS[config]
+    g = DataBlock(c["dim"], c["shape"](n), comps=c["comps"], mhd=True,
+                  recon=recon or c["recon"], solver="hlld", emf=c["emf"], precision=precision,
+                  nstages=c["nstages"], cfl=c["cfl"], xend=c["xend"])
+    Vc, Vs = getattr(problems, c["ic"])(g)
+    g.upload(Vc, Vs)
+    if params is not None:
+        g.set_mlp_params(params)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.run(tend)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = int(np.prod(c["shape"](n)))
+    return g, {"wall_s": wall, "cycles": g.ncycles, "t": g.t,
+               "zone_updates_per_s": cells * g.ncycles / wall}
+
+
+def l1(ga, gb):
+    """Mean over variables of the mean absolute difference over interior cells/faces."""
+    b, e = ga.beg, ga.end
+    sl = (slice(b[2], e[2]), slice(b[1], e[1]), slice(b[0], e[0]))
+    errs = []
+    dim = ga.dim
+    for n in range(ga.nvar):
+        if n >= ga.comps + 2 and n - (ga.comps + 2) < dim:
+            continue  # face-held components compared on the faces below
+        errs.append(float((ga.Vc[n][sl] - gb.Vc[n][sl]).abs().mean()))
+    for d in range(dim):
+        fs = list(sl)
+        fs[2 - d] = slice(b[d], e[d] + 1)
+        errs.append(float((ga.Vs[d][tuple(fs)] - gb.Vs[d][tuple(fs)]).abs().mean()))
+    return float(np.mean(errs)), errs
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="ot")
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--tend", type=float, default=0.5)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    g64, r64 = run(a.config, a.n, a.tend, "fp64")
+    g32, r32 = run(a.config, a.n, a.tend, "fp32")
+    err, per = l1(g32, g64)
+    rep = {"config": a.config, "n": a.n, "shape": CONFIGS[a.config]["shape"](a.n),
+           "tend": a.tend, "fp64": r64, "fp32": r32, "L1_fp32_vs_fp64": err,
+           "L1_per_variable": per, "divB_fp64": g64.check_divb(), "divB_fp32": g32.check_divb()}
+    print(json.dumps(rep))
+    if a.out:
+        os.makedirs(os.path.dirname(a.out) or ".", exist_ok=True)
+        with open(a.out, "w") as fh:
+            json.dump(rep, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -232,6 +232,33 @@ def turbulence(g, seed=13763, kmin=1, kmax=8, v_rms=0.5, db_rms=0.1, b0=0.1, gam
     return Vc, Vs
 
 
+def kelvin_helmholtz(g, seed=13763, v0=0.5, a=0.05, rho=1.0, p=2.5, b0=0.05, noise=0.01):
+    """3-D MHD Kelvin-Helmholtz shear layer (three components): vx = v0 tanh((y - 1/2)/a),
+    rho, p uniform, uniform Bx = b0, and white-noise vy = noise (2 u - 1)
+    exp(-((y - 1/2)/a)^2) with u drawn from ParkMiller(seed) once per interior cell in
+    (k, j, i) order."""
+    Vc, Vs = _empty(g)
+    x, y, z = _coords(g)
+    comps = _comps(g)
+    assert comps == 3 and _is_mhd(g)
+    nx, ny, nz = g.np_tot
+    Y = y[None, :, None] + 0 * x[None, None, :] + 0 * z[:, None, None]
+    Vc[0] = rho
+    Vc[4] = p
+    Vc[1] = v0 * np.tanh((Y - 0.5) / a)
+    rng = ParkMiller(seed)
+    b, e = g.beg, g.end
+    u = np.empty((e[2] - b[2], e[1] - b[1], e[0] - b[0]))
+    flat = u.reshape(-1)
+    for q in range(flat.size):
+        flat[q] = rng()
+    env = np.exp(-((y[b[1]:e[1]] - 0.5) / a) ** 2)[None, :, None]
+    Vc[2][b[2]:e[2], b[1]:e[1], b[0]:e[0]] = noise * (2.0 * u - 1.0) * env
+    Vs[0] = b0
+    Vc[5] = b0
+    return Vc, Vs
+
+
 def _widths(g, d):
     """Cell widths along direction d over np_tot cells."""
     if hasattr(g, "dx"):

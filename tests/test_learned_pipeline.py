@@ -1,0 +1,77 @@
+"""CPU checks of the learned limiter's data pipeline (torch on the CPU): block averaging
+conserves the totals and keeps the discrete div B at zero, interface targets are exact
+for linear data, and the samples' features and targets follow the specification."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from idefix_amd import learned  # noqa: E402
+from oracle import mlp  # noqa: E402
+
+
+def _fine(n=32, seed=0):
+    rng = np.random.default_rng(seed)
+    V = torch.tensor(rng.uniform(0.5, 1.5, (8, n, n)))
+    # divergence-free faces from a corner potential A (periodic)
+    A = rng.normal(0, 1, (n, n))
+    bx = np.zeros((n, n + 1))
+    by = np.zeros((n + 1, n))
+    bx[:, :n] = (np.roll(A, -1, axis=0) - A) * n
+    bx[:, n] = bx[:, 0]
+    by[:n, :] = -(np.roll(A, -1, axis=1) - A) * n
+    by[n, :] = by[0, :]
+    V[5] = torch.tensor(0.5 * (bx[:, :-1] + bx[:, 1:]))
+    V[6] = torch.tensor(0.5 * (by[:-1] + by[1:]))
+    return V, torch.tensor(bx), torch.tensor(by)
+
+
+def test_coarse_state_conserves_and_keeps_div_b():
+    gamma = 5.0 / 3.0
+    V, bx, by = _fine()
+    Vc, bxc, byc = learned.coarse_state(V, bx, by, 4, gamma)
+    rho, v, p, B = V[0], V[1:4], V[4], V[5:8]
+    E = p / (gamma - 1) + 0.5 * rho * (v * v).sum(0) + 0.5 * (B * B).sum(0)
+    rc, vc, pc, Bc = Vc[0], Vc[1:4], Vc[4], Vc[5:8]
+    assert float(rc.sum()) * 16 == pytest.approx(float(rho.sum()), rel=1e-13)
+    for c in range(3):
+        assert float((rc * vc[c]).sum()) * 16 == pytest.approx(float((rho * v[c]).sum()), rel=1e-12, abs=1e-12)
+    # energy with the coarse cell field from the coarse faces is the averaged energy
+    Ec = pc / (gamma - 1) + 0.5 * rc * (vc * vc).sum(0) + 0.5 * (Bc * Bc).sum(0)
+    assert float(Ec.sum()) * 16 == pytest.approx(float(E.sum()), rel=1e-12)
+    div = (bxc[:, 1:] - bxc[:, :-1]) + (byc[1:] - byc[:-1])
+    assert float(div.abs().max()) < 1e-11 * float(bxc.abs().max())
+
+
+def test_face_targets_exact_for_linear_data():
+    n, f = 32, 4
+    x = (np.arange(n) + 0.5) / n
+    V = torch.zeros((8, n, n), dtype=torch.float64)
+    for q in range(8):
+        V[q] = torch.tensor(np.tile(1.0 + 0.1 * q + 0.3 * x, (n, 1)))      # linear in x
+    bx = torch.zeros((n, n + 1), dtype=torch.float64)
+    by = torch.zeros((n + 1, n), dtype=torch.float64)
+    tx, ty = learned.face_targets(V, bx, by, f)
+    xf = np.arange(n // f) * f / n   # coarse x faces
+    for q in range(8):
+        if q == 5:
+            continue  # the normal field comes from the faces
+        want = 1.0 + 0.1 * q + 0.3 * xf
+        got = tx[q].numpy()
+        # interior faces (the periodic wrap makes face 0 a jump)
+        assert np.allclose(got[:, 1:], want[None, 1:], rtol=0, atol=1e-14)
+
+
+def test_samples_follow_the_feature_specification():
+    V, bx, by = _fine(16, seed=3)
+    Vc, _, _ = learned.coarse_state(V, bx, by, 2, 5.0 / 3.0)
+    tgt = torch.rand((8, 8, 8), dtype=torch.float64)
+    X, T = learned.samples(Vc, tgt, 0)
+    # face (J=2, I=5): stencil cells I-3 .. I+2 along x in face-frame order
+    S = Vc[learned.FRAME[0]][:, 2, :].numpy()
+    st = np.stack([S[:, (5 - 3 + s) % 8] for s in range(6)], axis=-1)
+    x, c, s = mlp.features(st)
+    k = 2 * 8 + 5
+    assert np.allclose(X[k].numpy(), x.astype(np.float32), atol=1e-6)
+    tau = np.clip((tgt[:, 2, 5].numpy()[learned.FRAME[0]] - c) / s, -1, 1)
+    assert np.allclose(T[k].numpy(), tau, atol=1e-6)
