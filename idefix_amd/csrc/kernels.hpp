@@ -389,11 +389,20 @@ struct MarchGeom {
   int nxb;        // 64-wide x blocks
 };
 
-#ifndef IDFX_MARCH_WAVES_PER_EU
-#define IDFX_MARCH_WAVES_PER_EU 2
-#endif
+// The stencil rows consecutive marching steps share are carried in LDS for the 3-cell
+// window (PLM, learned limiter); PPM keeps its 5-row window in registers.
+template <class C>
+constexpr bool march_lds_window() {
+  return window<C>() == 3;
+}
+// 3 waves per SIMD fit the 2-D march without spills; the 3-D one would spill (measured
+// slower), so it stays at 2
+template <class C>
+constexpr int march_waves_per_eu() {
+  return (march_lds_window<C>() && C::DIM == 2) ? 3 : 2;
+}
 template <class C, typename T, int DIR, bool FUSED, int EMF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IDFX_MARCH_WAVES_PER_EU)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(march_waves_per_eu<C>())))
 void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                      MarchGeom m) {
   static_assert(DIR == JDIR || DIR == KDIR, "marching sweep is for y and z");
@@ -427,12 +436,20 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
     const unsigned vcol = (unsigned)idxs(g, 0, kk, jj, ic);
     const double *VsDir = a.Vs + DIR * g.svs;  // the normal component
 
+    constexpr bool CARRY = march_lds_window<C>();
+    // window rows f-1, f of thread t at lds[(row & 1) * NV + n][t]
+    __shared__ T lds[CARRY ? 2 * NV : 1][256];
+    const int tid = threadIdx.x;
     T win[NW][NV];
 #pragma unroll
     for (int s = 0; s < NW; s++)
 #pragma unroll
-      for (int n = 0; n < NV; n++)
+      for (int n = 0; n < NV; n++) {
         win[s][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(c0 - 1 - H + s) * sd);
+        if constexpr (CARRY) {
+          if (s >= 1) lds[((c0 - 1 - H + s) & 1) * NV + n][tid] = win[s][n];
+        }
+      }
     // right-face value of cell c0-1: the left state of the first face
     T carry[NV], Fprev[NV], cprev = 0;
 #pragma unroll
@@ -452,13 +469,23 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
     }
     for (int f = c0; f <= c1; f++) {
       // shift the window to cell f and load its leading row
+      if constexpr (CARRY) {
 #pragma unroll
-      for (int s = 0; s < NW - 1; s++)
+        for (int n = 0; n < NV; n++) {
+          win[0][n] = lds[((f - 1) & 1) * NV + n][tid];
+          win[1][n] = lds[(f & 1) * NV + n][tid];
+          win[2][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + 1) * sd);
+          lds[((f + 1) & 1) * NV + n][tid] = win[2][n];  // replaces row f-1, read above
+        }
+      } else {
 #pragma unroll
-        for (int n = 0; n < NV; n++) win[s][n] = win[s + 1][n];
+        for (int s = 0; s < NW - 1; s++)
 #pragma unroll
-      for (int n = 0; n < NV; n++)
-        win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
+          for (int n = 0; n < NV; n++) win[s][n] = win[s + 1][n];
+#pragma unroll
+        for (int n = 0; n < NV; n++)
+          win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
+      }
       T vL[NV], vR[NV], vr_next[NV];
 #pragma unroll
       for (int n = 0; n < NV; n++) {
