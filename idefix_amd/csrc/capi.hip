@@ -590,14 +590,30 @@ int idfx_check_divb(idfx_ctx *c, const double *Vs, double *divb, void *stream) {
 
 // One RK stage on the fused path.  When newdt_host is given in stage 0, the InvDt
 // maximum is copied to pinned memory asynchronously and ctx->ev_dt records it.
+// Boundary step of a stage.  In a box periodic along every active direction whose
+// interior cell field equals the face average (every stage after the first of a cycle:
+// the previous stage's ConsToPrim wrote it), one kernel fills every ghost zone with the
+// same values as the per-direction fill followed by ReconstructVcField.
+static bool all_periodic(const idfx_ctx *c) {
+  for (int d = 0; d < c->d.dim; d++)
+    if (c->d.lbound[d] != IDFX_BC_PERIODIC || c->d.rbound[d] != IDFX_BC_PERIODIC) return false;
+  return true;
+}
+static int stage_boundaries(idfx_ctx *c, double *Vc, double *Vs, bool cell_field_consistent,
+                            hipStream_t s) {
+  if (all_periodic(c) && (cell_field_consistent || !c->d.mhd))
+    return check(c->ops->boundary_periodic(c->g, Vc, Vs, s));
+  return idfx_set_boundaries(c, Vc, Vs, s);
+}
+
 static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double *Vc, double *Vs,
                          double *Uc, double *InvDt, int skip_boundaries, int write_uc,
-                         bool want_dt, hipStream_t s) {
+                         bool want_dt, hipStream_t s, bool consistent = false) {
   static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
   static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
   if (!skip_boundaries) {
     Region r(c, IDFX_REGION_BOUNDARIES, s);
-    IDFX_TRY(idfx_set_boundaries(c, Vc, Vs, s));
+    IDFX_TRY(stage_boundaries(c, Vc, Vs, consistent, s));
   }
   double wc = 0.0, w0 = 0.0;
   if (stage > 0) {
@@ -667,12 +683,13 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
 // which writes the interior of the output pair.  Same arithmetic as stage_enqueue.
 static int stage_fused2d(idfx_ctx *c, int stage, int nstages, double dt, double *Vc_in,
                          double *Vs_in, double *Vc_out, double *Vs_out, double *Uc,
-                         double *InvDt, int write_uc, bool want_dt, hipStream_t s) {
+                         double *InvDt, int write_uc, bool want_dt, hipStream_t s,
+                         bool consistent) {
   static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
   static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
   {
     Region r(c, IDFX_REGION_BOUNDARIES, s);
-    IDFX_TRY(idfx_set_boundaries(c, Vc_in, Vs_in, s));
+    IDFX_TRY(stage_boundaries(c, Vc_in, Vs_in, consistent, s));
   }
   if (want_dt) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, c->dtmax);
   int flags = SW_INIT_UC;
@@ -804,12 +821,12 @@ int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long nc
   for (int stage = 0; stage < nstages; stage++) {
     if (!fused || stage < first_fused) {
       IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0,
-                             stage == nstages - 1, stage == 0, s));
+                             stage == nstages - 1, stage == 0, s, stage > 0));
     } else {
       const bool fwd = (stage - first_fused) % 2 == 0;
       IDFX_TRY(stage_fused2d(c, stage, nstages, *dt, fwd ? Vc : c->Vc2, fwd ? Vs : c->Vs2,
                              fwd ? c->Vc2 : Vc, fwd ? c->Vs2 : Vs, Uc, InvDt,
-                             stage == nstages - 1, stage == 0, s));
+                             stage == nstages - 1, stage == 0, s, stage > 0));
     }
   }
   // the dt readback of stage 0 completes while the later stages still run

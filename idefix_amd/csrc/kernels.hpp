@@ -903,6 +903,88 @@ __global__ __launch_bounds__(256) void k_vc_field(GridDesc g, const double *Vs, 
     X[C::BX3 * g.sv + p] = 0.5 * (Vs[idxs(g, 2, k, j, i)] + Vs[idxs(g, 2, k + 1, j, i)]);
 }
 
+// Ghost zones of a box periodic in every active direction, all directions at once: every
+// ghost cell of Vc and ghost face of each Vs component takes the value at the periodic
+// image of its ghost coordinates (faces along their own direction wrap with period
+// nint, face end being interior).  Equals the direction-by-direction fill of k_boundary for periodic
+// sides (boundary.hpp:576-673), whose corner values are the images in every direction,
+// and the following ReconstructVcField: ghost cells take the average of their two faces as
+// the fill leaves them, interior cells keep theirs, which equals the face average when
+// the previous stage's ConsToPrim wrote it.  The ghost sites are
+// enumerated slab by slab: slab d holds the sites whose index is a ghost along d and
+// interior along every direction before d.
+template <class C>
+__global__ __launch_bounds__(256) void k_boundary_periodic(GridDesc g, double *Vc, double *Vs) {
+  constexpr int NF = C::MHD ? C::DIM : 0;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int f = -1; f < NF; f++) {  // f = -1: cells, else face component f
+    int lo[3], hi[3], ext[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = g.beg[d];
+      hi[d] = g.end[d] + ((f == d) ? 1 : 0);      // interior index range [lo, hi)
+      ext[d] = g.nt[d] + ((f == d) ? 1 : 0);      // full index range [0, ext)
+    }
+    for (int d = 0; d < C::DIM; d++) {
+      long long sz = 2LL * g.beg[d];  // ghosts along d: beg below and beg above
+      for (int d2 = 0; d2 < 3; d2++)
+        if (d2 != d) sz *= (d2 < d) ? (hi[d2] - lo[d2]) : ext[d2];
+      if (t >= sz) {
+        t -= sz;
+        continue;
+      }
+      // decode t into the slab: along d the ghost index, before d interior, after d any
+      int q[3];
+      long long r = t;
+      for (int d2 = 0; d2 < 3; d2++) {
+        const long long n = (d2 == d) ? (2LL * g.beg[d]) : (d2 < d) ? (hi[d2] - lo[d2]) : ext[d2];
+        const int v = (int)(r % n);
+        r /= n;
+        if (d2 == d) q[d2] = v < g.beg[d] ? v : hi[d] + (v - g.beg[d]);
+        else q[d2] = (d2 < d) ? lo[d2] + v : v;
+      }
+      // the image wraps the ghost coordinates only: an interior index keeps its own value
+      // (the last face along a face component's direction is interior and need not
+      // equal the first one, e.g. where the sweeps leave face data unwritten)
+      int src[3];
+      for (int d2 = 0; d2 < 3; d2++) {
+        const int n = g.end[d2] - g.beg[d2];
+        const bool ghost = d2 < C::DIM && (q[d2] < lo[d2] || q[d2] >= hi[d2]);
+        src[d2] = ghost ? g.beg[d2] + (((q[d2] - g.beg[d2]) % n) + n) % n : q[d2];
+      }
+      if (f < 0) {
+        const long long ps = idx3(g, src[2], src[1], src[0]);
+        const long long pd = idx3(g, q[2], q[1], q[0]);
+#pragma unroll
+        for (int n = 0; n < C::NVAR; n++) {
+          if (C::MHD && n >= C::BX1 && n < C::BX1 + C::DIM) {
+            // ReconstructVcField: the average of the two faces as the fill leaves them
+            // (each face at the image of its ghost coordinates)
+            const int fd = n - C::BX1;
+            double v[2];
+            for (int side = 0; side < 2; side++) {
+              int fi[3];
+              for (int d2 = 0; d2 < 3; d2++) {
+                const int qq = q[d2] + ((d2 == fd) ? side : 0);
+                const int fhi = g.end[d2] + ((d2 == fd) ? 1 : 0);
+                const int nn = g.end[d2] - g.beg[d2];
+                const bool gh = d2 < C::DIM && (qq < g.beg[d2] || qq >= fhi);
+                fi[d2] = gh ? g.beg[d2] + (((qq - g.beg[d2]) % nn) + nn) % nn : qq;
+              }
+              v[side] = Vs[idxs(g, fd, fi[2], fi[1], fi[0])];
+            }
+            Vc[n * g.sv + pd] = 0.5 * (v[0] + v[1]);
+          } else {
+            Vc[n * g.sv + pd] = Vc[n * g.sv + ps];
+          }
+        }
+      } else {
+        Vs[idxs(g, f, q[2], q[1], q[0])] = Vs[idxs(g, f, src[2], src[1], src[0])];
+      }
+      return;
+    }
+  }
+}
+
 // AddAndStore over whole arrays (stateContainer.cpp:91-98)
 static __global__ __launch_bounds__(256) void k_add_and_store(double *U, const double *U0, long long n,
                                                        double wl, double wr) {

@@ -29,6 +29,9 @@ struct Ops {
                              double wc, double w0, hipStream_t s) = nullptr;
   hipError_t (*vc_field)(const GridDesc &g, const double *Vs, double *X,
                          hipStream_t s) = nullptr;
+  // all-periodic box: every ghost zone in one pass (see k_boundary_periodic)
+  hipError_t (*boundary_periodic)(const GridDesc &g, double *Vc, double *Vs,
+                                  hipStream_t s) = nullptr;
   hipError_t (*boundary)(const GridDesc &g, double *Vc, double *Vs, int dir, int lbc, int rbc,
                          hipStream_t s) = nullptr;
   hipError_t (*normal_field)(const GridDesc &g, double *Vs, int dir, int recL, int recR,
@@ -228,6 +231,24 @@ hipError_t launch_boundary(const GridDesc &g, double *Vc, double *Vs, int dir, i
 }
 
 template <class C>
+hipError_t launch_boundary_periodic(const GridDesc &g, double *Vc, double *Vs, hipStream_t s) {
+  constexpr int NF = C::MHD ? C::DIM : 0;
+  long long total = 0;
+  for (int f = -1; f < NF; f++)
+    for (int d = 0; d < C::DIM; d++) {
+      long long sz = 2LL * g.beg[d];
+      for (int d2 = 0; d2 < 3; d2++) {
+        if (d2 == d) continue;
+        const int hi = g.end[d2] + ((f == d2) ? 1 : 0);
+        sz *= (d2 < d) ? (hi - g.beg[d2]) : (g.nt[d2] + ((f == d2) ? 1 : 0));
+      }
+      total += sz;
+    }
+  hipLaunchKernelGGL((k_boundary_periodic<C>), dim3(blocks_for(total)), dim3(256), 0, s, g, Vc, Vs);
+  return hipGetLastError();
+}
+
+template <class C>
 hipError_t launch_normal_field(const GridDesc &g, double *Vs, int dir, int recL, int recR,
                                hipStream_t s) {
   if constexpr (C::MHD) {
@@ -386,6 +407,7 @@ Ops make_ops() {
   o.cons_to_prim = &launch_cons_to_prim<C>;
   o.vc_field = &launch_vc_field<C>;
   o.boundary = &launch_boundary<C>;
+  o.boundary_periodic = &launch_boundary_periodic<C>;
   o.normal_field = &launch_normal_field<C>;
   o.correct_flux = &launch_correct_flux<C>;
   o.rhs = &launch_rhs<C>;

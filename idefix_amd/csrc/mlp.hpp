@@ -13,8 +13,7 @@
 // deviation from its cell value has the sign of the cell's other one-sided difference
 // and does not exceed it, and 0 <= w <= 1.  A variable that fails, any variable of an
 // interface with non-finite input, and every interface when no network is loaded take
-// the weights of van Leer PLM (slopeLimiter.hpp:54-57), the member
-// wL = vl(dvp, dvm) / (2 (V[i] - V[i-1])).
+// the weights of PPM (the member w = (v_PPM - V[i-1]) / (V[i] - V[i-1])).
 //
 // MFMA mapping (v_mfma_f32_16x16x16_bf16): a layer computes Z^T = W^T H^T with the
 // weights as the A operand (rows = output units) and the activations as the B operand
@@ -66,15 +65,21 @@ __device__ __forceinline__ constexpr int mlp_q(int n) {
   return q;
 }
 
-// van Leer member of the family for one variable: cells a = V[i-2], b = V[i-1],
-// c = V[i], d = V[i+1]
-__device__ __forceinline__ void mlp_vanleer(float a, float b, float c, float d, float &wl,
-                                            float &wr) {
-  const float jump = c - b;
-  const float sl = vanleer(c - b, b - a);
-  const float sr = vanleer(d - c, c - b);
-  wl = jump != 0.0f ? 0.5f * sl / jump : 0.0f;
-  wr = jump != 0.0f ? 0.5f * sr / jump : 0.0f;
+// PPM member of the family for one variable from the stencil s[0..5] = V[i-3..i+2]:
+// the left state is cell i-1's right value, the right state cell i's left value
+// (getPPMStates, slopeLimiter.hpp:142-191), with the van Leer fallback where rho or p
+// would not be positive (extrapolateToFaces.hpp:260-304); w = 0 where V[i] = V[i-1].
+__device__ __forceinline__ void mlp_ppm(const float *s, bool positive, float &wl, float &wr) {
+  float vl_c, left, right, vr_d;
+  ppm_states(s[0], s[1], s[2], s[3], s[4], vl_c, left);
+  ppm_states(s[1], s[2], s[3], s[4], s[5], right, vr_d);
+  if (positive) {
+    if (left <= 0.0f) left = s[2] + 0.5f * vanleer(s[3] - s[2], s[2] - s[1]);
+    if (right <= 0.0f) right = s[3] - 0.5f * vanleer(s[4] - s[3], s[3] - s[2]);
+  }
+  const float jump = s[3] - s[2];
+  wl = jump != 0.0f ? (left - s[2]) / jump : 0.0f;
+  wr = jump != 0.0f ? (s[3] - right) / jump : 0.0f;
 }
 
 // Interface weights W[q][face] (SoA, face index = index of the cell right of the face)
@@ -185,12 +190,14 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
 #pragma unroll
     for (int q = 0; q < MLP_NO; q++) out[q] = outs[lane * 17 + q];
   }
-  // per-variable range check, van Leer where it fails
+  // per-variable TVD check, PPM where it fails
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int n = mlp_var<C, DIR>(q);
-    const float a = (float)Vc[n * g.sv + pf - 2 * sd], b = (float)Vc[n * g.sv + pf - sd];
-    const float c = (float)Vc[n * g.sv + pf], d = (float)Vc[n * g.sv + pf + sd];
+    float st[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) st[t] = (float)Vc[n * g.sv + pf + (t - 3) * sd];
+    const float a = st[1], b = st[2], c = st[3], d = st[4];
     bool good = ok;
     if (good) {
       const float wl = out[2 * q], wr = out[2 * q + 1];
@@ -202,7 +209,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
                        (dr == 0.0f || (dr * (d - c) > 0.0f && fabsf(dr) <= fabsf(d - c)));
       good = (okl && okr) || (jump == 0.0f && wl == wl && wr == wr);  // NaN fails
     }
-    if (!good) mlp_vanleer(a, b, c, d, out[2 * q], out[2 * q + 1]);
+    if (!good) mlp_ppm(st, q == 0 || q == 4, out[2 * q], out[2 * q + 1]);
   }
   if (face_ok) {
 #pragma unroll

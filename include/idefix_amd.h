@@ -41,7 +41,7 @@ extern "C" {
 #define IDFX_RECON_PLM 2
 #define IDFX_RECON_PPM 4
 /* No counterpart in the reference: the learned limiter (per-interface MLP weights inside
- * the flux form, van Leer PLM fallback), 3 ghost cells, three-component MHD only. */
+ * the flux form, PPM fallback), 3 ghost cells, three-component MHD only. */
 #define IDFX_RECON_MLP 8
 /* [Hydro] emf (constrainedTransport.hpp:156-176) */
 #define IDFX_EMF_ARITHMETIC 0
@@ -229,7 +229,7 @@ int idfx_mlp_train_step(idfx_mlp *h, const float *X, const float *tau, int B, fl
                         void *stream);
 int idfx_mlp_loss(idfx_mlp *h, double *loss, void *stream);
 /* Loads packed FP32 parameters (host) into an IDFX_RECON_MLP context as BF16 for MFMA
- * inference; params == NULL unloads them (every interface then takes van Leer weights). */
+ * inference; params == NULL unloads them (every interface then takes the PPM weights). */
 int idfx_ctx_set_mlp_params(idfx_ctx *ctx, const float *params);
 /* Interface weights of direction dir for the faces beg..end along dir (every transverse
  * index) from Vc, into W (device float [16][np_tot cells], or the context's own array

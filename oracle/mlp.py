@@ -16,13 +16,14 @@ gelu(z) = z * sigmoid(1.702 z) (Hendrycks & Gimpel 2016, sigmoid form).
 Output: y in (0, 1)^16 are the limiter weights w: w[2v] = wL_v, w[2v+1] = wR_v per
 variable v, and the interface states are
     vL_v = V_v[i-1] + wL_v (V_v[i] - V_v[i-1]),   vR_v = V_v[i] - wR_v (V_v[i] - V_v[i-1]).
-Van Leer PLM is the member wL = dvm / (dvm + dvp) (monotone data) or 0 (extremum).  Each
+Van Leer PLM is the member wL = dvm / (dvm + dvp) (monotone data) or 0 (extremum); PPM is
+the member w = (v_PPM - V[i-1]) / (V[i] - V[i-1]) (0 where V[i] = V[i-1]).  Each
 variable's pair of weights is checked against the TVD region of slope limiters (Sweby
 1984: limited slope between 0 and twice each one-sided difference): the left state's
 deviation wL (V[i] - V[i-1]) must have the sign of V[i-1] - V[i-2] and not exceed it in
 magnitude (or be 0), the right state's deviation wR (V[i] - V[i-1]) likewise against
 V[i+1] - V[i], and 0 <= w <= 1.  A variable that fails, or has non-finite weights, takes
-the van Leer weights.
+the PPM weights (the fallback; van Leer with `fallback="plm"`).
 
 Training target: the interface value t_v of a finer solution; loss per interface
 sum_v ((vL_v - t_v)/s_v)^2 + ((vR_v - t_v)/s_v)^2 (both terms 0 when s_v = 0), mean
@@ -129,9 +130,9 @@ def face_states(S, w):
     return a + wl * (b - a), b - wr * (b - a)
 
 
-def checked_weights(S, w):
-    """The weights the solver uses: per variable, van Leer where a weight leaves the TVD
-    region or is not finite."""
+def checked_weights(S, w, fallback="ppm"):
+    """The weights the solver uses: per variable, the fallback's weights where a weight
+    leaves the TVD region or is not finite."""
     a, b, c, d = S[..., 1], S[..., 2], S[..., 3], S[..., 4]
     jump = c - b
     wl = w[..., 0::2]
@@ -142,11 +143,38 @@ def checked_weights(S, w):
     okr = (wr >= 0) & (wr <= 1) & ((dr == 0) | ((dr * (d - c) > 0) & (np.abs(dr) <= np.abs(d - c))))
     ok = (okl & okr) | (jump == 0)
     ok &= np.isfinite(wl) & np.isfinite(wr)
-    vl = van_leer_weights(S)
+    fb = ppm_weights(S) if fallback == "ppm" else van_leer_weights(S)
     out = np.array(w, dtype=np.float64, copy=True)
-    out[..., 0::2] = np.where(ok, wl, vl[..., 0::2])
-    out[..., 1::2] = np.where(ok, wr, vl[..., 1::2])
+    out[..., 0::2] = np.where(ok, wl, fb[..., 0::2])
+    out[..., 1::2] = np.where(ok, wr, fb[..., 1::2])
     return out
+
+
+def ppm_weights(S):
+    """The weights that reproduce PPM (getPPMStates, slopeLimiter.hpp:142-191, with the
+    van Leer fallback of extrapolateToFaces.hpp:260-304 where rho or p would not be
+    positive): the left state of the interface is cell i-1's right value (stencil cells
+    0..4), the right state cell i's left value (cells 1..5)."""
+    from . import oracle as O
+    ppm = np.vectorize(O.ppm_states)
+    vl_vec = np.vectorize(O.vanleer)
+    _, left = ppm(S[..., 0], S[..., 1], S[..., 2], S[..., 3], S[..., 4])
+    right, _ = ppm(S[..., 1], S[..., 2], S[..., 3], S[..., 4], S[..., 5])
+    left = np.array(left, dtype=np.float64)
+    right = np.array(right, dtype=np.float64)
+    for q in (0, 4):  # rho, p in the face frame
+        bad = left[..., q] <= 0
+        plm = S[..., q, 2] + 0.5 * vl_vec(S[..., q, 3] - S[..., q, 2], S[..., q, 2] - S[..., q, 1])
+        left[..., q] = np.where(bad, plm, left[..., q])
+        bad = right[..., q] <= 0
+        plm = S[..., q, 3] - 0.5 * vl_vec(S[..., q, 4] - S[..., q, 3], S[..., q, 3] - S[..., q, 2])
+        right[..., q] = np.where(bad, plm, right[..., q])
+    jump = S[..., 3] - S[..., 2]
+    safe = np.where(jump != 0, jump, 1.0)
+    w = np.empty(S.shape[:-2] + (16,))
+    w[..., 0::2] = np.where(jump != 0, (left - S[..., 2]) / safe, 0.0)
+    w[..., 1::2] = np.where(jump != 0, (S[..., 3] - right) / safe, 0.0)
+    return w
 
 
 def loss_and_grad(P, x, tau):

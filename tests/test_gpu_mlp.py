@@ -1,6 +1,6 @@
 """GPU checks of the learned limiter through the C ABI: FP32 training gradients and Adam
 steps against oracle/mlp.py, BF16 MFMA inference against the oracle's BF16-rounded
-forward, the van Leer fallback, the MLP-mode solver against PLM (fallback weights), and
+forward, the PPM fallback, the MLP-mode solver against PPM (fallback weights), and
 the flux-form invariants (conservation, div B) with a loaded network."""
 import numpy as np
 import pytest
@@ -102,32 +102,32 @@ def test_mfma_inference_matches_bf16_oracle(d):
 
 
 @pytest.mark.parametrize("d", [0, 1])
-def test_no_network_gives_van_leer_weights(d):
+def test_no_network_gives_ppm_weights(d):
     g = _mlp_block()
     g.set_mlp_params(None)
     W = g.mlp_faces(d).cpu().numpy()
     Vc, _ = g.download()
     lo, hi = g.beg[d], g.end[d] + 1
-    S = learned.stencils(Vc, d, lo, hi)
-    y = mlp.van_leer_weights(S.astype(np.float32).astype(np.float64))
+    S = learned.stencils(Vc, d, lo, hi).astype(np.float32).astype(np.float64)
+    w = mlp.ppm_weights(S)
     Wk = np.moveaxis(np.moveaxis(np.take(W, np.arange(lo, hi), axis=3 - d), 3 - d, 0), 1, -1)
-    # the interface states the weights produce agree with van Leer PLM in float precision
-    jump = (S[..., 3] - S[..., 2])
-    assert np.max(np.abs((Wk[..., 0::2] - y[..., 0::2]) * jump)) < 1e-5
-    assert np.max(np.abs((Wk[..., 1::2] - y[..., 1::2]) * jump)) < 1e-5
+    # the interface states the weights produce agree with PPM in float precision
+    vL, vR = mlp.face_states(S, Wk.astype(np.float64))
+    rL, rR = mlp.face_states(S, w)
+    scale = np.max(np.abs(S), axis=(-1,))
+    assert np.max(np.abs(vL - rL) / (scale + 1e-30)) < 1e-5
+    assert np.max(np.abs(vR - rR) / (scale + 1e-30)) < 1e-5
 
 
-def test_mlp_mode_with_fallback_weights_tracks_plm():
-    """MLP mode without a network runs van Leer PLM through the learned-limiter path: the
-    state after five RK2 cycles matches the PLM-mode state to the float precision of the
-    stored weights (an index or variable-mapping error would show at O(1))."""
+def test_mlp_mode_with_fallback_weights_tracks_ppm():
+    """MLP mode without a network runs PPM through the learned-limiter path: the state
+    after five RK2 cycles matches the PPM-mode state to the float precision of the stored
+    weights (an index or variable-mapping error would show at O(1))."""
     from idefix_amd.datablock import DataBlock
     a = _mlp_block()
     a.set_mlp_params(None)
-    b = DataBlock(2, (40, 36), comps=3, mhd=True, recon="plm", solver="hlld",
+    b = DataBlock(2, (40, 36), comps=3, mhd=True, recon="ppm", solver="hlld",
                   emf="uct_contact", precision="fp64", cfl=0.4)
-    # same initial data on the PLM block (2 ghost cells instead of 3)
-    Va, Sa = a.download()
     Vb, Sb = problems.turbulence(b, seed=5)
     b.upload(Vb, Sb)
     for _ in range(5):

@@ -71,7 +71,7 @@ def test_checked_weights_are_tvd():
     S = rng.normal(0, 1, (400, 8, 6))
     w = mlp.weights(rng.uniform(0, 1, (400, 16)))
     w[0, 3] = np.nan
-    wc = mlp.checked_weights(S, w)
+    wc = mlp.checked_weights(S, w, fallback="plm")
     vL, vR = mlp.face_states(S, wc)
     a, b, c, d = S[..., 1], S[..., 2], S[..., 3], S[..., 4]
     dl = vL - b   # deviation of cell i-1's right state
@@ -87,7 +87,31 @@ def test_checked_weights_are_tvd():
     vl = mlp.van_leer_weights(S)
     assert np.allclose(wc[~ok], vl[~ok])
     # the van Leer member passes its own check
-    assert np.allclose(mlp.checked_weights(S, vl), vl)
+    assert np.allclose(mlp.checked_weights(S, vl, fallback="plm"), vl)
+
+
+def test_ppm_member_reproduces_ppm_and_is_the_default_fallback():
+    from oracle import oracle as O
+    rng = np.random.default_rng(12)
+    S = rng.normal(0, 1, (60, 8, 6))
+    S[:, 0] = np.abs(S[:, 0]) + 0.1   # density positive
+    S[:, 4] = np.abs(S[:, 4]) + 0.1   # pressure positive
+    w = mlp.ppm_weights(S)
+    vL, vR = mlp.face_states(S, w)
+    for f in range(0, 60, 7):
+        for q in range(8):
+            _, left = O.ppm_states(*S[f, q, 0:5])
+            right, _ = O.ppm_states(*S[f, q, 1:6])
+            if q in (0, 4):
+                if left <= 0:
+                    left = S[f, q, 2] + 0.5 * O.vanleer(S[f, q, 3] - S[f, q, 2], S[f, q, 2] - S[f, q, 1])
+                if right <= 0:
+                    right = S[f, q, 3] - 0.5 * O.vanleer(S[f, q, 4] - S[f, q, 3], S[f, q, 3] - S[f, q, 2])
+            assert vL[f, q] == pytest.approx(left, rel=1e-13, abs=1e-13)
+            assert vR[f, q] == pytest.approx(right, rel=1e-13, abs=1e-13)
+    # weights that fail the check are replaced by the PPM ones
+    bad = np.full((60, 16), 2.0)
+    assert np.allclose(mlp.checked_weights(S, bad), w)
 
 
 def test_features_shift_and_scale_invariant():
