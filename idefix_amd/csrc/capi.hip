@@ -24,6 +24,8 @@ void register_mhd3_f32(Ops *t);
 void register_mhd_roe(Ops *t);
 void register_mhd2c3(Ops *t);
 void register_mlp(Ops *t);
+void register_hll(Ops *t);
+void register_tvdlf(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -39,6 +41,8 @@ Ops *ops_table() {
     register_mhd_roe(t);
     register_mhd2c3(t);
     register_mlp(t);
+    register_hll(t);
+    register_tvdlf(t);
     return t;
   }();
   return table;
@@ -188,11 +192,12 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
     a.dtdV = 0.0;
   }
   if (c->d.mhd) {
-    // face range extension across DIR required by CT (hlldMHD.hpp:32-49); RoeMHD
-    // always extends by one cell (roeMHD.hpp:62-73)
+    // face range extension across DIR required by CT: nghost for the uct_hll(d) averages
+    // of HllD and Hll (hlldMHD.hpp:32-49, hllMHD.hpp:32-49); RoeMHD and TvdlfMHD always
+    // extend by one cell (roeMHD.hpp:62-73, tvdlfMHD.hpp:30-41)
     int perp = 1;
     if ((c->d.emf == IDFX_EMF_UCT_HLL || c->d.emf == IDFX_EMF_UCT_HLLD) &&
-        c->d.solver != IDFX_SOLVER_ROE)
+        (c->d.solver == IDFX_SOLVER_HLLD || c->d.solver == IDFX_SOLVER_HLL))
       perp = c->nghost[dir];
     for (int d = 0; d < c->d.dim; d++) a.ext[d] = (d == dir) ? 0 : perp;
   }
@@ -306,9 +311,16 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   const idfx_desc &d = *desc;
   if (d.dim < 1 || d.dim > 3 || d.comps < d.dim || d.comps > 3) return IDFX_E_ARGS;
   if (d.recon != 1 && d.recon != 2 && d.recon != 4 && d.recon != IDFX_RECON_MLP) return IDFX_E_ARGS;
-  if (d.mhd && d.solver != IDFX_SOLVER_HLLD && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
-  if (!d.mhd && d.solver != IDFX_SOLVER_HLLC && d.solver != IDFX_SOLVER_ROE) return IDFX_E_ARGS;
+  const bool any_mode = d.solver == IDFX_SOLVER_ROE || d.solver == IDFX_SOLVER_HLL ||
+                        d.solver == IDFX_SOLVER_TVDLF;
+  if (d.mhd && d.solver != IDFX_SOLVER_HLLD && !any_mode) return IDFX_E_ARGS;
+  if (!d.mhd && d.solver != IDFX_SOLVER_HLLC && !any_mode) return IDFX_E_ARGS;
   if (d.emf < 0 || d.emf > 4) return IDFX_E_ARGS;
+  // HllMHD and TvdlfMHD write no uct_hlld face data (hllMHD.hpp:449-452,
+  // tvdlfMHD.hpp:230-233): the reference would average unwritten arrays
+  if (d.mhd && (d.solver == IDFX_SOLVER_HLL || d.solver == IDFX_SOLVER_TVDLF) &&
+      d.emf == IDFX_EMF_UCT_HLLD)
+    return IDFX_E_ARGS;
   const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver, d.precision == IDFX_PREC_FP32);
   if (key < 0 || key >= kOpsTableSize) return IDFX_E_ARGS;
   const Ops *ops = &ops_table()[key];

@@ -17,7 +17,8 @@ namespace idfx {
 constexpr int IDIR = 0, JDIR = 1, KDIR = 2;
 // RECON_MLP: the learned limiter (no counterpart in the reference), 3 ghost cells
 enum Recon { RECON_CONSTANT = 1, RECON_PLM = 2, RECON_PPM = 4, RECON_MLP = 8 };
-enum Solver { SOLVER_HLLC = 1, SOLVER_HLLD = 2, SOLVER_ROE = 3 };
+enum Solver { SOLVER_HLLC = 1, SOLVER_HLLD = 2, SOLVER_ROE = 3, SOLVER_HLL = 4,
+              SOLVER_TVDLF = 5 };
 enum EmfScheme { EMF_ARITHMETIC = 0, EMF_UCT0 = 1, EMF_UCT_CONTACT = 2, EMF_UCT_HLL = 3,
                  EMF_UCT_HLLD = 4 };
 
@@ -289,6 +290,58 @@ __device__ __forceinline__ void hllc_hd(const T *vL, const T *vR, T *flux, T &cm
       for (int n = 0; n < NV; n++) flux[n] = fR[n] + SR * (usR[n] - uR[n]);
     }
   }
+}
+
+// ------------------------------------------------------------------------------------
+// HllHD, src/fluid/RiemannSolver/HDsolvers/hllHD.hpp:35-120 (HAVE_ENERGY branch)
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hll_hd(const T *vL, const T *vR, T *flux, T &cmax, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fL[NV], fR[NV];
+  const T cL = sqrt_(gamma * (vL[C::ENG] / vL[C::RHO]));
+  const T cR = sqrt_(gamma * (vR[C::ENG] / vR[C::RHO]));
+  const T SL = fmn(vL[Xn] - cL, vR[Xn] - cR);
+  const T SR = fmx(vL[Xn] + cL, vR[Xn] + cR);
+  cmax = fmx(fab(SL), fab(SR));
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fL, vL, uL);
+  kflux<C, DIR>(fR, vR, uR);
+  if (SL > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fL[n];
+  } else if (SR < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fR[n];
+  } else {
+    const T sls = SL * SR, dS = SR - SL;
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = (sls * uR[n] - sls * uL[n] + SR * fL[n] - SL * fR[n]) / dS;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// TvdlfHD, src/fluid/RiemannSolver/HDsolvers/tvdlfHD.hpp:37-96 (HAVE_ENERGY branch):
+// Rusanov flux with the sound speed of the arithmetic mean state.
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void tvdlf_hd(const T *vL, const T *vR, T *flux, T &cmax, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fL[NV], fR[NV];
+  const T rho = T(0.5) * (vL[C::RHO] + vR[C::RHO]);
+  const T prs = T(0.5) * (vL[C::ENG] + vR[C::ENG]);
+  const T vn = T(0.5) * (vL[Xn] + vR[Xn]);
+  const T cRL = sqrt_(gamma * (prs / rho));
+  cmax = fmx(fab(vn + cRL), fab(vn - cRL));
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fL, vL, uL);
+  kflux<C, DIR>(fR, vR, uR);
+#pragma unroll
+  for (int n = 0; n < NV; n++) flux[n] = T(0.5) * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
 }
 
 // ------------------------------------------------------------------------------------
@@ -983,6 +1036,107 @@ __device__ __forceinline__ void roe_mhd(const T *vL, const T *vR, T *flux, T &cm
   store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, pL, pR, E, emf, pre);
 }
 
+// Fast magnetosonic speed of one state (hlldMHD.hpp:183-226, hllMHD.hpp:205-231).
+template <class C, int DIR, typename T>
+__device__ __forceinline__ T fast_speed(const T *v, T gamma) {
+  constexpr int CO = C::COMP;
+  const T gpr = gamma * v[C::ENG];
+  T b1 = v[C::template BXn<DIR>()], b2 = 0, b3 = 0;
+  if constexpr (CO >= 2) b2 = v[C::template BXt<DIR>()];
+  if constexpr (CO >= 3) b3 = v[C::template BXb<DIR>()];
+  const T Btmag2 = b2 * b2 + b3 * b3;
+  const T Bmag2 = b1 * b1 + Btmag2;
+  T c = gpr - Bmag2;
+  c = gpr + Bmag2 + sqrt_(c * c + T(4) * gpr * Btmag2);
+  return sqrt_(T(0.5) * c / v[C::RHO]);
+}
+
+// ------------------------------------------------------------------------------------
+// HllMHD, src/fluid/RiemannSolver/MHDsolvers/hllMHD.hpp:170-453 (HAVE_ENERGY, no Hall).
+// Density and momentum fluxes divide by (sr - sl); field and energy fluxes multiply by
+// 1/(sr - sl).  The reference has no uct_hlld face store for this solver, so the
+// context rejects that pairing.
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hll_mhd(const T *vL, const T *vR, T *flux, T &cmax,
+                                        FaceEmf<T> &E, int emf, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
+  const T cL = fast_speed<C, DIR>(vL, gamma);
+  const T cR = fast_speed<C, DIR>(vR, gamma);
+  const T sl = fmn(vL[Xn] - cL, vR[Xn] - cR);
+  const T sr = fmx(vL[Xn] + cL, vR[Xn] + cR);
+  cmax = fmx(fab(sl), fab(sr));
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fluxL, vL, uL);
+  kflux<C, DIR>(fluxR, vR, uR);
+  if (sl > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxL[n];
+  } else if (sr < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxR[n];
+  } else {
+    const T sls = sl * sr, dS = sr - sl, rdS = T(1) / dS;
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      const T f = sls * uR[n] - sls * uL[n] + sr * fluxL[n] - sl * fluxR[n];
+      flux[n] = (n < C::ENG) ? f / dS : f * rdS;
+    }
+  }
+  StarPre<T> pre;
+  pre.valid = false;
+  // uct_hlld is rejected at context creation; the total pressures are unused below
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, T(0), T(0), E,
+                         emf == EMF_UCT_HLLD ? EMF_UCT_HLL : emf, pre);
+}
+
+// ------------------------------------------------------------------------------------
+// TvdlfMHD, src/fluid/RiemannSolver/MHDsolvers/tvdlfMHD.hpp:138-234 (HAVE_ENERGY, no
+// Hall): Rusanov flux with the fast speed of the arithmetic mean state; the uct_hll
+// store sees sl = -cmax, sr = cmax.  No uct_hlld store (rejected at context creation).
+// ------------------------------------------------------------------------------------
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void tvdlf_mhd(const T *vL, const T *vR, T *flux, T &cmax,
+                                          FaceEmf<T> &E, int emf, T gamma) {
+  constexpr int NV = C::NVAR;
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
+  const T rho = T(0.5) * (vL[C::RHO] + vR[C::RHO]);
+  const T gpr = gamma * (T(0.5) * (vL[C::ENG] + vR[C::ENG]));
+  const T bn = vL[C::template BXn<DIR>()];  // vL and vR share the face value
+  T Bt2 = T(0);
+  if constexpr (CO >= 2) {
+    constexpr int BXt = C::template BXt<DIR>();
+    const T bt = T(0.5) * (vL[BXt] + vR[BXt]);
+    Bt2 = Bt2 + bt * bt;
+  }
+  if constexpr (CO >= 3) {
+    constexpr int BXb = C::template BXb<DIR>();
+    const T bb = T(0.5) * (vL[BXb] + vR[BXb]);
+    Bt2 = Bt2 + bb * bb;
+  }
+  const T B2 = Bt2 + bn * bn;
+  T cRL = gpr - B2;
+  cRL = cRL + B2 + sqrt_(cRL * cRL + T(4) * gpr * Bt2);
+  cRL = sqrt_(T(0.5) * cRL / rho);
+  const T vn = T(0.5) * (vL[Xn] + vR[Xn]);
+  cmax = fmx(fab(vn + cRL), fab(vn - cRL));
+  prim2cons<C>(vL, uL, gamma);
+  prim2cons<C>(vR, uR, gamma);
+  kflux<C, DIR>(fluxL, vL, uL);
+  kflux<C, DIR>(fluxR, vR, uR);
+#pragma unroll
+  for (int n = 0; n < NV; n++) flux[n] = T(0.5) * (fluxL[n] + fluxR[n] + cmax * (uL[n] - uR[n]));
+  StarPre<T> pre;
+  pre.valid = false;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, -cmax, cmax, T(0), T(0), E,
+                         emf == EMF_UCT_HLLD ? EMF_UCT_HLL : emf, pre);
+}
+
 // Dispatch to the configured Riemann solver.  Returns false only in the 1-D Roe case
 // where the reference leaves the flux unwritten.
 template <class C, int DIR, typename T>
@@ -991,6 +1145,10 @@ __device__ __forceinline__ bool riemann(const T *vL, const T *vR, T *flux, T &cm
   if constexpr (C::MHD) {
     if constexpr (C::SOLVER == SOLVER_ROE) {
       roe_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+    } else if constexpr (C::SOLVER == SOLVER_HLL) {
+      hll_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+    } else if constexpr (C::SOLVER == SOLVER_TVDLF) {
+      tvdlf_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
     } else {
       static_assert(C::SOLVER == SOLVER_HLLD, "MHD solver not built");
       hlld_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
@@ -998,6 +1156,12 @@ __device__ __forceinline__ bool riemann(const T *vL, const T *vR, T *flux, T &cm
     return true;
   } else if constexpr (C::SOLVER == SOLVER_HLLC) {
     hllc_hd<C, DIR>(vL, vR, flux, cmax, gamma);
+    return true;
+  } else if constexpr (C::SOLVER == SOLVER_HLL) {
+    hll_hd<C, DIR>(vL, vR, flux, cmax, gamma);
+    return true;
+  } else if constexpr (C::SOLVER == SOLVER_TVDLF) {
+    tvdlf_hd<C, DIR>(vL, vR, flux, cmax, gamma);
     return true;
   } else {
     return roe_hd<C, DIR>(vL, vR, flux, cmax, gamma);

@@ -36,6 +36,8 @@ extern "C" {
 #define IDFX_SOLVER_HLLC 1
 #define IDFX_SOLVER_HLLD 2
 #define IDFX_SOLVER_ROE 3
+#define IDFX_SOLVER_HLL 4    /* HllHD / HllMHD; with mhd, emf must not be UCT_HLLD */
+#define IDFX_SOLVER_TVDLF 5  /* TvdlfHD / TvdlfMHD; with mhd, emf must not be UCT_HLLD */
 /* reconstruction, the reference's ORDER (CMakeLists.txt:173-213) */
 #define IDFX_RECON_CONSTANT 1
 #define IDFX_RECON_PLM 2

@@ -302,6 +302,54 @@ void hllc_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, co
 }
 
 // --------------------------------------------------------------------------------------
+// HllHD: src/fluid/RiemannSolver/HDsolvers/hllHD.hpp:35-120 (HAVE_ENERGY branch)
+// --------------------------------------------------------------------------------------
+void hll_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, const double *vR,
+            double *flux, double &cmax) {
+  const int nv_ = V.nvar;
+  double uL[8], uR[8], fL[8], fR[8];
+  const double cL = std::sqrt(c.gamma * (vL[V.ENG] / vL[V.RHO]));
+  const double cR = std::sqrt(c.gamma * (vR[V.ENG] / vR[V.RHO]));
+  const double cminL = vL[D.Xn] - cL, cmaxL = vL[D.Xn] + cL;
+  const double cminR = vR[D.Xn] - cR, cmaxR = vR[D.Xn] + cR;
+  const double SL = FMIN(cminL, cminR);
+  const double SR = FMAX(cmaxL, cmaxR);
+  cmax = FMAX(std::fabs(SL), std::fabs(SR));
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  kflux(c, V, D, fL, vL, uL);
+  kflux(c, V, D, fR, vR, uR);
+  if (SL > 0) {
+    for (int n = 0; n < nv_; n++) flux[n] = fL[n];
+  } else if (SR < 0) {
+    for (int n = 0; n < nv_; n++) flux[n] = fR[n];
+  } else {
+    for (int n = 0; n < nv_; n++) {
+      flux[n] = SL * SR * uR[n] - SL * SR * uL[n] + SR * fL[n] - SL * fR[n];
+      flux[n] /= (SR - SL);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------------------
+// TvdlfHD: src/fluid/RiemannSolver/HDsolvers/tvdlfHD.hpp:37-96 (HAVE_ENERGY branch):
+// Rusanov flux with the sound speed of the arithmetic mean state.
+// --------------------------------------------------------------------------------------
+void tvdlf_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL,
+              const double *vR, double *flux, double &cmax) {
+  const int nv_ = V.nvar;
+  double v[8], uL[8], uR[8], fL[8], fR[8];
+  for (int n = 0; n < nv_; n++) v[n] = 0.5 * (vL[n] + vR[n]);
+  const double cRL = std::sqrt(c.gamma * (v[V.ENG] / v[V.RHO]));
+  cmax = FMAX(std::fabs(v[D.Xn] + cRL), std::fabs(v[D.Xn] - cRL));
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  kflux(c, V, D, fL, vL, uL);
+  kflux(c, V, D, fR, vR, uR);
+  for (int n = 0; n < nv_; n++) flux[n] = 0.5 * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
+}
+
+// --------------------------------------------------------------------------------------
 // RoeHD: src/fluid/RiemannSolver/HDsolvers/roeHD.hpp:55-364 (HAVE_ENERGY,
 // ROE_AVERAGE == 0).  Returns false when the reference leaves the flux unwritten
 // (1-D strong shock, roeHD.hpp:315-328 under DIMENSIONS > 1 only).
@@ -727,6 +775,98 @@ void store_face_emf(const orc_cfg &c, const Vars &V, const Dir &D, int dir, cons
       }
     }
   }}
+
+// --------------------------------------------------------------------------------------
+// HllMHD: src/fluid/RiemannSolver/MHDsolvers/hllMHD.hpp:170-453 (HAVE_ENERGY, no Hall):
+// density and momentum fluxes divide by (sr - sl), field and energy fluxes multiply by
+// 1/(sr - sl); the face stores cover arithmetic, uct0, uct_contact and uct_hll (the
+// reference has no uct_hlld store for this solver).
+// --------------------------------------------------------------------------------------
+void hll_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
+             const double *vR, double *flux, double &cmax, FaceEmf &E) {
+  const int comps = c.comps;
+  double uL[8], uR[8], fluxL[8], fluxR[8];
+  const double gamma = c.gamma;
+  double gpr = gamma * vL[V.ENG];
+  double b1 = vL[D.BXn], b2 = at(vL, D.BXt), b3 = at(vL, D.BXb);
+  double Btmag2 = b2 * b2 + b3 * b3;
+  double Bmag2 = b1 * b1 + Btmag2;
+  double cL = gpr - Bmag2;
+  cL = gpr + Bmag2 + std::sqrt(cL * cL + 4.0 * gpr * Btmag2);
+  cL = std::sqrt(0.5 * cL / vL[V.RHO]);
+  gpr = gamma * vR[V.ENG];
+  b1 = vR[D.BXn];
+  b2 = at(vR, D.BXt);
+  b3 = at(vR, D.BXb);
+  Btmag2 = b2 * b2 + b3 * b3;
+  Bmag2 = b1 * b1 + Btmag2;
+  double cR = gpr - Bmag2;
+  cR = gpr + Bmag2 + std::sqrt(cR * cR + 4.0 * gpr * Btmag2);
+  cR = std::sqrt(0.5 * cR / vR[V.RHO]);
+  const double cminL = vL[D.Xn] - cL, cmaxL = vL[D.Xn] + cL;
+  const double cminR = vR[D.Xn] - cR, cmaxR = vR[D.Xn] + cR;
+  const double sl = FMIN(cminL, cminR);
+  const double sr = FMAX(cmaxL, cmaxR);
+  cmax = FMAX(std::fabs(sl), std::fabs(sr));
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  kflux(c, V, D, fluxL, vL, uL);
+  kflux(c, V, D, fluxR, vR, uR);
+  // density and momentum
+  for (int n = V.RHO; n <= V.RHO + comps; n++) {
+    if (sl > 0) flux[n] = fluxL[n];
+    else if (sr < 0) flux[n] = fluxR[n];
+    else flux[n] = (sl * sr * uR[n] - sl * sr * uL[n] + sr * fluxL[n] - sl * fluxR[n]) / (sr - sl);
+  }
+  // magnetic field and energy
+  for (int n = V.ENG; n < V.nvar; n++) {
+    if (sl > 0) {
+      flux[n] = fluxL[n];
+    } else if (sr < 0) {
+      flux[n] = fluxR[n];
+    } else {
+      flux[n] = sl * sr * uR[n] - sl * sr * uL[n] + sr * fluxL[n] - sl * fluxR[n];
+      flux[n] *= (1.0 / (sr - sl));
+    }
+  }
+  double b2L = vL[V.BX1] * vL[V.BX1];
+  if (comps >= 2) b2L = b2L + vL[V.BX2] * vL[V.BX2];
+  if (comps >= 3) b2L = b2L + vL[V.BX3] * vL[V.BX3];
+  double b2R = vR[V.BX1] * vR[V.BX1];
+  if (comps >= 2) b2R = b2R + vR[V.BX2] * vR[V.BX2];
+  if (comps >= 3) b2R = b2R + vR[V.BX3] * vR[V.BX3];
+  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, vL[V.ENG] + 0.5 * b2L,
+                 vR[V.ENG] + 0.5 * b2R, E);
+}
+
+// --------------------------------------------------------------------------------------
+// TvdlfMHD: src/fluid/RiemannSolver/MHDsolvers/tvdlfMHD.hpp:138-234 (HAVE_ENERGY, no
+// Hall): Rusanov flux with the fast speed of the arithmetic mean state; the face range
+// always extends by one transverse cell and the uct_hll store sees sl = -cmax, sr = cmax.
+// --------------------------------------------------------------------------------------
+void tvdlf_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
+               const double *vR, double *flux, double &cmax, FaceEmf &E) {
+  const int nv_ = V.nvar;
+  const int comps = c.comps;
+  double v[8], uL[8], uR[8], fluxL[8], fluxR[8];
+  for (int n = 0; n < nv_; n++) v[n] = 0.5 * (vL[n] + vR[n]);
+  const double gpr = c.gamma * v[V.ENG];
+  double Bt2 = 0.0;
+  if (comps >= 2) Bt2 = Bt2 + v[D.BXt] * v[D.BXt];
+  if (comps >= 3) Bt2 = Bt2 + v[D.BXb] * v[D.BXb];
+  const double B2 = Bt2 + v[D.BXn] * v[D.BXn];
+  double cRL = gpr - B2;
+  cRL = cRL + B2 + std::sqrt(cRL * cRL + 4.0 * gpr * Bt2);
+  cRL = std::sqrt(0.5 * cRL / v[V.RHO]);
+  cmax = std::fmax(std::fabs(v[D.Xn] + cRL), std::fabs(v[D.Xn] - cRL));
+  const double sl = -cmax, sr = cmax;
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  kflux(c, V, D, fluxL, vL, uL);
+  kflux(c, V, D, fluxR, vR, uR);
+  for (int n = 0; n < nv_; n++) flux[n] = 0.5 * (fluxL[n] + fluxR[n] + cmax * (uL[n] - uR[n]));
+  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, 0.0, 0.0, E);
+}
 
 // --------------------------------------------------------------------------------------
 // RoeMHD: src/fluid/RiemannSolver/MHDsolvers/roeMHD.hpp:175-651 (HAVE_ENERGY,
@@ -1350,10 +1490,12 @@ struct Oracle {
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
     int ext[3] = {0, 0, 0};
     if (c.mhd) {
-      // hlldMHD.hpp:32-49 (roeMHD.hpp has the same extension)
-      // RoeMHD always extends by one cell (roeMHD.hpp:62-73)
+      // hlldMHD.hpp:32-49 and hllMHD.hpp:32-49 extend by nghost for the uct_hll(d)
+      // averages; RoeMHD and TvdlfMHD always extend by one cell (roeMHD.hpp:62-73,
+      // tvdlfMHD.hpp:30-41)
       int perp = 1;
-      if ((c.emf == ORC_EMF_UCT_HLL || c.emf == ORC_EMF_UCT_HLLD) && c.solver != ORC_SOLVER_ROE)
+      if ((c.emf == ORC_EMF_UCT_HLL || c.emf == ORC_EMF_UCT_HLLD) &&
+          (c.solver == ORC_SOLVER_HLLD || c.solver == ORC_SOLVER_HLL))
         perp = nghost[dir];
       for (int d = 0; d < 3; d++) ext[d] = (d == dir || d >= c.dim) ? 0 : perp;
     }
@@ -1373,9 +1515,15 @@ struct Oracle {
             vL[D.BXn] = vs(dir, k, j, i);
             vR[D.BXn] = vL[D.BXn];
             if (c.solver == ORC_SOLVER_ROE) roe_mhd(c, V, D, dir, vL, vR, F, cmax, E);
+            else if (c.solver == ORC_SOLVER_HLL) hll_mhd(c, V, D, dir, vL, vR, F, cmax, E);
+            else if (c.solver == ORC_SOLVER_TVDLF) tvdlf_mhd(c, V, D, dir, vL, vR, F, cmax, E);
             else hlld_mhd(c, V, D, dir, vL, vR, F, cmax, E);
           } else if (c.solver == ORC_SOLVER_HLLC) {
             hllc_hd(c, V, D, vL, vR, F, cmax);
+          } else if (c.solver == ORC_SOLVER_HLL) {
+            hll_hd(c, V, D, vL, vR, F, cmax);
+          } else if (c.solver == ORC_SOLVER_TVDLF) {
+            tvdlf_hd(c, V, D, vL, vR, F, cmax);
           } else {
             wrote = roe_hd(c, V, D, vL, vR, F, cmax);
           }
@@ -1776,6 +1924,11 @@ void *orc_new(const orc_cfg *cfg) {
   if (cfg->dim < 1 || cfg->dim > 3 || cfg->comps < cfg->dim || cfg->comps > 3) return nullptr;
   if (cfg->mhd && cfg->solver == ORC_SOLVER_HLLC) return nullptr;
   if (!cfg->mhd && cfg->solver == ORC_SOLVER_HLLD) return nullptr;
+  /* HllMHD and TvdlfMHD have no uct_hlld face store (hllMHD.hpp:449-452,
+     tvdlfMHD.hpp:230-233) */
+  if (cfg->mhd && (cfg->solver == ORC_SOLVER_HLL || cfg->solver == ORC_SOLVER_TVDLF) &&
+      cfg->emf == ORC_EMF_UCT_HLLD)
+    return nullptr;
   return new Oracle(*cfg);
 }
 void orc_free(void *o) { delete static_cast<Oracle *>(o); }
@@ -1887,9 +2040,15 @@ int orc_face_flux(const orc_cfg *cfg, int dir, const double *vL_in, const double
   bool wrote = true;
   if (cfg->mhd) {
     if (cfg->solver == ORC_SOLVER_ROE) roe_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
+    else if (cfg->solver == ORC_SOLVER_HLL) hll_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
+    else if (cfg->solver == ORC_SOLVER_TVDLF) tvdlf_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
     else hlld_mhd(*cfg, V, D, dir, vL, vR, flux, *cmax, E);
   } else if (cfg->solver == ORC_SOLVER_HLLC) {
     hllc_hd(*cfg, V, D, vL, vR, flux, *cmax);
+  } else if (cfg->solver == ORC_SOLVER_HLL) {
+    hll_hd(*cfg, V, D, vL, vR, flux, *cmax);
+  } else if (cfg->solver == ORC_SOLVER_TVDLF) {
+    tvdlf_hd(*cfg, V, D, vL, vR, flux, *cmax);
   } else {
     wrote = roe_hd(*cfg, V, D, vL, vR, flux, *cmax);
   }

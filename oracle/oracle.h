@@ -18,8 +18,10 @@ extern "C" {
 
 /* Reconstruction order, same values as the reference's ORDER macro. */
 enum { ORC_RECON_CONSTANT = 1, ORC_RECON_PLM = 2, ORC_RECON_PPM = 4 };
-/* Riemann solvers ([Hydro] solver). ROE is RoeHD or RoeMHD depending on mhd. */
-enum { ORC_SOLVER_HLLC = 1, ORC_SOLVER_HLLD = 2, ORC_SOLVER_ROE = 3 };
+/* Riemann solvers ([Hydro] solver). ROE, HLL and TVDLF are the HD or MHD variant
+   depending on mhd. */
+enum { ORC_SOLVER_HLLC = 1, ORC_SOLVER_HLLD = 2, ORC_SOLVER_ROE = 3, ORC_SOLVER_HLL = 4,
+       ORC_SOLVER_TVDLF = 5 };
 /* EMF averaging ([Hydro] emf). */
 enum { ORC_EMF_ARITHMETIC = 0, ORC_EMF_UCT0 = 1, ORC_EMF_UCT_CONTACT = 2,
        ORC_EMF_UCT_HLL = 3, ORC_EMF_UCT_HLLD = 4 };

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
 RECON = {"constant": 1, "plm": 2, "linear": 2, "ppm": 4, "parabolic": 4}
-SOLVER = {"hllc": 1, "hlld": 2, "roe": 3}
+SOLVER = {"hllc": 1, "hlld": 2, "roe": 3, "hll": 4, "tvdlf": 5}
 EMF = {"arithmetic": 0, "uct0": 1, "uct_contact": 2, "uct_hll": 3, "uct_hlld": 4}
 BC = {"periodic": 0, "outflow": 1, "reflective": 2}
 

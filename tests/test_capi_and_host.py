@@ -44,6 +44,10 @@ def test_ctx_create_rejects_bad_configs_without_gpu_work():
     d.solver = capi.SOLVER["hlld"]
     d.recon = 3  # LimO3 is not built
     assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
+    # HllMHD has no uct_hlld face store (hllMHD.hpp:449-452)
+    for solver in ("hll", "tvdlf"):
+        d.recon, d.solver, d.emf = 2, capi.SOLVER[solver], capi.EMF["uct_hlld"]
+        assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
 
 
 @pytest.mark.parametrize("dim,nx,recon", [(1, (100,), "plm"), (2, (48, 40), "ppm"),

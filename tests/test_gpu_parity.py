@@ -78,6 +78,51 @@ def test_hd_multid_cycles(dim, solver):
     compare_state(o, g, what="HD%dD %s" % (dim, solver))
 
 
+@pytest.mark.parametrize("solver", ["hll", "tvdlf"])
+@pytest.mark.parametrize("emf", ["uct_contact", "uct_hll", "arithmetic", "uct0"])
+def test_orszag_tang_hll_tvdlf_cycles(emf, solver):
+    """HllMHD and TvdlfMHD (test/MHD/OrszagTang/idefix-hll.ini, idefix-tvdlf.ini: PLM, RK2,
+    CFL 0.6)."""
+    kw = cfg_kwargs(2, (48, 40), emf=emf, solver=solver, cfl=0.6)
+    o, g = make_pair(kw, problems.orszag_tang)
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="OT %s %s" % (solver, emf))
+
+
+@pytest.mark.parametrize("solver", ["hll", "tvdlf"])
+@pytest.mark.parametrize("emf", ["uct_hll", "uct_contact"])
+def test_mhd3d_hll_tvdlf_cycles(emf, solver):
+    kw = cfg_kwargs(3, (16, 12, 10), emf=emf, solver=solver, cfl=0.4, nstages=3)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=4, amp=0.2))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="3D %s %s" % (solver, emf))
+
+
+@pytest.mark.parametrize("solver", ["hll", "tvdlf"])
+def test_sod_hll_tvdlf_cycles(solver):
+    """HllHD and TvdlfHD on the Sod tube (test/HD/sod/idefix-hll.ini, idefix-tvdlf.ini)."""
+    kw = cfg_kwargs(1, (1000,), mhd=False, solver=solver, recon="plm", bc="outflow",
+                    gamma=1.4, cfl=0.8)
+    o, g = make_pair(kw, problems.sod)
+    o.set_boundaries()
+    _cycle_both(o, g, 40)
+    compare_state(o, g, what="sod %s" % solver)
+
+
+@pytest.mark.parametrize("solver", ["hll", "tvdlf"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hd_multid_hll_tvdlf_cycles(dim, solver):
+    nx = (40, 24) if dim == 2 else (16, 12, 10)
+    bc = ["outflow", "periodic"] if dim == 2 else "periodic"
+    kw = cfg_kwargs(dim, nx, mhd=False, solver=solver, recon="plm", bc=bc, gamma=1.4, cfl=0.5)
+    o, g = make_pair(kw, lambda gg: problems.random_hd(gg, seed=6, amp=0.3))
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="HD%dD %s" % (dim, solver))
+
+
 def test_outflow_mhd2d():
     kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))

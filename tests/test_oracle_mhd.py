@@ -11,8 +11,8 @@ from idefix_amd import problems
 EMFS = ["uct_hlld", "uct_contact", "arithmetic", "uct0", "uct_hll"]
 
 
-def _ot(emf, recon="plm", n=48, nstages=2):
-    cfg = O.make_cfg(2, mhd=True, recon=recon, solver="hlld", emf=emf, nx=(n, n),
+def _ot(emf, recon="plm", n=48, nstages=2, solver="hlld"):
+    cfg = O.make_cfg(2, mhd=True, recon=recon, solver=solver, emf=emf, nx=(n, n),
                      nstages=nstages, cfl=0.6)
     o = O.Oracle(cfg)
     Vc, Vs = problems.orszag_tang(O.OracleGrid(o))
@@ -32,10 +32,12 @@ def _totals(o):
                      Vs[0, 0, b[1]:e[1], b[0]:e[0]].sum(), Vs[1, 0, b[1]:e[1], b[0]:e[0]].sum()])
 
 
-@pytest.mark.parametrize("emf", EMFS)
-@pytest.mark.parametrize("recon", ["plm", "ppm"])
-def test_ot_divb_and_conservation(emf, recon):
-    o = _ot(emf, recon)
+@pytest.mark.parametrize("emf,recon,solver",
+                         [(e, r, "hlld") for e in EMFS for r in ("plm", "ppm")] +
+                         [(e, "plm", "hll") for e in ("uct_contact", "uct_hll", "arithmetic")] +
+                         [(e, "plm", "tvdlf") for e in ("uct_contact", "uct_hll", "uct0")])
+def test_ot_divb_and_conservation(emf, recon, solver):
+    o = _ot(emf, recon, solver=solver)
     t0 = _totals(o)
     assert o.check_divb() < 1e-13
     for _ in range(25):
@@ -66,7 +68,9 @@ def _wave_error(n, emf, recon="plm", solver="hlld"):
                                                 ("uct_contact", "plm", 32, "hlld"),
                                                 ("uct_hlld", "ppm", 16, "hlld"),
                                                 ("uct_contact", "plm", 32, "roe"),
-                                                ("arithmetic", "ppm", 16, "roe")])
+                                                ("arithmetic", "ppm", 16, "roe"),
+                                                ("uct_hll", "plm", 32, "hll"),
+                                                ("uct_contact", "plm", 32, "tvdlf")])
 def test_linear_fast_wave_second_order(emf, recon, n, solver):
     """After one period the wave returns to its initial state; the L1 error must fall by
     more than 3x per halving of the cell size (observed: 3.5 for PLM, 3.5 for PPM)."""

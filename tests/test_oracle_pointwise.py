@@ -142,7 +142,7 @@ def _rand_mhd():
     return v
 
 
-@pytest.mark.parametrize("solver", ["hllc", "roe"])
+@pytest.mark.parametrize("solver", ["hllc", "roe", "hll", "tvdlf"])
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_hd_flux_consistency(solver, d):
     cfg = _cfg(3, False, solver, comps=3, gamma=1.4)
@@ -155,8 +155,10 @@ def test_hd_flux_consistency(solver, d):
         assert cmax == pytest.approx(abs(v[1 + d]) + a, rel=1e-12)
 
 
-@pytest.mark.parametrize("solver", ["hlld", "roe"])
-@pytest.mark.parametrize("emf", ["uct_contact", "uct_hlld"])
+@pytest.mark.parametrize("solver,emf", [("hlld", "uct_contact"), ("hlld", "uct_hlld"),
+                                        ("roe", "uct_contact"), ("roe", "uct_hlld"),
+                                        ("hll", "uct_contact"), ("hll", "uct_hll"),
+                                        ("tvdlf", "uct_contact"), ("tvdlf", "uct_hll")])
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_mhd_flux_consistency(solver, emf, d):
     cfg = _cfg(3, True, solver, comps=3, emf=emf)
@@ -178,11 +180,12 @@ def _mirror(v, d, mhd):
     return w
 
 
+@pytest.mark.parametrize("solver", ["hllc", "hll", "tvdlf"])
 @pytest.mark.parametrize("d", [0, 1, 2])
-def test_hllc_mirror_symmetry(d):
+def test_hllc_mirror_symmetry(solver, d):
     """Swapping and reflecting the two states reflects the flux: mass and energy fluxes
     change sign, the normal momentum flux does not."""
-    cfg = _cfg(3, False, "hllc", comps=3, gamma=1.4)
+    cfg = _cfg(3, False, solver, comps=3, gamma=1.4)
     for _ in range(300):
         vL, vR = _rand_hd(3), _rand_hd(3)
         F, _, _, _ = O.face_flux(cfg, d, vL, vR)
@@ -192,7 +195,7 @@ def test_hllc_mirror_symmetry(d):
         assert G[4] == pytest.approx(-F[4], rel=1e-12, abs=1e-13)
 
 
-@pytest.mark.parametrize("solver", ["hlld", "roe"])
+@pytest.mark.parametrize("solver", ["hlld", "roe", "hll", "tvdlf"])
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_mhd_mirror_symmetry(solver, d):
     cfg = _cfg(3, True, solver, comps=3)
@@ -206,15 +209,72 @@ def test_mhd_mirror_symmetry(solver, d):
         assert G[4] == pytest.approx(-F[4], rel=1e-11, abs=1e-12)
 
 
-@pytest.mark.parametrize("solver", ["hllc", "roe"])
+@pytest.mark.parametrize("solver", ["hllc", "roe", "hll"])
 def test_supersonic_upwinding(solver):
     cfg = _cfg(3, False, solver, comps=3, gamma=1.4)
     for _ in range(100):
         vL, vR = _rand_hd(3), _rand_hd(3)
         vL[1] = vR[1] = 10.0  # flow to the right faster than any wave
         F, _, _, _ = O.face_flux(cfg, 0, vL, vR)
-        if solver == "hllc":
+        if solver != "roe":
             assert np.allclose(F[:5], euler_flux(vL, 0, 1.4, 3), rtol=1e-13)
+
+
+def _fast_speed(v, d, gamma):
+    """Textbook fast magnetosonic speed: c_f^2 = (a^2 + b^2 + sqrt((a^2 + b^2)^2 -
+    4 a^2 b_n^2)) / 2 with a^2 = gamma p / rho, b^2 = |B|^2 / rho, b_n^2 = B_d^2 / rho."""
+    a2 = gamma * v[4] / v[0]
+    b2 = (v[5:8] @ v[5:8]) / v[0]
+    bn2 = v[5 + d] ** 2 / v[0]
+    return np.sqrt(0.5 * (a2 + b2 + np.sqrt((a2 + b2) ** 2 - 4 * a2 * bn2)))
+
+
+def _cons(v, gamma, mhd):
+    u = v.copy()
+    u[1:4] = v[0] * v[1:4]
+    e = v[4] / (gamma - 1) + 0.5 * v[0] * v[1:4] @ v[1:4]
+    u[4] = e + (0.5 * v[5:8] @ v[5:8] if mhd else 0.0)
+    return u
+
+
+@pytest.mark.parametrize("mhd", [False, True])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hll_satisfies_both_outer_jump_conditions(mhd, d):
+    """The HLL flux is the one whose single intermediate state U* meets the
+    Rankine-Hugoniot conditions across both outer waves (Harten, Lax & van Leer 1983):
+    F = F_L + S_L (U* - U_L) = F_R + S_R (U* - U_R), with S_L, S_R the Davis estimates
+    min/max(v_n -+ c) over the two states and U* their integral average.  The wave speeds
+    come from the textbook formulas, not from the oracle's expressions."""
+    gamma = 5.0 / 3.0 if mhd else 1.4
+    cfg = _cfg(3, mhd, "hll", comps=3, gamma=gamma, **({"emf": "uct_contact"} if mhd else {}))
+    nv = 8 if mhd else 5
+    nsub = 0
+    for _ in range(300):
+        vL, vR = (_rand_mhd(), _rand_mhd()) if mhd else (_rand_hd(3), _rand_hd(3))
+        if mhd:
+            vR[5 + d] = vL[5 + d]
+            c = (_fast_speed(vL, d, gamma), _fast_speed(vR, d, gamma))
+            fL, fR = mhd_flux(vL, d, gamma), mhd_flux(vR, d, gamma)
+        else:
+            c = tuple(np.sqrt(gamma * v[4] / v[0]) for v in (vL, vR))
+            fL, fR = euler_flux(vL, d, gamma, 3), euler_flux(vR, d, gamma, 3)
+        SL = min(vL[1 + d] - c[0], vR[1 + d] - c[1])
+        SR = max(vL[1 + d] + c[0], vR[1 + d] + c[1])
+        F, cmax, _, _ = O.face_flux(cfg, d, vL, vR)
+        F = F[:nv]
+        assert cmax == pytest.approx(max(abs(SL), abs(SR)), rel=1e-12)
+        uL, uR = _cons(vL, gamma, mhd)[:nv], _cons(vR, gamma, mhd)[:nv]
+        if SL > 0:
+            assert np.allclose(F, fL, rtol=1e-12, atol=1e-12)
+        elif SR < 0:
+            assert np.allclose(F, fR, rtol=1e-12, atol=1e-12)
+        else:
+            nsub += 1
+            us = (SR * uR - SL * uL - (fR - fL)) / (SR - SL)
+            scale = 1e-11 * (1 + np.abs(fL) + np.abs(fR) + abs(SL) * np.abs(uL) + abs(SR) * np.abs(uR))
+            assert np.all(np.abs(F - (fL + SL * (us - uL))) < scale)
+            assert np.all(np.abs(F - (fR + SR * (us - uR))) < scale)
+    assert nsub > 100
 
 
 @pytest.mark.parametrize("solver", ["hllc", "hlld"])
@@ -273,3 +333,36 @@ def test_roe_mhd_2d_consistency():
             ref = mhd_flux(v3, d, 5.0 / 3.0)
             assert np.allclose(F[:6], [ref[0], ref[1], ref[2], ref[4], ref[5], ref[6]],
                                rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("mhd", [False, True])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_tvdlf_is_rusanov_flux_of_mean_state(mhd, d):
+    """The local Lax-Friedrichs (Rusanov) flux, Toro 3rd ed. sec. 10.5.1:
+    F = (F_L + F_R)/2 - a (U_R - U_L)/2 with a = |v_n| + c of the arithmetic mean of the
+    two primitive states.  c is the sound speed in hydro.  In MHD the reference's speed
+    (tvdlfMHD.hpp:192-194: cRL = gpr - B2; cRL = cRL + B2 + sqrt(..)) lacks the B^2 term of
+    the fast speed: c^2 = c_f^2 - |B|^2 / (2 rho), with c_f the textbook fast speed."""
+    gamma = 5.0 / 3.0 if mhd else 1.4
+    cfg = _cfg(3, mhd, "tvdlf", comps=3, gamma=gamma)
+    nv = 8 if mhd else 5
+    for _ in range(300):
+        vL, vR = (_rand_mhd(), _rand_mhd()) if mhd else (_rand_hd(3), _rand_hd(3))
+        if mhd:
+            vR[5 + d] = vL[5 + d]
+        vm = 0.5 * (vL + vR)
+        if mhd:
+            c = np.sqrt(_fast_speed(vm, d, gamma) ** 2 - 0.5 * (vm[5:8] @ vm[5:8]) / vm[0])
+        else:
+            c = np.sqrt(gamma * vm[4] / vm[0])
+        a = abs(vm[1 + d]) + c
+        if mhd:
+            fL, fR = mhd_flux(vL, d, gamma), mhd_flux(vR, d, gamma)
+        else:
+            fL, fR = euler_flux(vL, d, gamma, 3), euler_flux(vR, d, gamma, 3)
+        uL, uR = _cons(vL, gamma, mhd)[:nv], _cons(vR, gamma, mhd)[:nv]
+        F, cmax, E, _ = O.face_flux(cfg, d, vL, vR)
+        assert cmax == pytest.approx(a, rel=1e-12)
+        want = 0.5 * (fL + fR) - 0.5 * a * (uR - uL)
+        scale = 1e-11 * (1 + np.abs(fL) + np.abs(fR) + a * (np.abs(uL) + np.abs(uR)))
+        assert np.all(np.abs(F[:nv] - want) < scale)
