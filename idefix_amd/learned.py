@@ -337,7 +337,7 @@ def main(argv=None):
     cases = [("learned_fp32_%d" % N, N, "mlp", "fp32", params),
              ("plain_ppm_fp64_%d" % N, N, "ppm", "fp64", None),
              ("plain_plm_fp64_%d" % N, N, "plm", "fp64", None),
-             ("vanleer_via_mlp_fp32_%d" % N, N, "mlp", "fp32", None),
+             ("ppm_via_mlp_fp32_%d" % N, N, "mlp", "fp32", None),
              ("plain_ppm_fp64_%d" % (2 * N), 2 * N, "ppm", "fp64", None)]
     for name, n, recon, prec, p in cases:
         st = init if n == N else coarse_state(V0, bx0, by0, f // 2, gamma)
