@@ -1,0 +1,101 @@
+"""Command-line driver with the reference's main loop (src/main.cpp:120-260):
+
+    python -m idefix_amd.run -i idefix.ini --setup orszag_tang [-restart [n]]
+                             [-maxcycles N] [-nowrite] [--recon plm|ppm] [--dir DIR]
+
+Reads the input file (Input, src/input.cpp), builds the DataBlock, creates the initial
+state with the named setup (the reference compiles one setup.cpp per test; here the
+setups are functions of idefix_amd.problems) or restarts from a dump, then cycles to
+tstop with the dt clipping of main.cpp:187 and calls the output cadence after every
+cycle.  Options keep the reference's spelling (-i, -restart, -maxcycles, -nowrite).
+"""
+import argparse
+import sys
+import time
+
+from . import capi, ini, problems
+from .output import Output
+
+SETUPS = {
+    # name -> (function, mhd): the reference's tests these setups reproduce
+    "orszag_tang": (problems.orszag_tang, True),        # test/MHD/OrszagTang
+    "orszag_tang_3d": (problems.orszag_tang_3d, True),  # test/MHD/OrszagTang3D
+    "sod": (problems.sod, False),                       # test/HD/sod
+    "kelvin_helmholtz": (problems.kelvin_helmholtz, True),
+    "turbulence": (problems.turbulence, True),
+}
+
+
+def build(blocks, setup, recon="plm", precision="fp64", comps=None):
+    from .datablock import DataBlock
+    fn, mhd = SETUPS[setup]
+    kw, run = ini.block_config(blocks, recon=recon, mhd=mhd, precision=precision, comps=comps)
+    g = DataBlock(**kw)
+    periodic = [all(b == "periodic" for b in kw["bc"][d]) if d < kw["dim"] else True
+                for d in range(3)]
+    return g, fn, run, periodic
+
+
+def main(argv=None):
+    return simulate(argv)[0]
+
+
+def simulate(argv=None):
+    """The main loop; returns (return code, DataBlock, Output)."""
+    ap = argparse.ArgumentParser(prog="idefix_amd.run")
+    ap.add_argument("-i", dest="ini", default="idefix.ini")
+    ap.add_argument("--setup", required=True, choices=sorted(SETUPS))
+    ap.add_argument("--recon", default="plm", choices=["constant", "plm", "ppm"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--comps", type=int, default=None)
+    ap.add_argument("--dir", default=".")
+    ap.add_argument("-restart", nargs="?", type=int, const=-1, default=None)
+    ap.add_argument("-maxcycles", type=int, default=-1)
+    ap.add_argument("-nowrite", action="store_true")
+    a = ap.parse_args(argv)
+
+    blocks = ini.read_ini(a.ini)
+    g, fn, run, periodic = build(blocks, a.setup, a.recon, a.precision, a.comps)
+    out = Output(run["output"], g, directory=a.dir, force_no_write=a.nowrite,
+                 periodic=periodic)
+    restarted = False
+    if a.restart is not None:
+        restarted = out.restart_from_dump(g, a.restart)
+        print("Main: restart %s." % ("done" if restarted else "aborted"))
+        if restarted:
+            g.set_boundaries()
+    if not restarted:
+        Vc, Vs = fn(g)
+        g.upload(Vc, Vs)
+        g.set_boundaries()
+        out.check_for_writes(g)
+    tstop = run["tstop"]
+    rc = 0
+    t0 = time.perf_counter()
+    n0 = g.ncycles
+    while g.t < tstop:
+        if tstop - g.t < g.dt:
+            g.dt = tstop - g.t
+        try:
+            g.cycle()
+        except capi.IdfxError as e:  # the TimeIntegrator's exceptions (NaN, small dt)
+            print("Main: WARNING! Caught an exception in TimeIntegrator.\n%s" % e)
+            out.force_write_vtk(g)
+            rc = 1
+            break
+        out.check_for_writes(g)
+        if 0 <= a.maxcycles <= g.ncycles:
+            print("Main: Reached maximum number of integration cycles.")
+            break
+    g.torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = g.nx[0] * g.nx[1] * g.nx[2]
+    ncyc = g.ncycles - n0
+    print("Main: Reached t=%.10g after %d cycles" % (g.t, ncyc))
+    if ncyc > 0:
+        print("Main: Perfs are %.4g cell updates/second" % (cells * ncyc / wall))
+    return rc, g, out
+
+
+if __name__ == "__main__":
+    sys.exit(main())
