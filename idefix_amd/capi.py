@@ -14,7 +14,7 @@ OK = 0
 E_ARGS, E_LAUNCH, E_NAN, E_DT_SMALL, E_FIRST_DT, E_ALLOC = 1, 2, 3, 4, 5, 6
 
 SOLVER = {"hllc": 1, "hlld": 2, "roe": 3, "hll": 4, "tvdlf": 5}
-RECON = {"constant": 1, "plm": 2, "linear": 2, "ppm": 4, "parabolic": 4, "mlp": 8}
+RECON = {"constant": 1, "plm": 2, "linear": 2, "limo3": 3, "ppm": 4, "parabolic": 4, "mlp": 8}
 EMF = {"arithmetic": 0, "uct0": 1, "uct_contact": 2, "uct_hll": 3, "uct_hlld": 4}
 BC = {"periodic": 0, "outflow": 1, "reflective": 2, "userdef": 3}
 PREC = {"fp64": 0, "fp32": 1}

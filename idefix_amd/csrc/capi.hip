@@ -26,6 +26,9 @@ void register_mhd2c3(Ops *t);
 void register_mlp(Ops *t);
 void register_hll(Ops *t);
 void register_tvdlf(Ops *t);
+void register_limo3_hd(Ops *t);
+void register_limo3_mhd(Ops *t);
+void register_limo3_mhd_b(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -43,6 +46,9 @@ Ops *ops_table() {
     register_mlp(t);
     register_hll(t);
     register_tvdlf(t);
+    register_limo3_hd(t);
+    register_limo3_mhd(t);
+    register_limo3_mhd_b(t);
     return t;
   }();
   return table;
@@ -310,7 +316,7 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   *out = nullptr;
   const idfx_desc &d = *desc;
   if (d.dim < 1 || d.dim > 3 || d.comps < d.dim || d.comps > 3) return IDFX_E_ARGS;
-  if (d.recon != 1 && d.recon != 2 && d.recon != 4 && d.recon != IDFX_RECON_MLP) return IDFX_E_ARGS;
+  if (d.recon < 1 || (d.recon > 4 && d.recon != IDFX_RECON_MLP)) return IDFX_E_ARGS;
   const bool any_mode = d.solver == IDFX_SOLVER_ROE || d.solver == IDFX_SOLVER_HLL ||
                         d.solver == IDFX_SOLVER_TVDLF;
   if (d.mhd && d.solver != IDFX_SOLVER_HLLD && !any_mode) return IDFX_E_ARGS;

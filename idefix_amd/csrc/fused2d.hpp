@@ -111,7 +111,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       }
       if (n == BX2) continue;
       T dummy;
-      recon_cell<C>(wv, n, dummy, carry[n]);
+      recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, 1, c0 - 1));
     }
     if constexpr (CARRY) {
       if constexpr (HLLE) {
@@ -158,7 +158,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         T wv[NW];
 #pragma unroll
         for (int q = 0; q < NW; q++) wv[q] = win[q][n];
-        recon_cell<C>(wv, n, vR[n], vr_next[n]);
+        recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, 1, f));
       }
       const double b2old = ld32(Vs1_in, (unsigned)f * vsd + col);
       vL[BX2] = (T)b2old;

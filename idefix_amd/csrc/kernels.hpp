@@ -154,7 +154,7 @@ __device__ __forceinline__ void store_face_emf(const EmfArrays<T> &e, unsigned p
 // ExtrapolatePrimVar computes them for the two faces (extrapolateToFaces.hpp:88-124,
 // :247-306).
 template <class C, typename T>
-__device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr) {
+__device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr, T dxc = T(0)) {
   if constexpr (C::ORDER == RECON_CONSTANT) {
     vl = w[0];
     vr = w[0];
@@ -165,6 +165,20 @@ __device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr) {
     const T dv = vanleer(dvp, dvm);
     vl = w[1] - T(0.5) * dv;
     vr = w[1] + T(0.5) * dv;
+  } else if constexpr (C::ORDER == RECON_LIMO3) {
+    // extrapolateToFaces.hpp:170-246 (no shock flattening); dxc is the cell's width.  The
+    // positivity fallback calls MinModLim, which as written always returns 0
+    // (slopeLimiter.hpp:16-23): the state falls back to the cell value.
+    const T dvm = w[1] - w[0];
+    const T dvp = w[2] - w[1];
+    T dv = dvp * limo3_lim(dvp, dvm, dxc);
+    vr = w[1] + T(0.5) * dv;
+    dv = dvm * limo3_lim(dvm, dvp, dxc);
+    vl = w[1] - T(0.5) * dv;
+    if (nv == C::RHO || nv == C::ENG) {
+      if (vr <= T(0)) vr = w[1] + T(0.5) * T(0);
+      if (vl <= T(0)) vl = w[1] - T(0.5) * T(0);
+    }
   } else {
     // w[0..4] = V(c-2) .. V(c+2)
     ppm_states(w[0], w[1], w[2], w[3], w[4], vl, vr);
@@ -182,7 +196,15 @@ __device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr) {
 }
 template <class C>
 constexpr int window() {
-  return C::ORDER == RECON_CONSTANT ? 1 : ((C::ORDER == RECON_PLM || C::ORDER == RECON_MLP) ? 3 : 5);
+  return C::ORDER == RECON_CONSTANT
+             ? 1
+             : ((C::ORDER == RECON_PLM || C::ORDER == RECON_LIMO3 || C::ORDER == RECON_MLP) ? 3 : 5);
+}
+// Width of cell `cell` along dir, the only geometric input of a reconstruction (LimO3).
+template <class C, typename T>
+__device__ __forceinline__ T recon_dx(const GridDesc &g, int dir, int cell) {
+  if constexpr (C::ORDER == RECON_LIMO3) return (T)ld32(g.dx[dir], (unsigned)cell);
+  else return T(0);
 }
 
 // Learned-limiter reconstruction of one cell from the window V(c-1), V(c), V(c+1) and
@@ -332,7 +354,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (C::MHD && n == C::BX1) { vl[n] = vr[n] = wv[H]; continue; }  // normal field: Vs
         recon_mlp<C, IDIR>(wv, n, a.mlpw, g.sv, row + (unsigned)cc, 1u, vl[n], vr[n]);
       } else {
-        recon_cell<C>(wv, n, vl[n], vr[n]);
+        recon_cell<C>(wv, n, vl[n], vr[n], recon_dx<C, T>(g, 0, cc));
       }
     }
     T vL[NV], vR[NV];
@@ -464,7 +486,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (C::MHD && n == C::template BXn<DIR>()) { carry[n] = wv[H]; continue; }
         recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)(c0 - 1) * sd, sd, dummy, carry[n]);
       } else {
-        recon_cell<C>(wv, n, dummy, carry[n]);
+        recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, DIR, c0 - 1));
       }
     }
     for (int f = c0; f <= c1; f++) {
@@ -499,7 +521,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
             recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)f * sd, sd, vR[n], vr_next[n]);
           }
         } else {
-          recon_cell<C>(wv, n, vR[n], vr_next[n]);
+          recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, DIR, f));
         }
         vL[n] = carry[n];
       }

@@ -16,7 +16,7 @@ namespace idfx {
 
 constexpr int IDIR = 0, JDIR = 1, KDIR = 2;
 // RECON_MLP: the learned limiter (no counterpart in the reference), 3 ghost cells
-enum Recon { RECON_CONSTANT = 1, RECON_PLM = 2, RECON_PPM = 4, RECON_MLP = 8 };
+enum Recon { RECON_CONSTANT = 1, RECON_PLM = 2, RECON_LIMO3 = 3, RECON_PPM = 4, RECON_MLP = 8 };
 enum Solver { SOLVER_HLLC = 1, SOLVER_HLLD = 2, SOLVER_ROE = 3, SOLVER_HLL = 4,
               SOLVER_TVDLF = 5 };
 enum EmfScheme { EMF_ARITHMETIC = 0, EMF_UCT0 = 1, EMF_UCT_CONTACT = 2, EMF_UCT_HLL = 3,
@@ -78,6 +78,25 @@ __device__ __forceinline__ T mclim(T dvp, T dvm) {  // slopeLimiter.hpp:67-75
     dq = fab(d2q) < fab(dqc) ? d2q : dqc;
   }
   return dq;
+}
+// LimO3Lim, slopeLimiter.hpp:25-51: the third-order limiter of Cada & Torrilhon (2009)
+// with r = 0.1; the limited slope of a cell is dvp * limo3_lim(dvp, dvm, dx).
+template <typename T>
+__device__ __forceinline__ T limo3_lim(T dvp, T dvm, T dx) {
+  const T r = T(0.1), eps = T(1.e-12);
+  const T th = dvm / (dvp + T(1.e-16));
+  const T q = (T(2) + th) / T(3);
+  T a = fmn(T(1.5), T(2) * th);
+  a = fmn(q, a);
+  const T b = fmx(T(-0.5) * th, a);
+  const T c = fmn(q, b);
+  T psi = fmx(T(0), c);
+  T eta = r * dx;
+  eta = (dvm * dvm + dvp * dvp) / (eta * eta);
+  if (eta <= T(1) - eps) return q;
+  if (eta >= T(1) + eps) return psi;
+  psi = (T(1) - (eta - T(1)) / eps) * q + (T(1) + (eta - T(1)) / eps) * psi;
+  return T(0.5) * psi;
 }
 template <typename T>
 __device__ __forceinline__ int sgn(T v) { return (T(0) < v) - (v < T(0)); }

@@ -41,6 +41,7 @@ extern "C" {
 /* reconstruction, the reference's ORDER (CMakeLists.txt:173-213) */
 #define IDFX_RECON_CONSTANT 1
 #define IDFX_RECON_PLM 2
+#define IDFX_RECON_LIMO3 3  /* LimO3 (extrapolateToFaces.hpp:170-246), 2 ghost cells */
 #define IDFX_RECON_PPM 4
 /* No counterpart in the reference: the learned limiter (per-interface MLP weights inside
  * the flux form, PPM fallback), 3 ghost cells, three-component MHD only. */

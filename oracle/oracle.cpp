@@ -78,6 +78,34 @@ inline double FMAX(double a, double b) { return std::fmax(a, b); }
 inline double vanleer(double dvp, double dvm) {
   return (dvp * dvm > 0.0 ? 2.0 * dvp * dvm / (dvp + dvm) : 0.0);
 }
+// LimO3Lim, slopeLimiter.hpp:25-51 (Cada & Torrilhon 2009 third-order limiter, r = 0.1):
+// the limited slope of a cell is dvp * LimO3Lim(dvp, dvm, dx).
+inline double limo3lim(double dvp, double dvm, double dx) {
+  const double r = 0.1, eps = 1.e-12;
+  const double th = dvm / (dvp + 1.e-16);
+  const double q = (2.0 + th) / 3.0;
+  double a = FMIN(1.5, 2.0 * th);
+  a = FMIN(q, a);
+  const double b = FMAX(-0.5 * th, a);
+  const double cc = FMIN(q, b);
+  double psi = FMAX(0.0, cc);
+  double eta = r * dx;
+  eta = (dvm * dvm + dvp * dvp) / (eta * eta);
+  double lim;
+  if (eta <= 1.0 - eps) {
+    lim = q;
+  } else if (eta >= 1.0 + eps) {
+    lim = psi;
+  } else {
+    psi = (1.0 - (eta - 1.0) / eps) * q + (1.0 + (eta - 1.0) / eps) * psi;
+    lim = 0.5 * psi;
+  }
+  return lim;
+}
+// MinModLim, slopeLimiter.hpp:16-23, as written: the limited value is assigned to a
+// variable declared inside the if-block, so the function always returns 0 and the
+// positivity fallback of LimO3 is first order.
+inline double minmod_as_built(double, double) { return 0.0; }
 // McLim, slopeLimiter.hpp:67-75
 inline double mclim(double dvp, double dvm) {
   double dq = 0;
@@ -1449,6 +1477,25 @@ struct Oracle {
         dvp = q(1) - q(0);
         dv = vanleer(dvp, dvm);
         vR[nv] = q(0) - 0.5 * dv;
+      } else if (c.order == 3) {
+        // LimO3, extrapolateToFaces.hpp:170-246 (no shock flattening)
+        const int idx = io * i + jo * j + ko * k;
+        double dvm = q(-1) - q(-2);
+        double dvp = q(0) - q(-1);
+        double dv = dvp * limo3lim(dvp, dvm, dx[dir][idx - 1]);
+        vL[nv] = q(-1) + 0.5 * dv;
+        if ((nv == V.RHO || nv == V.ENG) && vL[nv] <= 0.0) {
+          dv = minmod_as_built(dvp, dvm);
+          vL[nv] = q(-1) + 0.5 * dv;
+        }
+        dvm = dvp;
+        dvp = q(1) - q(0);
+        dv = dvm * limo3lim(dvm, dvp, dx[dir][idx]);
+        vR[nv] = q(0) - 0.5 * dv;
+        if ((nv == V.RHO || nv == V.ENG) && vR[nv] <= 0.0) {
+          dv = minmod_as_built(dvp, dvm);
+          vR[nv] = q(0) - 0.5 * dv;
+        }
       } else {
         double vm2 = q(-3), vm1 = q(-2), v0 = q(-1), vp1 = q(0), vp2 = q(1);
         double vr, vl;
@@ -2060,6 +2107,7 @@ int orc_face_flux(const orc_cfg *cfg, int dir, const double *vL_in, const double
 }
 double orc_vanleer(double dvp, double dvm) { return vanleer(dvp, dvm); }
 double orc_mclim(double dvp, double dvm) { return mclim(dvp, dvm); }
+double orc_limo3lim(double dvp, double dvm, double dx) { return limo3lim(dvp, dvm, dx); }
 void orc_ppm_states(double vm2, double vm1, double v0, double vp1, double vp2, double *vl,
                     double *vr) {
   ppm_states(vm2, vm1, v0, vp1, vp2, *vl, *vr);

@@ -17,7 +17,7 @@ extern "C" {
 #endif
 
 /* Reconstruction order, same values as the reference's ORDER macro. */
-enum { ORC_RECON_CONSTANT = 1, ORC_RECON_PLM = 2, ORC_RECON_PPM = 4 };
+enum { ORC_RECON_CONSTANT = 1, ORC_RECON_PLM = 2, ORC_RECON_LIMO3 = 3, ORC_RECON_PPM = 4 };
 /* Riemann solvers ([Hydro] solver). ROE, HLL and TVDLF are the HD or MHD variant
    depending on mhd. */
 enum { ORC_SOLVER_HLLC = 1, ORC_SOLVER_HLLD = 2, ORC_SOLVER_ROE = 3, ORC_SOLVER_HLL = 4,
@@ -95,6 +95,7 @@ int orc_face_flux(const orc_cfg *cfg, int dir, const double *vL, const double *v
                   double *flux, double *cmax, double *emf7);
 double orc_vanleer(double dvp, double dvm);
 double orc_mclim(double dvp, double dvm);
+double orc_limo3lim(double dvp, double dvm, double dx);
 void orc_ppm_states(double vm2, double vm1, double v0, double vp1, double vp2,
                     double *vl, double *vr);
 void orc_prim2cons_pt(const orc_cfg *cfg, const double *v, double *u);

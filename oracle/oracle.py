@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
-RECON = {"constant": 1, "plm": 2, "linear": 2, "ppm": 4, "parabolic": 4}
+RECON = {"constant": 1, "plm": 2, "linear": 2, "limo3": 3, "ppm": 4, "parabolic": 4}
 SOLVER = {"hllc": 1, "hlld": 2, "roe": 3, "hll": 4, "tvdlf": 5}
 EMF = {"arithmetic": 0, "uct0": 1, "uct_contact": 2, "uct_hll": 3, "uct_hlld": 4}
 BC = {"periodic": 0, "outflow": 1, "reflective": 2}
@@ -68,7 +68,7 @@ def lib():
             "orc_get_dt": (d, [vp]), "orc_set_t": (None, [vp, d]),
             "orc_set_dt": (None, [vp, d]), "orc_get_ncycles": (ctypes.c_long, [vp]),
             "orc_face_flux": (i, [cp, i, dp, dp, dp, dp, dp]),
-            "orc_vanleer": (d, [d, d]), "orc_mclim": (d, [d, d]),
+            "orc_vanleer": (d, [d, d]), "orc_mclim": (d, [d, d]), "orc_limo3lim": (d, [d, d, d]),
             "orc_ppm_states": (None, [d, d, d, d, d, dp, dp]),
             "orc_prim2cons_pt": (None, [cp, dp, dp]), "orc_cons2prim_pt": (None, [cp, dp, dp]),
         }
@@ -277,6 +277,10 @@ def vanleer(dvp, dvm):
 
 def mclim(dvp, dvm):
     return lib().orc_mclim(dvp, dvm)
+
+
+def limo3lim(dvp, dvm, dx):
+    return lib().orc_limo3lim(dvp, dvm, dx)
 
 
 def ppm_states(vm2, vm1, v0, vp1, vp2):

@@ -42,7 +42,9 @@ def test_ctx_create_rejects_bad_configs_without_gpu_work():
     h = ctypes.c_void_p()
     assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
     d.solver = capi.SOLVER["hlld"]
-    d.recon = 3  # LimO3 is not built
+    d.recon = 5  # no such reconstruction
+    assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
+    d.recon, d.solver = 3, capi.SOLVER["roe"]  # LimO3 with Roe MHD is not built
     assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
     # HllMHD has no uct_hlld face store (hllMHD.hpp:449-452)
     for solver in ("hll", "tvdlf"):

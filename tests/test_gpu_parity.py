@@ -123,6 +123,53 @@ def test_hd_multid_hll_tvdlf_cycles(dim, solver):
     compare_state(o, g, what="HD%dD %s" % (dim, solver))
 
 
+@pytest.mark.parametrize("solver,emf,recon", [("hlld", "uct_contact", "limo3"),
+                                              ("hlld", "uct_hlld", "limo3"),
+                                              ("hll", "uct_hll", "limo3"),
+                                              ("tvdlf", "uct_contact", "limo3"),
+                                              ("hll", "uct_contact", "ppm"),
+                                              ("tvdlf", "uct_hll", "ppm")])
+def test_orszag_tang_reconstruction_matrix(solver, emf, recon):
+    """The reference's OrszagTang test runs every solver input with reconstructions 2, 3
+    and 4 (test/MHD/OrszagTang/testme.py:46-53); the LimO3 and PPM members here."""
+    kw = cfg_kwargs(2, (48, 40), emf=emf, solver=solver, recon=recon, cfl=0.6)
+    o, g = make_pair(kw, problems.orszag_tang)
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="OT %s %s %s" % (solver, emf, recon))
+
+
+def test_mhd3d_limo3_cycles():
+    kw = cfg_kwargs(3, (16, 12, 10), emf="uct_contact", solver="hlld", recon="limo3",
+                    cfl=0.4, nstages=3)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=8, amp=0.2))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="3D hlld limo3")
+
+
+@pytest.mark.parametrize("solver,recon", [("hllc", "limo3"), ("roe", "limo3"),
+                                          ("hll", "limo3"), ("tvdlf", "limo3"),
+                                          ("hll", "ppm"), ("tvdlf", "ppm")])
+def test_sod_reconstruction_matrix(solver, recon):
+    """test/HD/sod/testme.py runs its solver inputs with reconstructions 2-4."""
+    kw = cfg_kwargs(1, (1000,), mhd=False, solver=solver, recon=recon, bc="outflow",
+                    gamma=1.4, cfl=0.8)
+    o, g = make_pair(kw, problems.sod)
+    o.set_boundaries()
+    _cycle_both(o, g, 40)
+    compare_state(o, g, what="sod %s %s" % (solver, recon))
+
+
+def test_hd2d_limo3_cycles():
+    kw = cfg_kwargs(2, (40, 24), mhd=False, solver="hllc", recon="limo3",
+                    bc=["outflow", "periodic"], gamma=1.4, cfl=0.5)
+    o, g = make_pair(kw, lambda gg: problems.random_hd(gg, seed=9, amp=0.3))
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="HD2D limo3")
+
+
 def test_outflow_mhd2d():
     kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))

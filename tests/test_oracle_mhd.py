@@ -35,7 +35,11 @@ def _totals(o):
 @pytest.mark.parametrize("emf,recon,solver",
                          [(e, r, "hlld") for e in EMFS for r in ("plm", "ppm")] +
                          [(e, "plm", "hll") for e in ("uct_contact", "uct_hll", "arithmetic")] +
-                         [(e, "plm", "tvdlf") for e in ("uct_contact", "uct_hll", "uct0")])
+                         [(e, "plm", "tvdlf") for e in ("uct_contact", "uct_hll", "uct0")] +
+                         [(e, "limo3", sv) for e, sv in (("uct_contact", "hlld"),
+                                                         ("uct_hlld", "hlld"),
+                                                         ("uct_hll", "hll"),
+                                                         ("uct_contact", "tvdlf"))])
 def test_ot_divb_and_conservation(emf, recon, solver):
     o = _ot(emf, recon, solver=solver)
     t0 = _totals(o)
@@ -70,7 +74,8 @@ def _wave_error(n, emf, recon="plm", solver="hlld"):
                                                 ("uct_contact", "plm", 32, "roe"),
                                                 ("arithmetic", "ppm", 16, "roe"),
                                                 ("uct_hll", "plm", 32, "hll"),
-                                                ("uct_contact", "plm", 32, "tvdlf")])
+                                                ("uct_contact", "plm", 32, "tvdlf"),
+                                                ("uct_hlld", "limo3", 32, "hlld")])
 def test_linear_fast_wave_second_order(emf, recon, n, solver):
     """After one period the wave returns to its initial state; the L1 error must fall by
     more than 3x per halving of the cell size (observed: 3.5 for PLM, 3.5 for PPM)."""

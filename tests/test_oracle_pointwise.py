@@ -37,6 +37,49 @@ def test_mc_limiter_is_minmod_of_three():
         assert O.mclim(a, b) == pytest.approx(minmod3(2 * a, 2 * b, 0.5 * (a + b)), rel=1e-15, abs=0)
 
 
+def _limo3_face(vm, v0, vp, dx):
+    """Left state of the face between cells 0 and +1 from the LimO3 slope of cell 0."""
+    dvm, dvp = v0 - vm, vp - v0
+    return v0 + 0.5 * dvp * O.limo3lim(dvp, dvm, dx)
+
+
+def test_limo3_third_order_on_smooth_quadratics():
+    """Where the differences are small against r dx (r = 0.1) LimO3 is the unlimited
+    third-order interface value (-V[-1] + 5 V[0] + 2 V[+1]) / 6, exact for the cell
+    averages of any quadratic (Cada & Torrilhon 2009, eq. 3.6)."""
+    h = 0.01
+    used = 0
+    for _ in range(500):
+        c = rng.normal(size=3) * np.array([1.0, 0.01, 0.01])
+        x0 = rng.uniform(0.1, 0.9)
+        avg = [_cell_avg_poly(c, x0 + (s - 0.5) * h, x0 + (s + 0.5) * h) for s in (-1, 0, 1)]
+        dvm, dvp = avg[1] - avg[0], avg[2] - avg[1]
+        if (dvm ** 2 + dvp ** 2) / (0.1 * h) ** 2 > 1.0 - 1e-9:
+            continue  # outside the smooth regime
+        used += 1
+        exact = c[0] + c[1] * (x0 + h / 2) + c[2] * (x0 + h / 2) ** 2
+        assert _limo3_face(*avg, h) == pytest.approx(exact, rel=0, abs=1e-13)
+    assert used > 400
+
+
+def test_limo3_limited_regime_properties():
+    """Above the smoothness threshold the limiter lies in [0, 1.5], never exceeds twice the
+    slope ratio on monotone data (no new extremum at the face), and reconstructs linear
+    data exactly in both regimes."""
+    for _ in range(3000):
+        dvp, dvm = rng.normal(size=2)
+        dx = 10 ** rng.uniform(-3, -1)  # differences of order 1: the limited regime
+        lim = O.limo3lim(dvp, dvm, dx)
+        th = dvm / dvp
+        assert 0.0 <= lim <= 1.5
+        if th >= 0:
+            assert lim <= 2 * th * (1 + 1e-12) + 1e-300
+        else:
+            assert lim <= -0.5 * th * (1 + 1e-12)
+    for dx in (1e-3, 1.0, 1e3):
+        assert O.limo3lim(0.7, 0.7, dx) == pytest.approx(1.0, rel=1e-15)
+
+
 def _cell_avg_poly(c, xl, xr):
     # average of sum c_k x^k over [xl, xr]
     return sum(ck * (xr ** (k + 1) - xl ** (k + 1)) / (k + 1) for k, ck in enumerate(c)) / (xr - xl)

@@ -38,7 +38,9 @@ def _sod(solver, recon="plm", n=1000):
 
 
 @pytest.mark.parametrize("solver,recon", [("hllc", "plm"), ("hllc", "ppm"), ("roe", "plm"),
-                                          ("hll", "plm"), ("tvdlf", "plm")])
+                                          ("hll", "plm"), ("tvdlf", "plm"),
+                                          ("hllc", "limo3"), ("roe", "limo3"),
+                                          ("hll", "limo3"), ("tvdlf", "limo3")])
 def test_sod_pressure_error(solver, recon):
     """Mean |p - p_exact| < 2e-3: the reference's own acceptance bound for its Sod test
     (test/HD/sod/python/testidefix.py:71-77), here at the 1000-cell configuration."""
