@@ -327,6 +327,10 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   if (d.mhd && (d.solver == IDFX_SOLVER_HLL || d.solver == IDFX_SOLVER_TVDLF) &&
       d.emf == IDFX_EMF_UCT_HLLD)
     return IDFX_E_ARGS;
+  // passive tracers ride on the fused sweeps; the flux-array path of the 1-D Roe HD
+  // solver has no tracer step
+  if (d.ntracer < 0 || d.ntracer > 64) return IDFX_E_ARGS;
+  if (d.ntracer > 0 && d.dim == 1 && !d.mhd && d.solver == IDFX_SOLVER_ROE) return IDFX_E_ARGS;
   const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver, d.precision == IDFX_PREC_FP32);
   if (key < 0 || key >= kOpsTableSize) return IDFX_E_ARGS;
   const Ops *ops = &ops_table()[key];
@@ -350,7 +354,8 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
     c->g.end[q] = c->nghost[q] + (act ? d.nx[q] : 1);
     c->g.vsn[q] = c->vsn[q];
   }
-  c->nvar = ops->nvar;
+  c->nvar = ops->nvar + d.ntracer;
+  c->g.ntr = d.ntracer;
   c->n3 = (long long)c->np_tot[0] * c->np_tot[1] * c->np_tot[2];
   // kernels address each variable (and each Vs component) with 32-bit byte offsets
   if ((long long)c->vsn[0] * c->vsn[1] * c->vsn[2] >= (1LL << 29)) {
@@ -398,7 +403,8 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
     c->net.b3 = c->mlp_b + 128;
     c->net.valid = 0;
   }
-  if (st == IDFX_OK && d.mhd && d.dim == 2 && ops->fused2d_y && !needs_unfused(c)) {
+  if (st == IDFX_OK && d.mhd && d.dim == 2 && ops->fused2d_y && !needs_unfused(c) &&
+      d.ntracer == 0) {
     bool user = false;
     for (int q = 0; q < 2; q++)
       user = user || d.lbound[q] == IDFX_BC_USERDEF || d.rbound[q] == IDFX_BC_USERDEF;

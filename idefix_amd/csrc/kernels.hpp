@@ -28,6 +28,7 @@ struct GridDesc {
   long long svs;  // stride between components of Vs
   const double *dx[3];
   double gamma;
+  int ntr;      // passive tracers, stored after the fluid variables of Vc/Uc
 };
 
 template <typename T>
@@ -257,6 +258,46 @@ __device__ __forceinline__ void block_max_atomic(unsigned long long v, unsigned 
   }
 }
 
+// Passive tracers of one cell (tracer.hpp:58-111, tracer.cpp:13-47): the face value of
+// each tracer is upwinded by the sign of the Riemann mass flux with a van Leer slope, its
+// flux is F_rho * vface * A, and rho*tracer is updated like the fluid variables.  Both
+// faces of the cell are evaluated here; the neighbouring cell evaluates the shared face
+// from the same inputs, so the update stays conservative.
+template <class C>
+__device__ __forceinline__ double tracer_face(const double *V, unsigned p, unsigned sd,
+                                              double frho) {
+  // face between the cells at p - sd and p
+  if (frho > 0) {
+    const double vm2 = ld32(V, p - 2 * sd), vm1 = ld32(V, p - sd), v0 = ld32(V, p);
+    const double dv = vanleer(v0 - vm1, vm1 - vm2);
+    return vm1 + 0.5 * dv;
+  }
+  const double vm1 = ld32(V, p - sd), v0 = ld32(V, p), vp1 = ld32(V, p + sd);
+  const double dv = vanleer(vp1 - v0, v0 - vm1);
+  return v0 - 0.5 * dv;
+}
+template <class C, int DIR>
+__device__ __forceinline__ void tracer_update(const GridDesc &g, const SweepArgs &a, unsigned p,
+                                           double dtdV, double A, double fm, double fp) {
+  const unsigned sd = DIR == 0 ? 1u : DIR == 1 ? (unsigned)g.nt[0] : (unsigned)(g.nt[0] * g.nt[1]);
+  const double rho = ld32(a.Vc + C::RHO * g.sv, p);
+  for (int t = 0; t < g.ntr; t++) {
+    const long long n = C::NVAR + t;
+    const double *V = a.Vc + n * g.sv;
+    double u;
+    if (a.flags & (SW_INIT_UC | SW_SAVE_UC0)) {
+      const double u0 = ld32(V, p) * rho;  // Tracer::ConvertPrimToCons
+      if (a.flags & SW_SAVE_UC0) st32(a.Uc0 + n * g.sv, p, u0);
+      u = (a.flags & SW_INIT_UC) ? u0 : ld32((const double *)a.Uc + n * g.sv, p);
+    } else {
+      u = ld32((const double *)a.Uc + n * g.sv, p);
+    }
+    const double ftp = fp * tracer_face<C>(V, p + sd, sd, fp) * A;
+    const double ftm = fm * tracer_face<C>(V, p, sd, fm) * A;
+    st32(a.Uc + n * g.sv, p, u + -dtdV * (ftp - ftm));
+  }
+}
+
 // Update of one cell by the flux difference along DIR (calcRightHandSide.hpp:288-549,
 // Cartesian, with the area correction of :156-194 applied to both faces first).
 template <class C, int DIR, typename T>
@@ -296,6 +337,7 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     const double rhs = -dtdV * (fp - fm);
     st32(a.Uc + n * g.sv, p, u[n] + rhs);
   }
+  if (g.ntr > 0) tracer_update<C, DIR>(g, a, p, dtdV, A, (double)Fm[C::RHO], (double)Fp[C::RHO]);
   const int ig = DIR == 0 ? i : DIR == 1 ? j : k;
   const double dl = g.dx[DIR][ig];
   const double term = 0.5 * ((double)cmaxp + (double)cmaxm) / (dl);
@@ -870,6 +912,8 @@ __global__ __launch_bounds__(256) void k_prim_to_cons(GridDesc g, const double *
   cell_p2c<C>(g, Vc, p, u);
 #pragma unroll
   for (int n = 0; n < C::NVAR; n++) Uc[n * g.sv + p] = u[n];
+  // Tracer::ConvertPrimToCons (tracer.cpp:32-47)
+  for (int n = C::NVAR; n < C::NVAR + g.ntr; n++) Uc[n * g.sv + p] = Vc[n * g.sv + p] * Vc[C::RHO * g.sv + p];
 }
 
 // ReconstructVcField (boundary.hpp:388-404) + AddAndStore of the cell state
@@ -906,6 +950,13 @@ __global__ __launch_bounds__(256) void k_cons_to_prim(GridDesc g, double *Uc, co
   if (flags & 2) {
 #pragma unroll
     for (int n = 0; n < C::NVAR; n++) Uc[n * g.sv + p] = u[n];
+  }
+  // tracers: AddAndStore, then Tracer::ConvertConsToPrim (tracer.cpp:13-28)
+  for (int n = C::NVAR; n < C::NVAR + g.ntr; n++) {
+    double ut = Uc[n * g.sv + p];
+    if (flags & 1) ut = wc * ut + w0 * Uc0[n * g.sv + p];
+    Vc[n * g.sv + p] = ut / u[C::RHO];
+    if (flags & 2) Uc[n * g.sv + p] = ut;
   }
 }
 
@@ -976,8 +1027,7 @@ __global__ __launch_bounds__(256) void k_boundary_periodic(GridDesc g, double *V
       if (f < 0) {
         const long long ps = idx3(g, src[2], src[1], src[0]);
         const long long pd = idx3(g, q[2], q[1], q[0]);
-#pragma unroll
-        for (int n = 0; n < C::NVAR; n++) {
+        for (int n = 0; n < C::NVAR + g.ntr; n++) {
           if (C::MHD && n >= C::BX1 && n < C::BX1 + C::DIM) {
             // ReconstructVcField: the average of the two faces as the fill leaves them
             // (each face at the image of its ghost coordinates)
@@ -1120,7 +1170,9 @@ __global__ __launch_bounds__(256) void k_boundary(GridDesc g, double *Vc, double
   q[dir] = side ? (ng + nxd + q[dir]) : q[dir];
   int r[3] = {q[0], q[1], q[2]};
   if (f < 0) {
-    for (int n = 0; n < C::NVAR; n++) {
+    // tracers included (boundary.hpp:135-136); with MHD the reflective sign test n >= BX1
+    // also flips them (boundary.hpp:697)
+    for (int n = 0; n < C::NVAR + g.ntr; n++) {
       double val;
       if (bc == BC_PERIODIC) {
         r[dir] = ng + (q[dir] + ng * (nint - 1)) % nint;

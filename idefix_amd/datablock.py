@@ -43,7 +43,7 @@ class DataBlock:
     def __init__(self, dim, nx, xbeg=(0.0, 0.0, 0.0), xend=(1.0, 1.0, 1.0), comps=None,
                  mhd=True, recon="plm", solver="hlld", emf="uct_contact", precision="fp64",
                  bc="periodic", gamma=5.0 / 3.0, nstages=2, cfl=0.6, cfl_max_var=1.1,
-                 first_dt=1e-4, device="cuda:0"):
+                 first_dt=1e-4, device="cuda:0", ntracer=0):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("DataBlock needs a GPU: the hot path has no CPU fallback")
@@ -81,6 +81,8 @@ class DataBlock:
         desc.emf = capi.EMF[emf]
         desc.precision = capi.PREC[precision]
         desc.gamma = gamma
+        desc.ntracer = ntracer
+        self.ntracer = ntracer
         L = capi.load()
         self._L = L
         h = ctypes.c_void_p()

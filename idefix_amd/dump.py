@@ -21,8 +21,9 @@ TYPES = {0: ("d", np.float64), 1: ("f", np.float32), 2: ("i", np.int32), 3: ("?"
 VERSION = "v2.2.02"
 
 
-def _cvar_names(dim, comps, mhd):
-    """Vc variables saved in dumps (fluid.hpp:587-627): name -> variable index."""
+def _cvar_names(dim, comps, mhd, ntracer=0):
+    """Vc variables saved in dumps (fluid.hpp:587-638): name -> variable index; passive
+    tracers follow the fluid variables as Vc-TR0, Vc-TR1, ..."""
     names = {"Vc-RHO": 0, "Vc-VX1": 1}
     if comps >= 2:
         names["Vc-VX2"] = 2
@@ -34,6 +35,9 @@ def _cvar_names(dim, comps, mhd):
             names["Vc-BX2"] = comps + 3
         if comps >= 3 and dim < 3:
             names["Vc-BX3"] = comps + 4
+    nfluid = 2 * comps + 2 if mhd else comps + 2
+    for t in range(ntracer):
+        names["Vc-TR%d" % t] = nfluid + t
     return names
 
 
@@ -66,7 +70,7 @@ def write_dump(filename, g, extra=None):
             _field(fh, "xl%d" % (d + 1), xl)
             _field(fh, "xr%d" % (d + 1), xrr)
         entries = {}
-        for name, n in _cvar_names(g.dim, g.comps, g.mhd).items():
+        for name, n in _cvar_names(g.dim, g.comps, g.mhd, getattr(g, "ntracer", 0)).items():
             entries[name] = ("cell", n)
         if g.mhd:
             for d in range(g.dim):
@@ -123,7 +127,7 @@ def restart(g, filename):
     _, F = read_dump(filename)
     Vc, Vs = g.download()
     b, e = g.beg, g.end
-    for name, n in _cvar_names(g.dim, g.comps, g.mhd).items():
+    for name, n in _cvar_names(g.dim, g.comps, g.mhd, getattr(g, "ntracer", 0)).items():
         Vc[n][b[2]:e[2], b[1]:e[1], b[0]:e[0]] = F[name]
     if g.mhd:
         for d in range(g.dim):

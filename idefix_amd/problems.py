@@ -25,7 +25,14 @@ def _empty(g):
 
 
 def _is_mhd(g):
-    return g.nvar == 2 * _comps(g) + 2
+    if hasattr(g, "mhd"):
+        return bool(g.mhd)
+    return bool(g.cfg.mhd)
+
+
+def _nfluid(g):
+    """Number of fluid variables; passive tracers follow them in Vc."""
+    return 2 * _comps(g) + 2 if _is_mhd(g) else _comps(g) + 2
 
 
 def _comps(g):
@@ -91,6 +98,12 @@ def orszag_tang_3d(g):
     Vs[0, k0:k1, j0:j1, i0:i1] = (0.0 + 1 / d2 * (Ajp[2] - Ai[2])) - 1 / d3 * (Akp[1] - Ai[1])
     Vs[1, k0:k1, j0:j1, i0:i1] = -1 / d1 * (Aip[2] - Ai[2]) + 1 / d3 * (Akp[0] - Ai[0])
     Vs[2, k0:k1, j0:j1, i0:i1] = 1 / d1 * (Aip[1] - Ai[1]) - 1 / d2 * (Ajp[0] - Ai[0])
+    # passive tracers (setup.cpp:250-253): TRG = [x > 1/2], TRG+1 = [z > 1/2]
+    trg = _nfluid(g)
+    if g.nvar > trg:
+        Vc[trg] = np.where(X > 0.5, 1.0, 0.0) + 0 * Y + 0 * Z
+    if g.nvar > trg + 1:
+        Vc[trg + 1] = np.where(Z > 0.5, 1.0, 0.0) + 0 * X + 0 * Y
     return Vc, Vs
 
 

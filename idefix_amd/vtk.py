@@ -16,11 +16,11 @@ VERSION = "v2.2.02"
 GEOMETRY_CARTESIAN = 0  # idefix.hpp geometry codes
 
 
-def _names(comps, mhd):
+def _names(comps, mhd, ntracer=0):
     n = ["RHO", "VX1", "VX2", "VX3"][:1 + comps] + ["PRS"]
     if mhd:
         n += ["BX1", "BX2", "BX3"][:comps]
-    return n
+    return n + ["TR%d" % t for t in range(ntracer)]  # fluid.hpp:633-638
 
 
 def write_vtk(filename, g, periodic=(True, True, True)):
@@ -67,7 +67,7 @@ def write_vtk(filename, g, periodic=(True, True, True)):
             w(("" if d == 0 else "\n") + "%s_COORDINATES %d float\n" % (lab, nx[d] + off[d]))
             fh.write(be(nodes[d]))
         w("\nCELL_DATA %d\n" % (nx[0] * nx[1] * nx[2]))
-        names = _names(g.comps, g.mhd)
+        names = _names(g.comps, g.mhd, getattr(g, "ntracer", 0))
         for name in sorted(names):
             n = names.index(name)
             w("\nSCALARS %s float\n" % name)

@@ -81,6 +81,11 @@ typedef struct {
   int lbound[3];  /* IDFX_BC_* per direction, left side */
   int rbound[3];  /* IDFX_BC_* per direction, right side */
   double gamma;   /* adiabatic index (eos_adiabatic.hpp:21) */
+  int ntracer;    /* passive tracers ([Hydro] tracer, fluid.hpp:315-325), 0..64: Vc/Uc hold
+                     nvar = fluid variables + ntracer (Tracer, tracer.hpp:58-111).  Tracers are
+                     advanced by idfx_cycle / idfx_stage / idfx_sweep; the flux-array entry
+                     points (idfx_calc_flux, idfx_calc_rhs) cover the fluid variables only,
+                     and the 1-D Roe HD configuration rejects tracers. */
 } idfx_desc;
 
 /* Context: replaces the Fluid<Phys>/RiemannSolver/ConstrainedTransport constructors

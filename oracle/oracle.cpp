@@ -1145,6 +1145,7 @@ struct Oracle {
   orc_cfg c;
   Vars V;
   int np_tot[3], np_int[3], nghost[3], beg[3], end[3];
+  int ntot;  // fluid variables + passive tracers
   int offs[3];  // IOFFSET, JOFFSET, KOFFSET (macros.hpp:31-53)
   int vsn[3];   // Vs extents (i, j, k)
   std::vector<double> x[3], xl[3], xr[3], dx[3];
@@ -1194,10 +1195,12 @@ struct Oracle {
       }
     }
     const size_t n3 = (size_t)np_tot[0] * np_tot[1] * np_tot[2];
-    Vc.assign(V.nvar * n3, 0.0);
-    Uc.assign(V.nvar * n3, 0.0);
-    Uc0.assign(V.nvar * n3, 0.0);
-    Flux.assign(V.nvar * n3, 0.0);
+    // passive tracers follow the fluid variables (fluid.hpp:529-545)
+    ntot = V.nvar + c.ntracer;
+    Vc.assign(ntot * n3, 0.0);
+    Uc.assign(ntot * n3, 0.0);
+    Uc0.assign(ntot * n3, 0.0);
+    Flux.assign(ntot * n3, 0.0);
     InvDt.assign(n3, 0.0);
     cMax.assign(n3, 0.0);
     if (c.mhd) {
@@ -1275,7 +1278,7 @@ struct Oracle {
     auto ref = [&](int d, int q) {
       return (d == dir) ? nghost[d] + (q + nghost[d] * (np_int[d] - 1)) % np_int[d] : q;
     };
-    for (int n = 0; n < V.nvar; n++)
+    for (int n = 0; n < ntot; n++)  // boundary.hpp:135-136: tracers included
       for (int k = lo[2]; k < hi[2]; k++)
         for (int j = lo[1]; j < hi[1]; j++)
           for (int i = lo[0]; i < hi[0]; i++)
@@ -1296,7 +1299,7 @@ struct Oracle {
     side_range(dir, side, -1, lo, hi);
     auto ref = [&](int d, int q) { return (d == dir) ? nghost[d] + side * (np_int[d] - 1) : q; };
     const int sign = 1 - 2 * side;
-    for (int n = 0; n < V.nvar; n++)
+    for (int n = 0; n < ntot; n++)
       for (int k = lo[2]; k < hi[2]; k++)
         for (int j = lo[1]; j < hi[1]; j++)
           for (int i = lo[0]; i < hi[0]; i++) {
@@ -1324,7 +1327,8 @@ struct Oracle {
     auto ref = [&](int d, int q) {
       return (d == dir) ? 2 * (nghost[d] + side * np_int[d]) - q - 1 : q;
     };
-    for (int n = 0; n < V.nvar; n++) {
+    // with MHD the sign test n >= BX1 also flips tracers (boundary.hpp:697)
+    for (int n = 0; n < ntot; n++) {
       const double sign =
           (n == V.MX1 + dir || (V.BX1 >= 0 && n >= V.BX1 && n != V.BX1 + dir)) ? -1.0 : 1.0;
       for (int k = lo[2]; k < hi[2]; k++)
@@ -1445,6 +1449,8 @@ struct Oracle {
           for (int n = 0; n < V.nvar; n++) u[n] = uc(n, k, j, i);
           cons2prim(c, V, u, v);
           for (int n = 0; n < V.nvar; n++) vc(n, k, j, i) = v[n];
+          // Tracer::ConvertConsToPrim, tracer.cpp:13-28
+          for (int n = V.nvar; n < ntot; n++) vc(n, k, j, i) = uc(n, k, j, i) / uc(V.RHO, k, j, i);
         }
   }
   // ConvertPrimToCons, convertConsToPrim.hpp:152-189
@@ -1457,6 +1463,8 @@ struct Oracle {
           for (int n = 0; n < V.nvar; n++) v[n] = vc(n, k, j, i);
           prim2cons(c, V, v, u);
           for (int n = 0; n < V.nvar; n++) uc(n, k, j, i) = u[n];
+          // Tracer::ConvertPrimToCons, tracer.cpp:32-47
+          for (int n = V.nvar; n < ntot; n++) uc(n, k, j, i) = vc(n, k, j, i) * vc(V.RHO, k, j, i);
         }
   }
 
@@ -1595,6 +1603,43 @@ struct Oracle {
   }
 
   // ---------------- CalcRightHandSide (calcRightHandSide.hpp:556-596) ----------------
+  // Tracer::CalcFlux, tracer.hpp:58-87: face value upwinded by the sign of the Riemann
+  // mass flux, van Leer slope; flux = F_rho * vface * A
+  void tracer_flux(int dir) {
+    const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
+    for (int nv = V.nvar; nv < ntot; nv++)
+#pragma omp parallel for collapse(2)
+      for (int k = beg[2]; k < end[2] + ko; k++)
+        for (int j = beg[1]; j < end[1] + jo; j++)
+          for (int i = beg[0]; i < end[0] + io; i++) {
+            auto q = [&](int s) { return vc(nv, k + s * ko, j + s * jo, i + s * io); };
+            const double frho = Flux[i4(V.RHO, k, j, i)];
+            double vface;
+            if (frho > 0) {
+              const double dvm = q(-1) - q(-2);
+              const double dvp = q(0) - q(-1);
+              const double dv = vanleer(dvp, dvm);
+              vface = q(-1) + 0.5 * dv;
+            } else {
+              const double dvm = q(0) - q(-1);
+              const double dvp = q(1) - q(0);
+              const double dv = vanleer(dvp, dvm);
+              vface = q(0) - 0.5 * dv;
+            }
+            Flux[i4(nv, k, j, i)] = frho * vface * Ax(dir, k, j, i);
+          }
+  }
+  // Tracer::CalcRightHandSide, tracer.hpp:89-111
+  void tracer_rhs(int dir, double dtl) {
+    const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
+    for (int nv = V.nvar; nv < ntot; nv++)
+#pragma omp parallel for collapse(2)
+      for (int k = beg[2]; k < end[2]; k++)
+        for (int j = beg[1]; j < end[1]; j++)
+          for (int i = beg[0]; i < end[0]; i++)
+            uc(nv, k, j, i) = uc(nv, k, j, i) + -dtl / dV(k, j, i) *
+                              (Flux[i4(nv, k + ko, j + jo, i + io)] - Flux[i4(nv, k, j, i)]);
+  }
   void calc_rhs(int dir, double dtl) {
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
     // "Correct Flux": Flux *= A (Cartesian, no Fargo), :156-194
@@ -1854,9 +1899,11 @@ struct Oracle {
   // Fluid::EvolveStage, evolveStage.hpp:40-90 (ideal MHD / HD, Cartesian)
   void evolve_stage(double tl, double dtl) {
     (void)tl;
-    for (int dir = 0; dir < c.dim; dir++) {
+    for (int dir = 0; dir < c.dim; dir++) {  // Fluid::LoopDir, evolveStage.hpp:14-33
       calc_flux(dir);
+      if (c.ntracer > 0) tracer_flux(dir);
       calc_rhs(dir, dtl);
+      if (c.ntracer > 0) tracer_rhs(dir, dtl);
     }
     if (c.mhd && c.dim >= 2) {
       calc_corner_emf();
@@ -1997,7 +2044,7 @@ void orc_layout(void *p, int *np_tot, int *nghost, int *nvar, int *vs_shape) {
     nghost[d] = o->nghost[d];
     vs_shape[d] = o->vsn[d];
   }
-  *nvar = o->V.nvar;
+  *nvar = o->ntot;
 }
 double *orc_array(void *p, int which, long *nelem) {
   auto *o = static_cast<Oracle *>(p);

@@ -54,6 +54,7 @@ typedef struct {
   double cfl;
   double cfl_max_var;
   double first_dt;
+  int ntracer;        /* passive tracers ([Hydro] tracer, fluid.hpp:315-325) */
 } orc_cfg;
 
 void *orc_new(const orc_cfg *cfg);

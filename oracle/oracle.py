@@ -31,7 +31,8 @@ class OrcCfg(ctypes.Structure):
                 ("xbeg", ctypes.c_double * 3), ("xend", ctypes.c_double * 3),
                 ("lbound", ctypes.c_int * 3), ("rbound", ctypes.c_int * 3),
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
-                ("cfl_max_var", ctypes.c_double), ("first_dt", ctypes.c_double)]
+                ("cfl_max_var", ctypes.c_double), ("first_dt", ctypes.c_double),
+                ("ntracer", ctypes.c_int)]
 
 
 def build():
@@ -82,7 +83,8 @@ def lib():
 
 def make_cfg(dim, comps=None, mhd=True, recon="plm", solver="hlld", emf="uct_contact",
              nstages=2, nx=(64, 1, 1), xbeg=(0.0, 0.0, 0.0), xend=(1.0, 1.0, 1.0),
-             bc="periodic", gamma=5.0 / 3.0, cfl=0.6, cfl_max_var=1.1, first_dt=1e-4):
+             bc="periodic", gamma=5.0 / 3.0, cfl=0.6, cfl_max_var=1.1, first_dt=1e-4,
+             ntracer=0):
     c = OrcCfg()
     c.dim = dim
     c.comps = dim if comps is None else comps
@@ -107,6 +109,7 @@ def make_cfg(dim, comps=None, mhd=True, recon="plm", solver="hlld", emf="uct_con
     c.cfl = cfl
     c.cfl_max_var = cfl_max_var
     c.first_dt = first_dt
+    c.ntracer = ntracer
     return c
 
 
@@ -247,6 +250,7 @@ class OracleGrid:
         self.nvar = o.nvar
         self.dim = o.dim
         self.comps = o.cfg.comps
+        self.mhd = bool(o.cfg.mhd)
         self.x = [o.coord(d, "x") for d in range(3)]
         self.xl = [o.coord(d, "xl") for d in range(3)]
         self.dx = [o.coord(d, "dx") for d in range(3)]

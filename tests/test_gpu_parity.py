@@ -170,6 +170,87 @@ def test_hd2d_limo3_cycles():
     compare_state(o, g, what="HD2D limo3")
 
 
+def _with_tracers(ic, nfluid, ntr):
+    """Initial state of `ic` plus ntr tracers: a step in x and a smooth profile in y."""
+    def f(gg):
+        Vc, Vs = ic(gg)
+        x = np.asarray(gg.x[0])[None, None, :]
+        y = np.asarray(gg.x[1])[None, :, None]
+        for t in range(ntr):
+            Vc[nfluid + t] = (np.where(x > 0.5, 1.0, 0.0) + 0 * y) if t % 2 == 0 else \
+                (0.5 + 0.4 * np.sin(2 * np.pi * (y + 0.3 * t)) + 0 * x)
+        return Vc, Vs
+    return f
+
+
+def test_orszag_tang_3d_tracers_reference_config():
+    """test/MHD/OrszagTang3D/idefix.ini: 32 x 64 x 32, hlld, CFL 0.9, 2 tracers
+    (TRG = [x > 1/2], TRG+1 = [z > 1/2] as its setup)."""
+    kw = cfg_kwargs(3, (32, 64, 32), emf="uct_contact", solver="hlld", cfl=0.9, ntracer=2)
+    o, g = make_pair(kw, problems.orszag_tang_3d)
+    assert g.nvar == 10 and o.nvar == 10
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="OT3D tracers")
+
+
+@pytest.mark.parametrize("emf,recon", [("uct_hlld", "plm"), ("uct_contact", "ppm"),
+                                       ("uct_hll", "limo3")])
+def test_orszag_tang_2d_tracers(emf, recon):
+    kw = cfg_kwargs(2, (48, 40), emf=emf, recon=recon, cfl=0.6, ntracer=2)
+    o, g = make_pair(kw, _with_tracers(problems.orszag_tang, 6, 2))
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="OT tracers %s %s" % (emf, recon))
+
+
+def test_uniform_tracer_stays_one_on_gpu():
+    kw = cfg_kwargs(2, (48, 40), emf="uct_contact", cfl=0.6, ntracer=1)
+    def ic(gg):
+        Vc, Vs = problems.orszag_tang(gg)
+        Vc[6] = 1.0
+        return Vc, Vs
+    o, g = make_pair(kw, ic)
+    g.set_boundaries()
+    for _ in range(10):
+        g.cycle()
+    Vc, _ = g.download()
+    b, e = g.beg, g.end
+    assert np.all(Vc[6, 0, b[1]:e[1], b[0]:e[0]] == 1.0)
+
+
+@pytest.mark.parametrize("bc", [["outflow", "periodic"], ["reflective", "periodic"]])
+def test_tracers_with_outflow_and_reflective_sides(bc):
+    """Tracers in the ghost fill of every side type; with MHD the reflective fill flips
+    their sign like the transverse field (boundary.hpp:697)."""
+    kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=bc, cfl=0.4, ntracer=2)
+    o, g = make_pair(kw, _with_tracers(lambda gg: problems.random_mhd(gg, seed=12, amp=0.1), 6, 2))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="tracers bc %s" % bc)
+
+
+@pytest.mark.parametrize("dim,solver", [(1, "hllc"), (2, "hll"), (3, "tvdlf")])
+def test_hd_tracers(dim, solver):
+    nx = {1: (400,), 2: (40, 24), 3: (16, 12, 10)}[dim]
+    bc = {1: "outflow", 2: ["outflow", "periodic"], 3: "periodic"}[dim]
+    ic = problems.sod if dim == 1 else (lambda gg: problems.random_hd(gg, seed=5, amp=0.3))
+    kw = cfg_kwargs(dim, nx, mhd=False, solver=solver, recon="plm", bc=bc, gamma=1.4,
+                    cfl=0.5, ntracer=2)
+    nfluid = dim + 2
+
+    def f(gg):
+        Vc, Vs = ic(gg)
+        x = np.asarray(gg.x[0])[None, None, :]
+        Vc[nfluid] = np.where(x > 0.3, 1.0, 0.0) + 0 * Vc[0]
+        Vc[nfluid + 1] = 0.5 + 0.4 * np.cos(6 * np.pi * x) + 0 * Vc[0]
+        return Vc, Vs
+    o, g = make_pair(kw, f)
+    o.set_boundaries()
+    _cycle_both(o, g, 8)
+    compare_state(o, g, what="HD%dD tracers" % dim)
+
+
 def test_outflow_mhd2d():
     kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))
