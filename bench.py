@@ -40,6 +40,9 @@ def parse():
                    help="skip the reference's own 3-D benchmark configuration")
     p.add_argument("--headline-only", action="store_true",
                    help="only time the reference's own 3-D benchmark configuration")
+    p.add_argument("--matrix", action="store_true",
+                   help="only time the Orszag-Tang workload across solvers, reconstructions, "
+                        "EMF averages and tracers (one JSON line per configuration)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB device copy 10 times (counter calibration)")
     return p.parse_args()
@@ -58,15 +61,42 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8, "launches": 10}))
 
 
-def make_block(nx, precision):
+def make_block(nx, precision, solver="hlld", recon="plm", emf="uct_hlld", ntracer=0):
     from idefix_amd.datablock import DataBlock
     from idefix_amd import problems
-    g = DataBlock(2, (nx, nx), mhd=True, recon="plm", solver="hlld", emf="uct_hlld",
+    g = DataBlock(2, (nx, nx), mhd=True, recon=recon, solver=solver, emf=emf,
                   precision=precision, bc="periodic", gamma=5.0 / 3.0, nstages=2, cfl=0.6,
-                  first_dt=1e-4)
+                  first_dt=1e-4, ntracer=ntracer)
     Vc, Vs = problems.orszag_tang(g)
+    if ntracer:
+        x = g.x[0][None, None, :]
+        for t in range(ntracer):
+            Vc[6 + t] = (x > 0.5) + 0.0 * Vc[0]
     g.upload(Vc, Vs)
     return g
+
+
+# Orszag-Tang configurations of the reference's test inputs (test/MHD/OrszagTang/*.ini
+# with reconstructions 2-4) and of its OrszagTang3D input (2 tracers), timed by --matrix
+MATRIX = [("hlld", "plm", "uct_hlld", 0), ("hlld", "plm", "uct_contact", 0),
+          ("hlld", "plm", "uct0", 0), ("hlld", "plm", "arithmetic", 0),
+          ("hlld", "plm", "uct_hll", 0), ("hlld", "ppm", "uct_contact", 0),
+          ("hlld", "limo3", "uct_contact", 0), ("hll", "plm", "uct_contact", 0),
+          ("hll", "plm", "uct_hll", 0), ("tvdlf", "plm", "uct_contact", 0),
+          ("roe", "plm", "uct_contact", 0), ("hlld", "plm", "uct_contact", 2)]
+
+
+def matrix(nx, steps, warmup):
+    import torch
+    for solver, recon, emf, ntr in MATRIX:
+        g = make_block(nx, "fp64", solver, recon, emf, ntr)
+        el = time_cycles(g, steps, warmup)
+        print(json.dumps({"workload": "2D Orszag-Tang %dx%d, RK2, CFL 0.6, FP64" % (nx, nx),
+                          "solver": solver, "recon": recon, "emf": emf, "tracers": ntr,
+                          "zone_updates_per_s": nx * nx * steps / el,
+                          "ms_per_step": 1e3 * el / steps}), flush=True)
+        del g
+        torch.cuda.empty_cache()
 
 
 def time_cycles(g, steps, warmup):
@@ -187,6 +217,9 @@ def main():
     import torch
     if args.headline_only:
         print(json.dumps(reference_headline(args.steps, args.warmup)))
+        return
+    if args.matrix:
+        matrix(args.nx, args.steps, args.warmup)
         return
     nx = args.nx
     g = make_block(nx, "fp64")
