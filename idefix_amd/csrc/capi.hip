@@ -29,6 +29,7 @@ void register_tvdlf(Ops *t);
 void register_limo3_hd(Ops *t);
 void register_limo3_mhd(Ops *t);
 void register_limo3_mhd_b(Ops *t);
+void register_mhd1(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -49,6 +50,7 @@ Ops *ops_table() {
     register_limo3_hd(t);
     register_limo3_mhd(t);
     register_limo3_mhd_b(t);
+    register_mhd1(t);
     return t;
   }();
   return table;

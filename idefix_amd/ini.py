@@ -9,7 +9,8 @@ arguments, with the reference's defaults:
   [TimeIntegrator] CFL, CFL_max_var (1.1), first_dt (1e-10), nstages, tstop
                    (timeIntegrator.cpp:30-45, main.cpp:184)
   [Hydro]          solver (riemannSolver.hpp:98-138), gamma (5/3, eos_adiabatic.hpp:21),
-                   emf (uct_contact, constrainedTransport.hpp:156-177)
+                   emf (uct_contact, constrainedTransport.hpp:156-177), tracer (0,
+                   fluid.hpp:315-325)
   [Boundary]       X1-beg .. X3-end
 The reconstruction is a build option of the reference (ORDER), so it is an argument here.
 """
@@ -17,7 +18,7 @@ The reconstruction is a build option of the reference (ORDER), so it is an argum
 UNSUPPORTED = {
     # [Hydro] entries that switch on physics this library does not implement
     "Hydro": ("hall", "resistivity", "ambipolar", "viscosity", "bragTDiffusion",
-              "bragViscosity", "rotation", "shearingBox", "fargo", "tracer",
+              "bragViscosity", "rotation", "shearingBox", "fargo",
               "shockFlattening", "csiso", "gravPotential", "bodyForce", "drag"),
 }
 
@@ -99,7 +100,8 @@ def block_config(blocks, recon="plm", mhd=True, precision="fp64", comps=None):
               nstages=_get(blocks, "TimeIntegrator", "nstages", cast=int),
               cfl=_get(blocks, "TimeIntegrator", "CFL", cast=float),
               cfl_max_var=_get(blocks, "TimeIntegrator", "CFL_max_var", cast=float, default=1.1),
-              first_dt=_get(blocks, "TimeIntegrator", "first_dt", cast=float, default=1.0e-10))
+              first_dt=_get(blocks, "TimeIntegrator", "first_dt", cast=float, default=1.0e-10),
+              ntracer=_get(blocks, "Hydro", "tracer", cast=int, default=0))
     if mhd:
         kw["emf"] = _get(blocks, "Hydro", "emf", default="uct_contact")
     run = dict(tstop=_get(blocks, "TimeIntegrator", "tstop", cast=float),

@@ -368,3 +368,41 @@ def linear_fast_wave_2d(g, eps=1e-6, gamma=5.0 / 3.0, Lx=2.0, Ly=1.0):
     cf = 2.0
     period = (1.0 / np.hypot(1.0 / Lx, 1.0 / Ly)) / cf
     return Vc, Vs, period
+
+
+def linear_fast_wave_1d(g, eps=1e-6, gamma=5.0 / 3.0, L=1.0):
+    """The fast magnetosonic eigenmode of linear_fast_wave_2d propagating along x on a
+    periodic [0, L] line (two-component MHD, B = (1, 3/2) in the background, fast speed 2):
+    Bx is the constant face field, By a cell variable.  Returns (Vc, Vs, period)."""
+    Vc, Vs = _empty(g)
+    x = _coords(g)[0]
+    k = 2 * np.pi / L
+    R0 = 1.0 / (2.0 * np.sqrt(5.0))
+    Rrho, Rvpar, Rvperp, RBperp, RE = 2.0, 4.0, -2.0, 4.0, 9.0
+    Bpar, Bperp = 1.0, 1.5
+    Rp = (gamma - 1.0) * (RE - Bperp * RBperp)
+    ph = np.cos(k * x)[None, None, :]
+    a = eps * R0
+    Vc[0] = 1.0 + a * Rrho * ph
+    Vc[1] = a * Rvpar * ph
+    Vc[2] = a * Rvperp * ph
+    Vc[3] = 1.0 / gamma + a * Rp * ph
+    Vc[4] = Bpar
+    Vc[5] = Bperp + a * RBperp * ph
+    Vs[0] = Bpar
+    return Vc, Vs, L / 2.0
+
+
+def brio_wu(g, x0=50.0):
+    """MHD shock tube of test/MHD/sod/setup.cpp:20-40 (Brio & Wu 1988 states, the
+    reference's gamma 5/3 and [0, 100] domain): rho, p = 1, 1 | 0.125, 0.1; Bx = 0.75;
+    By = 1 | -1; at rest."""
+    Vc, Vs = _empty(g)
+    x = _coords(g)[0][None, None, :]
+    comps = _comps(g)
+    Vc[0] = np.where(x < x0, 1.0, 0.125) + 0 * Vc[0]
+    Vc[comps + 1] = np.where(x < x0, 1.0, 0.1) + 0 * Vc[0]
+    Vc[comps + 2] = 0.75
+    Vc[comps + 3] = np.where(x < x0, 1.0, -1.0) + 0 * Vc[0]
+    Vs[0] = 0.75
+    return Vc, Vs

@@ -17,18 +17,21 @@ from . import capi, ini, problems
 from .output import Output
 
 SETUPS = {
-    # name -> (function, mhd): the reference's tests these setups reproduce
-    "orszag_tang": (problems.orszag_tang, True),        # test/MHD/OrszagTang
-    "orszag_tang_3d": (problems.orszag_tang_3d, True),  # test/MHD/OrszagTang3D
-    "sod": (problems.sod, False),                       # test/HD/sod
-    "kelvin_helmholtz": (problems.kelvin_helmholtz, True),
-    "turbulence": (problems.turbulence, True),
+    # name -> (function, mhd, COMPONENTS or None for DIMENSIONS): the reference's tests
+    # these setups reproduce
+    "orszag_tang": (problems.orszag_tang, True, None),        # test/MHD/OrszagTang
+    "orszag_tang_3d": (problems.orszag_tang_3d, True, None),  # test/MHD/OrszagTang3D
+    "sod": (problems.sod, False, None),                       # test/HD/sod
+    "brio_wu": (problems.brio_wu, True, 2),                   # test/MHD/sod
+    "kelvin_helmholtz": (problems.kelvin_helmholtz, True, 3),
+    "turbulence": (problems.turbulence, True, 3),
 }
 
 
 def build(blocks, setup, recon="plm", precision="fp64", comps=None):
     from .datablock import DataBlock
-    fn, mhd = SETUPS[setup]
+    fn, mhd, comps0 = SETUPS[setup]
+    comps = comps if comps is not None else comps0
     kw, run = ini.block_config(blocks, recon=recon, mhd=mhd, precision=precision, comps=comps)
     g = DataBlock(**kw)
     periodic = [all(b == "periodic" for b in kw["bc"][d]) if d < kw["dim"] else True

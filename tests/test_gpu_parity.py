@@ -251,6 +251,29 @@ def test_hd_tracers(dim, solver):
     compare_state(o, g, what="HD%dD tracers" % dim)
 
 
+@pytest.mark.parametrize("solver,recon,nstages", [
+    (s, r, 2) for s in ("roe", "hll", "hlld", "tvdlf") for r in ("plm", "limo3")] +
+    [("roe", "ppm", 3), ("hlld", "ppm", 3)])
+def test_brio_wu_reference_matrix(solver, recon, nstages):
+    """test/MHD/sod (Brio-Wu, 800 cells on [0, 100], CFL 0.8, gamma 5/3): its inputs
+    idefix.ini / -hll / -hlld / -tvdlf with reconstructions 2 and 3, and the RK3 inputs
+    with reconstruction 4 (test/MHD/sod/testme.py:18-20)."""
+    kw = cfg_kwargs(1, (800,), comps=2, solver=solver, recon=recon, bc="outflow",
+                    gamma=1.66666666666667, cfl=0.8, nstages=nstages, xend=(100, 1, 1))
+    o, g = make_pair(kw, problems.brio_wu)
+    o.set_boundaries()
+    _cycle_both(o, g, 40)
+    compare_state(o, g, what="Brio-Wu %s %s" % (solver, recon))
+
+
+def test_fast_wave_1d_periodic():
+    kw = cfg_kwargs(1, (128,), comps=2, solver="hlld", recon="plm", cfl=0.8)
+    o, g = make_pair(kw, lambda gg: problems.linear_fast_wave_1d(gg, eps=1e-3)[:2])
+    o.set_boundaries()
+    _cycle_both(o, g, 30)
+    compare_state(o, g, what="1-D fast wave")
+
+
 def test_outflow_mhd2d():
     kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))

@@ -81,6 +81,8 @@ def test_block_config_refusals():
         ini.block_config(ini.parse_ini(SOD), mhd=True)  # hllc needs a HD fluid
     with pytest.raises(NotImplementedError):
         ini.block_config(ini.parse_ini(OT_HLL + "[Hydro]\nhall constant 0.1\n"))
+    kw, _ = ini.block_config(ini.parse_ini(OT_HLL.replace("solver    hll", "solver hlld\ntracer 2")))
+    assert kw["ntracer"] == 2 and kw["solver"] == "hlld"
     with pytest.raises(NotImplementedError):
         ini.block_config(ini.parse_ini(OT_HLL.replace("128  u", "128  l")))
 
