@@ -43,7 +43,8 @@ class IdfxDesc(ctypes.Structure):
                 ("nx", ctypes.c_int * 3), ("recon", ctypes.c_int), ("solver", ctypes.c_int),
                 ("emf", ctypes.c_int), ("precision", ctypes.c_int),
                 ("lbound", ctypes.c_int * 3), ("rbound", ctypes.c_int * 3),
-                ("gamma", ctypes.c_double), ("ntracer", ctypes.c_int)]
+                ("gamma", ctypes.c_double), ("ntracer", ctypes.c_int),
+                ("shock_flattening", ctypes.c_int), ("shock_smoothing", ctypes.c_double)]
 
 
 class IdfxError(RuntimeError):

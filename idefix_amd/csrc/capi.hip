@@ -79,6 +79,7 @@ struct idfx_ctx {
   double *InvDt = nullptr;     // used when the caller passes no InvDt
   double *Uc0 = nullptr, *Vs0 = nullptr;  // Runge-Kutta "begin" state
   unsigned long long *dtmax = nullptr, *divb = nullptr;  // device scalars
+  unsigned char *shock = nullptr;  // shock flags (shock flattening on)
   unsigned long long *host = nullptr;                    // pinned readback
   hipEvent_t ev_dt = nullptr;
   int target_waves = 8192;
@@ -187,6 +188,7 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
   a.flags = flags;
   a.dtmax = c->dtmax;
   a.mlpw = c->mlp ? c->mlpW[dir] : nullptr;
+  a.shock = c->shock;
   for (int d = 0; d < 3; d++) a.ext[d] = 0;
   // uniform grid: every cell has the volume of cell (beg); dt/dV computed once here with
   // the device's association (dx1 * dx2) * dx3
@@ -225,8 +227,16 @@ bool needs_unfused(const idfx_ctx *c) {
   return c->d.dim == 1 && !c->d.mhd && c->d.solver == IDFX_SOLVER_ROE;
 }
 
+// Shock flags from Vc before the first direction's fluxes, as RiemannSolver::CalcFlux<IDIR>
+// does (calcFlux.hpp:30-33); the other directions reuse them.
+int shock_step(idfx_ctx *c, int dir, const double *Vc, hipStream_t s) {
+  if (!c->shock || dir != 0) return IDFX_OK;
+  return check(c->ops->find_shock(c->g, Vc, c->shock, c->d.shock_smoothing, s));
+}
+
 int do_sweep_unfused(idfx_ctx *c, int dir, double dt, const double *Vc, const double *Vs,
                      double *Uc, double *InvDt, double *Flux, hipStream_t s) {
+  IDFX_TRY(shock_step(c, dir, Vc, s));
   SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, 0);
   a.Flux = Flux;
   IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
@@ -332,6 +342,9 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   // passive tracers ride on the fused sweeps; the flux-array path of the 1-D Roe HD
   // solver has no tracer step
   if (d.ntracer < 0 || d.ntracer > 64) return IDFX_E_ARGS;
+  // shock flattening acts on the PLM and LimO3 slopes (extrapolateToFaces.hpp:98-226)
+  if (d.shock_flattening && d.recon != IDFX_RECON_PLM && d.recon != IDFX_RECON_LIMO3)
+    return IDFX_E_ARGS;
   if (d.ntracer > 0 && d.dim == 1 && !d.mhd && d.solver == IDFX_SOLVER_ROE) return IDFX_E_ARGS;
   const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver, d.precision == IDFX_PREC_FP32);
   if (key < 0 || key >= kOpsTableSize) return IDFX_E_ARGS;
@@ -391,6 +404,7 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   if (st == IDFX_OK && d.mhd) st = alloc_n(c, &c->Vs0, c->nvs);
   if (st == IDFX_OK && needs_unfused(c)) st = alloc_n(c, &c->Flux, c->n3 * c->nvar);
   if (st == IDFX_OK) st = alloc_n(c, &c->dtmax, 1);
+  if (st == IDFX_OK && d.shock_flattening) st = alloc_n(c, &c->shock, c->n3);
   if (st == IDFX_OK) st = alloc_n(c, &c->divb, 1);
   if (st == IDFX_OK && d.recon == IDFX_RECON_MLP) {
     c->mlp = true;
@@ -526,6 +540,7 @@ int idfx_calc_flux(idfx_ctx *c, int dir, const double *Vc, const double *Vs, dou
                    void *stream) {
   if (!c || !Vc || !Flux || dir < 0 || dir >= c->d.dim || (c->d.mhd && !Vs) || c->fp32)
     return IDFX_E_ARGS;
+  IDFX_TRY(shock_step(c, dir, Vc, (hipStream_t)stream));
   SweepArgs a = sweep_args(c, dir, 0.0, Vc, Vs, nullptr, nullptr, 0);
   a.Flux = Flux;
   IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, (hipStream_t)stream));
@@ -545,6 +560,7 @@ int idfx_sweep(idfx_ctx *c, int dir, double dt, const double *Vc, const double *
   if (!c || !Vc || !Uc || dir < 0 || dir >= c->d.dim || (c->d.mhd && !Vs)) return IDFX_E_ARGS;
   if (needs_unfused(c)) return IDFX_E_ARGS;
   const int f = flags & (SW_INIT_UC | SW_SAVE_UC0 | SW_INVDT_SET | SW_INVDT_ADD);
+  IDFX_TRY(shock_step(c, dir, Vc, (hipStream_t)stream));
   SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, f);
   IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, (hipStream_t)stream));
   return check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, (hipStream_t)stream));
@@ -569,6 +585,7 @@ int idfx_evolve_stage(idfx_ctx *c, double t, double dt, const double *Vc, double
     if (needs_unfused(c)) {
       IDFX_TRY(do_sweep_unfused(c, dir, dt, Vc, Vs, Uc, InvDt, c->Flux, s));
     } else {
+      IDFX_TRY(shock_step(c, dir, Vc, s));
       SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, SW_INVDT_ADD);
       IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
       IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
@@ -675,6 +692,7 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
         hipLaunchKernelGGL(k_invdt_max, dim3(1024), dim3(256), 0, s, c->g,
                            InvDt ? InvDt : c->InvDt, c->dtmax);
     } else {
+      IDFX_TRY(shock_step(c, dir, Vc, s));
       SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, flags);
       IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
       IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
@@ -721,6 +739,7 @@ static int stage_fused2d(idfx_ctx *c, int stage, int nstages, double dt, double 
   int flags = SW_INIT_UC;
   if (stage == 0 && nstages > 1) flags |= SW_SAVE_UC0;
   if (stage == 0) flags |= SW_INVDT_SET;
+  IDFX_TRY(shock_step(c, 0, Vc_in, s));
   SweepArgs ax = sweep_args(c, 0, dt, Vc_in, Vs_in, Uc, InvDt, flags);
   {
     Region r(c, IDFX_REGION_SWEEP_X, s);

@@ -111,7 +111,8 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       }
       if (n == BX2) continue;
       T dummy;
-      recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, 1, c0 - 1));
+      recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, 1, c0 - 1),
+                    recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
     }
     if constexpr (CARRY) {
       if constexpr (HLLE) {
@@ -148,6 +149,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         }
       }
       T vL[NV], vR[NV], vr_next[NV];
+      const bool flat = recon_flat<C>(a, col + (unsigned)f * sd);
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         vL[n] = carry[n];
@@ -158,7 +160,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         T wv[NW];
 #pragma unroll
         for (int q = 0; q < NW; q++) wv[q] = win[q][n];
-        recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, 1, f));
+        recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, 1, f), flat);
       }
       const double b2old = ld32(Vs1_in, (unsigned)f * vsd + col);
       vL[BX2] = (T)b2old;

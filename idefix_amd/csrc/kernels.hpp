@@ -66,6 +66,7 @@ struct SweepArgs {
   int uniform;    // all cells share one volume: dtdV below equals dt/dV(k,j,i)
   double dtdV;
   const float *mlpw;  // learned-limiter interface weights of this direction (RECON_MLP)
+  const unsigned char *shock;  // shock flags per cell (shock flattening) or nullptr
 };
 
 // Element i of an array whose base is uniform across the wave, addressed as base + 32-bit
@@ -155,15 +156,18 @@ __device__ __forceinline__ void store_face_emf(const EmfArrays<T> &e, unsigned p
 // ExtrapolatePrimVar computes them for the two faces (extrapolateToFaces.hpp:88-124,
 // :247-306).
 template <class C, typename T>
-__device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr, T dxc = T(0)) {
+__device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr, T dxc = T(0),
+                                           bool flat = false) {
   if constexpr (C::ORDER == RECON_CONSTANT) {
     vl = w[0];
     vr = w[0];
   } else if constexpr (C::ORDER == RECON_PLM) {
     // w[0] = V(c-1), w[1] = V(c), w[2] = V(c+1)
+    // a cell flagged by the shock detector takes MinModLim, which as built returns 0
+    // (slopeLimiter.hpp:16-23, extrapolateToFaces.hpp:98-120)
     const T dvm = w[1] - w[0];
     const T dvp = w[2] - w[1];
-    const T dv = vanleer(dvp, dvm);
+    const T dv = flat ? T(0) : vanleer(dvp, dvm);
     vl = w[1] - T(0.5) * dv;
     vr = w[1] + T(0.5) * dv;
   } else if constexpr (C::ORDER == RECON_LIMO3) {
@@ -172,9 +176,9 @@ __device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr, T d
     // (slopeLimiter.hpp:16-23): the state falls back to the cell value.
     const T dvm = w[1] - w[0];
     const T dvp = w[2] - w[1];
-    T dv = dvp * limo3_lim(dvp, dvm, dxc);
+    T dv = flat ? T(0) : dvp * limo3_lim(dvp, dvm, dxc);
     vr = w[1] + T(0.5) * dv;
-    dv = dvm * limo3_lim(dvm, dvp, dxc);
+    dv = flat ? T(0) : dvm * limo3_lim(dvm, dvp, dxc);
     vl = w[1] - T(0.5) * dv;
     if (nv == C::RHO || nv == C::ENG) {
       if (vr <= T(0)) vr = w[1] + T(0.5) * T(0);
@@ -206,6 +210,13 @@ template <class C, typename T>
 __device__ __forceinline__ T recon_dx(const GridDesc &g, int dir, int cell) {
   if constexpr (C::ORDER == RECON_LIMO3) return (T)ld32(g.dx[dir], (unsigned)cell);
   else return T(0);
+}
+// Shock flag of the cell at linear index p (only PLM and LimO3 read it).
+template <class C>
+__device__ __forceinline__ bool recon_flat(const SweepArgs &a, unsigned p) {
+  if constexpr (C::ORDER == RECON_PLM || C::ORDER == RECON_LIMO3)
+    return a.shock != nullptr && ld32(a.shock, p) != 0;
+  else return false;
 }
 
 // Learned-limiter reconstruction of one cell from the window V(c-1), V(c), V(c+1) and
@@ -387,6 +398,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
 
     // reconstruct the lane's cell: vl = right state of face c, vr = left state of face c+1
     T vl[NV], vr[NV];
+    const bool flat = recon_flat<C>(a, row + (unsigned)cc);
 #pragma unroll
     for (int n = 0; n < NV; n++) {
       T wv[NW];
@@ -396,7 +408,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (C::MHD && n == C::BX1) { vl[n] = vr[n] = wv[H]; continue; }  // normal field: Vs
         recon_mlp<C, IDIR>(wv, n, a.mlpw, g.sv, row + (unsigned)cc, 1u, vl[n], vr[n]);
       } else {
-        recon_cell<C>(wv, n, vl[n], vr[n], recon_dx<C, T>(g, 0, cc));
+        recon_cell<C>(wv, n, vl[n], vr[n], recon_dx<C, T>(g, 0, cc), flat);
       }
     }
     T vL[NV], vR[NV];
@@ -528,7 +540,8 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (C::MHD && n == C::template BXn<DIR>()) { carry[n] = wv[H]; continue; }
         recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)(c0 - 1) * sd, sd, dummy, carry[n]);
       } else {
-        recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, DIR, c0 - 1));
+        recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, DIR, c0 - 1),
+                      recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
       }
     }
     for (int f = c0; f <= c1; f++) {
@@ -551,6 +564,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
           win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
       }
       T vL[NV], vR[NV], vr_next[NV];
+      const bool flat = recon_flat<C>(a, col + (unsigned)f * sd);
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         T wv[NW];
@@ -563,7 +577,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
             recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)f * sd, sd, vR[n], vr_next[n]);
           }
         } else {
-          recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, DIR, f));
+          recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, DIR, f), flat);
         }
         vL[n] = carry[n];
       }
@@ -902,6 +916,50 @@ __global__ __launch_bounds__(256) void k_evolve_b(GridDesc g, double *Vs, double
 // ------------------------------------------------------------------------------------
 // Pointwise conversions
 // ------------------------------------------------------------------------------------
+// ShockFlattening::FindShock (shockFlattening.hpp:62-168, Cartesian, with an energy
+// equation) over the interior and one cell around it along the active directions: a cell
+// is flagged where div v < 0 and the pressure jump across it exceeds smoothing times the
+// smallest neighbouring pressure.
+template <class C>
+__global__ __launch_bounds__(256) void k_find_shock(GridDesc g, const double *Vc,
+                                                   unsigned char *flags, double smoothing) {
+  constexpr int jo = C::DIM >= 2 ? 1 : 0, ko = C::DIM == 3 ? 1 : 0;
+  const int ni = g.end[0] - g.beg[0] + 2, nj = g.end[1] - g.beg[1] + 2 * jo;
+  const int nk = g.end[2] - g.beg[2] + 2 * ko;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = g.beg[0] - 1 + t % ni;
+  const int j = g.beg[1] - jo + (t / ni) % nj;
+  const int k = g.beg[2] - ko + t / ((long long)ni * nj);
+  const long long p = idx3(g, k, j, i);
+  const long long sx = 1, sy = g.nt[0], sz = (long long)g.nt[0] * g.nt[1];
+  const double *P = Vc + C::ENG * g.sv;
+  double divV = (Vc[C::MX1 * g.sv + p + sx] - Vc[C::MX1 * g.sv + p - sx]) / (2 * g.dx[0][i]);
+  if constexpr (C::DIM >= 2)
+    divV = divV + (Vc[C::MX2 * g.sv + p + sy] - Vc[C::MX2 * g.sv + p - sy]) / (2 * g.dx[1][j]);
+  if constexpr (C::DIM == 3)
+    divV = divV + (Vc[C::MX3 * g.sv + p + sz] - Vc[C::MX3 * g.sv + p - sz]) / (2 * g.dx[2][k]);
+  unsigned char f = 0;
+  if (divV < 0.0) {
+    double pmin = P[p];
+    pmin = fmin(pmin, P[p + sx]);
+    pmin = fmin(pmin, P[p - sx]);
+    double gradP = fabs(P[p + sx] - P[p - sx]);
+    if constexpr (C::DIM >= 2) {
+      pmin = fmin(pmin, P[p + sy]);
+      pmin = fmin(pmin, P[p - sy]);
+      gradP += fabs(P[p + sy] - P[p - sy]);
+    }
+    if constexpr (C::DIM == 3) {
+      pmin = fmin(pmin, P[p + sz]);
+      pmin = fmin(pmin, P[p - sz]);
+      gradP += fabs(P[p + sz] - P[p - sz]);
+    }
+    if (gradP > smoothing * pmin) f = 1;
+  }
+  flags[p] = f;
+}
+
 // ConvertPrimToCons over a box (convertConsToPrim.hpp:152-189)
 template <class C>
 __global__ __launch_bounds__(256) void k_prim_to_cons(GridDesc g, const double *Vc, double *Uc) {

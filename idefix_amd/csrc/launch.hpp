@@ -47,6 +47,9 @@ struct Ops {
   // learned limiter (RECON_MLP): interface weights of direction dir
   hipError_t (*mlp_infer)(int dir, const GridDesc &g, const double *Vc, const MlpNet &net,
                           float *W, hipStream_t s) = nullptr;
+  // shock flattening: flags of the cells around the interior
+  hipError_t (*find_shock)(const GridDesc &g, const double *Vc, unsigned char *flags,
+                           double smoothing, hipStream_t s) = nullptr;
 };
 
 inline unsigned blocks_for(long long n, int bs = 256) {
@@ -394,6 +397,17 @@ hipError_t launch_mlp(int dir, const GridDesc &g, const double *Vc, const MlpNet
   return hipErrorInvalidValue;
 }
 
+template <class C>
+hipError_t launch_find_shock(const GridDesc &g, const double *Vc, unsigned char *flags,
+                             double smoothing, hipStream_t s) {
+  const long long n = (long long)(g.end[0] - g.beg[0] + 2) *
+                      (g.end[1] - g.beg[1] + (C::DIM >= 2 ? 2 : 0)) *
+                      (g.end[2] - g.beg[2] + (C::DIM == 3 ? 2 : 0));
+  hipLaunchKernelGGL((k_find_shock<C>), dim3(blocks_for(n)), dim3(256), 0, s, g, Vc, flags,
+                     smoothing);
+  return hipGetLastError();
+}
+
 template <class C, typename T>
 Ops make_ops() {
   Ops o;
@@ -415,6 +429,7 @@ Ops make_ops() {
   if constexpr (C::MHD && C::DIM == 2 && C::ORDER != RECON_MLP)
     o.fused2d_y = &launch_fused2d_y<C, T>;
   if constexpr (C::ORDER == RECON_MLP) o.mlp_infer = &launch_mlp<C>;
+  o.find_shock = &launch_find_shock<C>;
   return o;
 }
 

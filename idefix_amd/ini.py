@@ -19,7 +19,7 @@ UNSUPPORTED = {
     # [Hydro] entries that switch on physics this library does not implement
     "Hydro": ("hall", "resistivity", "ambipolar", "viscosity", "bragTDiffusion",
               "bragViscosity", "rotation", "shearingBox", "fargo",
-              "shockFlattening", "csiso", "gravPotential", "bodyForce", "drag"),
+              "csiso", "gravPotential", "bodyForce", "drag"),
 }
 
 
@@ -101,7 +101,9 @@ def block_config(blocks, recon="plm", mhd=True, precision="fp64", comps=None):
               cfl=_get(blocks, "TimeIntegrator", "CFL", cast=float),
               cfl_max_var=_get(blocks, "TimeIntegrator", "CFL_max_var", cast=float, default=1.1),
               first_dt=_get(blocks, "TimeIntegrator", "first_dt", cast=float, default=1.0e-10),
-              ntracer=_get(blocks, "Hydro", "tracer", cast=int, default=0))
+              ntracer=_get(blocks, "Hydro", "tracer", cast=int, default=0),
+              shock_flattening=(_get(blocks, "Hydro", "shockFlattening", cast=float)
+                                if "shockFlattening" in blocks.get("Hydro", {}) else None))
     if mhd:
         kw["emf"] = _get(blocks, "Hydro", "emf", default="uct_contact")
     run = dict(tstop=_get(blocks, "TimeIntegrator", "tstop", cast=float),

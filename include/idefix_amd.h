@@ -86,6 +86,10 @@ typedef struct {
                      advanced by idfx_cycle / idfx_stage / idfx_sweep; the flux-array entry
                      points (idfx_calc_flux, idfx_calc_rhs) cover the fluid variables only,
                      and the 1-D Roe HD configuration rejects tracers. */
+  int shock_flattening;   /* [Hydro] shockFlattening given (riemannSolver.hpp:151-157):
+                             cells flagged by ShockFlattening::FindShock take the first-order
+                             slope; PLM and LimO3 only */
+  double shock_smoothing; /* its value (the pressure-jump threshold) */
 } idfx_desc;
 
 /* Context: replaces the Fluid<Phys>/RiemannSolver/ConstrainedTransport constructors

@@ -1146,6 +1146,7 @@ struct Oracle {
   Vars V;
   int np_tot[3], np_int[3], nghost[3], beg[3], end[3];
   int ntot;  // fluid variables + passive tracers
+  std::vector<unsigned char> shock;  // FlagShock per cell (shockFlattening.hpp:14)
   int offs[3];  // IOFFSET, JOFFSET, KOFFSET (macros.hpp:31-53)
   int vsn[3];   // Vs extents (i, j, k)
   std::vector<double> x[3], xl[3], xr[3], dx[3];
@@ -1203,6 +1204,7 @@ struct Oracle {
     Flux.assign(ntot * n3, 0.0);
     InvDt.assign(n3, 0.0);
     cMax.assign(n3, 0.0);
+    shock.assign(n3, 0);
     if (c.mhd) {
       Vs.assign((size_t)c.dim * vsn[0] * vsn[1] * vsn[2], 0.0);
       Vs0.assign(Vs.size(), 0.0);
@@ -1468,6 +1470,48 @@ struct Oracle {
         }
   }
 
+  // ---------------- Shock flattening (shockFlattening.hpp:62-168) ----------------
+  bool flagged(int dir, int k, int j, int i, int s) const {
+    if (!c.shock_flattening) return false;
+    const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
+    return shock[i3(k + s * ko, j + s * jo, i + s * io)] != 0;
+  }
+  // ShockFlattening::FindShock, Cartesian with an energy equation: a cell is a shock
+  // where the flow converges and the pressure jump across it exceeds smoothing times the
+  // smallest neighbouring pressure
+  void find_shock() {
+    const int ko = c.dim == 3 ? 1 : 0, jo = c.dim >= 2 ? 1 : 0;
+#pragma omp parallel for collapse(2)
+    for (int k = beg[2] - ko; k < end[2] + ko; k++)
+      for (int j = beg[1] - jo; j < end[1] + jo; j++)
+        for (int i = beg[0] - 1; i < end[0] + 1; i++) {
+          unsigned char f = 0;
+          double divV = (vc(V.MX1, k, j, i + 1) - vc(V.MX1, k, j, i - 1)) / (2 * dx[0][i]);
+          if (c.dim >= 2)
+            divV = divV + (vc(V.MX2, k, j + 1, i) - vc(V.MX2, k, j - 1, i)) / (2 * dx[1][j]);
+          if (c.dim == 3)
+            divV = divV + (vc(V.MX3, k + 1, j, i) - vc(V.MX3, k - 1, j, i)) / (2 * dx[2][k]);
+          if (divV < 0.0) {
+            double pmin = vc(V.ENG, k, j, i);
+            pmin = FMIN(pmin, vc(V.ENG, k, j, i + 1));
+            pmin = FMIN(pmin, vc(V.ENG, k, j, i - 1));
+            double gradP = std::fabs(vc(V.ENG, k, j, i + 1) - vc(V.ENG, k, j, i - 1));
+            if (c.dim >= 2) {
+              pmin = FMIN(pmin, vc(V.ENG, k, j + 1, i));
+              pmin = FMIN(pmin, vc(V.ENG, k, j - 1, i));
+              gradP += std::fabs(vc(V.ENG, k, j + 1, i) - vc(V.ENG, k, j - 1, i));
+            }
+            if (c.dim == 3) {
+              pmin = FMIN(pmin, vc(V.ENG, k + 1, j, i));
+              pmin = FMIN(pmin, vc(V.ENG, k - 1, j, i));
+              gradP += std::fabs(vc(V.ENG, k + 1, j, i) - vc(V.ENG, k - 1, j, i));
+            }
+            if (gradP > c.shock_smoothing * pmin) f = 1;
+          }
+          shock[i3(k, j, i)] = f;
+        }
+  }
+
   // ---------------- Reconstruction (extrapolateToFaces.hpp:75-309) ----------------
   void extrapolate(int dir, int k, int j, int i, double *vL, double *vR) {
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
@@ -1477,20 +1521,23 @@ struct Oracle {
         vL[nv] = q(-1);
         vR[nv] = q(0);
       } else if (c.order == 2) {
+        // extrapolateToFaces.hpp:89-124: a flagged cell takes MinModLim, which as built
+        // returns 0
         double dvm = q(-1) - q(-2);
         double dvp = q(0) - q(-1);
-        double dv = vanleer(dvp, dvm);
+        double dv = flagged(dir, k, j, i, -1) ? minmod_as_built(dvp, dvm) : vanleer(dvp, dvm);
         vL[nv] = q(-1) + 0.5 * dv;
         dvm = dvp;
         dvp = q(1) - q(0);
-        dv = vanleer(dvp, dvm);
+        dv = flagged(dir, k, j, i, 0) ? minmod_as_built(dvp, dvm) : vanleer(dvp, dvm);
         vR[nv] = q(0) - 0.5 * dv;
       } else if (c.order == 3) {
         // LimO3, extrapolateToFaces.hpp:170-246 (no shock flattening)
         const int idx = io * i + jo * j + ko * k;
         double dvm = q(-1) - q(-2);
         double dvp = q(0) - q(-1);
-        double dv = dvp * limo3lim(dvp, dvm, dx[dir][idx - 1]);
+        double dv = flagged(dir, k, j, i, -1) ? minmod_as_built(dvp, dvm)
+                                              : dvp * limo3lim(dvp, dvm, dx[dir][idx - 1]);
         vL[nv] = q(-1) + 0.5 * dv;
         if ((nv == V.RHO || nv == V.ENG) && vL[nv] <= 0.0) {
           dv = minmod_as_built(dvp, dvm);
@@ -1498,7 +1545,8 @@ struct Oracle {
         }
         dvm = dvp;
         dvp = q(1) - q(0);
-        dv = dvm * limo3lim(dvm, dvp, dx[dir][idx]);
+        dv = flagged(dir, k, j, i, 0) ? minmod_as_built(dvp, dvm)
+                                      : dvm * limo3lim(dvm, dvp, dx[dir][idx]);
         vR[nv] = q(0) - 0.5 * dv;
         if ((nv == V.RHO || nv == V.ENG) && vR[nv] <= 0.0) {
           dv = minmod_as_built(dvp, dvm);
@@ -1542,6 +1590,8 @@ struct Oracle {
   std::vector<double> *dR_arr(int dir) { return dir == 0 ? &dxR : dir == 1 ? &dyR : &dzR; }
 
   void calc_flux(int dir) {
+    // RiemannSolver::CalcFlux<IDIR> refreshes the shock flags (calcFlux.hpp:30-33)
+    if (dir == IDIR && c.shock_flattening) find_shock();
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
     int ext[3] = {0, 0, 0};
     if (c.mhd) {
@@ -2102,6 +2152,11 @@ void orc_prim_to_cons(void *p) { static_cast<Oracle *>(p)->prim_to_cons(); }
 void orc_cons_to_prim(void *p) { static_cast<Oracle *>(p)->cons_to_prim(); }
 void orc_reset_stage(void *p) { static_cast<Oracle *>(p)->reset_stage(); }
 void orc_calc_flux(void *p, int dir) { static_cast<Oracle *>(p)->calc_flux(dir); }
+void orc_find_shock(void *p, unsigned char *out) {
+  Oracle *o = static_cast<Oracle *>(p);
+  o->find_shock();
+  std::copy(o->shock.begin(), o->shock.end(), out);
+}
 void orc_calc_rhs(void *p, int dir, double dt) { static_cast<Oracle *>(p)->calc_rhs(dir, dt); }
 void orc_calc_corner_emf(void *p) { static_cast<Oracle *>(p)->calc_corner_emf(); }
 void orc_evolve_mag_field(void *p, double dt) { static_cast<Oracle *>(p)->evolve_mag_field(dt); }

@@ -55,6 +55,8 @@ typedef struct {
   double cfl_max_var;
   double first_dt;
   int ntracer;        /* passive tracers ([Hydro] tracer, fluid.hpp:315-325) */
+  int shock_flattening;   /* [Hydro] shockFlattening given (riemannSolver.hpp:151-157) */
+  double shock_smoothing; /* its value */
 } orc_cfg;
 
 void *orc_new(const orc_cfg *cfg);
@@ -72,6 +74,8 @@ void orc_prim_to_cons(void *o);
 void orc_cons_to_prim(void *o);
 void orc_reset_stage(void *o);
 void orc_calc_flux(void *o, int dir);
+/* ShockFlattening::FindShock on the current Vc; copies the flags (np_tot cells) to out */
+void orc_find_shock(void *o, unsigned char *out);
 void orc_calc_rhs(void *o, int dir, double dt);
 void orc_calc_corner_emf(void *o);
 void orc_evolve_mag_field(void *o, double dt);

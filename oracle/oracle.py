@@ -32,7 +32,8 @@ class OrcCfg(ctypes.Structure):
                 ("lbound", ctypes.c_int * 3), ("rbound", ctypes.c_int * 3),
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("cfl_max_var", ctypes.c_double), ("first_dt", ctypes.c_double),
-                ("ntracer", ctypes.c_int)]
+                ("ntracer", ctypes.c_int), ("shock_flattening", ctypes.c_int),
+                ("shock_smoothing", ctypes.c_double)]
 
 
 def build():
@@ -60,6 +61,7 @@ def lib():
             "orc_set_boundaries": (None, [vp]), "orc_prim_to_cons": (None, [vp]),
             "orc_cons_to_prim": (None, [vp]), "orc_reset_stage": (None, [vp]),
             "orc_calc_flux": (None, [vp, i]), "orc_calc_rhs": (None, [vp, i, d]),
+            "orc_find_shock": (None, [vp, vp]),
             "orc_calc_corner_emf": (None, [vp]), "orc_evolve_mag_field": (None, [vp, d]),
             "orc_reconstruct_vc_field_uc": (None, [vp]),
             "orc_evolve_stage": (None, [vp, d, d]), "orc_compute_timestep": (d, [vp]),
@@ -84,7 +86,8 @@ def lib():
 def make_cfg(dim, comps=None, mhd=True, recon="plm", solver="hlld", emf="uct_contact",
              nstages=2, nx=(64, 1, 1), xbeg=(0.0, 0.0, 0.0), xend=(1.0, 1.0, 1.0),
              bc="periodic", gamma=5.0 / 3.0, cfl=0.6, cfl_max_var=1.1, first_dt=1e-4,
-             ntracer=0):
+             ntracer=0, shock_flattening=None):
+    """shock_flattening: None (off) or the smoothing value of [Hydro] shockFlattening."""
     c = OrcCfg()
     c.dim = dim
     c.comps = dim if comps is None else comps
@@ -110,6 +113,8 @@ def make_cfg(dim, comps=None, mhd=True, recon="plm", solver="hlld", emf="uct_con
     c.cfl_max_var = cfl_max_var
     c.first_dt = first_dt
     c.ntracer = ntracer
+    c.shock_flattening = 0 if shock_flattening is None else 1
+    c.shock_smoothing = 0.0 if shock_flattening is None else float(shock_flattening)
     return c
 
 
@@ -182,6 +187,13 @@ class Oracle:
 
     def calc_flux(self, d):
         lib().orc_calc_flux(self._h, d)
+
+    def find_shock(self):
+        """Shock flags (uint8, [nz][ny][nx]) of the current Vc."""
+        nx, ny, nz = self.np_tot
+        out = np.zeros((nz, ny, nx), dtype=np.uint8)
+        lib().orc_find_shock(self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return out
 
     def calc_rhs(self, d, dt):
         lib().orc_calc_rhs(self._h, d, dt)

@@ -11,17 +11,19 @@ RTOL = 1e-12
 
 def cfg_kwargs(dim, nx, mhd=True, recon="plm", solver="hlld", emf="uct_contact",
                bc="periodic", gamma=5.0 / 3.0, cfl=0.6, nstages=2, xbeg=(0, 0, 0),
-               xend=(1, 1, 1), first_dt=1e-4, comps=None, ntracer=0):
+               xend=(1, 1, 1), first_dt=1e-4, comps=None, ntracer=0, shock_flattening=None):
     return dict(dim=dim, nx=nx, mhd=mhd, recon=recon, solver=solver, emf=emf, bc=bc,
                 gamma=gamma, cfl=cfl, nstages=nstages, xbeg=xbeg, xend=xend, first_dt=first_dt,
-                comps=dim if comps is None else comps, ntracer=ntracer)
+                comps=dim if comps is None else comps, ntracer=ntracer,
+                shock_flattening=shock_flattening)
 
 
 def make_oracle(kw):
     cfg = O.make_cfg(kw["dim"], kw.get("comps", kw["dim"]), mhd=kw["mhd"], recon=kw["recon"], solver=kw["solver"],
                      emf=kw["emf"], nstages=kw["nstages"], nx=kw["nx"], xbeg=kw["xbeg"],
                      xend=kw["xend"], bc=kw["bc"], gamma=kw["gamma"], cfl=kw["cfl"],
-                     first_dt=kw["first_dt"], ntracer=kw.get("ntracer", 0))
+                     first_dt=kw["first_dt"], ntracer=kw.get("ntracer", 0),
+                     shock_flattening=kw.get("shock_flattening"))
     return O.Oracle(cfg)
 
 
@@ -32,7 +34,8 @@ def make_pair(kw, ic, precision="fp64"):
                   comps=kw.get("comps", kw["dim"]),
                   recon=kw["recon"], solver=kw["solver"], emf=kw["emf"], bc=kw["bc"],
                   gamma=kw["gamma"], cfl=kw["cfl"], nstages=kw["nstages"],
-                  first_dt=kw["first_dt"], precision=precision, ntracer=kw.get("ntracer", 0))
+                  first_dt=kw["first_dt"], precision=precision, ntracer=kw.get("ntracer", 0),
+                  shock_flattening=kw.get("shock_flattening"))
     og = OracleGrid(o)
     for d in range(3):
         assert np.array_equal(og.x[d], g.x[d]), "grid coordinates differ"

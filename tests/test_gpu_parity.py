@@ -274,6 +274,42 @@ def test_fast_wave_1d_periodic():
     compare_state(o, g, what="1-D fast wave")
 
 
+@pytest.mark.parametrize("solver,recon", [("hllc", "plm"), ("hll", "limo3"), ("tvdlf", "plm")])
+def test_sod_shock_flattening(solver, recon):
+    kw = cfg_kwargs(1, (1000,), mhd=False, solver=solver, recon=recon, bc="outflow",
+                    gamma=1.4, cfl=0.8, shock_flattening=0.5)
+    o, g = make_pair(kw, problems.sod)
+    o.set_boundaries()
+    _cycle_both(o, g, 60)
+    compare_state(o, g, what="sod flattening %s %s" % (solver, recon))
+
+
+@pytest.mark.parametrize("emf,recon", [("uct_hlld", "plm"), ("uct_contact", "limo3"),
+                                       ("uct_hll", "plm")])
+def test_orszag_tang_shock_flattening(emf, recon):
+    """Flags on the fused 2-D path (both sweeps read them), from a perturbed state that
+    has compressive cells from the start."""
+    kw = cfg_kwargs(2, (48, 40), emf=emf, recon=recon, cfl=0.6, shock_flattening=0.05)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=23, amp=0.3))
+    o.set_boundaries()
+    _cycle_both(o, g, 12)
+    compare_state(o, g, what="OT flattening %s %s" % (emf, recon))
+
+
+def test_shock_flattening_3d_and_hd2d():
+    kw = cfg_kwargs(3, (16, 12, 10), emf="uct_contact", cfl=0.4, shock_flattening=0.05)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=21, amp=0.3))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="3D flattening")
+    kw = cfg_kwargs(2, (40, 24), mhd=False, solver="hllc", bc=["outflow", "periodic"],
+                    gamma=1.4, cfl=0.5, shock_flattening=0.05)
+    o, g = make_pair(kw, lambda gg: problems.random_hd(gg, seed=22, amp=0.4))
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="HD2D flattening")
+
+
 def test_outflow_mhd2d():
     kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))
