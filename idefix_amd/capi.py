@@ -44,7 +44,8 @@ class IdfxDesc(ctypes.Structure):
                 ("emf", ctypes.c_int), ("precision", ctypes.c_int),
                 ("lbound", ctypes.c_int * 3), ("rbound", ctypes.c_int * 3),
                 ("gamma", ctypes.c_double), ("ntracer", ctypes.c_int),
-                ("shock_flattening", ctypes.c_int), ("shock_smoothing", ctypes.c_double)]
+                ("shock_flattening", ctypes.c_int), ("shock_smoothing", ctypes.c_double),
+                ("isothermal", ctypes.c_int), ("csiso", ctypes.c_double)]
 
 
 class IdfxError(RuntimeError):

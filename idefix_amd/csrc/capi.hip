@@ -30,6 +30,7 @@ void register_limo3_hd(Ops *t);
 void register_limo3_mhd(Ops *t);
 void register_limo3_mhd_b(Ops *t);
 void register_mhd1(Ops *t);
+void register_iso(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -51,6 +52,7 @@ Ops *ops_table() {
     register_limo3_mhd(t);
     register_limo3_mhd_b(t);
     register_mhd1(t);
+    register_iso(t);
     return t;
   }();
   return table;
@@ -346,7 +348,9 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   if (d.shock_flattening && d.recon != IDFX_RECON_PLM && d.recon != IDFX_RECON_LIMO3)
     return IDFX_E_ARGS;
   if (d.ntracer > 0 && d.dim == 1 && !d.mhd && d.solver == IDFX_SOLVER_ROE) return IDFX_E_ARGS;
-  const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver, d.precision == IDFX_PREC_FP32);
+  if (d.isothermal && !(d.csiso > 0.0)) return IDFX_E_ARGS;
+  const int key = ops_key(d.dim, d.comps, d.mhd, d.recon, d.solver,
+                          d.precision == IDFX_PREC_FP32, d.isothermal ? 1 : 0);
   if (key < 0 || key >= kOpsTableSize) return IDFX_E_ARGS;
   const Ops *ops = &ops_table()[key];
   if (!ops->valid) return IDFX_E_ARGS;
@@ -380,7 +384,9 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   c->nvs = d.mhd ? (long long)d.dim * c->vsn[0] * c->vsn[1] * c->vsn[2] : 0;
   c->g.sv = c->n3;
   c->g.svs = (long long)c->vsn[0] * c->vsn[1] * c->vsn[2];
-  c->g.gamma = d.gamma;
+  // the kernels read the equation of state's one constant from here: gamma, or the
+  // isothermal sound speed
+  c->g.gamma = d.isothermal ? d.csiso : d.gamma;
   int st = IDFX_OK;
   for (int q = 0; q < 3 && st == IDFX_OK; q++) {
     double *p = nullptr;

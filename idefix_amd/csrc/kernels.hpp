@@ -27,7 +27,7 @@ struct GridDesc {
   long long sv;   // stride between variables of Vc/Uc
   long long svs;  // stride between components of Vs
   const double *dx[3];
-  double gamma;
+  double gamma;  // adiabatic index, or the isothermal sound speed (isothermal configurations)
   int ntr;      // passive tracers, stored after the fluid variables of Vc/Uc
 };
 
@@ -933,7 +933,15 @@ __global__ __launch_bounds__(256) void k_find_shock(GridDesc g, const double *Vc
   const int k = g.beg[2] - ko + t / ((long long)ni * nj);
   const long long p = idx3(g, k, j, i);
   const long long sx = 1, sy = g.nt[0], sz = (long long)g.nt[0] * g.nt[1];
-  const double *P = Vc + C::ENG * g.sv;
+  // pressure: PRS, or rho cs^2 in isothermal configurations (g.gamma holds cs there)
+  auto P = [&](long long q) {
+    if constexpr (C::ISO) {
+      const double cs = g.gamma;
+      return Vc[C::RHO * g.sv + q] * cs * cs;
+    } else {
+      return Vc[C::ENG * g.sv + q];
+    }
+  };
   double divV = (Vc[C::MX1 * g.sv + p + sx] - Vc[C::MX1 * g.sv + p - sx]) / (2 * g.dx[0][i]);
   if constexpr (C::DIM >= 2)
     divV = divV + (Vc[C::MX2 * g.sv + p + sy] - Vc[C::MX2 * g.sv + p - sy]) / (2 * g.dx[1][j]);
@@ -941,19 +949,19 @@ __global__ __launch_bounds__(256) void k_find_shock(GridDesc g, const double *Vc
     divV = divV + (Vc[C::MX3 * g.sv + p + sz] - Vc[C::MX3 * g.sv + p - sz]) / (2 * g.dx[2][k]);
   unsigned char f = 0;
   if (divV < 0.0) {
-    double pmin = P[p];
-    pmin = fmin(pmin, P[p + sx]);
-    pmin = fmin(pmin, P[p - sx]);
-    double gradP = fabs(P[p + sx] - P[p - sx]);
+    double pmin = P(p);
+    pmin = fmin(pmin, P(p + sx));
+    pmin = fmin(pmin, P(p - sx));
+    double gradP = fabs(P(p + sx) - P(p - sx));
     if constexpr (C::DIM >= 2) {
-      pmin = fmin(pmin, P[p + sy]);
-      pmin = fmin(pmin, P[p - sy]);
-      gradP += fabs(P[p + sy] - P[p - sy]);
+      pmin = fmin(pmin, P(p + sy));
+      pmin = fmin(pmin, P(p - sy));
+      gradP += fabs(P(p + sy) - P(p - sy));
     }
     if constexpr (C::DIM == 3) {
-      pmin = fmin(pmin, P[p + sz]);
-      pmin = fmin(pmin, P[p - sz]);
-      gradP += fabs(P[p + sz] - P[p - sz]);
+      pmin = fmin(pmin, P(p + sz));
+      pmin = fmin(pmin, P(p - sz));
+      gradP += fabs(P(p + sz) - P(p - sz));
     }
     if (gradP > smoothing * pmin) f = 1;
   }

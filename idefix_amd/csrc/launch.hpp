@@ -434,10 +434,11 @@ Ops make_ops() {
 }
 
 // Registry key: one entry per built configuration and precision.
-constexpr int ops_key(int dim, int comps, int mhd, int order, int solver, int fp32) {
-  return ((((dim * 4 + comps) * 2 + mhd) * 16 + order) * 8 + solver) * 2 + fp32;
+constexpr int ops_key(int dim, int comps, int mhd, int order, int solver, int fp32,
+                      int iso = 0) {
+  return (((((dim * 4 + comps) * 2 + mhd) * 16 + order) * 8 + solver) * 2 + fp32) * 2 + iso;
 }
-constexpr int kOpsTableSize = ops_key(3, 3, 1, 15, 7, 1) + 1;
+constexpr int kOpsTableSize = ops_key(3, 3, 1, 15, 7, 1, 1) + 1;
 Ops *ops_table();  // defined in capi.hip
 
 }  // namespace idfx

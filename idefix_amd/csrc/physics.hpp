@@ -23,21 +23,24 @@ enum EmfScheme { EMF_ARITHMETIC = 0, EMF_UCT0 = 1, EMF_UCT_CONTACT = 2, EMF_UCT_
                  EMF_UCT_HLLD = 4 };
 
 // Compile-time configuration; variable indices follow src/idefix.hpp:74-108.
-template <int DIM_, int COMP_, bool MHD_, int ORDER_, int SOLVER_>
+// ISO: isothermal equation of state (ISOTHERMAL builds: no energy variable, ENG = -1).
+template <int DIM_, int COMP_, bool MHD_, int ORDER_, int SOLVER_, bool ISO_ = false>
 struct Cfg {
   static constexpr int DIM = DIM_;
   static constexpr int COMP = COMP_;
   static constexpr bool MHD = MHD_;
   static constexpr int ORDER = ORDER_;
   static constexpr int SOLVER = SOLVER_;
+  static constexpr bool ISO = ISO_;
+  static constexpr int HE = ISO ? 0 : 1;  // HAVE_ENERGY
   static constexpr int RHO = 0, MX1 = 1;
   static constexpr int MX2 = COMP >= 2 ? 2 : -1;
   static constexpr int MX3 = COMP == 3 ? 3 : -1;
-  static constexpr int ENG = COMP + 1, PRS = COMP + 1;
-  static constexpr int BX1 = MHD ? COMP + 2 : -1;
-  static constexpr int BX2 = (MHD && COMP >= 2) ? COMP + 3 : -1;
-  static constexpr int BX3 = (MHD && COMP == 3) ? COMP + 4 : -1;
-  static constexpr int NVAR = MHD ? 2 * COMP + 2 : COMP + 2;
+  static constexpr int ENG = ISO ? -1 : COMP + 1, PRS = ENG;
+  static constexpr int BX1 = MHD ? COMP + 1 + HE : -1;
+  static constexpr int BX2 = (MHD && COMP >= 2) ? BX1 + 1 : -1;
+  static constexpr int BX3 = (MHD && COMP == 3) ? BX1 + 2 : -1;
+  static constexpr int NVAR = MHD ? 2 * COMP + 1 + HE : COMP + 1 + HE;
   static constexpr int NG = ORDER < 4 ? 2 : 3;  // grid.cpp:60-65
   // per-direction indices (hlldMHD.hpp:153-159)
   template <int DIR> static constexpr int Xn() { return MX1 + DIR; }
@@ -160,20 +163,28 @@ __device__ __forceinline__ void prim2cons(const T *v, T *u, T gamma) {
   u[C::MX1] = v[C::MX1] * v[C::RHO];
   if constexpr (C::COMP >= 2) u[C::MX2] = v[C::MX2] * v[C::RHO];
   if constexpr (C::COMP >= 3) u[C::MX3] = v[C::MX3] * v[C::RHO];
-  T v2 = v[C::MX1] * v[C::MX1];
-  if constexpr (C::COMP >= 2) v2 = v2 + v[C::MX2] * v[C::MX2];
-  if constexpr (C::COMP >= 3) v2 = v2 + v[C::MX3] * v[C::MX3];
-  const T eint = v[C::ENG] / (gamma - T(1.0));
-  if constexpr (C::MHD) {
-    u[C::BX1] = v[C::BX1];
-    if constexpr (C::COMP >= 2) u[C::BX2] = v[C::BX2];
-    if constexpr (C::COMP >= 3) u[C::BX3] = v[C::BX3];
-    T b2 = u[C::BX1] * u[C::BX1];
-    if constexpr (C::COMP >= 2) b2 = b2 + u[C::BX2] * u[C::BX2];
-    if constexpr (C::COMP >= 3) b2 = b2 + u[C::BX3] * u[C::BX3];
-    u[C::ENG] = eint + T(0.5) * v[C::RHO] * v2 + T(0.5) * b2;
+  if constexpr (C::ISO) {
+    if constexpr (C::MHD) {
+      u[C::BX1] = v[C::BX1];
+      if constexpr (C::COMP >= 2) u[C::BX2] = v[C::BX2];
+      if constexpr (C::COMP >= 3) u[C::BX3] = v[C::BX3];
+    }
   } else {
-    u[C::ENG] = eint + T(0.5) * v[C::RHO] * v2;
+    T v2 = v[C::MX1] * v[C::MX1];
+    if constexpr (C::COMP >= 2) v2 = v2 + v[C::MX2] * v[C::MX2];
+    if constexpr (C::COMP >= 3) v2 = v2 + v[C::MX3] * v[C::MX3];
+    const T eint = v[C::ENG] / (gamma - T(1.0));
+    if constexpr (C::MHD) {
+      u[C::BX1] = v[C::BX1];
+      if constexpr (C::COMP >= 2) u[C::BX2] = v[C::BX2];
+      if constexpr (C::COMP >= 3) u[C::BX3] = v[C::BX3];
+      T b2 = u[C::BX1] * u[C::BX1];
+      if constexpr (C::COMP >= 2) b2 = b2 + u[C::BX2] * u[C::BX2];
+      if constexpr (C::COMP >= 3) b2 = b2 + u[C::BX3] * u[C::BX3];
+      u[C::ENG] = eint + T(0.5) * v[C::RHO] * v2 + T(0.5) * b2;
+    } else {
+      u[C::ENG] = eint + T(0.5) * v[C::RHO] * v2;
+    }
   }
 }
 
@@ -186,30 +197,39 @@ __device__ __forceinline__ void cons2prim(const double *u, double *v, double gam
   v[C::MX1] = u[C::MX1] / u[C::RHO];
   if constexpr (C::COMP >= 2) v[C::MX2] = u[C::MX2] / u[C::RHO];
   if constexpr (C::COMP >= 3) v[C::MX3] = u[C::MX3] / u[C::RHO];
-  double m2 = u[C::MX1] * u[C::MX1];
-  if constexpr (C::COMP >= 2) m2 = m2 + u[C::MX2] * u[C::MX2];
-  if constexpr (C::COMP >= 3) m2 = m2 + u[C::MX3] * u[C::MX3];
-  const double kin = 0.5 / u[C::RHO] * m2;
-  if constexpr (C::MHD) {
-    v[C::BX1] = u[C::BX1];
-    if constexpr (C::COMP >= 2) v[C::BX2] = u[C::BX2];
-    if constexpr (C::COMP >= 3) v[C::BX3] = u[C::BX3];
-    double b2 = u[C::BX1] * u[C::BX1];
-    if constexpr (C::COMP >= 2) b2 = b2 + u[C::BX2] * u[C::BX2];
-    if constexpr (C::COMP >= 3) b2 = b2 + u[C::BX3] * u[C::BX3];
-    const double mag = 0.5 * b2;
-    v[C::ENG] = (u[C::ENG] - kin - mag) * (gamma - 1.0);
+  if constexpr (C::ISO) {
+    if constexpr (C::MHD) {
+      v[C::BX1] = u[C::BX1];
+      if constexpr (C::COMP >= 2) v[C::BX2] = u[C::BX2];
+      if constexpr (C::COMP >= 3) v[C::BX3] = u[C::BX3];
+    }
   } else {
-    v[C::ENG] = (u[C::ENG] - kin) * (gamma - 1.0);
+    double m2 = u[C::MX1] * u[C::MX1];
+    if constexpr (C::COMP >= 2) m2 = m2 + u[C::MX2] * u[C::MX2];
+    if constexpr (C::COMP >= 3) m2 = m2 + u[C::MX3] * u[C::MX3];
+    const double kin = 0.5 / u[C::RHO] * m2;
+    if constexpr (C::MHD) {
+      v[C::BX1] = u[C::BX1];
+      if constexpr (C::COMP >= 2) v[C::BX2] = u[C::BX2];
+      if constexpr (C::COMP >= 3) v[C::BX3] = u[C::BX3];
+      double b2 = u[C::BX1] * u[C::BX1];
+      if constexpr (C::COMP >= 2) b2 = b2 + u[C::BX2] * u[C::BX2];
+      if constexpr (C::COMP >= 3) b2 = b2 + u[C::BX3] * u[C::BX3];
+      const double mag = 0.5 * b2;
+      v[C::ENG] = (u[C::ENG] - kin - mag) * (gamma - 1.0);
+    } else {
+      v[C::ENG] = (u[C::ENG] - kin) * (gamma - 1.0);
+    }
+    if (v[C::ENG] <= 0.0) v[C::ENG] = 1.0e-5;
   }
-  if (v[C::ENG] <= 0.0) v[C::ENG] = 1.0e-5;
 }
 
 // ------------------------------------------------------------------------------------
 // K_Flux<Phys,DIR>, src/fluid/RiemannSolver/flux.hpp:28-95
 // ------------------------------------------------------------------------------------
+// c2 is Cs2Iso, the squared isothermal sound speed (isothermal configurations only)
 template <class C, int DIR, typename T>
-__device__ __forceinline__ void kflux(T *F, const T *v, const T *u) {
+__device__ __forceinline__ void kflux(T *F, const T *v, const T *u, T c2 = T(0)) {
   constexpr int Xn = C::template Xn<DIR>();
   F[C::RHO] = u[Xn];
   if constexpr (C::MHD) {
@@ -227,19 +247,27 @@ __device__ __forceinline__ void kflux(T *F, const T *v, const T *u) {
     T b2 = v[C::BX1] * v[C::BX1];
     if constexpr (C::COMP >= 2) b2 = b2 + v[C::BX2] * v[C::BX2];
     if constexpr (C::COMP >= 3) b2 = b2 + v[C::BX3] * v[C::BX3];
-    const T ptot = v[C::ENG] + T(0.5) * b2;
-    T vb = v[C::MX1] * v[C::BX1];
-    if constexpr (C::COMP >= 2) vb = vb + v[C::MX2] * v[C::BX2];
-    if constexpr (C::COMP >= 3) vb = vb + v[C::MX3] * v[C::BX3];
-    F[C::ENG] = (u[C::ENG] + ptot) * v[Xn] - v[BXn] * vb;
-    F[Xn] += ptot;
+    if constexpr (C::ISO) {
+      F[Xn] += c2 * v[C::RHO] + T(0.5) * b2;
+    } else {
+      const T ptot = v[C::ENG] + T(0.5) * b2;
+      T vb = v[C::MX1] * v[C::BX1];
+      if constexpr (C::COMP >= 2) vb = vb + v[C::MX2] * v[C::BX2];
+      if constexpr (C::COMP >= 3) vb = vb + v[C::MX3] * v[C::BX3];
+      F[C::ENG] = (u[C::ENG] + ptot) * v[Xn] - v[BXn] * vb;
+      F[Xn] += ptot;
+    }
   } else {
     F[C::MX1] = u[C::MX1] * v[Xn];
     if constexpr (C::COMP >= 2) F[C::MX2] = u[C::MX2] * v[Xn];
     if constexpr (C::COMP >= 3) F[C::MX3] = u[C::MX3] * v[Xn];
-    const T ptot = v[C::ENG];
-    F[C::ENG] = (u[C::ENG] + ptot) * v[Xn];
-    F[Xn] += ptot;
+    if constexpr (C::ISO) {
+      F[Xn] += c2 * v[C::RHO];
+    } else {
+      const T ptot = v[C::ENG];
+      F[C::ENG] = (u[C::ENG] + ptot) * v[Xn];
+      F[Xn] += ptot;
+    }
   }
 }
 
@@ -539,14 +567,23 @@ __device__ __forceinline__ void store_face_emf(const T *vL, const T *vR, const T
     // K_StoreHLLD, storeFlux.hpp:77-208 (HAVE_ENERGY branch)
     bool revert_to_hll = false;
     const T Bn = (sr * vR[BXn] - sl * vL[BXn]) / (sr - sl);
-    StarPre<T> P = pre;
-    if (!P.valid) star_pre<C, DIR>(vL, vR, uL, uR, sl, sr, ptL, ptR, P);
-    T scrh;
-    const T Sc = P.Sc;
-    const T sqrL = P.sqrL;
-    const T sqrR = P.sqrR;
-    const T SaL = Sc - fab(Bn) / sqrL;
-    const T SaR = Sc + fab(Bn) / sqrR;
+    T scrh, Sc, SaL, SaR;
+    if constexpr (C::ISO) {
+      // storeFlux.hpp:139-147: one averaged density across the fan
+      const T duL = sl - vL[Xn], duR = sr - vR[Xn];
+      const T scrh0 = T(1) / (sr - sl);
+      const T rho_h = (uR[C::RHO] * duR - uL[C::RHO] * duL) * scrh0;
+      Sc = (sl * uR[C::RHO] * duR - sr * uL[C::RHO] * duL) * scrh0 / rho_h;
+      const T sqrho_h = sqrt_(rho_h);
+      SaL = Sc - fab(Bn) / sqrho_h;
+      SaR = Sc + fab(Bn) / sqrho_h;
+    } else {
+      StarPre<T> P = pre;
+      if (!P.valid) star_pre<C, DIR>(vL, vR, uL, uR, sl, sr, ptL, ptR, P);
+      Sc = P.Sc;
+      SaL = Sc - fab(Bn) / P.sqrL;
+      SaR = Sc + fab(Bn) / P.sqrR;
+    }
     if ((SaL - sl) < T(1e-4) * (Sc - sl)) revert_to_hll = true;
     if ((SaR - sr) > -T(1e-4) * (sr - Sc)) revert_to_hll = true;
     const T chiL = (vL[Xn] - Sc) * (sl - Sc) / (SaL + sl - T(2) * Sc);
@@ -1156,12 +1193,430 @@ __device__ __forceinline__ void tvdlf_mhd(const T *vL, const T *vR, T *flux, T &
                          emf == EMF_UCT_HLLD ? EMF_UCT_HLL : emf, pre);
 }
 
+// ====================================================================================
+// Isothermal branches of the solvers (HAVE_ENERGY == 0, constant csiso): the sound speed
+// of a face is the average of its two cells' (here equal) speeds, Cs2Iso its square; the
+// argument `cs` takes the place of gamma.
+// ====================================================================================
+template <typename T>
+__device__ __forceinline__ T face_cs(T cs) { return T(0.5) * (cs + cs); }
+
+// HllHD (hllHD.hpp:67-70) and TvdlfHD (tvdlfHD.hpp:72-75), isothermal
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hll_hd_iso(const T *vL, const T *vR, T *flux, T &cmax, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fL[NV], fR[NV];
+  const T cL = face_cs(cs), cR = cL;
+  const T SL = fmn(vL[Xn] - cL, vR[Xn] - cR);
+  const T SR = fmx(vL[Xn] + cL, vR[Xn] + cR);
+  cmax = fmx(fab(SL), fab(SR));
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fL, vL, uL, cL * cL);
+  kflux<C, DIR>(fR, vR, uR, cR * cR);
+  if (SL > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fL[n];
+  } else if (SR < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fR[n];
+  } else {
+    const T sls = SL * SR, dS = SR - SL;
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = (sls * uR[n] - sls * uL[n] + SR * fL[n] - SL * fR[n]) / dS;
+  }
+}
+
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void tvdlf_hd_iso(const T *vL, const T *vR, T *flux, T &cmax, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fL[NV], fR[NV];
+  const T cRL = face_cs(cs);
+  const T vn = T(0.5) * (vL[Xn] + vR[Xn]);
+  cmax = fmx(fab(vn + cRL), fab(vn - cRL));
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fL, vL, uL, cRL * cRL);
+  kflux<C, DIR>(fR, vR, uR, cRL * cRL);
+#pragma unroll
+  for (int n = 0; n < NV; n++) flux[n] = T(0.5) * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
+}
+
+// HllcHD isothermal star states (hllcHD.hpp:129-143)
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hllc_hd_iso(const T *vL, const T *vR, T *flux, T &cmax, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  T uL[NV], uR[NV], fL[NV], fR[NV];
+  const T cL = face_cs(cs), cR = cL;
+  const T SL = fmn(vL[Xn] - cL, vR[Xn] - cR);
+  const T SR = fmx(vL[Xn] + cL, vR[Xn] + cR);
+  cmax = fmx(fab(SL), fab(SR));
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fL, vL, uL, cL * cL);
+  kflux<C, DIR>(fR, vR, uR, cR * cR);
+  if (SL > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fL[n];
+  } else if (SR < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fR[n];
+  } else {
+    T usL[NV], usR[NV];
+    const T scrh = T(1) / (SR - SL);
+    const T rho = (SR * uR[C::RHO] - SL * uL[C::RHO] - fR[C::RHO] + fL[C::RHO]) * scrh;
+    const T mx = (SR * uR[Xn] - SL * uL[Xn] - fR[Xn] + fL[Xn]) * scrh;
+    usL[C::RHO] = usR[C::RHO] = rho;
+    usL[Xn] = usR[Xn] = mx;
+    T vs = (SR * fL[C::RHO] - SL * fR[C::RHO] + SR * SL * (uR[C::RHO] - uL[C::RHO]));
+    vs *= scrh;
+    vs /= rho;
+    if constexpr (C::COMP >= 2) {
+      usL[Xt] = rho * vL[Xt];
+      usR[Xt] = rho * vR[Xt];
+    }
+    if constexpr (C::COMP >= 3) {
+      usL[Xb] = rho * vL[Xb];
+      usR[Xb] = rho * vR[Xb];
+    }
+    if (vs >= T(0)) {
+#pragma unroll
+      for (int n = 0; n < NV; n++) flux[n] = fL[n] + SL * (usL[n] - uL[n]);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; n++) flux[n] = fR[n] + SR * (usR[n] - uR[n]);
+    }
+  }
+}
+
+// RoeHD isothermal branches (roeHD.hpp:29-36, :95-99, :164-167, :186-194, :254-264)
+template <class C, int DIR, typename T>
+__device__ __forceinline__ bool roe_hd_iso(const T *vL, const T *vR, T *flux, T &cmax, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  constexpr int I0 = 0, I1 = 1, I2 = 2, I3 = 3;
+  T dv[NV], uL[NV], uR[NV], fL[NV], fR[NV], um[NV], Rc[NV][NV];
+  T lambda[NV], alambda[NV], eta[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) dv[n] = vR[n] - vL[n];
+  T a2L = face_cs(cs);
+  T a2R = a2L;
+  a2L = a2L * a2L;
+  a2R = a2R * a2R;
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fL, vL, uL, a2L);
+  kflux<C, DIR>(fR, vR, uR, a2R);
+#pragma unroll
+  for (int n = 0; n < NV; n++) um[n] = T(0.5) * (vR[n] + vL[n]);
+  const T a2 = T(0.5) * (a2L + a2R);
+  const T a = sqrt_(a2);
+#pragma unroll
+  for (int n1 = 0; n1 < NV; n1++)
+#pragma unroll
+    for (int n2 = 0; n2 < NV; n2++) Rc[n1][n2] = T(0);
+  lambda[I0] = um[Xn] - a;
+  eta[I0] = T(0.5) * (dv[C::RHO] - um[C::RHO] * dv[Xn] / a);
+  Rc[C::RHO][I0] = T(1);
+  Rc[Xn][I0] = um[Xn] - a;
+  if constexpr (C::COMP >= 2) Rc[Xt][I0] = um[Xt];
+  if constexpr (C::COMP >= 3) Rc[Xb][I0] = um[Xb];
+  lambda[I1] = um[Xn] + a;
+  eta[I1] = T(0.5) * (dv[C::RHO] + um[C::RHO] * dv[Xn] / a);
+  Rc[C::RHO][I1] = T(1);
+  Rc[Xn][I1] = um[Xn] + a;
+  if constexpr (C::COMP >= 2) Rc[Xt][I1] = um[Xt];
+  if constexpr (C::COMP >= 3) Rc[Xb][I1] = um[Xb];
+  if constexpr (C::COMP > 1) {
+    lambda[I2] = um[Xn];
+    eta[I2] = um[C::RHO] * dv[Xt];
+    Rc[Xt][I2] = T(1);
+  }
+  if constexpr (C::COMP > 2) {
+    lambda[I3] = um[Xn];
+    eta[I3] = um[C::RHO] * dv[Xb];
+    Rc[Xb][I3] = T(1);
+  }
+  cmax = fab(um[Xn]) + a;
+  T scrh = fab(vL[C::RHO] - vR[C::RHO]);
+  scrh /= fmn(vL[C::RHO], vR[C::RHO]);
+  scrh *= a * a;
+  if (scrh > T(0.5) && (vR[Xn] < vL[Xn])) {
+    if constexpr (C::DIM > 1) {
+      const T bmin = fmn(T(0), lambda[0]);
+      const T bmax = fmx(T(0), lambda[1]);
+      const T scrh1 = T(1) / (bmax - bmin);
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        flux[n] = bmin * bmax * (uR[n] - uL[n]) + bmax * fL[n] - bmin * fR[n];
+        flux[n] *= scrh1;
+      }
+      return true;
+    } else {
+      return false;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NV; n++) alambda[n] = fab(lambda[n]);
+  const T delta = T(1.e-7);
+  if (alambda[0] <= delta) alambda[0] = T(0.5) * lambda[0] * lambda[0] / delta + T(0.5) * delta;
+  if (alambda[1] <= delta) alambda[1] = T(0.5) * lambda[1] * lambda[1] / delta + T(0.5) * delta;
+#pragma unroll
+  for (int n = 0; n < NV; n++) {
+    T f = fL[n] + fR[n];
+#pragma unroll
+    for (int n2 = 0; n2 < NV; n2++) f -= alambda[n2] * eta[n2] * Rc[n][n2];
+    flux[n] = f * T(0.5);
+  }
+  return true;
+}
+
+// Fast speed of one state with gpr = Cs2Iso rho (hlldMHD.hpp:188-226 isothermal)
+template <class C, int DIR, typename T>
+__device__ __forceinline__ T fast_speed_iso(const T *v, T c2) {
+  constexpr int CO = C::COMP;
+  const T gpr = c2 * v[C::RHO];
+  T b1 = v[C::template BXn<DIR>()], b2 = 0, b3 = 0;
+  if constexpr (CO >= 2) b2 = v[C::template BXt<DIR>()];
+  if constexpr (CO >= 3) b3 = v[C::template BXb<DIR>()];
+  const T Btmag2 = b2 * b2 + b3 * b3;
+  const T Bmag2 = b1 * b1 + Btmag2;
+  T c = gpr - Bmag2;
+  c = gpr + Bmag2 + sqrt_(c * c + T(4) * gpr * Btmag2);
+  return sqrt_(T(0.5) * c / v[C::RHO]);
+}
+
+template <class C, typename T>
+__device__ __forceinline__ T total_pressure_iso(const T *v, T c2) {
+  T b2 = v[C::BX1] * v[C::BX1];
+  if constexpr (C::COMP >= 2) b2 = b2 + v[C::BX2] * v[C::BX2];
+  if constexpr (C::COMP >= 3) b2 = b2 + v[C::BX3] * v[C::BX3];
+  return c2 * v[C::RHO] + T(0.5) * b2;
+}
+
+// HllMHD isothermal (hllMHD.hpp:213-231)
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hll_mhd_iso(const T *vL, const T *vR, T *flux, T &cmax,
+                                            FaceEmf<T> &E, int emf, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
+  const T c2 = face_cs(cs) * face_cs(cs);
+  const T cL = fast_speed_iso<C, DIR>(vL, c2);
+  const T cR = fast_speed_iso<C, DIR>(vR, c2);
+  const T sl = fmn(vL[Xn] - cL, vR[Xn] - cR);
+  const T sr = fmx(vL[Xn] + cL, vR[Xn] + cR);
+  cmax = fmx(fab(sl), fab(sr));
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fluxL, vL, uL, c2);
+  kflux<C, DIR>(fluxR, vR, uR, c2);
+  if (sl > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxL[n];
+  } else if (sr < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxR[n];
+  } else {
+    const T sls = sl * sr, dS = sr - sl, rdS = T(1) / dS;
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      const T f = sls * uR[n] - sls * uL[n] + sr * fluxL[n] - sl * fluxR[n];
+      flux[n] = (n < C::BX1) ? f / dS : f * rdS;
+    }
+  }
+  StarPre<T> pre;
+  pre.valid = false;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, total_pressure_iso<C>(vL, c2),
+                         total_pressure_iso<C>(vR, c2), E,
+                         emf == EMF_UCT_HLLD ? EMF_UCT_HLL : emf, pre);
+}
+
+// TvdlfMHD isothermal (tvdlfMHD.hpp:179-185)
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void tvdlf_mhd_iso(const T *vL, const T *vR, T *flux, T &cmax,
+                                              FaceEmf<T> &E, int emf, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
+  const T c2 = face_cs(cs) * face_cs(cs);
+  const T rho = T(0.5) * (vL[C::RHO] + vR[C::RHO]);
+  const T gpr = c2 * rho;
+  const T bn = vL[C::template BXn<DIR>()];
+  T Bt2 = T(0);
+  if constexpr (CO >= 2) {
+    constexpr int BXt = C::template BXt<DIR>();
+    const T bt = T(0.5) * (vL[BXt] + vR[BXt]);
+    Bt2 = Bt2 + bt * bt;
+  }
+  if constexpr (CO >= 3) {
+    constexpr int BXb = C::template BXb<DIR>();
+    const T bb = T(0.5) * (vL[BXb] + vR[BXb]);
+    Bt2 = Bt2 + bb * bb;
+  }
+  const T B2 = Bt2 + bn * bn;
+  T cRL = gpr - B2;
+  cRL = cRL + B2 + sqrt_(cRL * cRL + T(4) * gpr * Bt2);
+  cRL = sqrt_(T(0.5) * cRL / rho);
+  const T vn = T(0.5) * (vL[Xn] + vR[Xn]);
+  cmax = fmx(fab(vn + cRL), fab(vn - cRL));
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fluxL, vL, uL, c2);
+  kflux<C, DIR>(fluxR, vR, uR, c2);
+#pragma unroll
+  for (int n = 0; n < NV; n++) flux[n] = T(0.5) * (fluxL[n] + fluxR[n] + cmax * (uL[n] - uR[n]));
+  StarPre<T> pre;
+  pre.valid = false;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, -cmax, cmax, T(0), T(0), E,
+                         emf == EMF_UCT_HLLD ? EMF_UCT_HLL : emf, pre);
+}
+
+// HlldMHD isothermal (hlldMHD.hpp:459-571, Mignone 2007)
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void hlld_mhd_iso(const T *vL, const T *vR, T *flux, T &cmax,
+                                             FaceEmf<T> &E, int emf, T cs) {
+  constexpr int NV = C::NVAR;
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  constexpr int BXn = C::template BXn<DIR>();
+  constexpr int BXt = C::template BXt<DIR>();
+  constexpr int BXb = C::template BXb<DIR>();
+  T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
+  const T c2 = face_cs(cs) * face_cs(cs);
+  const T cL = fast_speed_iso<C, DIR>(vL, c2);
+  const T cR = fast_speed_iso<C, DIR>(vR, c2);
+  const T sl = fmn(vL[Xn] - cL, vR[Xn] - cR);
+  const T sr = fmx(vL[Xn] + cL, vR[Xn] + cR);
+  cmax = fmx(fab(sl), fab(sr));
+  prim2cons<C>(vL, uL, cs);
+  prim2cons<C>(vR, uR, cs);
+  kflux<C, DIR>(fluxL, vL, uL, c2);
+  kflux<C, DIR>(fluxR, vR, uR, c2);
+  if (sl > T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxL[n];
+  } else if (sr < T(0)) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) flux[n] = fluxR[n];
+  } else {
+    T scrh = T(1) / (sr - sl);
+    const T duL = sl - vL[Xn];
+    const T duR = sr - vR[Xn];
+    const T Bx = (sr * vR[BXn] - sl * vL[BXn]) * scrh;
+    const T rho = (uR[C::RHO] * duR - uL[C::RHO] * duL) * scrh;
+    flux[C::RHO] = (sl * uR[C::RHO] * duR - sr * uL[C::RHO] * duL) * scrh;
+    const T sqrho = sqrt_(rho);
+    const T SM = flux[C::RHO] / rho;
+    const T S1L = SM - fab(Bx) / sqrho;
+    const T S1R = SM + fab(Bx) / sqrho;
+    bool revert_to_hll = false;
+    if ((S1L - sl) < T(1.e-4) * (sr - sl)) revert_to_hll = true;
+    if ((S1R - sr) > -T(1.e-4) * (sr - sl)) revert_to_hll = true;
+    if (revert_to_hll) {
+      scrh = T(1) / (sr - sl);
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        flux[n] = sl * sr * (uR[n] - uL[n]) + sr * fluxL[n] - sl * fluxR[n];
+        flux[n] *= scrh;
+      }
+    } else {
+      flux[Xn] = (sr * fluxL[Xn] - sl * fluxR[Xn] + sr * sl * (uR[Xn] - uL[Xn])) * scrh;
+      flux[BXn] = sr * sl * (uR[BXn] - uL[BXn]) * scrh;
+      const T scrhL = T(1) / ((sl - S1L) * (sl - S1R));
+      const T scrhR = T(1) / ((sr - S1L) * (sr - S1R));
+      T usLt = 0, usRt = 0, usLbt = 0, usRbt = 0, usLb = 0, usRb = 0, usLbb = 0, usRbb = 0;
+      if constexpr (CO >= 2) {
+        usLt = rho * vL[Xt] - Bx * uL[BXt] * (SM - vL[Xn]) * scrhL;
+        usRt = rho * vR[Xt] - Bx * uR[BXt] * (SM - vR[Xn]) * scrhR;
+        usLbt = uL[BXt] / rho * (uL[C::RHO] * duL * duL - Bx * Bx) * scrhL;
+        usRbt = uR[BXt] / rho * (uR[C::RHO] * duR * duR - Bx * Bx) * scrhR;
+      }
+      if constexpr (CO >= 3) {
+        usLb = rho * vL[Xb] - Bx * uL[BXb] * (SM - vL[Xn]) * scrhL;
+        usRb = rho * vR[Xb] - Bx * uR[BXb] * (SM - vR[Xn]) * scrhR;
+        usLbb = uL[BXb] / rho * (uL[C::RHO] * duL * duL - Bx * Bx) * scrhL;
+        usRbb = uR[BXb] / rho * (uR[C::RHO] * duR * duR - Bx * Bx) * scrhR;
+      }
+      if (S1L >= T(0)) {
+        if constexpr (CO >= 2) {
+          flux[Xt] = fluxL[Xt] + sl * (usLt - uL[Xt]);
+          flux[BXt] = fluxL[BXt] + sl * (usLbt - uL[BXt]);
+        }
+        if constexpr (CO >= 3) {
+          flux[Xb] = fluxL[Xb] + sl * (usLb - uL[Xb]);
+          flux[BXb] = fluxL[BXb] + sl * (usLbb - uL[BXb]);
+        }
+      } else if (S1R <= T(0)) {
+        if constexpr (CO >= 2) {
+          flux[Xt] = fluxR[Xt] + sr * (usRt - uR[Xt]);
+          flux[BXt] = fluxR[BXt] + sr * (usRbt - uR[BXt]);
+        }
+        if constexpr (CO >= 3) {
+          flux[Xb] = fluxR[Xb] + sr * (usRb - uR[Xb]);
+          flux[BXb] = fluxR[BXb] + sr * (usRbb - uR[BXb]);
+        }
+      } else {
+        const T sBx = (Bx > T(0) ? T(1) : T(-1));
+        if constexpr (CO >= 2) {
+          const T uct = T(0.5) * (usRt + usLt + (usRbt - usLbt) * sBx * sqrho);
+          const T ucbt = T(0.5) * (usRbt + usLbt + (usRt - usLt) * sBx / sqrho);
+          flux[Xt] = uct * SM - Bx * ucbt;
+          flux[BXt] = ucbt * SM - Bx * uct / rho;
+        }
+        if constexpr (CO >= 3) {
+          const T ucb = T(0.5) * (usRb + usLb + (usRbb - usLbb) * sBx * sqrho);
+          const T ucbb = T(0.5) * (usRbb + usLbb + (usRb - usLb) * sBx / sqrho);
+          flux[Xb] = ucb * SM - Bx * ucbb;
+          flux[BXb] = ucbb * SM - Bx * ucb / rho;
+        }
+      }
+    }
+  }
+  StarPre<T> pre;
+  pre.valid = false;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, total_pressure_iso<C>(vL, c2),
+                         total_pressure_iso<C>(vR, c2), E, emf, pre);
+}
+
 // Dispatch to the configured Riemann solver.  Returns false only in the 1-D Roe case
 // where the reference leaves the flux unwritten.
 template <class C, int DIR, typename T>
 __device__ __forceinline__ bool riemann(const T *vL, const T *vR, T *flux, T &cmax,
                                         FaceEmf<T> &E, int emf, T gamma) {
-  if constexpr (C::MHD) {
+  if constexpr (C::ISO) {
+    // gamma carries the isothermal sound speed in isothermal configurations
+    if constexpr (C::MHD) {
+      if constexpr (C::SOLVER == SOLVER_HLL) hll_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+      else if constexpr (C::SOLVER == SOLVER_TVDLF) tvdlf_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+      else {
+        static_assert(C::SOLVER == SOLVER_HLLD, "isothermal MHD solver not built");
+        hlld_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+      }
+      return true;
+    } else if constexpr (C::SOLVER == SOLVER_HLLC) {
+      hllc_hd_iso<C, DIR>(vL, vR, flux, cmax, gamma);
+      return true;
+    } else if constexpr (C::SOLVER == SOLVER_HLL) {
+      hll_hd_iso<C, DIR>(vL, vR, flux, cmax, gamma);
+      return true;
+    } else if constexpr (C::SOLVER == SOLVER_TVDLF) {
+      tvdlf_hd_iso<C, DIR>(vL, vR, flux, cmax, gamma);
+      return true;
+    } else {
+      return roe_hd_iso<C, DIR>(vL, vR, flux, cmax, gamma);
+    }
+  } else if constexpr (C::MHD) {
     if constexpr (C::SOLVER == SOLVER_ROE) {
       roe_mhd<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
     } else if constexpr (C::SOLVER == SOLVER_HLL) {

@@ -43,7 +43,8 @@ class DataBlock:
     def __init__(self, dim, nx, xbeg=(0.0, 0.0, 0.0), xend=(1.0, 1.0, 1.0), comps=None,
                  mhd=True, recon="plm", solver="hlld", emf="uct_contact", precision="fp64",
                  bc="periodic", gamma=5.0 / 3.0, nstages=2, cfl=0.6, cfl_max_var=1.1,
-                 first_dt=1e-4, device="cuda:0", ntracer=0, shock_flattening=None):
+                 first_dt=1e-4, device="cuda:0", ntracer=0, shock_flattening=None,
+                 csiso=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("DataBlock needs a GPU: the hot path has no CPU fallback")
@@ -85,6 +86,10 @@ class DataBlock:
         # None: off; else the smoothing value of [Hydro] shockFlattening
         desc.shock_flattening = 0 if shock_flattening is None else 1
         desc.shock_smoothing = 0.0 if shock_flattening is None else float(shock_flattening)
+        # None: adiabatic; else the constant isothermal sound speed
+        desc.isothermal = 0 if csiso is None else 1
+        desc.csiso = 0.0 if csiso is None else float(csiso)
+        self.isothermal = csiso is not None
         self.ntracer = ntracer
         L = capi.load()
         self._L = L

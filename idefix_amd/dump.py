@@ -21,7 +21,7 @@ TYPES = {0: ("d", np.float64), 1: ("f", np.float32), 2: ("i", np.int32), 3: ("?"
 VERSION = "v2.2.02"
 
 
-def _cvar_names(dim, comps, mhd, ntracer=0):
+def _cvar_names(dim, comps, mhd, ntracer=0, isothermal=False):
     """Vc variables saved in dumps (fluid.hpp:587-638): name -> variable index; passive
     tracers follow the fluid variables as Vc-TR0, Vc-TR1, ..."""
     names = {"Vc-RHO": 0, "Vc-VX1": 1}
@@ -29,13 +29,15 @@ def _cvar_names(dim, comps, mhd, ntracer=0):
         names["Vc-VX2"] = 2
     if comps >= 3:
         names["Vc-VX3"] = 3
-    names["Vc-PRS"] = comps + 1
+    he = 0 if isothermal else 1  # no PRS in isothermal builds
+    if he:
+        names["Vc-PRS"] = comps + 1
     if mhd:
         if comps >= 2 and dim < 2:
-            names["Vc-BX2"] = comps + 3
+            names["Vc-BX2"] = comps + 2 + he
         if comps >= 3 and dim < 3:
-            names["Vc-BX3"] = comps + 4
-    nfluid = 2 * comps + 2 if mhd else comps + 2
+            names["Vc-BX3"] = comps + 3 + he
+    nfluid = 2 * comps + 1 + he if mhd else comps + 1 + he
     for t in range(ntracer):
         names["Vc-TR%d" % t] = nfluid + t
     return names
@@ -70,7 +72,8 @@ def write_dump(filename, g, extra=None):
             _field(fh, "xl%d" % (d + 1), xl)
             _field(fh, "xr%d" % (d + 1), xrr)
         entries = {}
-        for name, n in _cvar_names(g.dim, g.comps, g.mhd, getattr(g, "ntracer", 0)).items():
+        for name, n in _cvar_names(g.dim, g.comps, g.mhd, getattr(g, "ntracer", 0),
+                                     getattr(g, "isothermal", False)).items():
             entries[name] = ("cell", n)
         if g.mhd:
             for d in range(g.dim):
@@ -127,7 +130,8 @@ def restart(g, filename):
     _, F = read_dump(filename)
     Vc, Vs = g.download()
     b, e = g.beg, g.end
-    for name, n in _cvar_names(g.dim, g.comps, g.mhd, getattr(g, "ntracer", 0)).items():
+    for name, n in _cvar_names(g.dim, g.comps, g.mhd, getattr(g, "ntracer", 0),
+                                     getattr(g, "isothermal", False)).items():
         Vc[n][b[2]:e[2], b[1]:e[1], b[0]:e[0]] = F[name]
     if g.mhd:
         for d in range(g.dim):

@@ -19,7 +19,7 @@ UNSUPPORTED = {
     # [Hydro] entries that switch on physics this library does not implement
     "Hydro": ("hall", "resistivity", "ambipolar", "viscosity", "bragTDiffusion",
               "bragViscosity", "rotation", "shearingBox", "fargo",
-              "csiso", "gravPotential", "bodyForce", "drag"),
+              "gravPotential", "bodyForce", "drag"),
 }
 
 
@@ -104,6 +104,11 @@ def block_config(blocks, recon="plm", mhd=True, precision="fp64", comps=None):
               ntracer=_get(blocks, "Hydro", "tracer", cast=int, default=0),
               shock_flattening=(_get(blocks, "Hydro", "shockFlattening", cast=float)
                                 if "shockFlattening" in blocks.get("Hydro", {}) else None))
+    if "csiso" in blocks.get("Hydro", {}):
+        # ISOTHERMAL builds (eos_isothermal.hpp:16-27): "csiso constant <value>"
+        if _get(blocks, "Hydro", "csiso") != "constant":
+            raise NotImplementedError("[Hydro] csiso: only a constant sound speed")
+        kw["csiso"] = _get(blocks, "Hydro", "csiso", i=1, cast=float)
     if mhd:
         kw["emf"] = _get(blocks, "Hydro", "emf", default="uct_contact")
     run = dict(tstop=_get(blocks, "TimeIntegrator", "tstop", cast=float),

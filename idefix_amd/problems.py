@@ -30,9 +30,16 @@ def _is_mhd(g):
     return bool(g.cfg.mhd)
 
 
+def _is_iso(g):
+    if hasattr(g, "isothermal"):
+        return bool(g.isothermal)
+    return bool(getattr(getattr(g, "cfg", None), "isothermal", 0))
+
+
 def _nfluid(g):
-    """Number of fluid variables; passive tracers follow them in Vc."""
-    return 2 * _comps(g) + 2 if _is_mhd(g) else _comps(g) + 2
+    """Number of fluid variables (no energy when isothermal); passive tracers follow."""
+    he = 0 if _is_iso(g) else 1
+    return 2 * _comps(g) + 1 + he if _is_mhd(g) else _comps(g) + 1 + he
 
 
 def _comps(g):
@@ -300,12 +307,15 @@ def random_mhd(g, seed=0, amp=0.3):
     Vc[0] = 1.0 + amp * rng.uniform(-1, 1, shape)
     for c in range(comps):
         Vc[1 + c] = amp * rng.uniform(-1, 1, shape)
-    Vc[comps + 1] = 0.6 + amp * rng.uniform(-1, 1, shape)
+    he = 0 if _is_iso(g) else 1
+    prs = 0.6 + amp * rng.uniform(-1, 1, shape)  # drawn either way: same sequence
+    if he:
+        Vc[comps + 1] = prs
     if Vs is not None:
         for c in range(Vs.shape[0]):
             Vs[c] = (0.4 - 0.1 * c) + amp * rng.uniform(-1, 1, Vs.shape[1:])
         for c in range(Vs.shape[0], comps):
-            Vc[comps + 2 + c] = 0.2 + amp * rng.uniform(-1, 1, shape)
+            Vc[comps + 1 + he + c] = 0.2 + amp * rng.uniform(-1, 1, shape)
     return Vc, Vs
 
 
@@ -317,7 +327,9 @@ def random_hd(g, seed=0, amp=0.3):
     Vc[0] = 1.0 + amp * rng.uniform(-1, 1, shape)
     for c in range(comps):
         Vc[1 + c] = amp * rng.uniform(-1, 1, shape)
-    Vc[comps + 1] = 0.6 + amp * rng.uniform(-1, 1, shape)
+    prs = 0.6 + amp * rng.uniform(-1, 1, shape)
+    if not _is_iso(g):
+        Vc[comps + 1] = prs
     return Vc, None
 
 
@@ -404,5 +416,27 @@ def brio_wu(g, x0=50.0):
     Vc[comps + 1] = np.where(x < x0, 1.0, 0.1) + 0 * Vc[0]
     Vc[comps + 2] = 0.75
     Vc[comps + 3] = np.where(x < x0, 1.0, -1.0) + 0 * Vc[0]
+    Vs[0] = 0.75
+    return Vc, Vs
+
+
+def sod_iso(g, x0=0.5):
+    """Isothermal shock tube of test/HD/sod-iso/setup.cpp:20-30: rho = 1 | 0.125 at rest
+    (csiso 1 in its input)."""
+    Vc, Vs = _empty(g)
+    x = _coords(g)[0][None, None, :]
+    Vc[0] = np.where(x > x0, 0.125, 1.0) + 0 * Vc[0]
+    return Vc, Vs
+
+
+def brio_wu_iso(g, x0=50.0):
+    """Isothermal MHD shock tube of test/MHD/sod-iso/setup.cpp:20-40: rho = 1 | 0.125,
+    Bx = 0.75, By = 1 | -1, at rest, on [0, 100] (two components)."""
+    Vc, Vs = _empty(g)
+    x = _coords(g)[0][None, None, :]
+    comps = _comps(g)
+    Vc[0] = np.where(x < x0, 1.0, 0.125) + 0 * Vc[0]
+    Vc[comps + 1] = 0.75
+    Vc[comps + 2] = np.where(x < x0, 1.0, -1.0) + 0 * Vc[0]
     Vs[0] = 0.75
     return Vc, Vs

@@ -23,6 +23,8 @@ SETUPS = {
     "orszag_tang_3d": (problems.orszag_tang_3d, True, None),  # test/MHD/OrszagTang3D
     "sod": (problems.sod, False, None),                       # test/HD/sod
     "brio_wu": (problems.brio_wu, True, 2),                   # test/MHD/sod
+    "sod_iso": (problems.sod_iso, False, None),               # test/HD/sod-iso
+    "brio_wu_iso": (problems.brio_wu_iso, True, 2),           # test/MHD/sod-iso
     "kelvin_helmholtz": (problems.kelvin_helmholtz, True, 3),
     "turbulence": (problems.turbulence, True, 3),
 }

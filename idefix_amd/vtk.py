@@ -16,8 +16,8 @@ VERSION = "v2.2.02"
 GEOMETRY_CARTESIAN = 0  # idefix.hpp geometry codes
 
 
-def _names(comps, mhd, ntracer=0):
-    n = ["RHO", "VX1", "VX2", "VX3"][:1 + comps] + ["PRS"]
+def _names(comps, mhd, ntracer=0, isothermal=False):
+    n = ["RHO", "VX1", "VX2", "VX3"][:1 + comps] + ([] if isothermal else ["PRS"])
     if mhd:
         n += ["BX1", "BX2", "BX3"][:comps]
     return n + ["TR%d" % t for t in range(ntracer)]  # fluid.hpp:633-638
@@ -67,7 +67,7 @@ def write_vtk(filename, g, periodic=(True, True, True)):
             w(("" if d == 0 else "\n") + "%s_COORDINATES %d float\n" % (lab, nx[d] + off[d]))
             fh.write(be(nodes[d]))
         w("\nCELL_DATA %d\n" % (nx[0] * nx[1] * nx[2]))
-        names = _names(g.comps, g.mhd, getattr(g, "ntracer", 0))
+        names = _names(g.comps, g.mhd, getattr(g, "ntracer", 0), getattr(g, "isothermal", False))
         for name in sorted(names):
             n = names.index(name)
             w("\nSCALARS %s float\n" % name)

@@ -90,6 +90,9 @@ typedef struct {
                              cells flagged by ShockFlattening::FindShock take the first-order
                              slope; PLM and LimO3 only */
   double shock_smoothing; /* its value (the pressure-jump threshold) */
+  int isothermal;         /* ISOTHERMAL build (idefix.hpp:63-92): no energy variable, so
+                             nvar = 1 + comps (+ comps with mhd); gamma is ignored */
+  double csiso;           /* [Hydro] csiso constant <value> (eos_isothermal.hpp:16-27), > 0 */
 } idfx_desc;
 
 /* Context: replaces the Fluid<Phys>/RiemannSolver/ConstrainedTransport constructors

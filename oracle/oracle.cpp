@@ -52,20 +52,28 @@ struct Vars {
     MX1 = 1;
     MX2 = c.comps >= 2 ? 2 : -1;
     MX3 = c.comps == 3 ? 3 : -1;
-    ENG = c.comps + 1;
+    // ISOTHERMAL builds have no energy variable (idefix.hpp:63-92)
+    const int he = c.isothermal ? 0 : 1;
+    ENG = he ? c.comps + 1 : -1;
     if (c.mhd) {
-      BX1 = c.comps + 2;
+      BX1 = c.comps + 1 + he;
       BX2 = c.comps >= 2 ? BX1 + 1 : -1;
       BX3 = c.comps == 3 ? BX1 + 2 : -1;
-      nvar = 2 * c.comps + 2;
+      nvar = 2 * c.comps + 1 + he;
     } else {
       BX1 = BX2 = BX3 = -1;
-      nvar = c.comps + 2;
+      nvar = c.comps + 1 + he;
     }
   }
 };
 
 inline double at(const double *a, int idx) { return idx >= 0 ? a[idx] : 0.0; }
+// Cs2Iso of a face: the square of the average of the sound speeds of its two cells
+// (hlldMHD.hpp:188-191); constant here (eos_isothermal.hpp, "csiso constant").
+inline double c2iso(const orc_cfg &c) {
+  const double cs = 0.5 * (c.csiso + c.csiso);
+  return cs * cs;
+}
 inline void put(double *a, int idx, double v) { if (idx >= 0) a[idx] = v; }
 // FMIN/FMAX/FABS are fmin/fmax/fabs in double precision (real_types.hpp:49-51)
 inline double FMIN(double a, double b) { return std::fmin(a, b); }
@@ -177,6 +185,14 @@ inline void prim2cons(const orc_cfg &c, const Vars &V, const double *v, double *
   double v2 = v[V.MX1] * v[V.MX1];
   if (c.comps >= 2) v2 = v2 + v[V.MX2] * v[V.MX2];
   if (c.comps >= 3) v2 = v2 + v[V.MX3] * v[V.MX3];
+  if (c.isothermal) {
+    if (c.mhd) {
+      u[V.BX1] = v[V.BX1];
+      if (c.comps >= 2) u[V.BX2] = v[V.BX2];
+      if (c.comps >= 3) u[V.BX3] = v[V.BX3];
+    }
+    return;
+  }
   const double eint = v[V.ENG] / (c.gamma - 1.0);
   if (c.mhd) {
     u[V.BX1] = v[V.BX1];
@@ -203,6 +219,14 @@ inline void cons2prim(const orc_cfg &c, const Vars &V, const double *u_in, doubl
   double m2 = u[V.MX1] * u[V.MX1];
   if (c.comps >= 2) m2 = m2 + u[V.MX2] * u[V.MX2];
   if (c.comps >= 3) m2 = m2 + u[V.MX3] * u[V.MX3];
+  if (c.isothermal) {
+    if (c.mhd) {
+      v[V.BX1] = u[V.BX1];
+      if (c.comps >= 2) v[V.BX2] = u[V.BX2];
+      if (c.comps >= 3) v[V.BX3] = u[V.BX3];
+    }
+    return;
+  }
   const double kin = 0.5 / u[V.RHO] * m2;
   if (c.mhd) {
     v[V.BX1] = u[V.BX1];
@@ -239,8 +263,9 @@ struct Dir {
   }
 };
 
+// c2 is Cs2Iso, the squared isothermal sound speed (used only when isothermal)
 inline void kflux(const orc_cfg &c, const Vars &V, const Dir &D, double *F, const double *v,
-                  const double *u) {
+                  const double *u, double c2 = 0.0) {
   F[V.RHO] = u[D.Xn];
   if (c.mhd) {
     F[V.MX1] = u[V.MX1] * v[D.Xn] - v[D.BXn] * v[V.BX1];
@@ -252,6 +277,10 @@ inline void kflux(const orc_cfg &c, const Vars &V, const Dir &D, double *F, cons
     double b2 = v[V.BX1] * v[V.BX1];
     if (c.comps >= 2) b2 = b2 + v[V.BX2] * v[V.BX2];
     if (c.comps >= 3) b2 = b2 + v[V.BX3] * v[V.BX3];
+    if (c.isothermal) {
+      F[D.Xn] += c2 * v[V.RHO] + 0.5 * b2;
+      return;
+    }
     const double ptot = v[V.ENG] + 0.5 * b2;
     double vb = v[V.MX1] * v[V.BX1];
     if (c.comps >= 2) vb = vb + v[V.MX2] * v[V.BX2];
@@ -262,6 +291,10 @@ inline void kflux(const orc_cfg &c, const Vars &V, const Dir &D, double *F, cons
     F[V.MX1] = u[V.MX1] * v[D.Xn];
     if (c.comps >= 2) F[V.MX2] = u[V.MX2] * v[D.Xn];
     if (c.comps >= 3) F[V.MX3] = u[V.MX3] * v[D.Xn];
+    if (c.isothermal) {
+      F[D.Xn] += c2 * v[V.RHO];
+      return;
+    }
     const double ptot = v[V.ENG];
     F[V.ENG] = (u[V.ENG] + ptot) * v[D.Xn];
     F[D.Xn] += ptot;
@@ -281,8 +314,10 @@ void hllc_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, co
              double *flux, double &cmax) {
   const int nv_ = V.nvar;
   double uL[8], uR[8], fL[8], fR[8];
-  const double cL = std::sqrt(c.gamma * (vL[V.ENG] / vL[V.RHO]));
-  const double cR = std::sqrt(c.gamma * (vR[V.ENG] / vR[V.RHO]));
+  // isothermal: the face average of the sound speed at the two cells (hllcHD.hpp:67-70)
+  const double cL = c.isothermal ? 0.5 * (c.csiso + c.csiso)
+                                 : std::sqrt(c.gamma * (vL[V.ENG] / vL[V.RHO]));
+  const double cR = c.isothermal ? cL : std::sqrt(c.gamma * (vR[V.ENG] / vR[V.RHO]));
   const double cminL = vL[D.Xn] - cL, cmaxL = vL[D.Xn] + cL;
   const double cminR = vR[D.Xn] - cR, cmaxR = vR[D.Xn] + cR;
   const double SL = FMIN(cminL, cminR);
@@ -290,12 +325,36 @@ void hllc_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, co
   cmax = FMAX(std::fabs(SL), std::fabs(SR));
   prim2cons(c, V, vL, uL);
   prim2cons(c, V, vR, uR);
-  kflux(c, V, D, fL, vL, uL);
-  kflux(c, V, D, fR, vR, uR);
+  kflux(c, V, D, fL, vL, uL, cL * cL);
+  kflux(c, V, D, fR, vR, uR, cR * cR);
   if (SL > 0) {
     for (int n = 0; n < nv_; n++) flux[n] = fL[n];
   } else if (SR < 0) {
     for (int n = 0; n < nv_; n++) flux[n] = fR[n];
+  } else if (c.isothermal) {
+    // hllcHD.hpp:129-143
+    double usL[8], usR[8];
+    const double scrh = 1.0 / (SR - SL);
+    const double rho = (SR * uR[V.RHO] - SL * uL[V.RHO] - fR[V.RHO] + fL[V.RHO]) * scrh;
+    const double mx = (SR * uR[D.Xn] - SL * uL[D.Xn] - fR[D.Xn] + fL[D.Xn]) * scrh;
+    usL[V.RHO] = usR[V.RHO] = rho;
+    usL[D.Xn] = usR[D.Xn] = mx;
+    double vs = (SR * fL[V.RHO] - SL * fR[V.RHO] + SR * SL * (uR[V.RHO] - uL[V.RHO]));
+    vs *= scrh;
+    vs /= rho;
+    if (c.comps >= 2) {
+      usL[D.Xt] = rho * vL[D.Xt];
+      usR[D.Xt] = rho * vR[D.Xt];
+    }
+    if (c.comps >= 3) {
+      usL[D.Xb] = rho * vL[D.Xb];
+      usR[D.Xb] = rho * vR[D.Xb];
+    }
+    if (vs >= 0.0) {
+      for (int n = 0; n < nv_; n++) flux[n] = fL[n] + SL * (usL[n] - uL[n]);
+    } else {
+      for (int n = 0; n < nv_; n++) flux[n] = fR[n] + SR * (usR[n] - uR[n]);
+    }
   } else {
     double usL[8], usR[8];
     const double qL = vL[V.ENG] + uL[D.Xn] * (vL[D.Xn] - SL);
@@ -336,8 +395,9 @@ void hll_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, con
             double *flux, double &cmax) {
   const int nv_ = V.nvar;
   double uL[8], uR[8], fL[8], fR[8];
-  const double cL = std::sqrt(c.gamma * (vL[V.ENG] / vL[V.RHO]));
-  const double cR = std::sqrt(c.gamma * (vR[V.ENG] / vR[V.RHO]));
+  const double cL = c.isothermal ? 0.5 * (c.csiso + c.csiso)
+                                 : std::sqrt(c.gamma * (vL[V.ENG] / vL[V.RHO]));
+  const double cR = c.isothermal ? cL : std::sqrt(c.gamma * (vR[V.ENG] / vR[V.RHO]));
   const double cminL = vL[D.Xn] - cL, cmaxL = vL[D.Xn] + cL;
   const double cminR = vR[D.Xn] - cR, cmaxR = vR[D.Xn] + cR;
   const double SL = FMIN(cminL, cminR);
@@ -345,8 +405,8 @@ void hll_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, con
   cmax = FMAX(std::fabs(SL), std::fabs(SR));
   prim2cons(c, V, vL, uL);
   prim2cons(c, V, vR, uR);
-  kflux(c, V, D, fL, vL, uL);
-  kflux(c, V, D, fR, vR, uR);
+  kflux(c, V, D, fL, vL, uL, cL * cL);
+  kflux(c, V, D, fR, vR, uR, cR * cR);
   if (SL > 0) {
     for (int n = 0; n < nv_; n++) flux[n] = fL[n];
   } else if (SR < 0) {
@@ -368,12 +428,13 @@ void tvdlf_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL,
   const int nv_ = V.nvar;
   double v[8], uL[8], uR[8], fL[8], fR[8];
   for (int n = 0; n < nv_; n++) v[n] = 0.5 * (vL[n] + vR[n]);
-  const double cRL = std::sqrt(c.gamma * (v[V.ENG] / v[V.RHO]));
+  const double cRL = c.isothermal ? 0.5 * (c.csiso + c.csiso)
+                                  : std::sqrt(c.gamma * (v[V.ENG] / v[V.RHO]));
   cmax = FMAX(std::fabs(v[D.Xn] + cRL), std::fabs(v[D.Xn] - cRL));
   prim2cons(c, V, vL, uL);
   prim2cons(c, V, vR, uR);
-  kflux(c, V, D, fL, vL, uL);
-  kflux(c, V, D, fR, vR, uR);
+  kflux(c, V, D, fL, vL, uL, cRL * cRL);
+  kflux(c, V, D, fR, vR, uR, cRL * cRL);
   for (int n = 0; n < nv_; n++) flux[n] = 0.5 * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
 }
 
@@ -382,8 +443,83 @@ void tvdlf_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL,
 // ROE_AVERAGE == 0).  Returns false when the reference leaves the flux unwritten
 // (1-D strong shock, roeHD.hpp:315-328 under DIMENSIONS > 1 only).
 // --------------------------------------------------------------------------------------
+// RoeHD isothermal branches (roeHD.hpp:29-36, :95-99, :164-167, :186-194, :254-264):
+// modes u -+ c and the transverse velocities, a^2 the squared sound speed.
+bool roe_hd_iso(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL,
+                const double *vR, double *flux, double &cmax) {
+  const int nv_ = V.nvar;
+  double dv[8], uL[8], uR[8], fL[8], fR[8], Rc[8][8], um[8];
+  for (int n = 0; n < nv_; n++) dv[n] = vR[n] - vL[n];
+  double a2L = 0.5 * (c.csiso + c.csiso);
+  double a2R = a2L;
+  a2L = a2L * a2L;
+  a2R = a2R * a2R;
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  kflux(c, V, D, fL, vL, uL, a2L);
+  kflux(c, V, D, fR, vR, uR, a2R);
+  for (int n = 0; n < nv_; n++) um[n] = 0.5 * (vR[n] + vL[n]);
+  const double a2 = 0.5 * (a2L + a2R);
+  const double a = std::sqrt(a2);
+  const int I0 = 0, I1 = 1, I2 = 2, I3 = 3;
+  double lambda[8], alambda[8], eta[8];
+  for (int a1 = 0; a1 < nv_; a1++)
+    for (int a3 = 0; a3 < nv_; a3++) Rc[a1][a3] = 0;
+  for (int n = 0; n < 8; n++) { lambda[n] = 0; eta[n] = 0; }
+  lambda[I0] = um[D.Xn] - a;
+  eta[I0] = 0.5 * (dv[V.RHO] - um[V.RHO] * dv[D.Xn] / a);
+  Rc[V.RHO][I0] = 1.0;
+  Rc[D.Xn][I0] = um[D.Xn] - a;
+  if (c.comps >= 2) Rc[D.Xt][I0] = um[D.Xt];
+  if (c.comps >= 3) Rc[D.Xb][I0] = um[D.Xb];
+  lambda[I1] = um[D.Xn] + a;
+  eta[I1] = 0.5 * (dv[V.RHO] + um[V.RHO] * dv[D.Xn] / a);
+  Rc[V.RHO][I1] = 1.0;
+  Rc[D.Xn][I1] = um[D.Xn] + a;
+  if (c.comps >= 2) Rc[D.Xt][I1] = um[D.Xt];
+  if (c.comps >= 3) Rc[D.Xb][I1] = um[D.Xb];
+  if (c.comps > 1) {
+    lambda[I2] = um[D.Xn];
+    eta[I2] = um[V.RHO] * dv[D.Xt];
+    Rc[D.Xt][I2] = 1.0;
+  }
+  if (c.comps > 2) {
+    lambda[I3] = um[D.Xn];
+    eta[I3] = um[V.RHO] * dv[D.Xb];
+    Rc[D.Xb][I3] = 1.0;
+  }
+  cmax = std::fabs(um[D.Xn]) + a;
+  double scrh = std::fabs(vL[V.RHO] - vR[V.RHO]);
+  scrh /= FMIN(vL[V.RHO], vR[V.RHO]);
+  scrh *= a * a;
+  if (scrh > 0.5 && (vR[D.Xn] < vL[D.Xn])) {
+    if (c.dim > 1) {
+      const double bmin = FMIN(0.0, lambda[0]);
+      const double bmax = FMAX(0.0, lambda[1]);
+      const double scrh1 = 1.0 / (bmax - bmin);
+      for (int n = 0; n < nv_; n++) {
+        flux[n] = bmin * bmax * (uR[n] - uL[n]) + bmax * fL[n] - bmin * fR[n];
+        flux[n] *= scrh1;
+      }
+      return true;
+    }
+    return false;
+  }
+  for (int n = 0; n < nv_; n++) alambda[n] = std::fabs(lambda[n]);
+  const double delta = 1.e-7;
+  if (alambda[0] <= delta) alambda[0] = 0.5 * lambda[0] * lambda[0] / delta + 0.5 * delta;
+  if (alambda[1] <= delta) alambda[1] = 0.5 * lambda[1] * lambda[1] / delta + 0.5 * delta;
+  for (int n = 0; n < nv_; n++) {
+    double f = fL[n] + fR[n];
+    for (int n2 = 0; n2 < nv_; n2++) f -= alambda[n2] * eta[n2] * Rc[n][n2];
+    flux[n] = f * 0.5;
+  }
+  return true;
+}
+
 bool roe_hd(const orc_cfg &c, const Vars &V, const Dir &D, const double *vL, const double *vR,
             double *flux, double &cmax) {
+  if (c.isothermal) return roe_hd_iso(c, V, D, vL, vR, flux, cmax);
   const int nv_ = V.nvar;
   double dv[8], uL[8], uR[8], fL[8], fR[8], Rc[8][8], um[8];
   for (int n = 0; n < nv_; n++) dv[n] = vR[n] - vL[n];
@@ -485,8 +621,134 @@ void store_face_emf(const orc_cfg &c, const Vars &V, const Dir &D, int dir, cons
 // HlldMHD: src/fluid/RiemannSolver/MHDsolvers/hlldMHD.hpp:147-589 (HAVE_ENERGY branch),
 // with the face-EMF stores of storeFlux.hpp:14-208.
 // --------------------------------------------------------------------------------------
+// HlldMHD isothermal branch (hlldMHD.hpp:188-213, :459-571; Mignone 2007): the outer
+// waves of the fast speeds with gpr = Cs2Iso rho, one averaged density, the rotational
+// discontinuities S1L, S1R, and a reversion to HLL near degeneracy.
+void hlld_mhd_iso(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
+                  const double *vR, double *flux, double &cmax, FaceEmf &E) {
+  const int nv_ = V.nvar;
+  const int comps = c.comps;
+  double uL[8], uR[8], fluxL[8], fluxR[8];
+  const double c2 = c2iso(c);
+  double gpr = c2 * vL[V.RHO];
+  double b1 = vL[D.BXn], b2 = at(vL, D.BXt), b3 = at(vL, D.BXb);
+  double Btmag2 = b2 * b2 + b3 * b3;
+  double Bmag2 = b1 * b1 + Btmag2;
+  double cL = gpr - Bmag2;
+  cL = gpr + Bmag2 + std::sqrt(cL * cL + 4.0 * gpr * Btmag2);
+  cL = std::sqrt(0.5 * cL / vL[V.RHO]);
+  gpr = c2 * vR[V.RHO];
+  b1 = vR[D.BXn];
+  b2 = at(vR, D.BXt);
+  b3 = at(vR, D.BXb);
+  Btmag2 = b2 * b2 + b3 * b3;
+  Bmag2 = b1 * b1 + Btmag2;
+  double cR = gpr - Bmag2;
+  cR = gpr + Bmag2 + std::sqrt(cR * cR + 4.0 * gpr * Btmag2);
+  cR = std::sqrt(0.5 * cR / vR[V.RHO]);
+  const double cminL = vL[D.Xn] - cL, cmaxL = vL[D.Xn] + cL;
+  const double cminR = vR[D.Xn] - cR, cmaxR = vR[D.Xn] + cR;
+  const double sl = FMIN(cminL, cminR);
+  const double sr = FMAX(cmaxL, cmaxR);
+  cmax = std::fmax(std::fabs(sl), std::fabs(sr));
+  prim2cons(c, V, vL, uL);
+  prim2cons(c, V, vR, uR);
+  for (int n = 0; n < nv_; n++) {
+    fluxL[n] = uL[n];
+    fluxR[n] = uR[n];
+  }
+  kflux(c, V, D, fluxL, vL, fluxL, c2);
+  kflux(c, V, D, fluxR, vR, fluxR, c2);
+  if (sl > 0) {
+    for (int n = 0; n < nv_; n++) flux[n] = fluxL[n];
+  } else if (sr < 0) {
+    for (int n = 0; n < nv_; n++) flux[n] = fluxR[n];
+  } else {
+    double usL[8], usR[8], usc[8];
+    double scrh = 1.0 / (sr - sl);
+    const double duL = sl - vL[D.Xn];
+    const double duR = sr - vR[D.Xn];
+    const double Bx = (sr * vR[D.BXn] - sl * vL[D.BXn]) * scrh;
+    const double rho = (uR[V.RHO] * duR - uL[V.RHO] * duL) * scrh;
+    flux[V.RHO] = (sl * uR[V.RHO] * duR - sr * uL[V.RHO] * duL) * scrh;
+    const double sqrho = std::sqrt(rho);
+    const double SM = flux[V.RHO] / rho;
+    const double S1L = SM - std::fabs(Bx) / sqrho;
+    const double S1R = SM + std::fabs(Bx) / sqrho;
+    int revert_to_hll = 0;
+    if ((S1L - sl) < 1.e-4 * (sr - sl)) revert_to_hll = 1;
+    if ((S1R - sr) > -1.e-4 * (sr - sl)) revert_to_hll = 1;
+    if (revert_to_hll) {
+      scrh = 1.0 / (sr - sl);
+      for (int n = 0; n < nv_; n++) {
+        flux[n] = sl * sr * (uR[n] - uL[n]) + sr * fluxL[n] - sl * fluxR[n];
+        flux[n] *= scrh;
+      }
+    } else {
+      flux[D.Xn] = (sr * fluxL[D.Xn] - sl * fluxR[D.Xn] + sr * sl * (uR[D.Xn] - uL[D.Xn])) * scrh;
+      flux[D.BXn] = sr * sl * (uR[D.BXn] - uL[D.BXn]) * scrh;
+      const double scrhL = 1.0 / ((sl - S1L) * (sl - S1R));
+      const double scrhR = 1.0 / ((sr - S1L) * (sr - S1R));
+      if (comps >= 2) {
+        usL[D.Xt] = rho * vL[D.Xt] - Bx * uL[D.BXt] * (SM - vL[D.Xn]) * scrhL;
+        usR[D.Xt] = rho * vR[D.Xt] - Bx * uR[D.BXt] * (SM - vR[D.Xn]) * scrhR;
+        usL[D.BXt] = uL[D.BXt] / rho * (uL[V.RHO] * duL * duL - Bx * Bx) * scrhL;
+        usR[D.BXt] = uR[D.BXt] / rho * (uR[V.RHO] * duR * duR - Bx * Bx) * scrhR;
+      }
+      if (comps >= 3) {
+        usL[D.Xb] = rho * vL[D.Xb] - Bx * uL[D.BXb] * (SM - vL[D.Xn]) * scrhL;
+        usR[D.Xb] = rho * vR[D.Xb] - Bx * uR[D.BXb] * (SM - vR[D.Xn]) * scrhR;
+        usL[D.BXb] = uL[D.BXb] / rho * (uL[V.RHO] * duL * duL - Bx * Bx) * scrhL;
+        usR[D.BXb] = uR[D.BXb] / rho * (uR[V.RHO] * duR * duR - Bx * Bx) * scrhR;
+      }
+      if (S1L >= 0.0) {
+        if (comps >= 2) {
+          flux[D.Xt] = fluxL[D.Xt] + sl * (usL[D.Xt] - uL[D.Xt]);
+          flux[D.BXt] = fluxL[D.BXt] + sl * (usL[D.BXt] - uL[D.BXt]);
+        }
+        if (comps >= 3) {
+          flux[D.Xb] = fluxL[D.Xb] + sl * (usL[D.Xb] - uL[D.Xb]);
+          flux[D.BXb] = fluxL[D.BXb] + sl * (usL[D.BXb] - uL[D.BXb]);
+        }
+      } else if (S1R <= 0.0) {
+        if (comps >= 2) {
+          flux[D.Xt] = fluxR[D.Xt] + sr * (usR[D.Xt] - uR[D.Xt]);
+          flux[D.BXt] = fluxR[D.BXt] + sr * (usR[D.BXt] - uR[D.BXt]);
+        }
+        if (comps >= 3) {
+          flux[D.Xb] = fluxR[D.Xb] + sr * (usR[D.Xb] - uR[D.Xb]);
+          flux[D.BXb] = fluxR[D.BXb] + sr * (usR[D.BXb] - uR[D.BXb]);
+        }
+      } else {
+        const double sBx = (Bx > 0.0 ? 1.0 : -1.0);
+        if (comps >= 2) {
+          usc[D.Xt] = 0.5 * (usR[D.Xt] + usL[D.Xt] + (usR[D.BXt] - usL[D.BXt]) * sBx * sqrho);
+          usc[D.BXt] = 0.5 * (usR[D.BXt] + usL[D.BXt] + (usR[D.Xt] - usL[D.Xt]) * sBx / sqrho);
+          flux[D.Xt] = usc[D.Xt] * SM - Bx * usc[D.BXt];
+          flux[D.BXt] = usc[D.BXt] * SM - Bx * usc[D.Xt] / rho;
+        }
+        if (comps >= 3) {
+          usc[D.Xb] = 0.5 * (usR[D.Xb] + usL[D.Xb] + (usR[D.BXb] - usL[D.BXb]) * sBx * sqrho);
+          usc[D.BXb] = 0.5 * (usR[D.BXb] + usL[D.BXb] + (usR[D.Xb] - usL[D.Xb]) * sBx / sqrho);
+          flux[D.Xb] = usc[D.Xb] * SM - Bx * usc[D.BXb];
+          flux[D.BXb] = usc[D.BXb] * SM - Bx * usc[D.Xb] / rho;
+        }
+      }
+    }
+  }
+  double b2L = vL[V.BX1] * vL[V.BX1];
+  if (comps >= 2) b2L = b2L + vL[V.BX2] * vL[V.BX2];
+  if (comps >= 3) b2L = b2L + vL[V.BX3] * vL[V.BX3];
+  double b2R = vR[V.BX1] * vR[V.BX1];
+  if (comps >= 2) b2R = b2R + vR[V.BX2] * vR[V.BX2];
+  if (comps >= 3) b2R = b2R + vR[V.BX3] * vR[V.BX3];
+  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, c2 * vL[V.RHO] + 0.5 * b2L,
+                 c2 * vR[V.RHO] + 0.5 * b2R, E);
+}
+
 void hlld_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
               const double *vR, double *flux, double &cmax, FaceEmf &E) {
+  if (c.isothermal) return hlld_mhd_iso(c, V, D, dir, vL, vR, flux, cmax, E);
   const int nv_ = V.nvar;
   const int comps = c.comps;
   double uL[8], uR[8], fluxL[8], fluxR[8];
@@ -758,14 +1020,25 @@ void store_face_emf(const orc_cfg &c, const Vars &V, const Dir &D, int dir, cons
     const double Bn = (sr * vR[D.BXn] - sl * vL[D.BXn]) / (sr - sl);
     const double duL = sl - vL[D.Xn];
     const double duR = sr - vR[D.Xn];
-    double scrh = 1.0 / (duR * uR[V.RHO] - duL * uL[V.RHO]);
-    const double Sc = (duR * uR[D.Xn] - duL * uL[D.Xn] - ptR2 + ptL2) * scrh;
-    const double usLRHO = uL[V.RHO] * duL / (sl - Sc);
-    const double usRRHO = uR[V.RHO] * duR / (sr - Sc);
-    const double sqrL = std::sqrt(usLRHO);
-    const double sqrR = std::sqrt(usRRHO);
-    const double SaL = Sc - std::fabs(Bn) / sqrL;
-    const double SaR = Sc + std::fabs(Bn) / sqrR;
+    double scrh, Sc, SaL, SaR;
+    if (c.isothermal) {
+      // storeFlux.hpp:139-147
+      scrh = 1.0 / (sr - sl);
+      const double rho_h = (uR[V.RHO] * duR - uL[V.RHO] * duL) * scrh;
+      Sc = (sl * uR[V.RHO] * duR - sr * uL[V.RHO] * duL) * scrh / rho_h;
+      const double sqrho_h = std::sqrt(rho_h);
+      SaL = Sc - std::fabs(Bn) / sqrho_h;
+      SaR = Sc + std::fabs(Bn) / sqrho_h;
+    } else {
+      scrh = 1.0 / (duR * uR[V.RHO] - duL * uL[V.RHO]);
+      Sc = (duR * uR[D.Xn] - duL * uL[D.Xn] - ptR2 + ptL2) * scrh;
+      const double usLRHO = uL[V.RHO] * duL / (sl - Sc);
+      const double usRRHO = uR[V.RHO] * duR / (sr - Sc);
+      const double sqrL = std::sqrt(usLRHO);
+      const double sqrR = std::sqrt(usRRHO);
+      SaL = Sc - std::fabs(Bn) / sqrL;
+      SaR = Sc + std::fabs(Bn) / sqrR;
+    }
     if ((SaL - sl) < 1e-4 * (Sc - sl)) revert_to_hll = 1;
     if ((SaR - sr) > -1e-4 * (sr - Sc)) revert_to_hll = 1;
     const double chiL = (vL[D.Xn] - Sc) * (sl - Sc) / (SaL + sl - 2.0 * Sc);
@@ -815,14 +1088,15 @@ void hll_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
   const int comps = c.comps;
   double uL[8], uR[8], fluxL[8], fluxR[8];
   const double gamma = c.gamma;
-  double gpr = gamma * vL[V.ENG];
+  const double c2 = c.isothermal ? c2iso(c) : 0.0;
+  double gpr = c.isothermal ? c2 * vL[V.RHO] : gamma * vL[V.ENG];
   double b1 = vL[D.BXn], b2 = at(vL, D.BXt), b3 = at(vL, D.BXb);
   double Btmag2 = b2 * b2 + b3 * b3;
   double Bmag2 = b1 * b1 + Btmag2;
   double cL = gpr - Bmag2;
   cL = gpr + Bmag2 + std::sqrt(cL * cL + 4.0 * gpr * Btmag2);
   cL = std::sqrt(0.5 * cL / vL[V.RHO]);
-  gpr = gamma * vR[V.ENG];
+  gpr = c.isothermal ? c2 * vR[V.RHO] : gamma * vR[V.ENG];
   b1 = vR[D.BXn];
   b2 = at(vR, D.BXt);
   b3 = at(vR, D.BXb);
@@ -838,8 +1112,8 @@ void hll_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
   cmax = FMAX(std::fabs(sl), std::fabs(sr));
   prim2cons(c, V, vL, uL);
   prim2cons(c, V, vR, uR);
-  kflux(c, V, D, fluxL, vL, uL);
-  kflux(c, V, D, fluxR, vR, uR);
+  kflux(c, V, D, fluxL, vL, uL, c2);
+  kflux(c, V, D, fluxR, vR, uR, c2);
   // density and momentum
   for (int n = V.RHO; n <= V.RHO + comps; n++) {
     if (sl > 0) flux[n] = fluxL[n];
@@ -847,7 +1121,7 @@ void hll_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
     else flux[n] = (sl * sr * uR[n] - sl * sr * uL[n] + sr * fluxL[n] - sl * fluxR[n]) / (sr - sl);
   }
   // magnetic field and energy
-  for (int n = V.ENG; n < V.nvar; n++) {
+  for (int n = c.isothermal ? V.BX1 : V.ENG; n < V.nvar; n++) {
     if (sl > 0) {
       flux[n] = fluxL[n];
     } else if (sr < 0) {
@@ -863,8 +1137,9 @@ void hll_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
   double b2R = vR[V.BX1] * vR[V.BX1];
   if (comps >= 2) b2R = b2R + vR[V.BX2] * vR[V.BX2];
   if (comps >= 3) b2R = b2R + vR[V.BX3] * vR[V.BX3];
-  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, vL[V.ENG] + 0.5 * b2L,
-                 vR[V.ENG] + 0.5 * b2R, E);
+  store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr,
+                 (c.isothermal ? c2 * vL[V.RHO] : vL[V.ENG]) + 0.5 * b2L,
+                 (c.isothermal ? c2 * vR[V.RHO] : vR[V.ENG]) + 0.5 * b2R, E);
 }
 
 // --------------------------------------------------------------------------------------
@@ -878,7 +1153,8 @@ void tvdlf_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const dou
   const int comps = c.comps;
   double v[8], uL[8], uR[8], fluxL[8], fluxR[8];
   for (int n = 0; n < nv_; n++) v[n] = 0.5 * (vL[n] + vR[n]);
-  const double gpr = c.gamma * v[V.ENG];
+  const double c2 = c.isothermal ? c2iso(c) : 0.0;
+  const double gpr = c.isothermal ? c2 * v[V.RHO] : c.gamma * v[V.ENG];
   double Bt2 = 0.0;
   if (comps >= 2) Bt2 = Bt2 + v[D.BXt] * v[D.BXt];
   if (comps >= 3) Bt2 = Bt2 + v[D.BXb] * v[D.BXb];
@@ -890,8 +1166,8 @@ void tvdlf_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const dou
   const double sl = -cmax, sr = cmax;
   prim2cons(c, V, vL, uL);
   prim2cons(c, V, vR, uR);
-  kflux(c, V, D, fluxL, vL, uL);
-  kflux(c, V, D, fluxR, vR, uR);
+  kflux(c, V, D, fluxL, vL, uL, c2);
+  kflux(c, V, D, fluxR, vR, uR, c2);
   for (int n = 0; n < nv_; n++) flux[n] = 0.5 * (fluxL[n] + fluxR[n] + cmax * (uL[n] - uR[n]));
   store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, 0.0, 0.0, E);
 }
@@ -1492,19 +1768,30 @@ struct Oracle {
           if (c.dim == 3)
             divV = divV + (vc(V.MX3, k + 1, j, i) - vc(V.MX3, k - 1, j, i)) / (2 * dx[2][k]);
           if (divV < 0.0) {
-            double pmin = vc(V.ENG, k, j, i);
-            pmin = FMIN(pmin, vc(V.ENG, k, j, i + 1));
-            pmin = FMIN(pmin, vc(V.ENG, k, j, i - 1));
-            double gradP = std::fabs(vc(V.ENG, k, j, i + 1) - vc(V.ENG, k, j, i - 1));
+            // pressure of a cell: PRS, or rho cs^2 when isothermal (shockFlattening.hpp:92-128)
+            auto P = [&](int kk, int jj, int ii) {
+              if (c.isothermal) {
+                const double cs = c.csiso;
+                return vc(V.RHO, kk, jj, ii) * cs * cs;
+              }
+              return vc(V.ENG, kk, jj, ii);
+            };
+            double pmin = P(k, j, i);
+            const double pR = P(k, j, i + 1), pL = P(k, j, i - 1);
+            pmin = FMIN(pmin, c.isothermal ? pL : pR);
+            pmin = FMIN(pmin, c.isothermal ? pR : pL);
+            double gradP = std::fabs(pR - pL);
             if (c.dim >= 2) {
-              pmin = FMIN(pmin, vc(V.ENG, k, j + 1, i));
-              pmin = FMIN(pmin, vc(V.ENG, k, j - 1, i));
-              gradP += std::fabs(vc(V.ENG, k, j + 1, i) - vc(V.ENG, k, j - 1, i));
+              const double qR = P(k, j + 1, i), qL = P(k, j - 1, i);
+              pmin = FMIN(pmin, qL);
+              pmin = FMIN(pmin, qR);
+              gradP += std::fabs(qR - qL);
             }
             if (c.dim == 3) {
-              pmin = FMIN(pmin, vc(V.ENG, k + 1, j, i));
-              pmin = FMIN(pmin, vc(V.ENG, k - 1, j, i));
-              gradP += std::fabs(vc(V.ENG, k + 1, j, i) - vc(V.ENG, k - 1, j, i));
+              const double qR = P(k + 1, j, i), qL = P(k - 1, j, i);
+              pmin = FMIN(pmin, qL);
+              pmin = FMIN(pmin, qR);
+              gradP += std::fabs(qR - qL);
             }
             if (gradP > c.shock_smoothing * pmin) f = 1;
           }
@@ -2073,6 +2360,8 @@ void *orc_new(const orc_cfg *cfg) {
   if (cfg->mhd && (cfg->solver == ORC_SOLVER_HLL || cfg->solver == ORC_SOLVER_TVDLF) &&
       cfg->emf == ORC_EMF_UCT_HLLD)
     return nullptr;
+  /* the isothermal RoeMHD branches are not restated */
+  if (cfg->isothermal && cfg->mhd && cfg->solver == ORC_SOLVER_ROE) return nullptr;
   return new Oracle(*cfg);
 }
 void orc_free(void *o) { delete static_cast<Oracle *>(o); }

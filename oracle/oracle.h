@@ -57,6 +57,8 @@ typedef struct {
   int ntracer;        /* passive tracers ([Hydro] tracer, fluid.hpp:315-325) */
   int shock_flattening;   /* [Hydro] shockFlattening given (riemannSolver.hpp:151-157) */
   double shock_smoothing; /* its value */
+  int isothermal;         /* ISOTHERMAL build: no energy variable (idefix.hpp:63-92) */
+  double csiso;           /* [Hydro] csiso constant <value> (eos_isothermal.hpp:16-27) */
 } orc_cfg;
 
 void *orc_new(const orc_cfg *cfg);

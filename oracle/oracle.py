@@ -33,7 +33,8 @@ class OrcCfg(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("cfl_max_var", ctypes.c_double), ("first_dt", ctypes.c_double),
                 ("ntracer", ctypes.c_int), ("shock_flattening", ctypes.c_int),
-                ("shock_smoothing", ctypes.c_double)]
+                ("shock_smoothing", ctypes.c_double), ("isothermal", ctypes.c_int),
+                ("csiso", ctypes.c_double)]
 
 
 def build():
@@ -86,8 +87,9 @@ def lib():
 def make_cfg(dim, comps=None, mhd=True, recon="plm", solver="hlld", emf="uct_contact",
              nstages=2, nx=(64, 1, 1), xbeg=(0.0, 0.0, 0.0), xend=(1.0, 1.0, 1.0),
              bc="periodic", gamma=5.0 / 3.0, cfl=0.6, cfl_max_var=1.1, first_dt=1e-4,
-             ntracer=0, shock_flattening=None):
-    """shock_flattening: None (off) or the smoothing value of [Hydro] shockFlattening."""
+             ntracer=0, shock_flattening=None, csiso=None):
+    """shock_flattening: None (off) or the smoothing value of [Hydro] shockFlattening;
+    csiso: None (adiabatic) or the constant isothermal sound speed."""
     c = OrcCfg()
     c.dim = dim
     c.comps = dim if comps is None else comps
@@ -115,6 +117,8 @@ def make_cfg(dim, comps=None, mhd=True, recon="plm", solver="hlld", emf="uct_con
     c.ntracer = ntracer
     c.shock_flattening = 0 if shock_flattening is None else 1
     c.shock_smoothing = 0.0 if shock_flattening is None else float(shock_flattening)
+    c.isothermal = 0 if csiso is None else 1
+    c.csiso = 0.0 if csiso is None else float(csiso)
     return c
 
 
@@ -263,6 +267,7 @@ class OracleGrid:
         self.dim = o.dim
         self.comps = o.cfg.comps
         self.mhd = bool(o.cfg.mhd)
+        self.isothermal = bool(o.cfg.isothermal)
         self.x = [o.coord(d, "x") for d in range(3)]
         self.xl = [o.coord(d, "xl") for d in range(3)]
         self.dx = [o.coord(d, "dx") for d in range(3)]

@@ -96,3 +96,72 @@ def sample(x, t, x0, left, right, gamma=1.4):
                           pR * c ** (2.0 * gamma / (gamma - 1)))
         out[:, n] = st
     return out
+
+
+# ---------------------------------------------------------------------------------------
+# Isothermal gas (p = cs^2 rho): the same construction with the isothermal wave curves
+# (e.g. LeVeque, "Finite Volume Methods for Hyperbolic Problems", sec. 14.6 and 13.7):
+# across a shock into density rho_K, u changes by -cs (rho - rho_K) / sqrt(rho rho_K);
+# across a rarefaction by -cs ln(rho / rho_K).
+# ---------------------------------------------------------------------------------------
+def _fK_iso(rho, rk, cs):
+    if rho > rk:
+        f = cs * (rho - rk) / np.sqrt(rho * rk)
+        df = cs * (0.5 / np.sqrt(rho * rk) + 0.5 * rk / (rho * np.sqrt(rho * rk)))
+    else:
+        f = cs * np.log(rho / rk)
+        df = cs / rho
+    return f, df
+
+
+def star_state_iso(rL, uL, rR, uR, cs=1.0, tol=1e-14):
+    """(rho*, u*) of the isothermal Riemann problem."""
+    rho = np.sqrt(rL * rR)
+    for _ in range(200):
+        fL, dfL = _fK_iso(rho, rL, cs)
+        fR, dfR = _fK_iso(rho, rR, cs)
+        rn = rho - (fL + fR + uR - uL) / (dfL + dfR)
+        rn = max(rn, 1e-12)
+        if abs(rn - rho) / (0.5 * (rn + rho)) < tol:
+            rho = rn
+            break
+        rho = rn
+    fL, _ = _fK_iso(rho, rL, cs)
+    fR, _ = _fK_iso(rho, rR, cs)
+    return rho, 0.5 * (uL + uR) + 0.5 * (fR - fL)
+
+
+def sample_iso(x, t, x0, left, right, cs=1.0):
+    """(rho, u) at positions x and time t; left/right = (rho, u)."""
+    rL, uL = left
+    rR, uR = right
+    rs, us = star_state_iso(rL, uL, rR, uR, cs)
+    out = np.zeros((2, len(x)))
+    for n, xi in enumerate(x):
+        s = (xi - x0) / t
+        if s <= us:  # left of the contact (no contact in isothermal flow: u* splits)
+            if rs > rL:  # left shock
+                S = uL - cs * np.sqrt(rs / rL)
+                st = (rL, uL) if s < S else (rs, us)
+            else:  # left rarefaction: head uL - cs, tail us - cs
+                if s < uL - cs:
+                    st = (rL, uL)
+                elif s > us - cs:
+                    st = (rs, us)
+                else:
+                    u = s + cs
+                    st = (rL * np.exp(-(u - uL) / cs), u)
+        else:
+            if rs > rR:  # right shock
+                S = uR + cs * np.sqrt(rs / rR)
+                st = (rR, uR) if s > S else (rs, us)
+            else:
+                if s > uR + cs:
+                    st = (rR, uR)
+                elif s < us + cs:
+                    st = (rs, us)
+                else:
+                    u = s - cs
+                    st = (rR * np.exp((u - uR) / cs), u)
+        out[:, n] = st
+    return out

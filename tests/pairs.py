@@ -11,11 +11,12 @@ RTOL = 1e-12
 
 def cfg_kwargs(dim, nx, mhd=True, recon="plm", solver="hlld", emf="uct_contact",
                bc="periodic", gamma=5.0 / 3.0, cfl=0.6, nstages=2, xbeg=(0, 0, 0),
-               xend=(1, 1, 1), first_dt=1e-4, comps=None, ntracer=0, shock_flattening=None):
+               xend=(1, 1, 1), first_dt=1e-4, comps=None, ntracer=0, shock_flattening=None,
+               csiso=None):
     return dict(dim=dim, nx=nx, mhd=mhd, recon=recon, solver=solver, emf=emf, bc=bc,
                 gamma=gamma, cfl=cfl, nstages=nstages, xbeg=xbeg, xend=xend, first_dt=first_dt,
                 comps=dim if comps is None else comps, ntracer=ntracer,
-                shock_flattening=shock_flattening)
+                shock_flattening=shock_flattening, csiso=csiso)
 
 
 def make_oracle(kw):
@@ -23,7 +24,7 @@ def make_oracle(kw):
                      emf=kw["emf"], nstages=kw["nstages"], nx=kw["nx"], xbeg=kw["xbeg"],
                      xend=kw["xend"], bc=kw["bc"], gamma=kw["gamma"], cfl=kw["cfl"],
                      first_dt=kw["first_dt"], ntracer=kw.get("ntracer", 0),
-                     shock_flattening=kw.get("shock_flattening"))
+                     shock_flattening=kw.get("shock_flattening"), csiso=kw.get("csiso"))
     return O.Oracle(cfg)
 
 
@@ -35,7 +36,7 @@ def make_pair(kw, ic, precision="fp64"):
                   recon=kw["recon"], solver=kw["solver"], emf=kw["emf"], bc=kw["bc"],
                   gamma=kw["gamma"], cfl=kw["cfl"], nstages=kw["nstages"],
                   first_dt=kw["first_dt"], precision=precision, ntracer=kw.get("ntracer", 0),
-                  shock_flattening=kw.get("shock_flattening"))
+                  shock_flattening=kw.get("shock_flattening"), csiso=kw.get("csiso"))
     og = OracleGrid(o)
     for d in range(3):
         assert np.array_equal(og.x[d], g.x[d]), "grid coordinates differ"

@@ -310,6 +310,53 @@ def test_shock_flattening_3d_and_hd2d():
     compare_state(o, g, what="HD2D flattening")
 
 
+@pytest.mark.parametrize("solver,recon,nstages", [
+    ("roe", "plm", 2), ("hll", "plm", 2), ("hllc", "plm", 2), ("tvdlf", "plm", 2),
+    ("hllc", "limo3", 2), ("roe", "ppm", 3), ("hllc", "ppm", 3)])
+def test_sod_iso_reference_matrix(solver, recon, nstages):
+    """test/HD/sod-iso (500 cells, CFL 0.8, csiso 1): idefix.ini / -hll / -hllc / -tvdlf and
+    the RK3 inputs with PPM (test/HD/sod-iso/testme.py:18-20)."""
+    kw = cfg_kwargs(1, (500,), mhd=False, solver=solver, recon=recon, bc="outflow",
+                    cfl=0.8, nstages=nstages, csiso=1.0)
+    o, g = make_pair(kw, problems.sod_iso)
+    assert g.nvar == 2
+    o.set_boundaries()
+    _cycle_both(o, g, 40)
+    compare_state(o, g, what="sod-iso %s %s" % (solver, recon))
+
+
+@pytest.mark.parametrize("solver,recon,nstages", [
+    ("hlld", "plm", 2), ("hll", "plm", 2), ("tvdlf", "plm", 2), ("hlld", "limo3", 2),
+    ("hlld", "ppm", 3)])
+def test_brio_wu_iso_reference_matrix(solver, recon, nstages):
+    """test/MHD/sod-iso (800 cells on [0, 100], CFL 0.8, csiso 1)."""
+    kw = cfg_kwargs(1, (800,), comps=2, solver=solver, recon=recon, bc="outflow", cfl=0.8,
+                    nstages=nstages, xend=(100, 1, 1), csiso=1.0)
+    o, g = make_pair(kw, problems.brio_wu_iso)
+    o.set_boundaries()
+    _cycle_both(o, g, 40)
+    compare_state(o, g, what="Brio-Wu iso %s %s" % (solver, recon))
+
+
+@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact"])
+def test_mhd2d_iso_fused(emf):
+    """Isothermal 2-D MHD through the fused 2-D kernel."""
+    kw = cfg_kwargs(2, (48, 40), emf=emf, cfl=0.6, csiso=0.8)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=31, amp=0.2))
+    o.set_boundaries()
+    _cycle_both(o, g, 6)
+    compare_state(o, g, what="2-D MHD iso %s" % emf)
+
+
+def test_hd2d_iso_with_flattening():
+    kw = cfg_kwargs(2, (40, 24), mhd=False, solver="hllc", bc=["outflow", "periodic"],
+                    cfl=0.5, csiso=1.0, shock_flattening=0.05)
+    o, g = make_pair(kw, lambda gg: problems.random_hd(gg, seed=32, amp=0.4))
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="HD2D iso flattening")
+
+
 def test_outflow_mhd2d():
     kw = cfg_kwargs(2, (32, 24), emf="uct_contact", bc=["outflow", "periodic"], cfl=0.4)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=11, amp=0.1))

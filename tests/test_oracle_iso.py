@@ -1,0 +1,134 @@
+"""Pins of the oracle's isothermal equation of state (ISOTHERMAL builds, constant csiso:
+eos_isothermal.hpp and the HAVE_ENERGY == 0 branches of the solvers): the exact
+isothermal Riemann solution checked by its jump conditions, flux consistency with the
+textbook isothermal fluxes, the isothermal Sod tube against its exact solution with every
+solver, and the isothermal Brio-Wu tube: constant normal field, conservation, and every
+solver against a 4x finer solution."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from idefix_amd import problems
+from tests.exact_riemann import sample_iso, star_state_iso
+
+rng = np.random.default_rng(7)
+CS = 1.0
+
+
+def test_exact_isothermal_solver_jump_conditions():
+    for _ in range(200):
+        rL, rR = rng.uniform(0.1, 3.0, 2)
+        uL, uR = rng.normal(size=2) * 0.5
+        rs, us = star_state_iso(rL, uL, rR, uR, CS)
+        for rk, uk, side in ((rL, uL, -1), (rR, uR, 1)):
+            if rs > rk:  # shock: mass and momentum fluxes continuous in its frame
+                S = uk + side * CS * np.sqrt(rs / rk)
+                assert S * (rs - rk) == pytest.approx(rs * us - rk * uk, rel=1e-9, abs=1e-12)
+                lhs = S * (rs * us - rk * uk)
+                rhs = (rs * us * us + CS ** 2 * rs) - (rk * uk * uk + CS ** 2 * rk)
+                assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-12)
+            else:  # rarefaction: Riemann invariant u -+ cs ln rho conserved
+                assert us - side * CS * np.log(rs) == pytest.approx(
+                    uk - side * CS * np.log(rk), rel=1e-12, abs=1e-12)
+
+
+def _iso_flux_hd(v, d):
+    rho, vel = v[0], v[1:4]
+    F = np.zeros(4)
+    F[0] = rho * vel[d]
+    F[1:4] = rho * vel * vel[d]
+    F[1 + d] += CS ** 2 * rho
+    return F
+
+
+def _iso_flux_mhd(v, d):
+    rho, vel, B = v[0], v[1:4], v[4:7]
+    F = np.zeros(7)
+    F[0] = rho * vel[d]
+    F[1:4] = rho * vel * vel[d] - B * B[d]
+    F[1 + d] += CS ** 2 * rho + 0.5 * B @ B
+    F[4:7] = vel[d] * B - B[d] * vel
+    return F
+
+
+@pytest.mark.parametrize("solver", ["hllc", "hll", "tvdlf", "roe"])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_iso_hd_flux_consistency(solver, d):
+    cfg = O.make_cfg(3, mhd=False, solver=solver, nx=(8, 8, 8), comps=3, csiso=CS)
+    for _ in range(100):
+        v = np.zeros(8)
+        v[0] = rng.uniform(0.2, 3)
+        v[1:4] = rng.normal(size=3)
+        F, cmax, _, wrote = O.face_flux(cfg, d, v, v)
+        assert wrote
+        assert np.allclose(F[:4], _iso_flux_hd(v, d), rtol=1e-12, atol=1e-12)
+        if solver != "tvdlf":
+            assert cmax == pytest.approx(abs(v[1 + d]) + CS, rel=1e-12)
+
+
+@pytest.mark.parametrize("solver,emf", [("hlld", "uct_contact"), ("hlld", "uct_hlld"),
+                                        ("hll", "uct_hll"), ("tvdlf", "uct_contact")])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_iso_mhd_flux_consistency(solver, emf, d):
+    cfg = O.make_cfg(3, mhd=True, solver=solver, emf=emf, nx=(8, 8, 8), comps=3, csiso=CS)
+    for _ in range(100):
+        v = np.zeros(8)
+        v[0] = rng.uniform(0.2, 3)
+        v[1:4] = rng.normal(size=3)
+        v[4:7] = rng.normal(size=3)
+        F, _, _, _ = O.face_flux(cfg, d, v, v)
+        assert np.allclose(F[:7], _iso_flux_mhd(v, d), rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("solver,recon", [("roe", "plm"), ("hll", "plm"), ("hllc", "plm"),
+                                          ("tvdlf", "plm"), ("hllc", "limo3")])
+def test_iso_sod_against_exact_solution(solver, recon):
+    """test/HD/sod-iso (500 cells, CFL 0.8, t = 0.2, csiso 1)."""
+    cfg = O.make_cfg(1, mhd=False, recon=recon, solver=solver, nx=(500,), bc="outflow",
+                     cfl=0.8, csiso=CS)
+    o = O.Oracle(cfg)
+    assert o.nvar == 2
+    Vc, _ = problems.sod_iso(O.OracleGrid(o))
+    o.Vc[...] = Vc
+    o.set_boundaries()
+    assert o.run(0.2) == 0
+    x = o.coord(0, "x")[o.beg[0]:o.end[0]]
+    v = o.interior(o.Vc)[:, 0, 0]
+    ex = sample_iso(x, 0.2, 0.5, (1.0, 0.0), (0.125, 0.0), CS)
+    assert np.mean(np.abs(v[0] - ex[0])) < 3e-3
+    assert np.mean(np.abs(v[1] - ex[1])) < 6e-3
+
+
+def _bw_iso(solver, n=800, recon="plm", nstages=2, bc="outflow"):
+    cfg = O.make_cfg(1, comps=2, mhd=True, recon=recon, solver=solver, emf="uct_contact",
+                     nx=(n,), xend=(100, 1, 1), bc=bc, cfl=0.8, nstages=nstages, csiso=CS)
+    o = O.Oracle(cfg)
+    Vc, Vs = problems.brio_wu_iso(O.OracleGrid(o))
+    o.Vc[...] = Vc
+    o.Vs[...] = Vs
+    o.set_boundaries()
+    return o
+
+
+def test_iso_brio_wu_solvers_agree_with_fine_solution():
+    fine = _bw_iso("hlld", n=3200)
+    assert fine.run(10.0) == 0
+    ref = fine.interior(fine.Vc)[:, 0, 0].reshape(5, 800, 4).mean(axis=2)
+    for solver, recon in [("hlld", "plm"), ("hll", "plm"), ("tvdlf", "plm"), ("hlld", "limo3")]:
+        o = _bw_iso(solver, recon=recon)
+        assert o.run(10.0) == 0
+        v = o.interior(o.Vc)[:, 0, 0]
+        err = np.mean(np.abs(v - ref), axis=1)
+        assert err[0] < 4e-3 and err[4] < 6e-3, (solver, recon, err)
+        assert np.all(v[3] == 0.75)
+
+
+def test_iso_brio_wu_periodic_conservation():
+    o = _bw_iso("hlld", n=400, bc="periodic")
+    o.prim_to_cons()
+    t0 = o.interior(o.Uc).sum(axis=(1, 2, 3)).copy()
+    for _ in range(60):
+        assert o.cycle() == 0
+    o.prim_to_cons()
+    t1 = o.interior(o.Uc).sum(axis=(1, 2, 3))
+    assert np.all(np.abs(t1 - t0) <= 1e-12 * np.maximum(np.abs(t0), 400.0))
