@@ -39,7 +39,7 @@ __device__ __forceinline__ T lane_at(T v, int lane, int off) {
   return __shfl(v, lane + off);
 }
 
-template <class C, typename T, int EMF>
+template <class C, typename T, int EMF, bool XTRA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGeom m) {
   static_assert(C::DIM == 2 && C::MHD, "2-D MHD only");
@@ -112,7 +112,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       if (n == BX2) continue;
       T dummy;
       recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, 1, c0 - 1),
-                    recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
+                    XTRA && recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
     }
     if constexpr (CARRY) {
       if constexpr (HLLE) {
@@ -149,7 +149,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         }
       }
       T vL[NV], vR[NV], vr_next[NV];
-      const bool flat = recon_flat<C>(a, col + (unsigned)f * sd);
+      const bool flat = XTRA && recon_flat<C>(a, col + (unsigned)f * sd);
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         vL[n] = carry[n];

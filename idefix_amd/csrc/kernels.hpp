@@ -310,8 +310,10 @@ __device__ __forceinline__ void tracer_update(const GridDesc &g, const SweepArgs
 }
 
 // Update of one cell by the flux difference along DIR (calcRightHandSide.hpp:288-549,
-// Cartesian, with the area correction of :156-194 applied to both faces first).
-template <class C, int DIR, typename T>
+// Cartesian, with the area correction of :156-194 applied to both faces first).  XTRA
+// compiles in the optional per-cell work (passive tracers); without it the kernels carry
+// no trace of the options.
+template <class C, int DIR, bool XTRA, typename T>
 __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs &a, int k,
                                               int j, int i, const T *Fm, const T *Fp,
                                               T cmaxm, T cmaxp) {
@@ -348,7 +350,9 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     const double rhs = -dtdV * (fp - fm);
     st32(a.Uc + n * g.sv, p, u[n] + rhs);
   }
-  if (g.ntr > 0) tracer_update<C, DIR>(g, a, p, dtdV, A, (double)Fm[C::RHO], (double)Fp[C::RHO]);
+  if constexpr (XTRA) {
+    if (g.ntr > 0) tracer_update<C, DIR>(g, a, p, dtdV, A, (double)Fm[C::RHO], (double)Fp[C::RHO]);
+  }
   const int ig = DIR == 0 ? i : DIR == 1 ? j : k;
   const double dl = g.dx[DIR][ig];
   const double term = 0.5 * ((double)cmaxp + (double)cmaxm) / (dl);
@@ -374,7 +378,7 @@ template <class C>
 constexpr int x_waves_per_eu() {
   return C::DIM == 3 ? 3 : 4;
 }
-template <class C, typename T, bool FUSED, int EMF>
+template <class C, typename T, bool FUSED, int EMF, bool XTRA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(x_waves_per_eu<C>())))
 void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                  int nseg) {
@@ -398,7 +402,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
 
     // reconstruct the lane's cell: vl = right state of face c, vr = left state of face c+1
     T vl[NV], vr[NV];
-    const bool flat = recon_flat<C>(a, row + (unsigned)cc);
+    const bool flat = XTRA && recon_flat<C>(a, row + (unsigned)cc);
 #pragma unroll
     for (int n = 0; n < NV; n++) {
       T wv[NW];
@@ -444,7 +448,7 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
       const bool cell_ok = lane >= 1 && lane <= 62 && c < g.end[0] && j >= g.beg[1] &&
                            j < g.end[1] && k >= g.beg[2] && k < g.end[2];
       if (cell_ok) {
-        const double inv = cell_update<C, IDIR>(g, a, k, j, c, F, Fp, cmax, cmaxp);
+        const double inv = cell_update<C, IDIR, XTRA>(g, a, k, j, c, F, Fp, cmax, cmaxp);
         red = __double_as_longlong(inv);
       }
     }
@@ -477,7 +481,7 @@ template <class C>
 constexpr int march_waves_per_eu() {
   return (march_lds_window<C>() && C::DIM == 2) ? 3 : 2;
 }
-template <class C, typename T, int DIR, bool FUSED, int EMF>
+template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(march_waves_per_eu<C>())))
 void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                      MarchGeom m) {
@@ -541,7 +545,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         recon_mlp<C, DIR>(wv, n, a.mlpw, g.sv, col + (unsigned)(c0 - 1) * sd, sd, dummy, carry[n]);
       } else {
         recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, DIR, c0 - 1),
-                      recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
+                      XTRA && recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
       }
     }
     for (int f = c0; f <= c1; f++) {
@@ -564,7 +568,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
           win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
       }
       T vL[NV], vR[NV], vr_next[NV];
-      const bool flat = recon_flat<C>(a, col + (unsigned)f * sd);
+      const bool flat = XTRA && recon_flat<C>(a, col + (unsigned)f * sd);
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         T wv[NW];
@@ -604,7 +608,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (f > c0 && rhs_lane) {
           const int cf = f - 1;
           const int ci = ic, cj = (DIR == JDIR) ? cf : jj, ck = (DIR == JDIR) ? kk : cf;
-          const double inv = cell_update<C, DIR>(g, a, ck, cj, ci, Fprev, F, cprev, cmax);
+          const double inv = cell_update<C, DIR, XTRA>(g, a, ck, cj, ci, Fprev, F, cprev, cmax);
           const unsigned long long b = __double_as_longlong(inv);
           red = b > red ? b : red;
         }

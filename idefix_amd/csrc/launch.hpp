@@ -83,8 +83,8 @@ inline int march_segment(long long columns, int nd, long long slots) {
   return bt;
 }
 
-template <class C, typename T, int DIR, bool FUSED, int EMF>
-hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
+template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA>
+hipError_t launch_sweep_xtra(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
                             int target_waves, hipStream_t s) {
   if constexpr (DIR == IDIR) {
     const int nint = g.end[0] - g.beg[0];
@@ -92,7 +92,7 @@ hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArra
     const long long nj = g.end[1] - g.beg[1] + 2 * a.ext[1];
     const long long nk = g.end[2] - g.beg[2] + 2 * a.ext[2];
     const long long waves = (long long)nseg * nj * nk;
-    hipLaunchKernelGGL((k_sweep_x<C, T, FUSED, EMF>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g,
+    hipLaunchKernelGGL((k_sweep_x<C, T, FUSED, EMF, XTRA>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g,
                        a, e, nseg);
   } else {
     MarchGeom m;
@@ -105,14 +105,24 @@ hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArra
     const int nd = g.end[DIR] - g.beg[DIR];
     const long long columns = (long long)m.nxb * (m.thi - m.tlo);
     static const long long slots =
-        resident_waves(k_sweep_march<C, T, DIR, FUSED, EMF>, target_waves);
+        resident_waves(k_sweep_march<C, T, DIR, FUSED, EMF, XTRA>, target_waves);
     m.ts = march_segment(columns, nd, slots);
     m.nseg = (nd + m.ts - 1) / m.ts;
     const long long waves = columns * m.nseg;
-    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED, EMF>), dim3(blocks_for(waves, 4)), dim3(256),
+    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED, EMF, XTRA>), dim3(blocks_for(waves, 4)), dim3(256),
                        0, s, g, a, e, m);
   }
   return hipGetLastError();
+}
+
+// The options that add per-cell work (shock flags, passive tracers) select the XTRA
+// instantiation; runs without them use kernels compiled without that code.
+template <class C, typename T, int DIR, bool FUSED, int EMF>
+hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
+                            int target_waves, hipStream_t s) {
+  if (a.shock != nullptr || g.ntr > 0)
+    return launch_sweep_xtra<C, T, DIR, FUSED, EMF, true>(g, a, e, target_waves, s);
+  return launch_sweep_xtra<C, T, DIR, FUSED, EMF, false>(g, a, e, target_waves, s);
 }
 
 // One direction of the sweep with the corner-EMF scheme as a compile-time parameter (the
@@ -338,12 +348,21 @@ hipError_t launch_fused2d_y_emf(const GridDesc &g, const SweepArgs &a, const voi
   const int ncols = g.end[0] + 1 - g.beg[0];  // face columns beg .. end
   m.nxb = (ncols + f.w - 1) / f.w;
   const int nd = g.end[1] - g.beg[1];
-  static const long long slots = resident_waves(k_fused2d_y<C, T, EMF>, target_waves);
-  m.ts = march_segment(m.nxb, nd, slots);
-  m.nseg = (nd + m.ts - 1) / m.ts;
-  const long long waves = (long long)m.nxb * m.nseg;
-  hipLaunchKernelGGL((k_fused2d_y<C, T, EMF>), dim3(blocks_for(waves, 4)), dim3(256), 0, s, g, a,
-                     *static_cast<const EmfArrays<T> *>(emf), f, m);
+  if (a.shock != nullptr) {
+    static const long long slots = resident_waves(k_fused2d_y<C, T, EMF, true>, target_waves);
+    m.ts = march_segment(m.nxb, nd, slots);
+    m.nseg = (nd + m.ts - 1) / m.ts;
+    const long long waves = (long long)m.nxb * m.nseg;
+    hipLaunchKernelGGL((k_fused2d_y<C, T, EMF, true>), dim3(blocks_for(waves, 4)), dim3(256), 0,
+                       s, g, a, *static_cast<const EmfArrays<T> *>(emf), f, m);
+  } else {
+    static const long long slots = resident_waves(k_fused2d_y<C, T, EMF, false>, target_waves);
+    m.ts = march_segment(m.nxb, nd, slots);
+    m.nseg = (nd + m.ts - 1) / m.ts;
+    const long long waves = (long long)m.nxb * m.nseg;
+    hipLaunchKernelGGL((k_fused2d_y<C, T, EMF, false>), dim3(blocks_for(waves, 4)), dim3(256), 0,
+                       s, g, a, *static_cast<const EmfArrays<T> *>(emf), f, m);
+  }
   return hipGetLastError();
 }
 
