@@ -177,26 +177,27 @@ __device__ __forceinline__ void recon_cell(const T *w, int nv, T &vl, T &vr, T d
     const T dvm = w[1] - w[0];
     const T dvp = w[2] - w[1];
     T dv = flat ? T(0) : dvp * limo3_lim(dvp, dvm, dxc);
-    vr = w[1] + T(0.5) * dv;
+    T r = w[1] + T(0.5) * dv;
     dv = flat ? T(0) : dvm * limo3_lim(dvm, dvp, dxc);
-    vl = w[1] - T(0.5) * dv;
+    T l = w[1] - T(0.5) * dv;
     if (nv == C::RHO || nv == C::ENG) {
-      if (vr <= T(0)) vr = w[1] + T(0.5) * T(0);
-      if (vl <= T(0)) vl = w[1] - T(0.5) * T(0);
+      r = (r <= T(0)) ? w[1] + T(0.5) * T(0) : r;
+      l = (l <= T(0)) ? w[1] - T(0.5) * T(0) : l;
     }
+    vl = l;
+    vr = r;
   } else {
     // w[0..4] = V(c-2) .. V(c+2)
-    ppm_states(w[0], w[1], w[2], w[3], w[4], vl, vr);
+    T l, r;
+    ppm_states(w[0], w[1], w[2], w[3], w[4], l, r);
     if (nv == C::RHO || nv == C::ENG) {
-      if (vr <= T(0)) {
-        const T dv = vanleer(w[3] - w[2], w[2] - w[1]);
-        vr = w[2] + T(0.5) * dv;
-      }
-      if (vl <= T(0)) {
-        const T dv = vanleer(w[3] - w[2], w[2] - w[1]);
-        vl = w[2] - T(0.5) * dv;
-      }
+      // van Leer fallback of extrapolateToFaces.hpp:260-304 (value selects, see ppm_states)
+      const T dv = vanleer(w[3] - w[2], w[2] - w[1]);
+      r = (r <= T(0)) ? w[2] + T(0.5) * dv : r;
+      l = (l <= T(0)) ? w[2] - T(0.5) * dv : l;
     }
+    vl = l;
+    vr = r;
   }
 }
 template <class C>

@@ -104,8 +104,11 @@ __device__ __forceinline__ T limo3_lim(T dvp, T dvm, T dx) {
 template <typename T>
 __device__ __forceinline__ int sgn(T v) { return (T(0) < v) - (v < T(0)); }
 
+// The PPM helpers work on values, not references: conditional stores through reference
+// parameters let the compiler merge them into one store through a selected address,
+// which keeps the caller's state arrays in scratch memory.
 template <typename T>
-__device__ __forceinline__ void limit_ppm_face(T vm1, T v0, T vp1, T vp2, T &vph) {
+__device__ __forceinline__ T limit_ppm_face(T vm1, T v0, T vp1, T vp2, T vph) {
   // limitPPMFaceValues, slopeLimiter.hpp:121-140
   if ((vp1 - vph) * (vph - v0) < T(0)) {
     const T deltaL = (vm1 - 2 * v0 + vp1);
@@ -117,17 +120,19 @@ __device__ __forceinline__ void limit_ppm_face(T vm1, T v0, T vp1, T vp2, T &vph
       delta = Cc * fmn(fab(deltaL), fab(deltaR));
       delta = sgn(deltaC) * fmn(delta, fab(deltaC));
     }
-    vph = T(0.5) * (v0 + vp1) - delta / T(6.0);
+    return T(0.5) * (v0 + vp1) - delta / T(6.0);
   }
+  return vph;
 }
 template <typename T>
-__device__ __forceinline__ void ppm_states(T vm2, T vm1, T v0, T vp1, T vp2, T &vl, T &vr) {
+__device__ __forceinline__ void ppm_states(T vm2, T vm1, T v0, T vp1, T vp2, T &vl_out,
+                                           T &vr_out) {
   // getPPMStates, slopeLimiter.hpp:142-191
   const int n = 2;
-  vr = T(7.0) / T(12.0) * (v0 + vp1) - T(1.0) / T(12.0) * (vm1 + vp2);
-  vl = T(7.0) / T(12.0) * (vm1 + v0) - T(1.0) / T(12.0) * (vm2 + vp1);
-  limit_ppm_face(vm2, vm1, v0, vp1, vl);
-  limit_ppm_face(vm1, v0, vp1, vp2, vr);
+  T vr = T(7.0) / T(12.0) * (v0 + vp1) - T(1.0) / T(12.0) * (vm1 + vp2);
+  T vl = T(7.0) / T(12.0) * (vm1 + v0) - T(1.0) / T(12.0) * (vm2 + vp1);
+  vl = limit_ppm_face(vm2, vm1, v0, vp1, vl);
+  vr = limit_ppm_face(vm1, v0, vp1, vp2, vr);
   T d2qf = T(6.0) * (vl + vr - T(2.0) * v0);
   T d2qc0 = vm1 + vp1 - T(2.0) * v0;
   T d2qcp1 = v0 + vp2 - T(2.0) * vp1;
@@ -142,15 +147,19 @@ __device__ __forceinline__ void ppm_states(T vm2, T vm1, T v0, T vp1, T vp2, T &
   T qmax = fmx(fmx(fab(vm1), fab(v0)), fab(vp1));
   T rho = 0;
   if (fab(d2qf) > T(1e-12) * qmax) rho = d2q / d2qf;
+  T l = vl, r = vr;
   if (((vr - v0) * (v0 - vl) <= T(0)) || (vm1 - v0) * (v0 - vp1) <= T(0)) {
     if (rho <= (T(1.0) - T(1e-12))) {
-      vl = v0 - rho * (v0 - vl);
-      vr = v0 + rho * (vr - v0);
+      l = v0 - rho * (v0 - vl);
+      r = v0 + rho * (vr - v0);
     }
   } else {
-    if (fab(vr - v0) >= n * fab(v0 - vl)) vr = v0 + n * (v0 - vl);
-    if (fab(vl - v0) >= n * fab(v0 - vr)) vl = v0 + n * (v0 - vr);
+    // sequential as in the reference: the second test sees the updated right state
+    if (fab(vr - v0) >= n * fab(v0 - vl)) r = v0 + n * (v0 - vl);
+    if (fab(vl - v0) >= n * fab(v0 - r)) l = v0 + n * (v0 - r);
   }
+  vl_out = l;
+  vr_out = r;
 }
 
 // ------------------------------------------------------------------------------------
