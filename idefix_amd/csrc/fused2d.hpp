@@ -39,8 +39,14 @@ __device__ __forceinline__ T lane_at(T v, int lane, int off) {
   return __shfl(v, lane + off);
 }
 
+// 3 waves per SIMD for the 6-variable state; the 8-variable one (three vector components)
+// needs about 220 VGPRs and runs at 2
+template <class C>
+constexpr int fused2d_waves_per_eu() {
+  return C::NVAR >= 8 ? 2 : 3;
+}
 template <class C, typename T, int EMF, bool XTRA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused2d_waves_per_eu<C>())))
 void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGeom m) {
   static_assert(C::DIM == 2 && C::MHD, "2-D MHD only");
   constexpr int NV = C::NVAR;

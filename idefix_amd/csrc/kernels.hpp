@@ -374,7 +374,8 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
 // lane itself, and cell c takes the flux of face c+1 from lane l+1.  A wave therefore
 // solves 63 faces and updates the 62 cells between them.
 // ------------------------------------------------------------------------------------
-// occupancy target of the x sweep: 4 waves per SIMD (<= 128 VGPRs) in 2-D, 3 in 3-D
+// occupancy target of the x sweep: 4 waves per SIMD (<= 128 VGPRs) in 2-D, 3 in 3-D (the
+// 8-variable 2-D MHD state spills a little at 128 but still runs faster than at 3 waves)
 template <class C>
 constexpr int x_waves_per_eu() {
   return C::DIM == 3 ? 3 : 4;
@@ -476,11 +477,11 @@ template <class C>
 constexpr bool march_lds_window() {
   return window<C>() == 3;
 }
-// 3 waves per SIMD fit the 2-D march without spills; the 3-D one would spill (measured
-// slower), so it stays at 2
+// 3 waves per SIMD fit the 2-D march of the 6-variable MHD state without spills; the 3-D
+// one and the 8-variable 2-D one would spill, so they stay at 2
 template <class C>
 constexpr int march_waves_per_eu() {
-  return (march_lds_window<C>() && C::DIM == 2) ? 3 : 2;
+  return (march_lds_window<C>() && C::DIM == 2 && C::NVAR < 8) ? 3 : 2;
 }
 template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(march_waves_per_eu<C>())))
