@@ -43,6 +43,11 @@ def parse():
     p.add_argument("--matrix", action="store_true",
                    help="only time the Orszag-Tang workload across solvers, reconstructions, "
                         "EMF averages and tracers (one JSON line per configuration)")
+    p.add_argument("--configs", action="store_true",
+                   help="only time every configured workload at its full size (one JSON "
+                        "line per workload and precision)")
+    p.add_argument("--only", default=None,
+                   help="with --configs: comma-separated workload names to run")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 1 GiB device copy 10 times (counter calibration)")
     return p.parse_args()
@@ -97,6 +102,91 @@ def matrix(nx, steps, warmup):
                           "ms_per_step": 1e3 * el / steps}), flush=True)
         del g
         torch.cuda.empty_cache()
+
+
+# The configured workloads at their full sizes.  S = state words per cell (conserved
+# hydro words + face-held B components), k = state-sized HBM transfers per cycle of the
+# two-register RK scheme (RK2 5, RK3 7): B_zu = k * S * 8 bytes is the compulsory HBM
+# traffic of one zone-update.
+WORKLOADS = [
+    dict(name="sod1000", label="1-D Sod 1000, PLM, HLLC, RK2, CFL 0.8, outflow",
+         dim=1, shape=(1000,), mhd=False, recon="plm", solver="hllc", nstages=2, cfl=0.8,
+         gamma=1.4, bc="outflow", ic="sod", S=3, k=5, modes=("fp64",)),
+    dict(name="ot512", label="2-D Orszag-Tang 512^2, PLM, HLLD, UCT-HLLD, RK2, CFL 0.6",
+         dim=2, shape=(512, 512), comps=2, recon="plm", emf="uct_hlld", nstages=2, cfl=0.6,
+         ic="orszag_tang", S=6, k=5, modes=("fp64", "fp32")),
+    dict(name="ot2048", label="2-D Orszag-Tang 2048^2, PLM, HLLD, UCT-HLLD, RK2, CFL 0.6",
+         dim=2, shape=(2048, 2048), comps=2, recon="plm", emf="uct_hlld", nstages=2,
+         cfl=0.6, ic="orszag_tang", S=6, k=5, modes=("fp64", "fp32")),
+    dict(name="turb3d256", label="3-D decaying turbulence 256^3, PPM, HLLD, CT, RK3, CFL 0.4",
+         dim=3, shape=(256, 256, 256), comps=3, recon="ppm", emf="uct_contact", nstages=3,
+         cfl=0.4, ic="turbulence", S=8, k=7, modes=("fp64", "fp32")),
+    dict(name="turb3d512", label="3-D decaying turbulence 512^3, PPM, HLLD, CT, RK3, CFL 0.4",
+         dim=3, shape=(512, 512, 512), comps=3, recon="ppm", emf="uct_contact", nstages=3,
+         cfl=0.4, ic="turbulence", S=8, k=7, modes=("fp64", "fp32")),
+    dict(name="turb2d256", label="2-D decaying turbulence 256^2 (3 components), PPM, HLLD, "
+         "CT, RK3, CFL 0.4", dim=2, shape=(256, 256), comps=3, recon="ppm",
+         emf="uct_contact", nstages=3, cfl=0.4, ic="turbulence", S=7, k=7,
+         modes=("fp64", "fp32", "fp32+learned")),
+    dict(name="turb2d2048", label="2-D decaying turbulence 2048^2 (3 components), PPM, HLLD, "
+         "CT, RK3, CFL 0.4", dim=2, shape=(2048, 2048), comps=3, recon="ppm",
+         emf="uct_contact", nstages=3, cfl=0.4, ic="turbulence", S=7, k=7,
+         modes=("fp64", "fp32", "fp32+learned")),
+    dict(name="kh512", label="3-D MHD Kelvin-Helmholtz 512x512x256 on [0,1]^2 x [0,1/2], PPM, "
+         "HLLD, CT, RK2, CFL 0.4", dim=3, shape=(512, 512, 256), xend=(1.0, 1.0, 0.5),
+         comps=3, recon="ppm", emf="uct_contact", nstages=2, cfl=0.4, ic="kelvin_helmholtz",
+         S=8, k=5, modes=("fp64", "fp32", "fp32+learned")),
+]
+
+
+def workloads(only, steps, warmup):
+    """Throughput of each configured workload at its full size: `warmup` untimed cycles,
+    then `steps` timed cycles from the workload's initial state.  The learned mode uses
+    the network trained on the 2-D turbulence (idefix_amd/learned_turb2d.npy)."""
+    import numpy as np
+    import torch
+    from idefix_amd.datablock import DataBlock
+    from idefix_amd import problems
+    names = set(only.split(",")) if only else None
+    for w in WORKLOADS:
+        if names is not None and w["name"] not in names:
+            continue
+        cells = int(np.prod(w["shape"]))
+        print("setting up %s" % w["name"], file=sys.stderr, flush=True)
+        ic = None
+        for mode in w["modes"]:
+            precision = "fp32" if mode.startswith("fp32") else "fp64"
+            recon = "mlp" if mode.endswith("learned") else w["recon"]
+            kw = dict(mhd=w.get("mhd", True), recon=recon, solver=w["solver"] if "solver" in w
+                      else "hlld", precision=precision, bc=w.get("bc", "periodic"),
+                      gamma=w.get("gamma", 5.0 / 3.0), nstages=w["nstages"], cfl=w["cfl"],
+                      first_dt=1e-4)
+            if "comps" in w:
+                kw["comps"] = w["comps"]
+            if "emf" in w:
+                kw["emf"] = w["emf"]
+            if "xend" in w:
+                kw["xend"] = w["xend"]
+            t0 = time.perf_counter()
+            g = DataBlock(w["dim"], w["shape"], **kw)
+            if ic is None:
+                ic = getattr(problems, w["ic"])(g)
+            g.upload(*ic)
+            if recon == "mlp":
+                g.set_mlp_params(np.load(os.path.join(ROOT, "idefix_amd", "learned_turb2d.npy")))
+            setup_s = time.perf_counter() - t0
+            el = time_cycles(g, steps, warmup)
+            zu = cells * steps / el
+            bzu = w["k"] * w["S"] * 8
+            print(json.dumps({"workload": w["name"], "label": w["label"], "mode": mode,
+                              "cells": cells, "steps": steps, "warmup": warmup,
+                              "zone_updates_per_s": zu, "ms_per_step": 1e3 * el / steps,
+                              "bytes_per_zone_update": bzu,
+                              "cycle_hbm_frac": bzu * zu / (HBM_PEAK_GBS * 1e9),
+                              "setup_s": round(setup_s, 1)}), flush=True)
+            del g
+            torch.cuda.empty_cache()
+        del ic
 
 
 def time_cycles(g, steps, warmup):
@@ -220,6 +310,9 @@ def main():
         return
     if args.matrix:
         matrix(args.nx, args.steps, args.warmup)
+        return
+    if args.configs:
+        workloads(args.only, args.steps, args.warmup)
         return
     nx = args.nx
     g = make_block(nx, "fp64")

@@ -19,7 +19,8 @@
 // weights as the A operand (rows = output units) and the activations as the B operand
 // (columns = interfaces).  The 16x16 accumulator tile a lane holds (rows 4(l/16)+t,
 // column l%16) is exactly the B-operand fragment of the next layer's K-step, so the
-// three layers chain in registers; only the 48 input features pass through LDS.
+// three layers chain in registers; the lane-to-variable mapping of k_mlp_infer makes the
+// input features and the outputs register-resident too (no LDS).
 #pragma once
 
 #include "kernels.hpp"
@@ -38,14 +39,14 @@ struct MlpNet {
   const short *w2t;  // [64][64]
   const short *w3t;  // [16][64]
   const float *b1, *b2, *b3;
-  int valid;         // 0: no network, every interface takes the van Leer weights
+  int valid;         // 0: no network, every interface takes the PPM weights
 };
 
 __device__ __forceinline__ short mlp_bf16(float f) {
   return __builtin_bit_cast(short, (__bf16)f);  // round to nearest even
 }
 __device__ __forceinline__ float mlp_sigmoid(float z) {
-  return 1.0f / (1.0f + __expf(-z));
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));  // v_rcp_f32: 1 ulp
 }
 __device__ __forceinline__ float mlp_gelu(float z) { return z * mlp_sigmoid(MLP_GELU_K * z); }
 
@@ -82,63 +83,91 @@ __device__ __forceinline__ void mlp_ppm(const float *s, bool positive, float &wl
   wr = jump != 0.0f ? (s[3] - right) / jump : 0.0f;
 }
 
+// Features of one variable at one interface from its stencil S[0..5] = V[i-3..i+2]:
+// x_j = (S_j - c) / s with c = (S[2] + S[3]) / 2, s = max |S_j - c| (x = 0 when s = 0).
+// Returns false for non-finite input.
+__device__ __forceinline__ bool mlp_features6(const float *S, float *x) {
+  const float c = 0.5f * (S[2] + S[3]);
+  float sc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < 6; s++) sc = fmaxf(sc, fabsf(S[s] - c));
+  const float inv = sc > 0.0f ? 1.0f / sc : 0.0f;
+  bool fin = isfinite(inv);
+#pragma unroll
+  for (int s = 0; s < 6; s++) fin = fin && isfinite(S[s]);
+#pragma unroll
+  for (int s = 0; s < 6; s++) x[s] = (S[s] - c) * inv;
+  return fin;
+}
+
+// TVD-region check of one variable's weights (0 <= w <= 1, each state's deviation from
+// its cell value has the sign of the cell's other one-sided difference and does not
+// exceed it); the PPM weights replace them where it fails or when use_net is false.
+__device__ __forceinline__ void mlp_check(const float *st, bool use_net, bool positive,
+                                          float &wl, float &wr) {
+  bool good = use_net;
+  if (good) {
+    const float a = st[1], b = st[2], c = st[3], d = st[4];
+    const float jump = c - b;
+    const float dl = wl * jump, dr = wr * jump;
+    const bool okl = wl >= 0.0f && wl <= 1.0f &&
+                     (dl == 0.0f || (dl * (b - a) > 0.0f && fabsf(dl) <= fabsf(b - a)));
+    const bool okr = wr >= 0.0f && wr <= 1.0f &&
+                     (dr == 0.0f || (dr * (d - c) > 0.0f && fabsf(dr) <= fabsf(d - c)));
+    good = (okl && okr) || (jump == 0.0f && wl == wl && wr == wr);  // NaN fails
+  }
+  if (!good) mlp_ppm(st, positive, wl, wr);
+}
+
 // Interface weights W[q][face] (SoA, face index = index of the cell right of the face)
-// for the faces [lo, hi) of direction DIR.  A wave handles 64 faces consecutive in i.
+// for the faces [lo, hi) of direction DIR.  A wave handles 64 faces consecutive in i, in
+// four tiles of 16.  Lane l = 16 g + r works on face 16 nt + r of tile nt and on the
+// face-frame variables 2g and 2g+1: it loads their two stencils, builds their 12
+// features and, after the three layers, holds exactly their four outputs (wL, wR of
+// each; the accumulator rows 4g..4g+3), so features enter the MFMA and outputs leave it
+// without LDS, and the range check runs on the stencils already in registers.  The K
+// order of layer 1 is permuted to match: K index 16 ks + 4 g + t carries feature
+// 12 g + 4 ks + t (variable 2g + (4ks+t)/6, cell (4ks+t)%6).
 template <class C, int DIR>
 __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc, MlpNet net,
                                                    float *W, int lo0, int lo1, int lo2,
                                                    int n0, int n1, int n2) {
   static_assert(C::MHD && C::COMP == 3, "the learned reconstruction uses 8 primitives");
-  constexpr int LDS_STRIDE = 52;  // bf16 per feature row (104 B: 8-byte aligned)
-  __shared__ short feat[4][64 * LDS_STRIDE];
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int g4 = lane >> 4, rr = lane & 15;
   const int nxb = (n0 + 63) / 64;
-  const long long w = (long long)blockIdx.x * 4 + wv;
-  const bool wave_ok = w < (long long)nxb * n1 * n2;
+  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= (long long)nxb * n1 * n2) return;  // wave-uniform: the MFMAs need whole waves
   const int xb = (int)(w % nxb);
   const long long r = w / nxb;
   const int j = lo1 + (int)(r % n1);
   const int k = lo2 + (int)(r / n1);
-  const int i = lo0 + xb * 64 + lane;
-  const bool face_ok = wave_ok && i < lo0 + n0;
-  const int ic = face_ok ? i : lo0;
-  const int jc = wave_ok ? j : lo1;
-  const int kc = wave_ok ? k : lo2;
   const long long sd = DIR == 0 ? 1 : DIR == 1 ? (long long)g.nt[0] : (long long)g.nt[0] * g.nt[1];
-  const long long pf = idx3(g, kc, jc, ic);  // the face's index (cell right of it)
+  const int na = mlp_var<C, DIR>(2 * g4), nb = mlp_var<C, DIR>(2 * g4 + 1);
+  const bool pos_a = g4 == 0 || g4 == 2;  // rho (q = 0) and p (q = 4) stay positive
+  const double *Va = Vc + na * g.sv, *Vb = Vc + nb * g.sv;
 
-  // ---- features -----------------------------------------------------------------
-  bool finite_in = true;
-  short *myfeat = &feat[wv][lane * LDS_STRIDE];
+#pragma unroll 1
+  for (int nt = 0; nt < 4; nt++) {
+    const int i = lo0 + xb * 64 + 16 * nt + rr;
+    const bool face_ok = i < lo0 + n0;
+    const long long pf = idx3(g, k, j, face_ok ? i : lo0);  // the face's index
+    float sa[6], sb[6];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int n = mlp_var<C, DIR>(q);
-    float S[6];
+    for (int s = 0; s < 6; s++) {
+      sa[s] = (float)Va[pf + (s - 3) * sd];
+      sb[s] = (float)Vb[pf + (s - 3) * sd];
+    }
+    float x[12];
+    int fin = mlp_features6(sa, x) && mlp_features6(sb, x + 6);
+    // the face's eight variables sit in lanes r, r + 16, r + 32, r + 48
+    fin &= __shfl_xor(fin, 16);
+    fin &= __shfl_xor(fin, 32);
+    float wal = 0.0f, war = 0.0f, wbl = 0.0f, wbr = 0.0f;
+    if (net.valid) {
+      mlp_bf16x4 bx[3];
 #pragma unroll
-    for (int s = 0; s < 6; s++) S[s] = (float)Vc[n * g.sv + pf + (s - 3) * sd];
-    const float c = 0.5f * (S[2] + S[3]);
-    float sc = 0.0f;
-#pragma unroll
-    for (int s = 0; s < 6; s++) sc = fmaxf(sc, fabsf(S[s] - c));
-    const float inv = sc > 0.0f ? 1.0f / sc : 0.0f;
-#pragma unroll
-    for (int s = 0; s < 6; s++) finite_in = finite_in && isfinite(S[s]);
-    finite_in = finite_in && isfinite(inv);
-#pragma unroll
-    for (int s = 0; s < 6; s++) myfeat[q * 6 + s] = mlp_bf16((S[s] - c) * inv);
-  }
-  __syncthreads();
-
-  float out[MLP_NO];
-  bool ok = finite_in && net.valid;
-  if (net.valid) {
-    const int g4 = lane >> 4, rr = lane & 15;
-    // layer 3 results land in lane (rr, g4) for interface 16 nt + rr, outputs 4 g4 + t;
-    // they pass through LDS (the feature rows are no longer needed) to the face's lane
-    float res[4][4];
-#pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
+      for (int q = 0; q < 12; q++) bx[q >> 2][q & 3] = mlp_bf16(x[q]);
       mlp_bf16x4 h1[4], h2[4];
       // layer 1: K = 48 in three steps of 16
 #pragma unroll
@@ -147,10 +176,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
 #pragma unroll
         for (int ks = 0; ks < 3; ks++) {
           const mlp_bf16x4 a =
-              *reinterpret_cast<const mlp_bf16x4 *>(&net.w1t[(16 * mt + rr) * 48 + 16 * ks + 4 * g4]);
-          const mlp_bf16x4 b =
-              *reinterpret_cast<const mlp_bf16x4 *>(&feat[wv][(16 * nt + rr) * LDS_STRIDE + 16 * ks + 4 * g4]);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+              *reinterpret_cast<const mlp_bf16x4 *>(&net.w1t[(16 * mt + rr) * 48 + 12 * g4 + 4 * ks]);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[ks], acc, 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < 4; t++) h1[mt][t] = mlp_bf16(mlp_gelu(acc[t] + net.b1[16 * mt + 4 * g4 + t]));
@@ -168,7 +195,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
 #pragma unroll
         for (int t = 0; t < 4; t++) h2[mt][t] = mlp_bf16(mlp_gelu(acc[t] + net.b2[16 * mt + 4 * g4 + t]));
       }
-      // layer 3: 16 outputs
+      // layer 3: 16 outputs; this lane holds outputs 4 g4 .. 4 g4 + 3
       mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < 4; ks++) {
@@ -176,44 +203,20 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
             *reinterpret_cast<const mlp_bf16x4 *>(&net.w3t[rr * 64 + 16 * ks + 4 * g4]);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[ks], acc, 0, 0, 0);
       }
-#pragma unroll
-      for (int t = 0; t < 4; t++)
-        res[nt][t] = mlp_sigmoid(acc[t] + net.b3[4 * g4 + t]);
+      wal = mlp_sigmoid(acc[0] + net.b3[4 * g4]);
+      war = mlp_sigmoid(acc[1] + net.b3[4 * g4 + 1]);
+      wbl = mlp_sigmoid(acc[2] + net.b3[4 * g4 + 2]);
+      wbr = mlp_sigmoid(acc[3] + net.b3[4 * g4 + 3]);
     }
-    __syncthreads();  // every wave is done reading its features
-    float *outs = reinterpret_cast<float *>(&feat[wv][0]);  // [64][17] floats
-#pragma unroll
-    for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-      for (int t = 0; t < 4; t++) outs[(16 * nt + rr) * 17 + 4 * g4 + t] = res[nt][t];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < MLP_NO; q++) out[q] = outs[lane * 17 + q];
-  }
-  // per-variable TVD check, PPM where it fails
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int n = mlp_var<C, DIR>(q);
-    float st[6];
-#pragma unroll
-    for (int t = 0; t < 6; t++) st[t] = (float)Vc[n * g.sv + pf + (t - 3) * sd];
-    const float a = st[1], b = st[2], c = st[3], d = st[4];
-    bool good = ok;
-    if (good) {
-      const float wl = out[2 * q], wr = out[2 * q + 1];
-      const float jump = c - b;
-      const float dl = wl * jump, dr = wr * jump;
-      const bool okl = wl >= 0.0f && wl <= 1.0f &&
-                       (dl == 0.0f || (dl * (b - a) > 0.0f && fabsf(dl) <= fabsf(b - a)));
-      const bool okr = wr >= 0.0f && wr <= 1.0f &&
-                       (dr == 0.0f || (dr * (d - c) > 0.0f && fabsf(dr) <= fabsf(d - c)));
-      good = (okl && okr) || (jump == 0.0f && wl == wl && wr == wr);  // NaN fails
+    const bool use_net = fin && net.valid;
+    mlp_check(sa, use_net, pos_a, wal, war);
+    mlp_check(sb, use_net, false, wbl, wbr);
+    if (face_ok) {
+      W[(4 * g4) * g.sv + pf] = wal;
+      W[(4 * g4 + 1) * g.sv + pf] = war;
+      W[(4 * g4 + 2) * g.sv + pf] = wbl;
+      W[(4 * g4 + 3) * g.sv + pf] = wbr;
     }
-    if (!good) mlp_ppm(st, q == 0 || q == 4, out[2 * q], out[2 * q + 1]);
-  }
-  if (face_ok) {
-#pragma unroll
-    for (int q = 0; q < MLP_NO; q++) W[q * g.sv + pf] = out[q];
   }
 }
 

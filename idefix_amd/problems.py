@@ -126,6 +126,31 @@ class ParkMiller:
         self.state = (16807 * self.state) % 2147483647
         return self.state / 2147483647.0
 
+    def draws(self, n, block=4096):
+        """The next n values of the sequence at once (the same numbers as n calls).
+        Block jump-ahead: with s_q the state after q steps, s_{q+B} = a^B s_q mod M, so
+        each block is the previous one times a^B mod M (products below 2^62 in int64)."""
+        M, a = 2147483647, 16807
+        n = int(n)
+        out = np.empty(n, dtype=np.float64)
+        if n == 0:
+            return out
+        b = min(block, n)
+        first = np.empty(b, dtype=np.int64)
+        s = self.state
+        for q in range(b):
+            s = (a * s) % M
+            first[q] = s
+        aB = pow(a, b, M)
+        cur = first
+        for o in range(0, n, b):
+            m = min(b, n - o)
+            out[o:o + m] = cur[:m] / float(M)
+            last = int(cur[m - 1])
+            cur = (cur * aB) % M
+        self.state = last
+        return out
+
 
 def _half_space_modes(dim, kmin, kmax):
     """Integer wave vectors n with kmin <= |n| <= kmax, one of each +-n pair, in a fixed
@@ -268,10 +293,8 @@ def kelvin_helmholtz(g, seed=13763, v0=0.5, a=0.05, rho=1.0, p=2.5, b0=0.05, noi
     Vc[1] = v0 * np.tanh((Y - 0.5) / a)
     rng = ParkMiller(seed)
     b, e = g.beg, g.end
-    u = np.empty((e[2] - b[2], e[1] - b[1], e[0] - b[0]))
-    flat = u.reshape(-1)
-    for q in range(flat.size):
-        flat[q] = rng()
+    shape = (e[2] - b[2], e[1] - b[1], e[0] - b[0])
+    u = rng.draws(int(np.prod(shape))).reshape(shape)
     env = np.exp(-((y[b[1]:e[1]] - 0.5) / a) ** 2)[None, :, None]
     Vc[2][b[2]:e[2], b[1]:e[1], b[0]:e[0]] = noise * (2.0 * u - 1.0) * env
     Vs[0] = b0

@@ -16,6 +16,19 @@ def test_park_miller_published_check_value():
     assert r.state == 1043618065
 
 
+def test_park_miller_block_draws():
+    """draws(n) (block jump-ahead) gives the published check value and the same numbers
+    and final state as n single calls, for block sizes that do and do not divide n."""
+    r = problems.ParkMiller(1)
+    r.draws(10000, block=333)
+    assert r.state == 1043618065
+    for n, block in ((0, 8), (1, 8), (7, 3), (4096, 4096), (5000, 4096)):
+        a, b = problems.ParkMiller(), problems.ParkMiller()
+        x = np.array([a() for _ in range(n)])
+        assert np.array_equal(b.draws(n, block=block), x)
+        assert a.state == b.state and a() == b()
+
+
 def test_park_miller_reference_seed_first_draw():
     r = problems.ParkMiller()
     x = r()
