@@ -120,7 +120,7 @@ WORKLOADS = [
          cfl=0.6, ic="orszag_tang", S=6, k=5, modes=("fp64", "fp32")),
     dict(name="turb3d256", label="3-D decaying turbulence 256^3, PPM, HLLD, CT, RK3, CFL 0.4",
          dim=3, shape=(256, 256, 256), comps=3, recon="ppm", emf="uct_contact", nstages=3,
-         cfl=0.4, ic="turbulence", S=8, k=7, modes=("fp64", "fp32")),
+         cfl=0.4, ic="turbulence", S=8, k=7, modes=("fp64", "fp32", "fp32+learned")),
     dict(name="turb3d512", label="3-D decaying turbulence 512^3, PPM, HLLD, CT, RK3, CFL 0.4",
          dim=3, shape=(512, 512, 512), comps=3, recon="ppm", emf="uct_contact", nstages=3,
          cfl=0.4, ic="turbulence", S=8, k=7, modes=("fp64", "fp32")),
@@ -232,7 +232,7 @@ def algorithmic_bytes(nx):
 # --pmc WRITE_SIZE on this workload and scaled by the plain-copy calibration (reads x2.0,
 # writes x1.0); the summary and the commands are in
 # profiles/ot2048_hlld_uct_hlld_fp64_kernels.md.  None where not measured.
-MEASURED_TRAFFIC = {2048: {"fused2d_y": (859.8 + 473.9) * 1e6, "sweep_x": (276.1 + 393.0) * 1e6}}
+MEASURED_TRAFFIC = {2048: {"fused2d_y": (859.6 + 473.9) * 1e6, "sweep_x": (276.9 + 393.0) * 1e6}}
 
 
 # The reference's published benchmark (doc/source/performances.rst:5-10, bench.json): 3-D

@@ -18,5 +18,9 @@ void register_iso(Ops *t) {
   t[ops_key(1,2,1,3,2,0,1)] = make_ops<Cfg<1,2,true,3,2,true>, double>();
   t[ops_key(1,2,1,4,2,0,1)] = make_ops<Cfg<1,2,true,4,2,true>, double>();
   t[ops_key(2,2,1,2,2,0,1)] = make_ops<Cfg<2,2,true,2,2,true>, double>();
+  t[ops_key(1,2,1,2,3,0,1)] = make_ops<Cfg<1,2,true,2,3,true>, double>();
+  t[ops_key(1,2,1,3,3,0,1)] = make_ops<Cfg<1,2,true,3,3,true>, double>();
+  t[ops_key(1,2,1,4,3,0,1)] = make_ops<Cfg<1,2,true,4,3,true>, double>();
+  t[ops_key(2,2,1,2,3,0,1)] = make_ops<Cfg<2,2,true,2,3,true>, double>();
 }
 }  // namespace idfx

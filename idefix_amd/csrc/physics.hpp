@@ -1598,6 +1598,214 @@ __device__ __forceinline__ void hlld_mhd_iso(const T *vL, const T *vR, T *flux, 
                          total_pressure_iso<C>(vR, c2), E, emf, pre);
 }
 
+// RoeMHD without an energy equation (roeMHD.hpp:175-651, the !HAVE_ENERGY branches
+// :21-41, :225-232, :296-302, :325-328, :427-431, :454-461, :515-520, :541-546): 7 waves
+// fast-, fast+, div B, slow-, slow+, Alfven-, Alfven+ (no entropy wave), the sound speed
+// squared a2 = Cs2Iso + X, fast and slow wave strengths built on a2 dV[RHO].
+template <class C, int DIR, typename T>
+__device__ __forceinline__ void roe_mhd_iso(const T *vL, const T *vR, T *flux, T &cmax,
+                                            FaceEmf<T> &E, int emf, T csiso) {
+  constexpr int NV = C::NVAR;
+  constexpr int CO = C::COMP;
+  constexpr int Xn = C::template Xn<DIR>();
+  constexpr int Xt = C::template Xt<DIR>();
+  constexpr int Xb = C::template Xb<DIR>();
+  constexpr int BXn = C::template BXn<DIR>();
+  constexpr int BXt = C::template BXt<DIR>();
+  constexpr int BXb = C::template BXb<DIR>();
+  constexpr int KFASTM = 0, KFASTP = 1, KDIVB = 2, KSLOWM = 3, KSLOWP = 4, KALFVM = 5,
+                KALFVP = 6;
+  const T delta = T(1.e-6);
+  T dV[NV], uL[NV], uR[NV], fluxL[NV], fluxR[NV], Rc[NV][NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) dV[n] = vR[n] - vL[n];
+  prim2cons<C>(vL, uL, csiso);
+  prim2cons<C>(vR, uR, csiso);
+  T a2L = face_cs(csiso);
+  a2L = a2L * a2L;
+  const T a2R = a2L;
+  kflux<C, DIR>(fluxL, vL, uL, a2L);
+  kflux<C, DIR>(fluxR, vR, uR, a2R);
+#pragma unroll
+  for (int p = 0; p < NV; p++)
+#pragma unroll
+    for (int q = 0; q < NV; q++) Rc[p][q] = 0;
+  const T sqr_rho_L = sqrt_(vL[C::RHO]);
+  const T sqr_rho_R = sqrt_(vR[C::RHO]);
+  T sl = sqr_rho_L / (sqr_rho_L + sqr_rho_R);
+  T sr = sqr_rho_R / (sqr_rho_L + sqr_rho_R);
+  const T rho = sr * vL[C::RHO] + sl * vR[C::RHO];
+  const T sqrt_rho = sqrt_(rho);
+  T u, v = 0, w = 0, Bx, By = 0, Bz = 0, bx, by = 0, bz = 0;
+  u = sl * vL[Xn] + sr * vR[Xn];
+  if constexpr (CO >= 2) v = sl * vL[Xt] + sr * vR[Xt];
+  if constexpr (CO >= 3) w = sl * vL[Xb] + sr * vR[Xb];
+  Bx = sr * vL[BXn] + sl * vR[BXn];
+  if constexpr (CO >= 2) By = sr * vL[BXt] + sl * vR[BXt];
+  if constexpr (CO >= 3) Bz = sr * vL[BXb] + sl * vR[BXb];
+  (void)v;
+  (void)w;
+  const T sBx = (Bx >= T(0) ? T(1) : T(-1));
+  bx = Bx / sqrt_rho;
+  if constexpr (CO >= 2) by = By / sqrt_rho;
+  if constexpr (CO >= 3) bz = Bz / sqrt_rho;
+  T bt2 = T(0);
+  if constexpr (CO >= 2) bt2 = bt2 + by * by;
+  if constexpr (CO >= 3) bt2 = bt2 + bz * bz;
+  const T b2 = bx * bx + bt2;
+  const T Btmag = sqrt_(bt2 * rho);
+  T X = dV[BXn] * dV[BXn];
+  if constexpr (CO >= 2) X = X + dV[BXt] * dV[BXt];
+  if constexpr (CO >= 3) X = X + dV[BXb] * dV[BXb];
+  X /= (sqr_rho_L + sqr_rho_R) * (sqr_rho_L + sqr_rho_R) * T(2);
+  const T a2 = T(0.5) * (a2L + a2R) + X;
+  T scrh = a2 - b2;
+  const T ca2 = bx * bx;
+  scrh = scrh * scrh + T(4) * bt2 * a2;
+  scrh = sqrt_(scrh);
+  const T cf2 = T(0.5) * (a2 + b2 + scrh);
+  const T cs2 = a2 * ca2 / cf2;
+  const T cf = sqrt_(cf2);
+  const T cs = sqrt_(cs2);
+  const T ca = sqrt_(ca2);
+  const T a = sqrt_(a2);
+  (void)ca;
+  T alpha_f, alpha_s;
+  if (cf == cs) {
+    alpha_f = T(1);
+    alpha_s = T(0);
+  } else if (a <= cs) {
+    alpha_f = T(0);
+    alpha_s = T(1);
+  } else if (cf <= a) {
+    alpha_f = T(1);
+    alpha_s = T(0);
+  } else {
+    scrh = T(1) / (cf2 - cs2);
+    alpha_f = (a2 - cs2) * scrh;
+    alpha_s = (cf2 - a2) * scrh;
+    alpha_f = fmx(T(0), alpha_f);
+    alpha_s = fmx(T(0), alpha_s);
+    alpha_f = sqrt_(alpha_f);
+    alpha_s = sqrt_(alpha_s);
+  }
+  T beta_y = 0, beta_z = 0;
+  if (Btmag > T(1.e-9)) {
+    if constexpr (CO == 2) beta_y = (By >= T(0) ? T(1) : T(-1));
+    if constexpr (CO == 3) {
+      beta_y = By / Btmag;
+      beta_z = Bz / Btmag;
+    }
+  } else {
+    if constexpr (CO == 2) beta_y = T(1);
+    if constexpr (CO == 3) beta_z = beta_y = T(1);
+  }
+  T lambda[NV], eta[NV];
+  // fast wave u - c_f
+  lambda[KFASTM] = u - cf;
+  scrh = alpha_s * cs * sBx;
+  T beta_dv = T(0), beta_dB = T(0);
+  if constexpr (CO >= 2) {
+    beta_dv = beta_dv + beta_y * dV[Xt];
+    beta_dB = beta_dB + beta_y * dV[BXt];
+  }
+  if constexpr (CO >= 3) {
+    beta_dv = beta_dv + beta_z * dV[Xb];
+    beta_dB = beta_dB + beta_z * dV[BXb];
+  }
+  Rc[C::RHO][KFASTM] = alpha_f;
+  Rc[Xn][KFASTM] = alpha_f * lambda[KFASTM];
+  if constexpr (CO >= 2) Rc[Xt][KFASTM] = alpha_f * v + scrh * beta_y;
+  if constexpr (CO >= 3) Rc[Xb][KFASTM] = alpha_f * w + scrh * beta_z;
+  if constexpr (CO >= 2) Rc[BXt][KFASTM] = alpha_s * a * beta_y / sqrt_rho;
+  if constexpr (CO >= 3) Rc[BXb][KFASTM] = alpha_s * a * beta_z / sqrt_rho;
+  eta[KFASTM] = alpha_f * a2 * dV[C::RHO];
+  eta[KFASTM] += rho * scrh * beta_dv - rho * alpha_f * cf * dV[Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  eta[KFASTM] *= T(0.5) / a2;
+  // fast wave u + c_f
+  lambda[KFASTP] = u + cf;
+  Rc[C::RHO][KFASTP] = alpha_f;
+  Rc[Xn][KFASTP] = alpha_f * lambda[KFASTP];
+  if constexpr (CO >= 2) Rc[Xt][KFASTP] = alpha_f * v - scrh * beta_y;
+  if constexpr (CO >= 3) Rc[Xb][KFASTP] = alpha_f * w - scrh * beta_z;
+  if constexpr (CO >= 2) Rc[BXt][KFASTP] = Rc[BXt][KFASTM];
+  if constexpr (CO >= 3) Rc[BXb][KFASTP] = Rc[BXb][KFASTM];
+  eta[KFASTP] = alpha_f * (T(0) * X + a2) * dV[C::RHO] - rho * scrh * beta_dv +
+                rho * alpha_f * cf * dV[Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  eta[KFASTP] *= T(0.5) / a2;
+  // div B wave u (zero jump)
+  lambda[KDIVB] = u;
+  eta[KDIVB] = T(0);
+  if constexpr (CO > 1) {
+    scrh = alpha_f * cf * sBx;
+    lambda[KSLOWM] = u - cs;
+    Rc[C::RHO][KSLOWM] = alpha_s;
+    Rc[Xn][KSLOWM] = alpha_s * lambda[KSLOWM];
+    Rc[Xt][KSLOWM] = alpha_s * v - scrh * beta_y;
+    if constexpr (CO >= 3) Rc[Xb][KSLOWM] = alpha_s * w - scrh * beta_z;
+    Rc[BXt][KSLOWM] = -alpha_f * a * beta_y / sqrt_rho;
+    if constexpr (CO >= 3) Rc[BXb][KSLOWM] = -alpha_f * a * beta_z / sqrt_rho;
+    eta[KSLOWM] = alpha_s * (T(0) * X + a2) * dV[C::RHO] - rho * scrh * beta_dv -
+                  rho * alpha_s * cs * dV[Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    eta[KSLOWM] *= T(0.5) / a2;
+    lambda[KSLOWP] = u + cs;
+    Rc[C::RHO][KSLOWP] = alpha_s;
+    Rc[Xn][KSLOWP] = alpha_s * lambda[KSLOWP];
+    Rc[Xt][KSLOWP] = alpha_s * v + scrh * beta_y;
+    if constexpr (CO >= 3) Rc[Xb][KSLOWP] = alpha_s * w + scrh * beta_z;
+    Rc[BXt][KSLOWP] = Rc[BXt][KSLOWM];
+    if constexpr (CO >= 3) Rc[BXb][KSLOWP] = Rc[BXb][KSLOWM];
+    eta[KSLOWP] = alpha_s * (T(0) * X + a2) * dV[C::RHO] + rho * scrh * beta_dv +
+                  rho * alpha_s * cs * dV[Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    eta[KSLOWP] *= T(0.5) / a2;
+  }
+  if constexpr (CO == 3) {
+    lambda[KALFVM] = u - ca;
+    Rc[Xt][KALFVM] = -rho * beta_z;
+    Rc[Xb][KALFVM] = +rho * beta_y;
+    Rc[BXt][KALFVM] = -sBx * sqrt_rho * beta_z;
+    Rc[BXb][KALFVM] = sBx * sqrt_rho * beta_y;
+    eta[KALFVM] = +beta_y * dV[Xb] - beta_z * dV[Xt] +
+                  sBx / sqrt_rho * (beta_y * dV[BXb] - beta_z * dV[BXt]);
+    eta[KALFVM] *= T(0.5);
+    lambda[KALFVP] = u + ca;
+    Rc[Xt][KALFVP] = -Rc[Xt][KALFVM];
+    Rc[Xb][KALFVP] = -Rc[Xb][KALFVM];
+    Rc[BXt][KALFVP] = Rc[BXt][KALFVM];
+    Rc[BXb][KALFVP] = Rc[BXb][KALFVM];
+    eta[KALFVP] = -beta_y * dV[Xb] + beta_z * dV[Xt] +
+                  sBx / sqrt_rho * (beta_y * dV[BXb] - beta_z * dV[BXt]);
+    eta[KALFVP] *= T(0.5);
+  }
+  cmax = fab(u) + cf;
+  sl = lambda[KFASTM];
+  sr = lambda[KFASTP];
+  T alambda[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) alambda[n] = fab(lambda[n]);
+  if (alambda[KFASTM] < T(0.5) * delta)
+    alambda[KFASTM] = lambda[KFASTM] * lambda[KFASTM] / delta + T(0.25) * delta;
+  if (alambda[KFASTP] < T(0.5) * delta)
+    alambda[KFASTP] = lambda[KFASTP] * lambda[KFASTP] / delta + T(0.25) * delta;
+  if constexpr (CO > 1) {
+    if (alambda[KSLOWM] < T(0.5) * delta)
+      alambda[KSLOWM] = lambda[KSLOWM] * lambda[KSLOWM] / delta + T(0.25) * delta;
+    if (alambda[KSLOWP] < T(0.5) * delta)
+      alambda[KSLOWP] = lambda[KSLOWP] * lambda[KSLOWP] / delta + T(0.25) * delta;
+  }
+#pragma unroll
+  for (int n1 = 0; n1 < NV; n1++) {
+    T sc = T(0);
+#pragma unroll
+    for (int n2 = 0; n2 < NV; n2++) sc += alambda[n2] * eta[n2] * Rc[n1][n2];
+    flux[n1] = T(0.5) * (fluxL[n1] + fluxR[n1] - sc);
+  }
+  StarPre<T> pre;
+  pre.valid = false;
+  store_face_emf<C, DIR>(vL, vR, uL, uR, flux, sl, sr, total_pressure_iso<C>(vL, a2L),
+                         total_pressure_iso<C>(vR, a2R), E, emf, pre);
+}
+
 // Dispatch to the configured Riemann solver.  Returns false only in the 1-D Roe case
 // where the reference leaves the flux unwritten.
 template <class C, int DIR, typename T>
@@ -1608,6 +1816,7 @@ __device__ __forceinline__ bool riemann(const T *vL, const T *vR, T *flux, T &cm
     if constexpr (C::MHD) {
       if constexpr (C::SOLVER == SOLVER_HLL) hll_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
       else if constexpr (C::SOLVER == SOLVER_TVDLF) tvdlf_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
+      else if constexpr (C::SOLVER == SOLVER_ROE) roe_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);
       else {
         static_assert(C::SOLVER == SOLVER_HLLD, "isothermal MHD solver not built");
         hlld_mhd_iso<C, DIR>(vL, vR, flux, cmax, E, emf, gamma);

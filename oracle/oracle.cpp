@@ -1173,31 +1173,35 @@ void tvdlf_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const dou
 }
 
 // --------------------------------------------------------------------------------------
-// RoeMHD: src/fluid/RiemannSolver/MHDsolvers/roeMHD.hpp:175-651 (HAVE_ENERGY,
-// ROE_AVERAGE 0, 8 waves: fast-, fast+, div B, entropy, slow-, slow+, Alfven-, Alfven+).
+// RoeMHD: src/fluid/RiemannSolver/MHDsolvers/roeMHD.hpp:175-651 (ROE_AVERAGE 0).  With
+// an energy equation 8 waves: fast-, fast+, div B, entropy, slow-, slow+, Alfven-,
+// Alfven+; isothermal (the !HAVE_ENERGY branches, :21-41, :225-232, :296-302, :325-328,
+// :427-431, :454-461, :515-520, :541-546) 7 waves without the entropy wave, the sound
+// speed squared a2 = Cs2Iso + X, and the fast and slow wave strengths built on a2 dV[RHO].
 // --------------------------------------------------------------------------------------
 void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const double *vL,
              const double *vR, double *flux, double &cmax, FaceEmf &E) {
   const int nv_ = V.nvar;
   const int comps = c.comps;
-  const int KFASTM = 0, KFASTP = 1, KDIVB = 2, KENTRP = 3, KSLOWM = 4, KSLOWP = 5,
-            KALFVM = 6, KALFVP = 7;
+  const bool iso = c.isothermal != 0;
+  const int KFASTM = 0, KFASTP = 1, KDIVB = 2, KENTRP = 3, KSLOWM = iso ? 3 : 4,
+            KSLOWP = iso ? 4 : 5, KALFVM = iso ? 5 : 6, KALFVP = iso ? 6 : 7;
   const double delta = 1.e-6;
   auto DSIGN = [](double x) { return x >= 0.0 ? 1.0 : -1.0; };
   double dV[8], uL[8], uR[8], dU[8], fluxL[8], fluxR[8], Rc[8][8];
   for (int n = 0; n < nv_; n++) dV[n] = vR[n] - vL[n];
   prim2cons(c, V, vL, uL);
   prim2cons(c, V, vR, uR);
-  const double a2L = 1.0, a2R = 1.0;  // unused with an energy equation
-  (void)a2R;
+  // with an energy equation a2L, a2R are placeholders (roeMHD.hpp:223-227)
+  const double a2L = iso ? c2iso(c) : 1.0, a2R = iso ? a2L : 1.0;
   const double gamma = c.gamma;
   for (int n = 0; n < nv_; n++) {
     fluxL[n] = uL[n];
     fluxR[n] = uR[n];
     dU[n] = uR[n] - uL[n];
   }
-  kflux(c, V, D, fluxL, vL, fluxL);
-  kflux(c, V, D, fluxR, vR, fluxR);
+  kflux(c, V, D, fluxL, vL, fluxL, a2L);
+  kflux(c, V, D, fluxR, vR, fluxR, a2R);
   for (int p = 0; p < 8; p++)
     for (int q = 0; q < 8; q++) Rc[p][q] = 0;
 
@@ -1233,24 +1237,29 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
   double Bmag2R = vR[V.BX1] * vR[V.BX1];
   if (comps >= 2) Bmag2R = Bmag2R + vR[V.BX2] * vR[V.BX2];
   if (comps >= 3) Bmag2R = Bmag2R + vR[V.BX3] * vR[V.BX3];
-  const double pL = vL[V.ENG] + 0.5 * Bmag2L;
-  const double pR = vR[V.ENG] + 0.5 * Bmag2R;
+  const double pL = (iso ? a2L * vL[V.RHO] : vL[V.ENG]) + 0.5 * Bmag2L;
+  const double pR = (iso ? a2R * vR[V.RHO] : vR[V.ENG]) + 0.5 * Bmag2R;
 
-  double vdm = u * dU[D.Xn];
-  if (comps >= 2) vdm = vdm + v * dU[D.Xt];
-  if (comps >= 3) vdm = vdm + w * dU[D.Xb];
-  double BdB = Bx * dU[D.BXn];
-  if (comps >= 2) BdB = BdB + By * dU[D.BXt];
-  if (comps >= 3) BdB = BdB + Bz * dU[D.BXb];
-  double vel2 = u * u;
-  if (comps >= 2) vel2 = vel2 + v * v;
-  if (comps >= 3) vel2 = vel2 + w * w;
-  dV[V.ENG] = (gamma - 1.0) * ((0.5 * vel2 - X) * dV[V.RHO] - vdm + dU[V.ENG] - BdB);
-  const double HL = (uL[V.ENG] + pL) / vL[V.RHO];
-  const double HR = (uR[V.ENG] + pR) / vR[V.RHO];
-  const double H = sl * HL + sr * HR;
-  const double Hgas = H - b2;
-  const double a2 = (2.0 - gamma) * X + (gamma - 1.0) * (Hgas - 0.5 * vel2);
+  double vel2 = 0.0, Hgas = 0.0, a2;
+  if (!iso) {
+    double vdm = u * dU[D.Xn];
+    if (comps >= 2) vdm = vdm + v * dU[D.Xt];
+    if (comps >= 3) vdm = vdm + w * dU[D.Xb];
+    double BdB = Bx * dU[D.BXn];
+    if (comps >= 2) BdB = BdB + By * dU[D.BXt];
+    if (comps >= 3) BdB = BdB + Bz * dU[D.BXb];
+    vel2 = u * u;
+    if (comps >= 2) vel2 = vel2 + v * v;
+    if (comps >= 3) vel2 = vel2 + w * w;
+    dV[V.ENG] = (gamma - 1.0) * ((0.5 * vel2 - X) * dV[V.RHO] - vdm + dU[V.ENG] - BdB);
+    const double HL = (uL[V.ENG] + pL) / vL[V.RHO];
+    const double HR = (uR[V.ENG] + pR) / vR[V.RHO];
+    const double H = sl * HL + sr * HR;
+    Hgas = H - b2;
+    a2 = (2.0 - gamma) * X + (gamma - 1.0) * (Hgas - 0.5 * vel2);
+  } else {
+    a2 = 0.5 * (a2L + a2R) + X;
+  }
 
   double scrh = a2 - b2;
   const double ca2 = bx * bx;
@@ -1316,8 +1325,12 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
   if (comps >= 3) Rc[D.Xb][kk] = alpha_f * w + scrh * beta_z;
   if (comps >= 2) Rc[D.BXt][kk] = alpha_s * a * beta_y / sqrt_rho;
   if (comps >= 3) Rc[D.BXb][kk] = alpha_s * a * beta_z / sqrt_rho;
-  Rc[V.ENG][kk] = alpha_f * (Hgas - u * cf) + scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
-  eta[kk] = alpha_f * (X * dV[V.RHO] + dV[V.ENG]);
+  if (!iso) {
+    Rc[V.ENG][kk] = alpha_f * (Hgas - u * cf) + scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
+    eta[kk] = alpha_f * (X * dV[V.RHO] + dV[V.ENG]);
+  } else {
+    eta[kk] = alpha_f * a2 * dV[V.RHO];
+  }
   eta[kk] += rho * scrh * beta_dv - rho * alpha_f * cf * dV[D.Xn] + sqrt_rho * alpha_s * a * beta_dB;
   eta[kk] *= 0.5 / a2;
   // fast wave u + c_f
@@ -1329,19 +1342,26 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
   if (comps >= 3) Rc[D.Xb][kk] = alpha_f * w - scrh * beta_z;
   if (comps >= 2) Rc[D.BXt][kk] = Rc[D.BXt][KFASTM];
   if (comps >= 3) Rc[D.BXb][kk] = Rc[D.BXb][KFASTM];
-  Rc[V.ENG][kk] = alpha_f * (Hgas + u * cf) - scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
-  eta[kk] = alpha_f * (X * dV[V.RHO] + dV[V.ENG]) - rho * scrh * beta_dv +
-            rho * alpha_f * cf * dV[D.Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  if (!iso) {
+    Rc[V.ENG][kk] = alpha_f * (Hgas + u * cf) - scrh * beta_v + alpha_s * a * Btmag / sqrt_rho;
+    eta[kk] = alpha_f * (X * dV[V.RHO] + dV[V.ENG]) - rho * scrh * beta_dv +
+              rho * alpha_f * cf * dV[D.Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  } else {
+    eta[kk] = alpha_f * (0. * X + a2) * dV[V.RHO] - rho * scrh * beta_dv +
+              rho * alpha_f * cf * dV[D.Xn] + sqrt_rho * alpha_s * a * beta_dB;
+  }
   eta[kk] *= 0.5 / a2;
-  // entropy wave u
-  kk = KENTRP;
-  lambda[kk] = u;
-  Rc[V.RHO][kk] = 1.0;
-  Rc[D.Xn][kk] = u;
-  if (comps >= 2) Rc[D.Xt][kk] = v;
-  if (comps >= 3) Rc[D.Xb][kk] = w;
-  Rc[V.ENG][kk] = 0.5 * vel2 + (gamma - 2.0) / (gamma - 1.0) * X;
-  eta[kk] = ((a2 - X) * dV[V.RHO] - dV[V.ENG]) / a2;
+  // entropy wave u (with an energy equation only)
+  if (!iso) {
+    kk = KENTRP;
+    lambda[kk] = u;
+    Rc[V.RHO][kk] = 1.0;
+    Rc[D.Xn][kk] = u;
+    if (comps >= 2) Rc[D.Xt][kk] = v;
+    if (comps >= 3) Rc[D.Xb][kk] = w;
+    Rc[V.ENG][kk] = 0.5 * vel2 + (gamma - 2.0) / (gamma - 1.0) * X;
+    eta[kk] = ((a2 - X) * dV[V.RHO] - dV[V.ENG]) / a2;
+  }
   // div B wave u (zero jump)
   kk = KDIVB;
   lambda[kk] = u;
@@ -1356,9 +1376,14 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
     if (comps >= 3) Rc[D.Xb][kk] = alpha_s * w - scrh * beta_z;
     Rc[D.BXt][kk] = -alpha_f * a * beta_y / sqrt_rho;
     if (comps >= 3) Rc[D.BXb][kk] = -alpha_f * a * beta_z / sqrt_rho;
-    Rc[V.ENG][kk] = alpha_s * (Hgas - u * cs) - scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
-    eta[kk] = alpha_s * (X * dV[V.RHO] + dV[V.ENG]) - rho * scrh * beta_dv -
-              rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    if (!iso) {
+      Rc[V.ENG][kk] = alpha_s * (Hgas - u * cs) - scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
+      eta[kk] = alpha_s * (X * dV[V.RHO] + dV[V.ENG]) - rho * scrh * beta_dv -
+                rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    } else {
+      eta[kk] = alpha_s * (0. * X + a2) * dV[V.RHO] - rho * scrh * beta_dv -
+                rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    }
     eta[kk] *= 0.5 / a2;
     kk = KSLOWP;
     lambda[kk] = u + cs;
@@ -1368,9 +1393,14 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
     if (comps >= 3) Rc[D.Xb][kk] = alpha_s * w + scrh * beta_z;
     Rc[D.BXt][kk] = Rc[D.BXt][KSLOWM];
     if (comps >= 3) Rc[D.BXb][kk] = Rc[D.BXb][KSLOWM];
-    Rc[V.ENG][kk] = alpha_s * (Hgas + u * cs) + scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
-    eta[kk] = alpha_s * (X * dV[V.RHO] + dV[V.ENG]) + rho * scrh * beta_dv +
-              rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    if (!iso) {
+      Rc[V.ENG][kk] = alpha_s * (Hgas + u * cs) + scrh * beta_v - alpha_f * a * Btmag / sqrt_rho;
+      eta[kk] = alpha_s * (X * dV[V.RHO] + dV[V.ENG]) + rho * scrh * beta_dv +
+                rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    } else {
+      eta[kk] = alpha_s * (0. * X + a2) * dV[V.RHO] + rho * scrh * beta_dv +
+                rho * alpha_s * cs * dV[D.Xn] - sqrt_rho * alpha_f * a * beta_dB;
+    }
     eta[kk] *= 0.5 / a2;
   }
   if (comps == 3) {
@@ -1380,7 +1410,7 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
     Rc[D.Xb][kk] = +rho * beta_y;
     Rc[D.BXt][kk] = -sBx * sqrt_rho * beta_z;
     Rc[D.BXb][kk] = sBx * sqrt_rho * beta_y;
-    Rc[V.ENG][kk] = -rho * (v * beta_z - w * beta_y);
+    if (!iso) Rc[V.ENG][kk] = -rho * (v * beta_z - w * beta_y);
     eta[kk] = +beta_y * dV[D.Xb] - beta_z * dV[D.Xt] +
               sBx / sqrt_rho * (beta_y * dV[D.BXb] - beta_z * dV[D.BXt]);
     eta[kk] *= 0.5;
@@ -1390,7 +1420,7 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
     Rc[D.Xb][kk] = -Rc[D.Xb][KALFVM];
     Rc[D.BXt][kk] = Rc[D.BXt][KALFVM];
     Rc[D.BXb][kk] = Rc[D.BXb][KALFVM];
-    Rc[V.ENG][kk] = -Rc[V.ENG][KALFVM];
+    if (!iso) Rc[V.ENG][kk] = -Rc[V.ENG][KALFVM];
     eta[kk] = -beta_y * dV[D.Xb] + beta_z * dV[D.Xt] +
               sBx / sqrt_rho * (beta_y * dV[D.BXb] - beta_z * dV[D.BXt]);
     eta[kk] *= 0.5;
@@ -1410,7 +1440,6 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
     for (int n2 = 0; n2 < nv_; n2++) sc += alambda[n2] * eta[n2] * Rc[n1][n2];
     flux[n1] = 0.5 * (fluxL[n1] + fluxR[n1] - sc);
   }
-  (void)a2L;
   store_face_emf(c, V, D, dir, vL, vR, uL, uR, flux, sl, sr, pL, pR, E);
 }
 
@@ -2360,8 +2389,6 @@ void *orc_new(const orc_cfg *cfg) {
   if (cfg->mhd && (cfg->solver == ORC_SOLVER_HLL || cfg->solver == ORC_SOLVER_TVDLF) &&
       cfg->emf == ORC_EMF_UCT_HLLD)
     return nullptr;
-  /* the isothermal RoeMHD branches are not restated */
-  if (cfg->isothermal && cfg->mhd && cfg->solver == ORC_SOLVER_ROE) return nullptr;
   return new Oracle(*cfg);
 }
 void orc_free(void *o) { delete static_cast<Oracle *>(o); }

@@ -327,9 +327,11 @@ def test_sod_iso_reference_matrix(solver, recon, nstages):
 
 @pytest.mark.parametrize("solver,recon,nstages", [
     ("hlld", "plm", 2), ("hll", "plm", 2), ("tvdlf", "plm", 2), ("hlld", "limo3", 2),
-    ("hlld", "ppm", 3)])
+    ("hlld", "ppm", 3), ("roe", "plm", 2), ("roe", "limo3", 2), ("roe", "ppm", 3)])
 def test_brio_wu_iso_reference_matrix(solver, recon, nstages):
-    """test/MHD/sod-iso (800 cells on [0, 100], CFL 0.8, csiso 1)."""
+    """test/MHD/sod-iso (800 cells on [0, 100], CFL 0.8, csiso 1): idefix.ini (roe),
+    -hll, -hlld, -tvdlf and the RK3 inputs (roe, hlld) with PPM
+    (test/MHD/sod-iso/testme.py:17-20)."""
     kw = cfg_kwargs(1, (800,), comps=2, solver=solver, recon=recon, bc="outflow", cfl=0.8,
                     nstages=nstages, xend=(100, 1, 1), csiso=1.0)
     o, g = make_pair(kw, problems.brio_wu_iso)
@@ -338,14 +340,15 @@ def test_brio_wu_iso_reference_matrix(solver, recon, nstages):
     compare_state(o, g, what="Brio-Wu iso %s %s" % (solver, recon))
 
 
-@pytest.mark.parametrize("emf", ["uct_hlld", "uct_contact"])
-def test_mhd2d_iso_fused(emf):
+@pytest.mark.parametrize("solver,emf", [("hlld", "uct_hlld"), ("hlld", "uct_contact"),
+                                        ("roe", "uct_contact"), ("roe", "uct_hlld")])
+def test_mhd2d_iso_fused(solver, emf):
     """Isothermal 2-D MHD through the fused 2-D kernel."""
-    kw = cfg_kwargs(2, (48, 40), emf=emf, cfl=0.6, csiso=0.8)
+    kw = cfg_kwargs(2, (48, 40), solver=solver, emf=emf, cfl=0.6, csiso=0.8)
     o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=31, amp=0.2))
     o.set_boundaries()
     _cycle_both(o, g, 6)
-    compare_state(o, g, what="2-D MHD iso %s" % emf)
+    compare_state(o, g, what="2-D MHD iso %s %s" % (solver, emf))
 
 
 def test_hd2d_iso_with_flattening():

@@ -1,9 +1,9 @@
 """Pins of the oracle's isothermal equation of state (ISOTHERMAL builds, constant csiso:
 eos_isothermal.hpp and the HAVE_ENERGY == 0 branches of the solvers): the exact
 isothermal Riemann solution checked by its jump conditions, flux consistency with the
-textbook isothermal fluxes, the isothermal Sod tube against its exact solution with every
-solver, and the isothermal Brio-Wu tube: constant normal field, conservation, and every
-solver against a 4x finer solution."""
+textbook isothermal fluxes, the Roe property of the Roe MHD solver, the isothermal Sod tube
+against its exact solution with every solver, and the isothermal Brio-Wu tube: constant
+normal field, conservation, and every solver against a 4x finer solution."""
 import numpy as np
 import pytest
 
@@ -67,7 +67,8 @@ def test_iso_hd_flux_consistency(solver, d):
 
 
 @pytest.mark.parametrize("solver,emf", [("hlld", "uct_contact"), ("hlld", "uct_hlld"),
-                                        ("hll", "uct_hll"), ("tvdlf", "uct_contact")])
+                                        ("hll", "uct_hll"), ("tvdlf", "uct_contact"),
+                                        ("roe", "uct_contact")])
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_iso_mhd_flux_consistency(solver, emf, d):
     cfg = O.make_cfg(3, mhd=True, solver=solver, emf=emf, nx=(8, 8, 8), comps=3, csiso=CS)
@@ -78,6 +79,50 @@ def test_iso_mhd_flux_consistency(solver, emf, d):
         v[4:7] = rng.normal(size=3)
         F, _, _, _ = O.face_flux(cfg, d, v, v)
         assert np.allclose(F[:7], _iso_flux_mhd(v, d), rtol=1e-11, atol=1e-11)
+
+
+def _flux_mhd_energy(v, d, gamma=5.0 / 3.0):
+    rho, vel, p, B = v[0], v[1:4], v[4], v[5:8]
+    pt = p + 0.5 * B @ B
+    E = p / (gamma - 1.0) + 0.5 * rho * vel @ vel + 0.5 * B @ B
+    F = np.zeros(8)
+    F[0] = rho * vel[d]
+    F[1:4] = rho * vel * vel[d] - B * B[d]
+    F[1 + d] += pt
+    F[4] = (E + pt) * vel[d] - B[d] * (vel @ B)
+    F[5:8] = vel[d] * B - B[d] * vel
+    return F
+
+
+@pytest.mark.parametrize("iso", [True, False])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_roe_mhd_is_a_roe_linearisation(iso, d):
+    """Roe's property F(U_R) - F(U_L) = A(U_R - U_L), summed over the waves, for finite
+    jumps: when every wave speed has one sign the dissipation sum_k |lambda_k| eta_k R_k
+    equals +-(F_R - F_L), so the Roe flux is exactly the upwind physical flux.  This pins
+    the eigenvectors, the wave strengths and the X term of the sound speed (the exact
+    linearisation of roeMHD.hpp; without X the deviation is ~5e-3)."""
+    cfg = O.make_cfg(3, mhd=True, solver="roe", emf="uct_contact", nx=(8, 8, 8), comps=3,
+                     csiso=CS if iso else None)
+    nv, b0 = (7, 4) if iso else (8, 5)
+    flux = (lambda v: _iso_flux_mhd(v, d)) if iso else (lambda v: _flux_mhd_energy(v, d))
+    for _ in range(100):
+        vL, vR = np.zeros(8), np.zeros(8)
+        for v in (vL, vR):
+            v[0] = rng.uniform(0.5, 2.0)
+            v[1:4] = rng.normal(size=3) * 0.3
+            if iso:
+                v[4:7] = rng.normal(size=3) * 0.5
+            else:
+                v[4] = rng.uniform(0.5, 2.0)
+                v[5:8] = rng.normal(size=3) * 0.5
+        vR[b0 + d] = vL[b0 + d]  # the normal field is continuous (it comes from Vs)
+        sgn = rng.choice([-1.0, 1.0])
+        vL[1 + d] += 8.0 * sgn  # faster than every wave of both states
+        vR[1 + d] += 8.0 * sgn
+        F, _, _, _ = O.face_flux(cfg, d, vL, vR)
+        ref = flux(vL if sgn > 0 else vR)
+        assert np.max(np.abs(F[:nv] - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
 @pytest.mark.parametrize("solver,recon", [("roe", "plm"), ("hll", "plm"), ("hllc", "plm"),
@@ -114,7 +159,8 @@ def test_iso_brio_wu_solvers_agree_with_fine_solution():
     fine = _bw_iso("hlld", n=3200)
     assert fine.run(10.0) == 0
     ref = fine.interior(fine.Vc)[:, 0, 0].reshape(5, 800, 4).mean(axis=2)
-    for solver, recon in [("hlld", "plm"), ("hll", "plm"), ("tvdlf", "plm"), ("hlld", "limo3")]:
+    for solver, recon in [("hlld", "plm"), ("hll", "plm"), ("tvdlf", "plm"), ("hlld", "limo3"),
+                          ("roe", "plm"), ("roe", "limo3")]:
         o = _bw_iso(solver, recon=recon)
         assert o.run(10.0) == 0
         v = o.interior(o.Vc)[:, 0, 0]
