@@ -10,6 +10,7 @@ fields for the components held on faces) and the throughput of each run.
 import argparse
 import json
 import os
+import sys
 import time
 
 import numpy as np
@@ -44,7 +45,10 @@ def run(config, n, tend, precision, recon=None, params=None):
         g.set_mlp_params(params)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    g.run(tend)
+    while g.t < tend:  # the main loop in chunks of cycles, with a progress line each
+        g.run(tend, maxcycles=g.ncycles + 500)
+        print("%s %s: t = %.4g after %d cycles" % (config, precision, g.t, g.ncycles),
+              file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     cells = int(np.prod(c["shape"](n)))

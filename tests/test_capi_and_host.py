@@ -52,6 +52,43 @@ def test_ctx_create_rejects_bad_configs_without_gpu_work():
         assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
 
 
+def _desc(**kw):
+    d = capi.IdfxDesc()
+    d.dim, d.comps, d.mhd = 2, 2, 1
+    d.nx[0], d.nx[1], d.nx[2] = 8, 8, 1
+    d.recon, d.solver, d.emf = 2, capi.SOLVER["hlld"], capi.EMF["uct_contact"]
+    d.gamma = 5.0 / 3.0
+    for k, v in kw.items():
+        if k == "nx":
+            for q in range(3):
+                d.nx[q] = v[q]
+        else:
+            setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nx=(0, 8, 1)),                                   # empty grid
+    dict(nx=(8, -3, 1)),
+    dict(dim=3, comps=3, nx=(1024, 1024, 1024)),          # beyond 32-bit byte offsets
+    dict(dim=0), dict(dim=4), dict(comps=1),              # COMPONENTS < DIMENSIONS
+    dict(ntracer=-1), dict(ntracer=65),
+    dict(isothermal=1, csiso=0.0), dict(isothermal=1, csiso=-1.0),
+    dict(shock_flattening=1, shock_smoothing=0.05, recon=4),  # PPM has no flattening
+    dict(emf=5), dict(emf=-1),
+    dict(mhd=0, solver=capi.SOLVER["hlld"]),              # hlld needs a MHD fluid
+    dict(dim=1, comps=1, mhd=0, solver=capi.SOLVER["roe"], ntracer=1, nx=(64, 1, 1)),
+])
+def test_ctx_create_rejects_invalid_descriptions(kw):
+    """Empty or oversized grids and inconsistent options are refused with IDFX_E_ARGS
+    before any device call (this runs without a GPU)."""
+    L = capi.load()
+    h = ctypes.c_void_p()
+    d = _desc(**kw)
+    assert L.idfx_ctx_create(ctypes.byref(d), None, None, None, ctypes.byref(h)) == capi.E_ARGS
+    assert not h.value
+
+
 @pytest.mark.parametrize("dim,nx,recon", [(1, (100,), "plm"), (2, (48, 40), "ppm"),
                                           (3, (8, 12, 10), "plm")])
 def test_grid_matches_oracle(dim, nx, recon):
