@@ -70,10 +70,12 @@ __device__ __forceinline__ constexpr int mlp_q(int n) {
 // the left state is cell i-1's right value, the right state cell i's left value
 // (getPPMStates, slopeLimiter.hpp:142-191), with the van Leer fallback where rho or p
 // would not be positive (extrapolateToFaces.hpp:260-304); w = 0 where V[i] = V[i-1].
+// The limiter's quotients use v_rcp_f32 (1 ulp); a non-finite one only skips a limiter
+// step, since every comparison with NaN or inf there keeps the unlimited value.
 __device__ __forceinline__ void mlp_ppm(const float *s, bool positive, float &wl, float &wr) {
   float vl_c, left, right, vr_d;
-  ppm_states(s[0], s[1], s[2], s[3], s[4], vl_c, left);
-  ppm_states(s[1], s[2], s[3], s[4], s[5], right, vr_d);
+  ppm_states<float, true>(s[0], s[1], s[2], s[3], s[4], vl_c, left);
+  ppm_states<float, true>(s[1], s[2], s[3], s[4], s[5], right, vr_d);
   if (positive) {
     if (left <= 0.0f) left = s[2] + 0.5f * vanleer(s[3] - s[2], s[2] - s[1]);
     if (right <= 0.0f) right = s[3] - 0.5f * vanleer(s[4] - s[3], s[3] - s[2]);
@@ -91,7 +93,7 @@ __device__ __forceinline__ bool mlp_features6(const float *S, float *x) {
   float sc = 0.0f;
 #pragma unroll
   for (int s = 0; s < 6; s++) sc = fmaxf(sc, fabsf(S[s] - c));
-  const float inv = sc > 0.0f ? 1.0f / sc : 0.0f;
+  const float inv = sc > 0.0f ? __builtin_amdgcn_rcpf(sc) : 0.0f;
   bool fin = isfinite(inv);
 #pragma unroll
   for (int s = 0; s < 6; s++) fin = fin && isfinite(S[s]);

@@ -107,7 +107,14 @@ __device__ __forceinline__ int sgn(T v) { return (T(0) < v) - (v < T(0)); }
 // The PPM helpers work on values, not references: conditional stores through reference
 // parameters let the compiler merge them into one store through a selected address,
 // which keeps the caller's state arrays in scratch memory.
-template <typename T>
+// Quotient of the PPM helpers: correctly rounded, or, with FAST (the learned limiter's
+// FP32 fallback only), one v_rcp_f32 (1 ulp) and a product.
+template <typename T, bool FAST>
+__device__ __forceinline__ T ppm_div(T a, T b) {
+  if constexpr (FAST) return a * __builtin_amdgcn_rcpf(b);
+  else return a / b;
+}
+template <typename T, bool FAST = false>
 __device__ __forceinline__ T limit_ppm_face(T vm1, T v0, T vp1, T vp2, T vph) {
   // limitPPMFaceValues, slopeLimiter.hpp:121-140
   if ((vp1 - vph) * (vph - v0) < T(0)) {
@@ -120,19 +127,19 @@ __device__ __forceinline__ T limit_ppm_face(T vm1, T v0, T vp1, T vp2, T vph) {
       delta = Cc * fmn(fab(deltaL), fab(deltaR));
       delta = sgn(deltaC) * fmn(delta, fab(deltaC));
     }
-    return T(0.5) * (v0 + vp1) - delta / T(6.0);
+    return T(0.5) * (v0 + vp1) - ppm_div<T, FAST>(delta, T(6.0));
   }
   return vph;
 }
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void ppm_states(T vm2, T vm1, T v0, T vp1, T vp2, T &vl_out,
                                            T &vr_out) {
   // getPPMStates, slopeLimiter.hpp:142-191
   const int n = 2;
   T vr = T(7.0) / T(12.0) * (v0 + vp1) - T(1.0) / T(12.0) * (vm1 + vp2);
   T vl = T(7.0) / T(12.0) * (vm1 + v0) - T(1.0) / T(12.0) * (vm2 + vp1);
-  vl = limit_ppm_face(vm2, vm1, v0, vp1, vl);
-  vr = limit_ppm_face(vm1, v0, vp1, vp2, vr);
+  vl = limit_ppm_face<T, FAST>(vm2, vm1, v0, vp1, vl);
+  vr = limit_ppm_face<T, FAST>(vm1, v0, vp1, vp2, vr);
   T d2qf = T(6.0) * (vl + vr - T(2.0) * v0);
   T d2qc0 = vm1 + vp1 - T(2.0) * v0;
   T d2qcp1 = v0 + vp2 - T(2.0) * vp1;
@@ -146,7 +153,7 @@ __device__ __forceinline__ void ppm_states(T vm2, T vm1, T v0, T vp1, T vp2, T &
   }
   T qmax = fmx(fmx(fab(vm1), fab(v0)), fab(vp1));
   T rho = 0;
-  if (fab(d2qf) > T(1e-12) * qmax) rho = d2q / d2qf;
+  if (fab(d2qf) > T(1e-12) * qmax) rho = ppm_div<T, FAST>(d2q, d2qf);
   T l = vl, r = vr;
   if (((vr - v0) * (v0 - vl) <= T(0)) || (vm1 - v0) * (v0 - vp1) <= T(0)) {
     if (rho <= (T(1.0) - T(1e-12))) {

@@ -628,7 +628,7 @@ void hlld_mhd_iso(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const 
                   const double *vR, double *flux, double &cmax, FaceEmf &E) {
   const int nv_ = V.nvar;
   const int comps = c.comps;
-  double uL[8], uR[8], fluxL[8], fluxR[8];
+  double uL[8] = {}, uR[8] = {}, fluxL[8] = {}, fluxR[8] = {};
   const double c2 = c2iso(c);
   double gpr = c2 * vL[V.RHO];
   double b1 = vL[D.BXn], b2 = at(vL, D.BXt), b3 = at(vL, D.BXb);
