@@ -57,15 +57,6 @@ __device__ __forceinline__ constexpr int mlp_var(int q) {
        : q == 3 ? C::template Xb<DIR>() : q == 4 ? C::PRS : q == 5 ? C::template BXn<DIR>()
        : q == 6 ? C::template BXt<DIR>() : C::template BXb<DIR>();
 }
-// Vc index -> face-frame variable q along DIR
-template <class C, int DIR>
-__device__ __forceinline__ constexpr int mlp_q(int n) {
-  int q = 0;
-  for (int k = 0; k < 8; k++)
-    if (mlp_var<C, DIR>(k) == n) q = k;
-  return q;
-}
-
 // PPM member of the family for one variable from the stencil s[0..5] = V[i-3..i+2]:
 // the left state is cell i-1's right value, the right state cell i's left value
 // (getPPMStates, slopeLimiter.hpp:142-191), with the van Leer fallback where rho or p
