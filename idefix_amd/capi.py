@@ -31,7 +31,7 @@ EXPORTS = [
     "idfx_cycle", "idfx_ctx_set_fusion", "idfx_ctx_set_profiling", "idfx_ctx_region_times",
     "idfx_mlp_param_count", "idfx_mlp_create", "idfx_mlp_destroy", "idfx_mlp_set_params",
     "idfx_mlp_get_params", "idfx_mlp_loss_grad", "idfx_mlp_train_step", "idfx_mlp_loss",
-    "idfx_ctx_set_mlp_params", "idfx_mlp_faces",
+    "idfx_ctx_set_mlp_params", "idfx_mlp_faces", "idfx_ctx_set_iso_sound_speed",
 ]
 
 REGIONS = ["boundaries", "sweep_x", "sweep_y", "sweep_z", "corner_emf", "evolve_b",
@@ -101,6 +101,7 @@ def load():
         "idfx_mlp_train_step": [vp, vp, vp, i, ctypes.c_float, vp],
         "idfx_mlp_loss": [vp, dp, vp],
         "idfx_ctx_set_mlp_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "idfx_ctx_set_iso_sound_speed": [vp, vp],
         "idfx_mlp_faces": [vp, i, vp, vp, vp],
     }
     for name, args in sigs.items():

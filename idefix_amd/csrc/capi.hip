@@ -387,6 +387,7 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   // the kernels read the equation of state's one constant from here: gamma, or the
   // isothermal sound speed
   c->g.gamma = d.isothermal ? d.csiso : d.gamma;
+  c->g.csiso = nullptr;
   int st = IDFX_OK;
   for (int q = 0; q < 3 && st == IDFX_OK; q++) {
     double *p = nullptr;
@@ -803,6 +804,12 @@ static unsigned short host_bf16(float f) {
   if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
   const unsigned rounding = ((u >> 16) & 1u) + 0x7fffu;  // round to nearest even
   return (unsigned short)((u + rounding) >> 16);
+}
+
+int idfx_ctx_set_iso_sound_speed(idfx_ctx *c, const double *cs) {
+  if (!c || !c->d.isothermal) return IDFX_E_ARGS;
+  c->g.csiso = cs;  // every launch takes GridDesc by value, so the next one sees it
+  return IDFX_OK;
 }
 
 int idfx_ctx_set_mlp_params(idfx_ctx *c, const float *p) {

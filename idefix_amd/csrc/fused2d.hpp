@@ -173,7 +173,8 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       vR[BX2] = vL[BX2];
       T F[NV], cmax;
       FaceEmf<T> E;
-      riemann<C, JDIR>(vL, vR, F, cmax, E, emf, (T)g.gamma);
+      riemann<C, JDIR>(vL, vR, F, cmax, E, emf,
+                       eos_face<C, T>(g, col + (unsigned)(f - 1) * sd, col + (unsigned)f * sd));
       if (!yface_col) {
         E.Et = 0;
         E.sv = 0;

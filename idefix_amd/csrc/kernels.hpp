@@ -29,6 +29,9 @@ struct GridDesc {
   const double *dx[3];
   double gamma;  // adiabatic index, or the isothermal sound speed (isothermal configurations)
   int ntr;      // passive tracers, stored after the fluid variables of Vc/Uc
+  // isothermal configurations with a per-cell sound speed ("csiso userdef",
+  // eos_isothermal.hpp:29-33, :70-75): the caller's [np_tot cells] array; else nullptr
+  const double *csiso;
 };
 
 template <typename T>
@@ -81,6 +84,19 @@ __device__ __forceinline__ T ld32(const T *base, unsigned i) {
 template <typename T>
 __device__ __forceinline__ void st32(T *base, unsigned i, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (unsigned)(i * (unsigned)sizeof(T))) = v;
+}
+
+// The equation-of-state constant a Riemann solver takes at the face between cells pl and
+// pr: gamma, or in isothermal configurations the face sound speed, the average of the two
+// cells' values (hllcHD.hpp:67-70, hlldMHD.hpp:188-191); a constant cs gives
+// 0.5 (cs + cs) = cs.
+template <class C, typename T>
+__device__ __forceinline__ T eos_face(const GridDesc &g, unsigned pl, unsigned pr) {
+  if constexpr (C::ISO) {
+    if (g.csiso != nullptr)
+      return (T)(0.5 * (ld32(g.csiso, pr) + ld32(g.csiso, pl)));
+  }
+  return (T)g.gamma;
 }
 
 __device__ __forceinline__ long long idx3(const GridDesc &g, int k, int j, int i) {
@@ -429,7 +445,8 @@ void k_sweep_x(GridDesc g, SweepArgs a, EmfArrays<T> e,
     }
     T F[NV], cmax;
     FaceEmf<T> E;
-    const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, EMF, (T)g.gamma);
+    const bool wrote = riemann<C, IDIR>(vL, vR, F, cmax, E, EMF,
+                                        eos_face<C, T>(g, row + (unsigned)(cc - 1), row + (unsigned)cc));
     const bool face_ok = lane >= 1 && c <= g.end[0];
     const bool own_face = face_ok && (lane <= 62 || c == g.end[0]);
     const unsigned p = row + (unsigned)c;
@@ -594,7 +611,8 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
       }
       T F[NV], cmax;
       FaceEmf<T> E;
-      const bool wrote = riemann<C, DIR>(vL, vR, F, cmax, E, EMF, (T)g.gamma);
+      const bool wrote = riemann<C, DIR>(vL, vR, F, cmax, E, EMF,
+                                         eos_face<C, T>(g, col + (unsigned)(f - 1) * sd, col + (unsigned)f * sd));
       const unsigned p = col + (unsigned)f * sd;
       const bool own_face = lane_ok && (f < c1 || c1 == g.end[DIR]);
       if (own_face) store_face_emf<C, DIR>(e, p, EMF, E);
@@ -939,10 +957,11 @@ __global__ __launch_bounds__(256) void k_find_shock(GridDesc g, const double *Vc
   const int k = g.beg[2] - ko + t / ((long long)ni * nj);
   const long long p = idx3(g, k, j, i);
   const long long sx = 1, sy = g.nt[0], sz = (long long)g.nt[0] * g.nt[1];
-  // pressure: PRS, or rho cs^2 in isothermal configurations (g.gamma holds cs there)
+  // pressure: PRS, or rho cs^2 in isothermal configurations with the cell's own sound
+  // speed (g.gamma holds a constant one)
   auto P = [&](long long q) {
     if constexpr (C::ISO) {
-      const double cs = g.gamma;
+      const double cs = g.csiso != nullptr ? g.csiso[q] : g.gamma;
       return Vc[C::RHO * g.sv + q] * cs * cs;
     } else {
       return Vc[C::ENG * g.sv + q];

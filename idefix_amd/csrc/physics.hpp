@@ -1210,12 +1210,10 @@ __device__ __forceinline__ void tvdlf_mhd(const T *vL, const T *vR, T *flux, T &
 }
 
 // ====================================================================================
-// Isothermal branches of the solvers (HAVE_ENERGY == 0, constant csiso): the sound speed
-// of a face is the average of its two cells' (here equal) speeds, Cs2Iso its square; the
-// argument `cs` takes the place of gamma.
+// Isothermal branches of the solvers (HAVE_ENERGY == 0): the argument `cs` takes the
+// place of gamma and is the sound speed of the face, the average of its two cells' speeds
+// (computed by the caller, eos_face); Cs2Iso is its square.
 // ====================================================================================
-template <typename T>
-__device__ __forceinline__ T face_cs(T cs) { return T(0.5) * (cs + cs); }
 
 // HllHD (hllHD.hpp:67-70) and TvdlfHD (tvdlfHD.hpp:72-75), isothermal
 template <class C, int DIR, typename T>
@@ -1223,7 +1221,7 @@ __device__ __forceinline__ void hll_hd_iso(const T *vL, const T *vR, T *flux, T 
   constexpr int NV = C::NVAR;
   constexpr int Xn = C::template Xn<DIR>();
   T uL[NV], uR[NV], fL[NV], fR[NV];
-  const T cL = face_cs(cs), cR = cL;
+  const T cL = cs, cR = cL;
   const T SL = fmn(vL[Xn] - cL, vR[Xn] - cR);
   const T SR = fmx(vL[Xn] + cL, vR[Xn] + cR);
   cmax = fmx(fab(SL), fab(SR));
@@ -1249,7 +1247,7 @@ __device__ __forceinline__ void tvdlf_hd_iso(const T *vL, const T *vR, T *flux, 
   constexpr int NV = C::NVAR;
   constexpr int Xn = C::template Xn<DIR>();
   T uL[NV], uR[NV], fL[NV], fR[NV];
-  const T cRL = face_cs(cs);
+  const T cRL = cs;
   const T vn = T(0.5) * (vL[Xn] + vR[Xn]);
   cmax = fmx(fab(vn + cRL), fab(vn - cRL));
   prim2cons<C>(vL, uL, cs);
@@ -1268,7 +1266,7 @@ __device__ __forceinline__ void hllc_hd_iso(const T *vL, const T *vR, T *flux, T
   constexpr int Xt = C::template Xt<DIR>();
   constexpr int Xb = C::template Xb<DIR>();
   T uL[NV], uR[NV], fL[NV], fR[NV];
-  const T cL = face_cs(cs), cR = cL;
+  const T cL = cs, cR = cL;
   const T SL = fmn(vL[Xn] - cL, vR[Xn] - cR);
   const T SR = fmx(vL[Xn] + cL, vR[Xn] + cR);
   cmax = fmx(fab(SL), fab(SR));
@@ -1322,7 +1320,7 @@ __device__ __forceinline__ bool roe_hd_iso(const T *vL, const T *vR, T *flux, T 
   T lambda[NV], alambda[NV], eta[NV];
 #pragma unroll
   for (int n = 0; n < NV; n++) dv[n] = vR[n] - vL[n];
-  T a2L = face_cs(cs);
+  T a2L = cs;
   T a2R = a2L;
   a2L = a2L * a2L;
   a2R = a2R * a2R;
@@ -1424,7 +1422,7 @@ __device__ __forceinline__ void hll_mhd_iso(const T *vL, const T *vR, T *flux, T
   constexpr int NV = C::NVAR;
   constexpr int Xn = C::template Xn<DIR>();
   T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
-  const T c2 = face_cs(cs) * face_cs(cs);
+  const T c2 = cs * cs;
   const T cL = fast_speed_iso<C, DIR>(vL, c2);
   const T cR = fast_speed_iso<C, DIR>(vR, c2);
   const T sl = fmn(vL[Xn] - cL, vR[Xn] - cR);
@@ -1463,7 +1461,7 @@ __device__ __forceinline__ void tvdlf_mhd_iso(const T *vL, const T *vR, T *flux,
   constexpr int CO = C::COMP;
   constexpr int Xn = C::template Xn<DIR>();
   T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
-  const T c2 = face_cs(cs) * face_cs(cs);
+  const T c2 = cs * cs;
   const T rho = T(0.5) * (vL[C::RHO] + vR[C::RHO]);
   const T gpr = c2 * rho;
   const T bn = vL[C::template BXn<DIR>()];
@@ -1509,7 +1507,7 @@ __device__ __forceinline__ void hlld_mhd_iso(const T *vL, const T *vR, T *flux, 
   constexpr int BXt = C::template BXt<DIR>();
   constexpr int BXb = C::template BXb<DIR>();
   T uL[NV], uR[NV], fluxL[NV], fluxR[NV];
-  const T c2 = face_cs(cs) * face_cs(cs);
+  const T c2 = cs * cs;
   const T cL = fast_speed_iso<C, DIR>(vL, c2);
   const T cR = fast_speed_iso<C, DIR>(vR, c2);
   const T sl = fmn(vL[Xn] - cL, vR[Xn] - cR);
@@ -1628,7 +1626,7 @@ __device__ __forceinline__ void roe_mhd_iso(const T *vL, const T *vR, T *flux, T
   for (int n = 0; n < NV; n++) dV[n] = vR[n] - vL[n];
   prim2cons<C>(vL, uL, csiso);
   prim2cons<C>(vR, uR, csiso);
-  T a2L = face_cs(csiso);
+  T a2L = csiso;
   a2L = a2L * a2L;
   const T a2R = a2L;
   kflux<C, DIR>(fluxL, vL, uL, a2L);

@@ -216,9 +216,27 @@ class DataBlock:
                    "stage")
         return newdt.value
 
+    def set_iso_sound_speed(self, cs):
+        """Per-cell isothermal sound speed ("csiso userdef"): an array of shape
+        (nz, ny, nx) over all cells, ghosts included (numpy or torch), kept on the device
+        and read by every sweep; None returns to the constant csiso."""
+        if cs is None:
+            self._cs = None
+            capi.check(self._L.idfx_ctx_set_iso_sound_speed(self.ctx, None),
+                       "set_iso_sound_speed")
+            return
+        t = self.torch.as_tensor(np.asarray(cs) if not self.torch.is_tensor(cs) else cs,
+                                 dtype=self.torch.float64, device=self.device).contiguous()
+        nx, ny, nz = self.np_tot
+        if tuple(t.shape) != (nz, ny, nx):
+            raise ValueError("sound speed array must have shape %s" % ((nz, ny, nx),))
+        self._cs = t
+        capi.check(self._L.idfx_ctx_set_iso_sound_speed(self.ctx, ctypes.c_void_p(t.data_ptr())),
+                   "set_iso_sound_speed")
+
     def set_mlp_params(self, params):
         """Loads packed FP32 learned-limiter parameters (numpy, idfx_mlp_param_count()
-        floats) for BF16 inference, or unloads them with None (van Leer weights)."""
+        floats) for BF16 inference, or unloads them with None (PPM weights)."""
         if params is None:
             capi.check(self._L.idfx_ctx_set_mlp_params(self.ctx, None), "set_mlp_params")
             return

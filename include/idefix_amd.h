@@ -213,6 +213,16 @@ enum {
  * when on, every region of idfx_cycle / idfx_stage is bracketed by a HIP event pair on the
  * launch stream.  idfx_ctx_region_times waits for the recorded regions, returns the total
  * milliseconds and launch count of the first n regions, and resets the totals. */
+/* EquationOfState with "csiso userdef" (eos_isothermal.hpp:29-33 and GetWaveSpeed at
+ * :70-75; filled by the function of Fluid::EnrollIsoSoundSpeed, enroll.hpp:13-20, at
+ * EquationOfState::Refresh): cs is a device array of np_tot cells ([k][j][i], ghosts
+ * included) holding every cell's sound speed.  It is borrowed, not copied: the sweeps
+ * read it at each launch (face value: the average of the two cells', hllcHD.hpp:67-70,
+ * hlldMHD.hpp:188-191; shock flags: each cell's own, shockFlattening.hpp:96-128), so a
+ * caller refreshes it before a stage as the reference does.  NULL returns to the
+ * constant csiso of the description.  IDFX_E_ARGS for a context that is not isothermal. */
+int idfx_ctx_set_iso_sound_speed(idfx_ctx *ctx, const double *cs);
+
 int idfx_ctx_set_profiling(idfx_ctx *ctx, int on);
 int idfx_ctx_region_times(idfx_ctx *ctx, double *ms, long *count, int n);
 

@@ -1452,6 +1452,9 @@ struct Oracle {
   int np_tot[3], np_int[3], nghost[3], beg[3], end[3];
   int ntot;  // fluid variables + passive tracers
   std::vector<unsigned char> shock;  // FlagShock per cell (shockFlattening.hpp:14)
+  // isothermal sound speed per cell ("csiso userdef", eos_isothermal.hpp:29-33, :70-75);
+  // empty: the constant c.csiso
+  std::vector<double> cs_cell;
   int offs[3];  // IOFFSET, JOFFSET, KOFFSET (macros.hpp:31-53)
   int vsn[3];   // Vs extents (i, j, k)
   std::vector<double> x[3], xl[3], xr[3], dx[3];
@@ -1800,7 +1803,7 @@ struct Oracle {
             // pressure of a cell: PRS, or rho cs^2 when isothermal (shockFlattening.hpp:92-128)
             auto P = [&](int kk, int jj, int ii) {
               if (c.isothermal) {
-                const double cs = c.csiso;
+                const double cs = cs_cell.empty() ? c.csiso : cs_cell[i3(kk, jj, ii)];
                 return vc(V.RHO, kk, jj, ii) * cs * cs;
               }
               return vc(V.ENG, kk, jj, ii);
@@ -1931,22 +1934,32 @@ struct Oracle {
           FaceEmf E;
           E.hasEt = E.hasEb = E.hasSv = E.hasAD = false;
           extrapolate(dir, k, j, i, vL, vR);
+          // a per-cell sound speed enters as the face average of the two cells'
+          // (hllcHD.hpp:67-70, hlldMHD.hpp:188-191): a configuration whose constant is
+          // that average, since every solver takes 0.5 (cs + cs) = cs of it
+          orc_cfg cface;
+          const orc_cfg *cf = &c;
+          if (!cs_cell.empty()) {
+            cface = c;
+            cface.csiso = 0.5 * (cs_cell[i3(k, j, i)] + cs_cell[i3(k - ko, j - jo, i - io)]);
+            cf = &cface;
+          }
           bool wrote = true;
           if (c.mhd) {
             vL[D.BXn] = vs(dir, k, j, i);
             vR[D.BXn] = vL[D.BXn];
-            if (c.solver == ORC_SOLVER_ROE) roe_mhd(c, V, D, dir, vL, vR, F, cmax, E);
-            else if (c.solver == ORC_SOLVER_HLL) hll_mhd(c, V, D, dir, vL, vR, F, cmax, E);
-            else if (c.solver == ORC_SOLVER_TVDLF) tvdlf_mhd(c, V, D, dir, vL, vR, F, cmax, E);
-            else hlld_mhd(c, V, D, dir, vL, vR, F, cmax, E);
+            if (c.solver == ORC_SOLVER_ROE) roe_mhd(*cf, V, D, dir, vL, vR, F, cmax, E);
+            else if (c.solver == ORC_SOLVER_HLL) hll_mhd(*cf, V, D, dir, vL, vR, F, cmax, E);
+            else if (c.solver == ORC_SOLVER_TVDLF) tvdlf_mhd(*cf, V, D, dir, vL, vR, F, cmax, E);
+            else hlld_mhd(*cf, V, D, dir, vL, vR, F, cmax, E);
           } else if (c.solver == ORC_SOLVER_HLLC) {
-            hllc_hd(c, V, D, vL, vR, F, cmax);
+            hllc_hd(*cf, V, D, vL, vR, F, cmax);
           } else if (c.solver == ORC_SOLVER_HLL) {
-            hll_hd(c, V, D, vL, vR, F, cmax);
+            hll_hd(*cf, V, D, vL, vR, F, cmax);
           } else if (c.solver == ORC_SOLVER_TVDLF) {
-            tvdlf_hd(c, V, D, vL, vR, F, cmax);
+            tvdlf_hd(*cf, V, D, vL, vR, F, cmax);
           } else {
-            wrote = roe_hd(c, V, D, vL, vR, F, cmax);
+            wrote = roe_hd(*cf, V, D, vL, vR, F, cmax);
           }
           if (wrote) {
             for (int n = 0; n < V.nvar; n++) Flux[i4(n, k, j, i)] = F[n];
@@ -2464,6 +2477,16 @@ double *orc_coord(void *p, int dir, int kind) {
   }
 }
 void orc_set_boundaries(void *p) { static_cast<Oracle *>(p)->set_boundaries(); }
+int orc_set_iso_sound_speed(void *p, const double *cs) {
+  Oracle *o = static_cast<Oracle *>(p);
+  if (!o->c.isothermal) return 1;
+  if (!cs) {
+    o->cs_cell.clear();
+    return 0;
+  }
+  o->cs_cell.assign(cs, cs + (size_t)o->np_tot[0] * o->np_tot[1] * o->np_tot[2]);
+  return 0;
+}
 void orc_prim_to_cons(void *p) { static_cast<Oracle *>(p)->prim_to_cons(); }
 void orc_cons_to_prim(void *p) { static_cast<Oracle *>(p)->cons_to_prim(); }
 void orc_reset_stage(void *p) { static_cast<Oracle *>(p)->reset_stage(); }

@@ -72,6 +72,9 @@ double *orc_coord(void *o, int dir, int kind); /* kind 0:x 1:xl 2:xr 3:dx */
 
 /* Individual steps of the stage (same granularity as the reference kernels) */
 void orc_set_boundaries(void *o);
+/* per-cell isothermal sound speed (np_tot cells, copied), NULL: the constant csiso;
+ * returns nonzero for a configuration that is not isothermal */
+int orc_set_iso_sound_speed(void *o, const double *cs);
 void orc_prim_to_cons(void *o);
 void orc_cons_to_prim(void *o);
 void orc_reset_stage(void *o);

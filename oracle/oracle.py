@@ -60,6 +60,7 @@ def lib():
             "orc_layout": (None, [vp, ip, ip, ip, ip]),
             "orc_array": (dp, [vp, i, lp]), "orc_coord": (dp, [vp, i, i]),
             "orc_set_boundaries": (None, [vp]), "orc_prim_to_cons": (None, [vp]),
+            "orc_set_iso_sound_speed": (i, [vp, dp]),
             "orc_cons_to_prim": (None, [vp]), "orc_reset_stage": (None, [vp]),
             "orc_calc_flux": (None, [vp, i]), "orc_calc_rhs": (None, [vp, i, d]),
             "orc_find_shock": (None, [vp, vp]),
@@ -222,6 +223,19 @@ class Oracle:
 
     def copy_begin(self):
         lib().orc_copy_begin(self._h)
+
+    def set_iso_sound_speed(self, cs):
+        """Per-cell isothermal sound speed ([k][j][i] over the whole array, ghosts
+        included), or None for the constant csiso of the configuration."""
+        if cs is None:
+            ptr = None
+        else:
+            self._cs = np.ascontiguousarray(cs, dtype=np.float64)
+            nx, ny, nz = self.np_tot
+            assert self._cs.size == nx * ny * nz
+            ptr = self._cs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if lib().orc_set_iso_sound_speed(self._h, ptr) != 0:
+            raise ValueError("a per-cell sound speed needs an isothermal configuration")
 
     def cycle(self):
         return lib().orc_cycle(self._h)

@@ -568,3 +568,43 @@ def test_turbulence_3d_ppm_rk3():
     o.set_boundaries()
     _cycle_both(o, g, 3)
     compare_state(o, g, what="turb3d ppm rk3")
+
+
+def _cs_field(o, amp=0.3):
+    """A smooth positive per-cell sound speed over every cell of the array."""
+    nx, ny, nz = o.np_tot
+    x = o.coord(0, "x")[None, None, :]
+    y = o.coord(1, "x")[None, :, None] if o.dim >= 2 else 0.0
+    return (1.0 + amp * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)) * np.ones((nz, ny, nx))
+
+
+@pytest.mark.parametrize("dim,nx,mhd,solver,recon,emf,flat", [
+    (1, (200,), False, "hllc", "plm", "uct_contact", None),
+    (1, (200,), False, "roe", "ppm", "uct_contact", None),
+    (1, (200,), True, "hll", "plm", "uct_contact", None),
+    (1, (200,), True, "roe", "limo3", "uct_contact", None),
+    (2, (40, 24), False, "hllc", "plm", "uct_contact", 0.05),
+    (2, (48, 40), True, "hlld", "plm", "uct_hlld", None),
+    (2, (48, 40), True, "roe", "plm", "uct_contact", None),
+])
+def test_iso_cell_sound_speed(dim, nx, mhd, solver, recon, emf, flat):
+    """"csiso userdef": a spatially varying sound speed held per cell, the same array in
+    the oracle and on the GPU (face value = average of the two cells; shock flags with
+    each cell's own)."""
+    kw = cfg_kwargs(dim, nx, mhd=mhd, solver=solver, recon=recon, emf=emf, cfl=0.5,
+                    comps=2 if mhd else dim, csiso=1.0, shock_flattening=flat,
+                    bc="periodic" if dim == 2 else "outflow")
+    ic = (lambda gg: problems.random_mhd(gg, seed=41, amp=0.2)) if mhd else \
+        (lambda gg: problems.random_hd(gg, seed=42, amp=0.3))
+    o, g = make_pair(kw, ic)
+    cs = _cs_field(o)
+    o.set_iso_sound_speed(cs)
+    g.set_iso_sound_speed(cs)
+    o.set_boundaries()
+    _cycle_both(o, g, 5)
+    compare_state(o, g, what="iso cs(x) %dD %s %s" % (dim, solver, recon))
+    # back to the constant sound speed: both follow
+    o.set_iso_sound_speed(None)
+    g.set_iso_sound_speed(None)
+    _cycle_both(o, g, 2)
+    compare_state(o, g, what="iso cs constant again %dD %s" % (dim, solver))

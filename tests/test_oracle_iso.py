@@ -178,3 +178,92 @@ def test_iso_brio_wu_periodic_conservation():
     o.prim_to_cons()
     t1 = o.interior(o.Uc).sum(axis=(1, 2, 3))
     assert np.all(np.abs(t1 - t0) <= 1e-12 * np.maximum(np.abs(t0), 400.0))
+
+
+# ---- per-cell sound speed ("csiso userdef", eos_isothermal.hpp:29-33, :70-75) ---------
+
+def _cs_field(o, amp=0.3):
+    """A smooth positive sound speed over every cell of the array."""
+    nx, ny, nz = o.np_tot
+    x = o.coord(0, "x")[None, None, :]
+    y = o.coord(1, "x")[None, :, None] if o.dim >= 2 else 0.0
+    return (1.0 + amp * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)) * np.ones((nz, ny, nx))
+
+
+@pytest.mark.parametrize("mhd,solver", [(False, "hllc"), (False, "roe"), (False, "tvdlf"),
+                                        (True, "hlld"), (True, "roe"), (True, "hll")])
+def test_uniform_cell_sound_speed_is_the_constant_one(mhd, solver):
+    """An array holding csiso everywhere reproduces the constant path bit for bit."""
+    kw = dict(mhd=mhd, solver=solver, nx=(24, 20), comps=2, cfl=0.5, csiso=0.9,
+              emf="uct_contact")
+    runs = []
+    for use_array in (False, True):
+        o = O.Oracle(O.make_cfg(2, **kw))
+        Vc, Vs = (problems.random_mhd if mhd else problems.random_hd)(O.OracleGrid(o), seed=4)
+        o.Vc[...] = Vc
+        if mhd:
+            o.Vs[...] = Vs
+        if use_array:
+            nx, ny, nz = o.np_tot
+            o.set_iso_sound_speed(np.full((nz, ny, nx), 0.9))
+        o.set_boundaries()
+        for _ in range(3):
+            assert o.cycle() == 0
+        runs.append(o.Vc.copy())
+    assert np.array_equal(runs[0], runs[1])
+
+
+@pytest.mark.parametrize("mhd,solver", [(False, "hllc"), (False, "hll"), (False, "tvdlf"),
+                                        (False, "roe"), (True, "hlld"), (True, "hll"),
+                                        (True, "tvdlf"), (True, "roe")])
+def test_face_sound_speed_is_the_average_of_the_two_cells(mhd, solver):
+    """Gas at rest with uniform density (and field): every solver returns the physical
+    flux, whose normal momentum is rho c^2 (+ B^2/2) with c the face average
+    (cs_left + cs_right)/2 -- not either cell's speed, nor the mean of the squares."""
+    o = O.Oracle(O.make_cfg(2, mhd=mhd, solver=solver, nx=(16, 12), comps=2, csiso=1.0,
+                            emf="uct_contact"))
+    cs = _cs_field(o)
+    o.set_iso_sound_speed(cs)
+    Vc = np.zeros_like(o.Vc)
+    Vc[0] = 1.3
+    b2 = 0.0
+    if mhd:
+        Vc[3], Vc[4] = 0.2, -0.1  # BX1, BX2
+        o.Vs[0] = 0.2
+        o.Vs[1] = -0.1
+        b2 = 0.2 ** 2 + 0.1 ** 2
+    o.Vc[...] = Vc
+    o.set_boundaries()
+    for d in (0, 1):
+        o.calc_flux(d)
+        F = o.array("Flux")
+        b, e = o.beg, o.end
+        sl = (slice(b[2], e[2]), slice(b[1], e[1]), slice(b[0], e[0]))
+        shift = [0, 0, 0]
+        shift[2 - d] = 1
+        left = cs[tuple(slice(s.start - sh, s.stop - sh) for s, sh in zip(sl, shift))]
+        cface = 0.5 * (cs[sl] + left)
+        bn2 = (0.2 if d == 0 else -0.1) ** 2
+        expect = 1.3 * cface * cface + 0.5 * b2 - (bn2 if mhd else 0.0)
+        assert np.allclose(F[1 + d][sl], expect, rtol=1e-14, atol=1e-14), (solver, d)
+
+
+def test_shock_flags_use_each_cells_sound_speed():
+    """A converging flow of uniform density and a sound speed that jumps by 1.5x inside
+    the converging region: only the two cells beside the jump, whose neighbours' pressures
+    rho cs^2 differ by 125 %, exceed the smoothing threshold (shockFlattening.hpp:96-128);
+    with a uniform sound speed no cell is flagged."""
+    cfg = O.make_cfg(1, mhd=False, solver="hllc", nx=(64,), bc="periodic", cfl=0.5,
+                     csiso=1.0, shock_flattening=0.1)
+    o = O.Oracle(cfg)
+    x = o.coord(0, "x")
+    Vc = np.zeros_like(o.Vc)
+    Vc[0] = 1.0
+    Vc[1] = -0.5 * np.sin(2 * np.pi * x)[None, None, :]  # div v < 0 for x < 1/4
+    o.Vc[...] = Vc
+    o.set_boundaries()
+    assert not o.find_shock().any()
+    o.set_iso_sound_speed(np.where(x > 0.125, 1.5, 1.0)[None, None, :])
+    flagged = np.nonzero(o.find_shock()[0, 0])[0]
+    jump = np.nonzero(np.diff(x > 0.125))[0][0]  # cells jump, jump + 1 straddle it
+    assert list(flagged) == [jump, jump + 1]
