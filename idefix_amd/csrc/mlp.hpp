@@ -93,23 +93,18 @@ __device__ __forceinline__ bool mlp_features6(const float *S, float *x) {
   return fin;
 }
 
-// TVD-region check of one variable's weights (0 <= w <= 1, each state's deviation from
-// its cell value has the sign of the cell's other one-sided difference and does not
-// exceed it); the PPM weights replace them where it fails or when use_net is false.
-__device__ __forceinline__ void mlp_check(const float *st, bool use_net, bool positive,
-                                          float &wl, float &wr) {
-  bool good = use_net;
-  if (good) {
-    const float a = st[1], b = st[2], c = st[3], d = st[4];
-    const float jump = c - b;
-    const float dl = wl * jump, dr = wr * jump;
-    const bool okl = wl >= 0.0f && wl <= 1.0f &&
-                     (dl == 0.0f || (dl * (b - a) > 0.0f && fabsf(dl) <= fabsf(b - a)));
-    const bool okr = wr >= 0.0f && wr <= 1.0f &&
-                     (dr == 0.0f || (dr * (d - c) > 0.0f && fabsf(dr) <= fabsf(d - c)));
-    good = (okl && okr) || (jump == 0.0f && wl == wl && wr == wr);  // NaN fails
-  }
-  if (!good) mlp_ppm(st, positive, wl, wr);
+// TVD-region check of one variable's weights: 0 <= w <= 1, and each state's deviation
+// from its cell value has the sign of the cell's other one-sided difference and does not
+// exceed it.  False also for NaN weights.
+__device__ __forceinline__ bool mlp_tvd_ok(const float *st, float wl, float wr) {
+  const float a = st[1], b = st[2], c = st[3], d = st[4];
+  const float jump = c - b;
+  const float dl = wl * jump, dr = wr * jump;
+  const bool okl = wl >= 0.0f && wl <= 1.0f &&
+                   (dl == 0.0f || (dl * (b - a) > 0.0f && fabsf(dl) <= fabsf(b - a)));
+  const bool okr = wr >= 0.0f && wr <= 1.0f &&
+                   (dr == 0.0f || (dr * (d - c) > 0.0f && fabsf(dr) <= fabsf(d - c)));
+  return (okl && okr) || (jump == 0.0f && wl == wl && wr == wr);
 }
 
 // Interface weights W[q][face] (SoA, face index = index of the cell right of the face)
@@ -202,8 +197,27 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
       wbr = mlp_sigmoid(acc[3] + net.b3[4 * g4 + 3]);
     }
     const bool use_net = fin && net.valid;
-    mlp_check(sa, use_net, pos_a, wal, war);
-    mlp_check(sb, use_net, false, wbl, wbr);
+    const bool need_a = !(use_net && mlp_tvd_ok(sa, wal, war));
+    const bool need_b = !(use_net && mlp_tvd_ok(sb, wbl, wbr));
+    // PPM weights where the check fails.  The first pass serves every lane that needs
+    // them for one of its variables (a if it fails, else b), the second only lanes that
+    // need both, so a wave whose failures are spread over the two variables runs the
+    // long fallback once rather than twice.
+    if (need_a || need_b) {
+      float sx[6];
+#pragma unroll
+      for (int t = 0; t < 6; t++) sx[t] = need_a ? sa[t] : sb[t];
+      float l, r;
+      mlp_ppm(sx, need_a && pos_a, l, r);
+      if (need_a) {
+        wal = l;
+        war = r;
+      } else {
+        wbl = l;
+        wbr = r;
+      }
+    }
+    if (need_a && need_b) mlp_ppm(sb, false, wbl, wbr);
     if (face_ok) {
       W[(4 * g4) * g.sv + pf] = wal;
       W[(4 * g4 + 1) * g.sv + pf] = war;
