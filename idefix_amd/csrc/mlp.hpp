@@ -57,16 +57,12 @@ __device__ __forceinline__ constexpr int mlp_var(int q) {
        : q == 3 ? C::template Xb<DIR>() : q == 4 ? C::PRS : q == 5 ? C::template BXn<DIR>()
        : q == 6 ? C::template BXt<DIR>() : C::template BXb<DIR>();
 }
-// PPM member of the family for one variable from the stencil s[0..5] = V[i-3..i+2]:
-// the left state is cell i-1's right value, the right state cell i's left value
+// PPM member of the family at one face from the stencil s[0..5] = V[i-3..i+2] and the
+// PPM values around the face: left = cell i-1's right value, right = cell i's left value
 // (getPPMStates, slopeLimiter.hpp:142-191), with the van Leer fallback where rho or p
 // would not be positive (extrapolateToFaces.hpp:260-304); w = 0 where V[i] = V[i-1].
-// The limiter's quotients use v_rcp_f32 (1 ulp); a non-finite one only skips a limiter
-// step, since every comparison with NaN or inf there keeps the unlimited value.
-__device__ __forceinline__ void mlp_ppm(const float *s, bool positive, float &wl, float &wr) {
-  float vl_c, left, right, vr_d;
-  ppm_states<float, true>(s[0], s[1], s[2], s[3], s[4], vl_c, left);
-  ppm_states<float, true>(s[1], s[2], s[3], s[4], s[5], right, vr_d);
+__device__ __forceinline__ void mlp_ppm_weights(const float *s, bool positive, float left,
+                                                float right, float &wl, float &wr) {
   if (positive) {
     if (left <= 0.0f) left = s[2] + 0.5f * vanleer(s[3] - s[2], s[2] - s[1]);
     if (right <= 0.0f) right = s[3] - 0.5f * vanleer(s[4] - s[3], s[3] - s[2]);
@@ -74,6 +70,14 @@ __device__ __forceinline__ void mlp_ppm(const float *s, bool positive, float &wl
   const float jump = s[3] - s[2];
   wl = jump != 0.0f ? (left - s[2]) / jump : 0.0f;
   wr = jump != 0.0f ? (s[3] - right) / jump : 0.0f;
+}
+
+// The same from the stencil alone: the PPM values of cells i-1 (s[0..4]) and i (s[1..5]).
+__device__ __forceinline__ void mlp_ppm(const float *s, bool positive, float &wl, float &wr) {
+  float unused, left, right;
+  ppm_states<float, true>(s[0], s[1], s[2], s[3], s[4], unused, left);
+  ppm_states<float, true>(s[1], s[2], s[3], s[4], s[5], right, unused);
+  mlp_ppm_weights(s, positive, left, right, wl, wr);
 }
 
 // Features of one variable at one interface from its stencil S[0..5] = V[i-3..i+2]:
@@ -134,6 +138,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
   const int na = mlp_var<C, DIR>(2 * g4), nb = mlp_var<C, DIR>(2 * g4 + 1);
   const bool pos_a = g4 == 0 || g4 == 2;  // rho (q = 0) and p (q = 4) stay positive
   const double *Va = Vc + na * g.sv, *Vb = Vc + nb * g.sv;
+  float carry_a = 0.0f, carry_b = 0.0f;  // PPM right value of the cell left of the tile
 
 #pragma unroll 1
   for (int nt = 0; nt < 4; nt++) {
@@ -199,25 +204,61 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
     const bool use_net = fin && net.valid;
     const bool need_a = !(use_net && mlp_tvd_ok(sa, wal, war));
     const bool need_b = !(use_net && mlp_tvd_ok(sb, wbl, wbr));
-    // PPM weights where the check fails.  The first pass serves every lane that needs
-    // them for one of its variables (a if it fails, else b), the second only lanes that
-    // need both, so a wave whose failures are spread over the two variables runs the
-    // long fallback once rather than twice.
-    if (need_a || need_b) {
-      float sx[6];
-#pragma unroll
-      for (int t = 0; t < 6; t++) sx[t] = need_a ? sa[t] : sb[t];
-      float l, r;
-      mlp_ppm(sx, need_a && pos_a, l, r);
-      if (need_a) {
-        wal = l;
-        war = r;
-      } else {
-        wbl = l;
-        wbr = r;
+    // PPM weights where the check fails.  The limiter's quotients use v_rcp_f32 (1 ulp);
+    // a non-finite one only skips a limiter step, since every comparison with NaN or inf
+    // there keeps the unlimited value.
+    if constexpr (DIR == IDIR) {
+      // x faces: the lanes of a group hold consecutive faces along x, so the PPM values
+      // of a cell serve both of its faces: every lane computes them for its own cell
+      // (right of its face, s[1..5]) and hands the right value to the next lane, whose
+      // face it bounds; the tile's first face takes it from the previous tile's last
+      // lane (tile 0: its lane computes it from s[0..4])
+      if (nt == 0 && rr == 0) {
+        float unused;
+        ppm_states<float, true>(sa[0], sa[1], sa[2], sa[3], sa[4], unused, carry_a);
+        ppm_states<float, true>(sb[0], sb[1], sb[2], sb[3], sb[4], unused, carry_b);
       }
+      float la, ra, lb, rb;
+      ppm_states<float, true>(sa[1], sa[2], sa[3], sa[4], sa[5], la, ra);
+      ppm_states<float, true>(sb[1], sb[2], sb[3], sb[4], sb[5], lb, rb);
+      float pa = __shfl_up(ra, 1, 16), pb = __shfl_up(rb, 1, 16);
+      if (rr == 0) {
+        pa = carry_a;
+        pb = carry_b;
+      }
+      carry_a = __shfl(ra, 16 * g4 + 15);
+      carry_b = __shfl(rb, 16 * g4 + 15);
+      float ppl, ppr;
+      mlp_ppm_weights(sa, pos_a, pa, la, ppl, ppr);
+      if (need_a) {
+        wal = ppl;
+        war = ppr;
+      }
+      mlp_ppm_weights(sb, false, pb, lb, ppl, ppr);
+      if (need_b) {
+        wbl = ppl;
+        wbr = ppr;
+      }
+    } else {
+      // y and z faces: neighbouring lanes are neighbours across the faces, not along
+      // them.  One pass serves every lane that needs the fallback for one of its
+      // variables (a if it fails, else b), a second only lanes that need both.
+      if (need_a || need_b) {
+        float sx[6];
+#pragma unroll
+        for (int t = 0; t < 6; t++) sx[t] = need_a ? sa[t] : sb[t];
+        float l, r;
+        mlp_ppm(sx, need_a && pos_a, l, r);
+        if (need_a) {
+          wal = l;
+          war = r;
+        } else {
+          wbl = l;
+          wbr = r;
+        }
+      }
+      if (need_a && need_b) mlp_ppm(sb, false, wbl, wbr);
     }
-    if (need_a && need_b) mlp_ppm(sb, false, wbl, wbr);
     if (face_ok) {
       W[(4 * g4) * g.sv + pf] = wal;
       W[(4 * g4 + 1) * g.sv + pf] = war;
