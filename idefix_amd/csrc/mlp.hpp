@@ -20,7 +20,9 @@
 // (columns = interfaces).  The 16x16 accumulator tile a lane holds (rows 4(l/16)+t,
 // column l%16) is exactly the B-operand fragment of the next layer's K-step, so the
 // three layers chain in registers; the lane-to-variable mapping of k_mlp_infer makes the
-// input features and the outputs register-resident too (no LDS).
+// input features and the outputs register-resident too.  The weights (16 KB) are staged
+// in LDS once per block: read from global memory by every tile of every wave, their
+// fragment loads were the kernel's largest cost.
 #pragma once
 
 #include "kernels.hpp"
@@ -35,10 +37,10 @@ typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
 // Network for inference: transposed BF16 weights ([out][in], row-major) and FP32 biases.
 struct MlpNet {
-  const short *w1t;  // [64][48]
+  const short *w1t;  // [64][48]; w2t and w3t follow it in the same allocation
   const short *w2t;  // [64][64]
   const short *w3t;  // [16][64]
-  const float *b1, *b2, *b3;
+  const float *b1, *b2, *b3;  // 64, 64, 16; b2 and b3 follow b1
   int valid;         // 0: no network, every interface takes the PPM weights
 };
 
@@ -129,6 +131,18 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
   const int g4 = lane >> 4, rr = lane & 15;
   const int nxb = (n0 + 63) / 64;
   const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the network (16 KB of BF16 weights, 144 biases), staged in LDS once per block: every
+  // tile of every wave reads all of it as MFMA A fragments
+  constexpr int NWT = 64 * 48 + 64 * 64 + 16 * 64, W2 = 64 * 48, W3 = 64 * 48 + 64 * 64;
+  __shared__ uint4 wlds[NWT / 8];
+  __shared__ float blds[64 + 64 + 16];
+  if (net.valid) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(net.w1t);
+    for (int q = threadIdx.x; q < NWT / 8; q += blockDim.x) wlds[q] = src[q];
+    if (threadIdx.x < 144) blds[threadIdx.x] = net.b1[threadIdx.x];
+  }
+  __syncthreads();
+  const short *wsh = reinterpret_cast<const short *>(wlds);
   if (w >= (long long)nxb * n1 * n2) return;  // wave-uniform: the MFMAs need whole waves
   const int xb = (int)(w % nxb);
   const long long r = w / nxb;
@@ -169,11 +183,11 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
 #pragma unroll
         for (int ks = 0; ks < 3; ks++) {
           const mlp_bf16x4 a =
-              *reinterpret_cast<const mlp_bf16x4 *>(&net.w1t[(16 * mt + rr) * 48 + 12 * g4 + 4 * ks]);
+              *reinterpret_cast<const mlp_bf16x4 *>(&wsh[(16 * mt + rr) * 48 + 12 * g4 + 4 * ks]);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[ks], acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int t = 0; t < 4; t++) h1[mt][t] = mlp_bf16(mlp_gelu(acc[t] + net.b1[16 * mt + 4 * g4 + t]));
+        for (int t = 0; t < 4; t++) h1[mt][t] = mlp_bf16(mlp_gelu(acc[t] + blds[16 * mt + 4 * g4 + t]));
       }
       // layer 2: K = 64, the K-step ks is layer 1's output tile ks
 #pragma unroll
@@ -182,24 +196,24 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
           const mlp_bf16x4 a =
-              *reinterpret_cast<const mlp_bf16x4 *>(&net.w2t[(16 * mt + rr) * 64 + 16 * ks + 4 * g4]);
+              *reinterpret_cast<const mlp_bf16x4 *>(&wsh[W2 + (16 * mt + rr) * 64 + 16 * ks + 4 * g4]);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[ks], acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int t = 0; t < 4; t++) h2[mt][t] = mlp_bf16(mlp_gelu(acc[t] + net.b2[16 * mt + 4 * g4 + t]));
+        for (int t = 0; t < 4; t++) h2[mt][t] = mlp_bf16(mlp_gelu(acc[t] + blds[64 + 16 * mt + 4 * g4 + t]));
       }
       // layer 3: 16 outputs; this lane holds outputs 4 g4 .. 4 g4 + 3
       mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < 4; ks++) {
         const mlp_bf16x4 a =
-            *reinterpret_cast<const mlp_bf16x4 *>(&net.w3t[rr * 64 + 16 * ks + 4 * g4]);
+            *reinterpret_cast<const mlp_bf16x4 *>(&wsh[W3 + rr * 64 + 16 * ks + 4 * g4]);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[ks], acc, 0, 0, 0);
       }
-      wal = mlp_sigmoid(acc[0] + net.b3[4 * g4]);
-      war = mlp_sigmoid(acc[1] + net.b3[4 * g4 + 1]);
-      wbl = mlp_sigmoid(acc[2] + net.b3[4 * g4 + 2]);
-      wbr = mlp_sigmoid(acc[3] + net.b3[4 * g4 + 3]);
+      wal = mlp_sigmoid(acc[0] + blds[128 + 4 * g4]);
+      war = mlp_sigmoid(acc[1] + blds[128 + 4 * g4 + 1]);
+      wbl = mlp_sigmoid(acc[2] + blds[128 + 4 * g4 + 2]);
+      wbr = mlp_sigmoid(acc[3] + blds[128 + 4 * g4 + 3]);
     }
     const bool use_net = fin && net.valid;
     const bool need_a = !(use_net && mlp_tvd_ok(sa, wal, war));
