@@ -70,17 +70,17 @@ def test_adam_steps_match_oracle():
 
 def _mlp_block(nx=(40, 36), precision="fp64", seed=5):
     from idefix_amd.datablock import DataBlock
-    g = DataBlock(2, nx, comps=3, mhd=True, recon="mlp", solver="hlld", emf="uct_contact",
-                  precision=precision, cfl=0.4)
+    g = DataBlock(len(nx), nx, comps=3, mhd=True, recon="mlp", solver="hlld",
+                  emf="uct_contact", precision=precision, cfl=0.4)
     Vc, Vs = problems.turbulence(g, seed=seed)
     g.upload(Vc, Vs)
     g.set_boundaries()
     return g
 
 
-@pytest.mark.parametrize("d", [0, 1])
-def test_mfma_inference_matches_bf16_oracle(d):
-    g = _mlp_block()
+@pytest.mark.parametrize("nx,d", [((40, 36), 0), ((40, 36), 1), ((20, 12, 10), 2)])
+def test_mfma_inference_matches_bf16_oracle(nx, d):
+    g = _mlp_block(nx)
     p = _params(7)
     g.set_mlp_params(p)
     W = g.mlp_faces(d).cpu().numpy()
