@@ -15,12 +15,11 @@
 // interface with non-finite input, and every interface when no network is loaded take
 // the weights of PPM (the member w = (v_PPM - V[i-1]) / (V[i] - V[i-1])).
 //
-// MFMA mapping (v_mfma_f32_16x16x32_bf16, one v_mfma_f32_16x16x16_bf16 for the last 16
-// inputs of layer 1): a layer computes Z^T = W^T H^T with the weights as the A operand
-// (rows = output units) and the activations as the B operand (columns = interfaces).
-// The 16x16 accumulator tile a lane holds (rows 4(l/16)+t, column l%16) is, two tiles at
-// a time, the B-operand fragment of the next layer's 32-step once that step's K order is
-// permuted to match (in both operands), so the three layers chain in registers; the lane-to-variable mapping of k_mlp_infer makes the
+// MFMA mapping (v_mfma_f32_16x16x16_bf16): a layer computes Z^T = W^T H^T with the
+// weights as the A operand (rows = output units) and the activations as the B operand
+// (columns = interfaces).  The 16x16 accumulator tile a lane holds (rows 4(l/16)+t,
+// column l%16) is exactly the B-operand fragment of the next layer's K-step, so the
+// three layers chain in registers; the lane-to-variable mapping of k_mlp_infer makes the
 // input features and the outputs register-resident too.  The weights (16 KB) are staged
 // in LDS once per block: read from global memory by every tile of every wave, their
 // fragment loads were the kernel's largest cost.
@@ -34,16 +33,7 @@ constexpr int MLP_NF = 48, MLP_NH = 64, MLP_NO = 16;
 constexpr float MLP_GELU_K = 1.702f;
 
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
-typedef short mlp_s16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
-
-// Operand of v_mfma_f32_16x16x32_bf16 from two 4-element halves (lane l holds K indices
-// 8 (l / 16) + 0..7 of its row or column; the layout was checked on the GPU)
-__device__ __forceinline__ mlp_bf16x8 mlp_cat8(mlp_bf16x4 lo, mlp_bf16x4 hi) {
-  const mlp_s16x8 v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-  return __builtin_bit_cast(mlp_bf16x8, v);
-}
 
 // Network for inference: transposed BF16 weights ([out][in], row-major) and FP32 biases.
 struct MlpNet {
@@ -186,44 +176,39 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
 #pragma unroll
       for (int q = 0; q < 12; q++) bx[q >> 2][q & 3] = mlp_bf16(x[q]);
       mlp_bf16x4 h1[4], h2[4];
-      // A fragment of 4 weights at LDS element e
-      auto wfrag = [&](int e) { return *reinterpret_cast<const mlp_bf16x4 *>(&wsh[e]); };
-      // layer 1: K = 48 as one 32-step (features 12 g4 + 0..7 of this lane) and one
-      // 16-step (features 12 g4 + 8..11)
+      // layer 1: K = 48 in three steps of 16
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        const int row = (16 * mt + rr) * 48 + 12 * g4;
         mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mlp_cat8(wfrag(row), wfrag(row + 4)),
-                                                      mlp_cat8(bx[0], bx[1]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wfrag(row + 8), bx[2], acc, 0, 0, 0);
+#pragma unroll
+        for (int ks = 0; ks < 3; ks++) {
+          const mlp_bf16x4 a =
+              *reinterpret_cast<const mlp_bf16x4 *>(&wsh[(16 * mt + rr) * 48 + 12 * g4 + 4 * ks]);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[ks], acc, 0, 0, 0);
+        }
 #pragma unroll
         for (int t = 0; t < 4; t++) h1[mt][t] = mlp_bf16(mlp_gelu(acc[t] + blds[16 * mt + 4 * g4 + t]));
       }
-      // layer 2: K = 64 as two 32-steps; step s takes layer 1's output tiles 2s and 2s+1,
-      // whose rows 32 s + 4 g4 + t and 32 s + 16 + 4 g4 + t this lane holds (the K order
-      // of the step is permuted to match, in both operands)
+      // layer 2: K = 64, the K-step ks is layer 1's output tile ks
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        const int row = W2 + (16 * mt + rr) * 64 + 4 * g4;
         mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int st = 0; st < 2; st++)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              mlp_cat8(wfrag(row + 32 * st), wfrag(row + 32 * st + 16)),
-              mlp_cat8(h1[2 * st], h1[2 * st + 1]), acc, 0, 0, 0);
+        for (int ks = 0; ks < 4; ks++) {
+          const mlp_bf16x4 a =
+              *reinterpret_cast<const mlp_bf16x4 *>(&wsh[W2 + (16 * mt + rr) * 64 + 16 * ks + 4 * g4]);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[ks], acc, 0, 0, 0);
+        }
 #pragma unroll
         for (int t = 0; t < 4; t++) h2[mt][t] = mlp_bf16(mlp_gelu(acc[t] + blds[64 + 16 * mt + 4 * g4 + t]));
       }
-      // layer 3: 16 outputs the same way; this lane holds outputs 4 g4 .. 4 g4 + 3
+      // layer 3: 16 outputs; this lane holds outputs 4 g4 .. 4 g4 + 3
       mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      {
-        const int row = W3 + rr * 64 + 4 * g4;
 #pragma unroll
-        for (int st = 0; st < 2; st++)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              mlp_cat8(wfrag(row + 32 * st), wfrag(row + 32 * st + 16)),
-              mlp_cat8(h2[2 * st], h2[2 * st + 1]), acc, 0, 0, 0);
+      for (int ks = 0; ks < 4; ks++) {
+        const mlp_bf16x4 a =
+            *reinterpret_cast<const mlp_bf16x4 *>(&wsh[W3 + rr * 64 + 16 * ks + 4 * g4]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[ks], acc, 0, 0, 0);
       }
       wal = mlp_sigmoid(acc[0] + blds[128 + 4 * g4]);
       war = mlp_sigmoid(acc[1] + blds[128 + 4 * g4 + 1]);
