@@ -748,6 +748,10 @@ static int stage_fused2d(idfx_ctx *c, int stage, int nstages, double dt, double 
   if (stage == 0) flags |= SW_INVDT_SET;
   IDFX_TRY(shock_step(c, 0, Vc_in, s));
   SweepArgs ax = sweep_args(c, 0, dt, Vc_in, Vs_in, Uc, InvDt, flags);
+  // learned limiter: the interface weights of both directions come from the stage's
+  // input state, before either sweep
+  IDFX_TRY(mlp_faces(c, 0, Vc_in, nullptr, s));
+  IDFX_TRY(mlp_faces(c, 1, Vc_in, nullptr, s));
   {
     Region r(c, IDFX_REGION_SWEEP_X, s);
     IDFX_TRY(check(c->ops->sweep(0, true, c->g, ax, c->emf(), c->target_waves, s)));

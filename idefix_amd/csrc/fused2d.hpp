@@ -117,8 +117,11 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
       }
       if (n == BX2) continue;
       T dummy;
-      recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, 1, c0 - 1),
-                    XTRA && recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
+      if constexpr (C::ORDER == RECON_MLP)
+        recon_mlp<C, JDIR>(wv, n, a.mlpw, g.sv, col + (unsigned)(c0 - 1) * sd, sd, dummy, carry[n]);
+      else
+        recon_cell<C>(wv, n, dummy, carry[n], recon_dx<C, T>(g, 1, c0 - 1),
+                      XTRA && recon_flat<C>(a, col + (unsigned)(c0 - 1) * sd));
     }
     if constexpr (CARRY) {
       if constexpr (HLLE) {
@@ -166,7 +169,10 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         T wv[NW];
 #pragma unroll
         for (int q = 0; q < NW; q++) wv[q] = win[q][n];
-        recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, 1, f), flat);
+        if constexpr (C::ORDER == RECON_MLP)
+          recon_mlp<C, JDIR>(wv, n, a.mlpw, g.sv, col + (unsigned)f * sd, sd, vR[n], vr_next[n]);
+        else
+          recon_cell<C>(wv, n, vR[n], vr_next[n], recon_dx<C, T>(g, 1, f), flat);
       }
       const double b2old = ld32(Vs1_in, (unsigned)f * vsd + col);
       vL[BX2] = (T)b2old;

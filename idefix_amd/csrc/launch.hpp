@@ -445,8 +445,7 @@ Ops make_ops() {
   o.correct_flux = &launch_correct_flux<C>;
   o.rhs = &launch_rhs<C>;
   o.divb = &launch_divb<C>;
-  if constexpr (C::MHD && C::DIM == 2 && C::ORDER != RECON_MLP)
-    o.fused2d_y = &launch_fused2d_y<C, T>;
+  if constexpr (C::MHD && C::DIM == 2) o.fused2d_y = &launch_fused2d_y<C, T>;
   if constexpr (C::ORDER == RECON_MLP) o.mlp_infer = &launch_mlp<C>;
   o.find_shock = &launch_find_shock<C>;
   return o;
