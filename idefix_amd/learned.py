@@ -284,7 +284,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--out", default="gpurun_out/learned")
+    ap.add_argument("--out", default="learned_out")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     gamma = 5.0 / 3.0
