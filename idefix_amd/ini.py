@@ -104,13 +104,21 @@ def block_config(blocks, recon="plm", mhd=True, precision="fp64", comps=None):
               ntracer=_get(blocks, "Hydro", "tracer", cast=int, default=0),
               shock_flattening=(_get(blocks, "Hydro", "shockFlattening", cast=float)
                                 if "shockFlattening" in blocks.get("Hydro", {}) else None))
+    user_cs = False
     if "csiso" in blocks.get("Hydro", {}):
-        # ISOTHERMAL builds (eos_isothermal.hpp:16-27): "csiso constant <value>"
-        if _get(blocks, "Hydro", "csiso") != "constant":
-            raise NotImplementedError("[Hydro] csiso: only a constant sound speed")
-        kw["csiso"] = _get(blocks, "Hydro", "csiso", i=1, cast=float)
+        # ISOTHERMAL builds (eos_isothermal.hpp:23-35): "csiso constant <value>", or
+        # "csiso userdef", whose per-cell speeds the setup supplies
+        # (DataBlock.set_iso_sound_speed; EnrollIsoSoundSpeed in the reference)
+        kind = _get(blocks, "Hydro", "csiso")
+        if kind == "constant":
+            kw["csiso"] = _get(blocks, "Hydro", "csiso", i=1, cast=float)
+        elif kind == "userdef":
+            kw["csiso"] = 1.0  # placeholder until the setup hands over its array
+            user_cs = True
+        else:
+            raise ValueError("csiso admits only constant or userdef entries")
     if mhd:
         kw["emf"] = _get(blocks, "Hydro", "emf", default="uct_contact")
     run = dict(tstop=_get(blocks, "TimeIntegrator", "tstop", cast=float),
-               output=blocks.get("Output", {}))
+               output=blocks.get("Output", {}), csiso_userdef=user_cs)
     return kw, run

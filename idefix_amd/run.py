@@ -35,6 +35,10 @@ def build(blocks, setup, recon="plm", precision="fp64", comps=None):
     fn, mhd, comps0 = SETUPS[setup]
     comps = comps if comps is not None else comps0
     kw, run = ini.block_config(blocks, recon=recon, mhd=mhd, precision=precision, comps=comps)
+    if run.get("csiso_userdef"):
+        # none of the setups here enrolls a sound-speed function
+        raise NotImplementedError("csiso userdef: setup %r provides no per-cell sound speed "
+                                  "(DataBlock.set_iso_sound_speed)" % setup)
     g = DataBlock(**kw)
     periodic = [all(b == "periodic" for b in kw["bc"][d]) if d < kw["dim"] else True
                 for d in range(3)]

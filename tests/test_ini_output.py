@@ -85,6 +85,14 @@ def test_block_config_refusals():
     assert kw["ntracer"] == 2 and kw["solver"] == "hlld"
     with pytest.raises(NotImplementedError):
         ini.block_config(ini.parse_ini(OT_HLL.replace("128  u", "128  l")))
+    # isothermal sound speed: constant, userdef (eos_isothermal.hpp:23-35), else refused
+    hydro = lambda line: ini.parse_ini(SOD.replace("gamma     1.4", "gamma     1.4\n" + line))
+    kw, run = ini.block_config(hydro("csiso constant 0.5"), mhd=False)
+    assert kw["csiso"] == 0.5 and not run["csiso_userdef"]
+    kw, run = ini.block_config(hydro("csiso userdef"), mhd=False)
+    assert run["csiso_userdef"] and kw["csiso"] > 0
+    with pytest.raises(ValueError):
+        ini.block_config(hydro("csiso table"), mhd=False)
 
 
 class FakeBlock:
