@@ -428,6 +428,107 @@ def linear_fast_wave_1d(g, eps=1e-6, gamma=5.0 / 3.0, L=1.0):
     return Vc, Vs, L / 2.0
 
 
+# Gardiner & Stone (2008) eigenvectors of the reference's linear-wave test
+# (test/MHD/LinearWaveTest/setup.cpp:48-98): (R0, right eigenvector in the wave frame
+# (rho, v1, v2, v3, B1, B2, B3, E)) of each |mode|; the sign of `mode` picks the
+# propagation direction of the fast, Alfven and slow waves.  The entropy mode rides on a
+# mean flow v1 = 1.
+LINEAR_WAVE_MODES = {
+    1: ("fast", 1.0 / (2.0 * np.sqrt(5.0)), (2.0, 4.0, -2.0, 0.0, 0.0, 4.0, 0.0, 9.0), 2.0),
+    2: ("alfven", 1.0, (0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 1.0, 0.0), 1.0),
+    3: ("slow", 1.0 / (2.0 * np.sqrt(5.0)), (4.0, 2.0, 4.0, 0.0, 0.0, -2.0, 0.0, 3.0), 0.5),
+    4: ("entropy", 0.5, (2.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0), 1.0),
+}
+
+
+def linear_wave_3d(g, mode, eps=1e-6, gamma=5.0 / 3.0):
+    """The reference's oblique linear MHD wave (test/MHD/LinearWaveTest/setup.cpp:29-120):
+    background rho = 1, p = 1/gamma, B = (1, 3/2, 0) in the wave frame (plus v1 = 1 for the
+    entropy mode), perturbed by eps * R0 * R * cos(2 pi x1) along
+    x1 = cos(a) cos(b) x + cos(a) sin(b) y + sin(a) z with sin(a) = 2/3, sin(b) = 2/sqrt(5).
+    The face field is the discrete curl of the edge vector potential
+    A = (0, B3 x1 + R0 R_B3 phi, -(B2 x1 + R0 R_B2 phi) + B1 x2) (wave frame), phi =
+    eps sin(2 pi x1) / (2 pi), as MakeVsFromAmag builds it (dataBlockHost.cpp:186-214).
+    For the fast and slow modes the pressure perturbation is (gamma - 1) (R_E - B.R_B)
+    (plus - v1 R_rho / 2 for the entropy mode).  Returns (Vc, Vs, period): the wavelength
+    along x1 is 1, so the period is 1 / wave speed."""
+    name, R0, R, speed = LINEAR_WAVE_MODES[abs(mode)]
+    sign = 1.0 if mode > 0 else -1.0
+    Vc, Vs = _empty(g)
+    Bg = np.array([1.0, 1.5, 0.0])  # background B in the wave frame
+    vg = np.array([1.0 if abs(mode) == 4 else 0.0, 0.0, 0.0])
+    Rr, R1, R2, R3, RB1, RB2, RB3, RE = R
+    if abs(mode) in (1, 3):  # fast and slow modes propagate along sign
+        R1, R2 = sign * R1, sign * R2
+    if abs(mode) == 2:
+        R3 = sign * R3
+    Rp = (gamma - 1.0) * (RE - Bg @ np.array([RB1, RB2, RB3]) - vg[0] * Rr / 2.0)
+    sa, sb = 2.0 / 3.0, 2.0 / np.sqrt(5.0)
+    ca, cb = np.sqrt(1.0 - sa * sa), np.sqrt(1.0 - sb * sb)
+    e1 = np.array([ca * cb, ca * sb, sa])
+    e2 = np.array([-sb, cb, 0.0])
+    e3 = np.array([-sa * cb, -sa * sb, ca])
+    x, y, z = _coords(g)
+    xl, yl, zl = _coords(g, "xl")
+    X, Y, Z = x[None, None, :], y[None, :, None], z[:, None, None]
+    x1 = e1[0] * X + e1[1] * Y + e1[2] * Z
+    ph = eps * np.cos(2.0 * np.pi * x1)
+    Vc[0] = 1.0 + R0 * Rr * ph
+    Vc[4] = 1.0 / gamma + R0 * Rp * ph
+    for c in range(3):
+        Vc[1 + c] = (vg[0] * e1[c] + vg[1] * e2[c] + vg[2] * e3[c] +
+                     R0 * (R1 * e1[c] + R2 * e2[c] + R3 * e3[c]) * ph)
+
+    def wave_frame_A(xx, yy, zz):
+        q1 = e1[0] * xx + e1[1] * yy + e1[2] * zz
+        q2 = e2[0] * xx + e2[1] * yy
+        phase = eps * np.sin(2.0 * np.pi * q1) / (2.0 * np.pi)
+        return Bg[2] * q1 + R0 * RB3 * phase, -(Bg[1] * q1 + R0 * RB2 * phase) + Bg[0] * q2
+    nx, ny, nz = g.np_tot
+    A = np.zeros((3, nz, ny, nx))
+    # component c of A lives on the edge along c: (x, yl, zl), (xl, y, zl), (xl, yl, z)
+    for c, (xx, yy, zz) in enumerate(((X, yl[None, :, None], zl[:, None, None]),
+                                      (xl[None, None, :], Y, zl[:, None, None]),
+                                      (xl[None, None, :], yl[None, :, None], Z))):
+        A2, A3 = wave_frame_A(xx, yy, zz)
+        A[c] = A2 * e2[c] + A3 * e3[c] + 0 * X * Y * Z
+    b, e = g.beg, g.end
+    k0, k1 = b[2], e[2] + 1
+    j0, j1 = b[1], e[1] + 1
+    i0, i1 = b[0], e[0] + 1
+    d1 = _widths(g, 0)[None, None, i0:i1]
+    d2 = _widths(g, 1)[None, j0:j1, None]
+    d3 = _widths(g, 2)[k0:k1, None, None]
+    Ai = A[:, k0:k1, j0:j1, i0:i1]
+    Aip = A[:, k0:k1, j0:j1, i0 + 1:i1 + 1]
+    Ajp = A[:, k0:k1, j0 + 1:j1 + 1, i0:i1]
+    Akp = A[:, k0 + 1:k1 + 1, j0:j1, i0:i1]
+    Vs[0, k0:k1, j0:j1, i0:i1] = (0.0 + 1 / d2 * (Ajp[2] - Ai[2])) - 1 / d3 * (Akp[1] - Ai[1])
+    Vs[1, k0:k1, j0:j1, i0:i1] = -1 / d1 * (Aip[2] - Ai[2]) + 1 / d3 * (Akp[0] - Ai[0])
+    Vs[2, k0:k1, j0:j1, i0:i1] = 1 / d1 * (Aip[1] - Ai[1]) - 1 / d2 * (Ajp[0] - Ai[0])
+    return Vc, Vs, 1.0 / speed
+
+
+def linear_wave_error(V0, V1, beg, end):
+    """Error measure of the reference's linear-wave test
+    (test/MHD/LinearWaveTest/python/testidefix.py:26-49): for RHO, VX1..3, PRS and the
+    three face fields, the mean absolute difference of the mean-subtracted initial and
+    final fields; the result is the rms of the eight.  V0, V1 = (Vc, Vs) pairs."""
+    b, e = beg, end
+    errs = []
+    for n in (0, 1, 2, 3, 4):
+        q0 = V0[0][n, b[2]:e[2], b[1]:e[1], b[0]:e[0]]
+        q1 = V1[0][n, b[2]:e[2], b[1]:e[1], b[0]:e[0]]
+        errs.append(np.mean(np.abs((q0 - q0.mean()) - (q1 - q1.mean()))))
+    for d in range(3):
+        sl = [slice(b[2], e[2]), slice(b[1], e[1]), slice(b[0], e[0])]
+        sl[2 - d] = slice(b[d], e[d] + 1)
+        q0 = V0[1][d][tuple(sl)]
+        q1 = V1[1][d][tuple(sl)]
+        errs.append(np.mean(np.abs((q0 - q0.mean()) - (q1 - q1.mean()))))
+    return float(np.sqrt(np.mean(np.square(errs))))
+
+
 def brio_wu(g, x0=50.0):
     """MHD shock tube of test/MHD/sod/setup.cpp:20-40 (Brio & Wu 1988 states, the
     reference's gamma 5/3 and [0, 100] domain): rho, p = 1, 1 | 0.125, 0.1; Bx = 0.75;

@@ -43,6 +43,16 @@ namespace {
 
 constexpr int IDIR = 0, JDIR = 1, KDIR = 2;
 
+// Planted mistake (negative controls of the tests, never set in normal use): the tests
+// switch one of these on to show that a check fails when the restated branch is wrong.
+//   ORC_PLANT_HLLD_SBX      sign of B_n flipped in the HLLD double-star state
+//   ORC_PLANT_CONTACT_SV    upwind switch of uct_contact inverted (sv -> 1 - sv)
+//   ORC_PLANT_UCT0_INDEX    uct0's cell-EMF average of the x faces taken along y
+//   ORC_PLANT_RIEMANN_PHI   sign of the dissipative term of UCT-HLL/HLLD reversed
+//   ORC_PLANT_PPM_EXTREMUM  PPM clips every extremum (smooth-extremum test removed)
+//   ORC_PLANT_ROE_ALFVEN    Roe MHD: Alfven-wave strengths of both families swapped
+int g_plant = 0;
+
 // Variable indices of the reference (src/idefix.hpp:74-108), with -1 for the
 // components that COMPONENTS removes.
 struct Vars {
@@ -162,6 +172,7 @@ inline void ppm_states(double vm2, double vm1, double v0, double vp1, double vp2
   double qmax = FMAX(FMAX(std::fabs(vm1), std::fabs(v0)), std::fabs(vp1));
   double rho = 0.0;
   if (std::fabs(d2qf) > 1e-12 * qmax) rho = d2q / d2qf;
+  if (g_plant == ORC_PLANT_PPM_EXTREMUM) rho = 0.0;
   if (((vr - v0) * (v0 - vl) <= 0) || (vm1 - v0) * (v0 - vp1) <= 0) {
     if (rho <= (1.0 - 1e-12)) {
       vl = v0 - rho * (v0 - vl);
@@ -901,6 +912,7 @@ void hlld_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doub
     } else {
       double vss = 0, wss = 0, ussl[8], ussr[8];
       for (int n = 0; n < 8; n++) { ussl[n] = 0; ussr[n] = 0; }
+      if (g_plant == ORC_PLANT_HLLD_SBX) sBx = -sBx;
       ussl[V.RHO] = usL[V.RHO];
       ussr[V.RHO] = usR[V.RHO];
       if (comps >= 2) {
@@ -1425,6 +1437,11 @@ void roe_mhd(const orc_cfg &c, const Vars &V, const Dir &D, int dir, const doubl
               sBx / sqrt_rho * (beta_y * dV[D.BXb] - beta_z * dV[D.BXt]);
     eta[kk] *= 0.5;
   }
+  if (comps == 3 && g_plant == ORC_PLANT_ROE_ALFVEN) {
+    const double e = eta[KALFVM];
+    eta[KALFVM] = eta[KALFVP];
+    eta[KALFVP] = e;
+  }
   cmax = std::fabs(u) + cf;
   sl = lambda[KFASTM];
   sr = lambda[KFASTP];
@@ -1467,6 +1484,9 @@ struct Oracle {
   long ncycles = 0;
   double wc[2] = {0, 0}, w0[2] = {0, 0};
   bool last_flux_unwritten = false;
+  // threads only pay off on blocks of some size: small (1-D) blocks run serially, which
+  // also keeps them fast when other processes compete for the cores
+  bool par_ = true;
 
   explicit Oracle(const orc_cfg &cfg) : c(cfg), V(cfg) {
     for (int d = 0; d < 3; d++) {
@@ -1504,6 +1524,7 @@ struct Oracle {
       }
     }
     const size_t n3 = (size_t)np_tot[0] * np_tot[1] * np_tot[2];
+    par_ = n3 >= 16384;
     // passive tracers follow the fluid variables (fluid.hpp:529-545)
     ntot = V.nvar + c.ntracer;
     Vc.assign(ntot * n3, 0.0);
@@ -1724,7 +1745,7 @@ struct Oracle {
   // ReconstructVcField, boundary.hpp:388-404 (applied to Vc or Uc)
   void reconstruct_vc_field(std::vector<double> &X) {
     if (!c.mhd) return;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = 0; k < np_tot[2]; k++)
       for (int j = 0; j < np_tot[1]; j++)
         for (int i = 0; i < np_tot[0]; i++) {
@@ -1751,7 +1772,7 @@ struct Oracle {
   // ConvertConsToPrim, convertConsToPrim.hpp:105-149
   void cons_to_prim() {
     if (c.mhd) reconstruct_vc_field(Uc);
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = 0; k < np_tot[2]; k++)
       for (int j = 0; j < np_tot[1]; j++)
         for (int i = 0; i < np_tot[0]; i++) {
@@ -1765,7 +1786,7 @@ struct Oracle {
   }
   // ConvertPrimToCons, convertConsToPrim.hpp:152-189
   void prim_to_cons() {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = 0; k < np_tot[2]; k++)
       for (int j = 0; j < np_tot[1]; j++)
         for (int i = 0; i < np_tot[0]; i++) {
@@ -1789,7 +1810,7 @@ struct Oracle {
   // smallest neighbouring pressure
   void find_shock() {
     const int ko = c.dim == 3 ? 1 : 0, jo = c.dim >= 2 ? 1 : 0;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2] - ko; k < end[2] + ko; k++)
       for (int j = beg[1] - jo; j < end[1] + jo; j++)
         for (int i = beg[0] - 1; i < end[0] + 1; i++) {
@@ -1926,7 +1947,7 @@ struct Oracle {
     const Dir D(V, dir);
     last_flux_unwritten = false;
     bool unwritten = false;
-#pragma omp parallel for collapse(2) reduction(|| : unwritten)
+#pragma omp parallel for collapse(2) reduction(|| : unwritten) if (par_)
     for (int k = beg[2] - ext[2]; k < end[2] + ko + ext[2]; k++)
       for (int j = beg[1] - ext[1]; j < end[1] + jo + ext[1]; j++)
         for (int i = beg[0] - ext[0]; i < end[0] + io + ext[0]; i++) {
@@ -1987,7 +2008,7 @@ struct Oracle {
   void tracer_flux(int dir) {
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
     for (int nv = V.nvar; nv < ntot; nv++)
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
       for (int k = beg[2]; k < end[2] + ko; k++)
         for (int j = beg[1]; j < end[1] + jo; j++)
           for (int i = beg[0]; i < end[0] + io; i++) {
@@ -2012,7 +2033,7 @@ struct Oracle {
   void tracer_rhs(int dir, double dtl) {
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
     for (int nv = V.nvar; nv < ntot; nv++)
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
       for (int k = beg[2]; k < end[2]; k++)
         for (int j = beg[1]; j < end[1]; j++)
           for (int i = beg[0]; i < end[0]; i++)
@@ -2022,7 +2043,7 @@ struct Oracle {
   void calc_rhs(int dir, double dtl) {
     const int io = dir == IDIR, jo = dir == JDIR, ko = dir == KDIR;
     // "Correct Flux": Flux *= A (Cartesian, no Fargo), :156-194
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2]; k < end[2] + ko; k++)
       for (int j = beg[1]; j < end[1] + jo; j++)
         for (int i = beg[0]; i < end[0] + io; i++) {
@@ -2030,7 +2051,7 @@ struct Oracle {
           for (int n = 0; n < V.nvar; n++) Flux[i4(n, k, j, i)] = Flux[i4(n, k, j, i)] * a;
         }
     // "CalcRightHandSide", :288-549
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2]; k < end[2]; k++)
       for (int j = beg[1]; j < end[1]; j++)
         for (int i = beg[0]; i < end[0]; i++) {
@@ -2056,7 +2077,7 @@ struct Oracle {
   // ---------------- Constrained transport ----------------
   // CalcCellCenteredEMF, calcCornerEmf.hpp:84-126
   void calc_cell_emf() {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = 0; k < np_tot[2]; k++)
       for (int j = 0; j < np_tot[1]; j++)
         for (int i = 0; i < np_tot[0]; i++) {
@@ -2077,7 +2098,7 @@ struct Oracle {
   // CalcArithmeticAverage / CalcUCT0CornerEMF corner average, calcCornerEmf.hpp:62-78
   void arithmetic_corner() {
     const double w = 0.25;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2]; k < end[2] + offs[2]; k++)
       for (int j = beg[1]; j < end[1] + offs[1]; j++)
         for (int i = beg[0]; i < end[0] + offs[0]; i++) {
@@ -2091,7 +2112,7 @@ struct Oracle {
   }
   // CalcUCT0Average, calcCornerEmf.hpp:128-194
   void uct0_average() {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2] - offs[2]; k < end[2] + offs[2]; k++)
       for (int j = beg[1] - offs[1]; j < end[1] + offs[1]; j++)
         for (int i = beg[0] - offs[0]; i < end[0] + offs[0]; i++) {
@@ -2108,18 +2129,23 @@ struct Oracle {
           }
           ezi[p] *= 2.0;
           ezj[p] *= 2.0;
-          ezi[p] -= 0.5 * (Ex3[i3(k, j, i - 1)] + Ex3[p]);
+          if (g_plant == ORC_PLANT_UCT0_INDEX) ezi[p] -= 0.5 * (Ex3[i3(k, j - 1, i)] + Ex3[p]);
+          else ezi[p] -= 0.5 * (Ex3[i3(k, j, i - 1)] + Ex3[p]);
           ezj[p] -= 0.5 * (Ex3[i3(k, j - 1, i)] + Ex3[p]);
         }
     arithmetic_corner();
   }
   // CalcContactAverage, calcCornerEmf.hpp:196-282
   void contact_average() {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2]; k < end[2] + offs[2]; k++)
       for (int j = beg[1]; j < end[1] + offs[1]; j++)
         for (int i = beg[0]; i < end[0] + offs[0]; i++) {
-          auto A = [&](std::vector<double> &a, int kk, int jj, int ii) { return a[i3(kk, jj, ii)]; };
+          auto A = [&](std::vector<double> &a, int kk, int jj, int ii) {
+            const double v = a[i3(kk, jj, ii)];
+            const bool sv = &a == &svx || &a == &svy || &a == &svz;
+            return (sv && g_plant == ORC_PLANT_CONTACT_SV) ? 1.0 - v : v;
+          };
           const double ez_l2 = (1 - A(svx, k, j - 1, i)) * (A(ezj, k, j, i) - A(Ex3, k, j - 1, i)) +
                                (A(svx, k, j - 1, i)) * (A(ezj, k, j, i - 1) - A(Ex3, k, j - 1, i - 1));
           const double ez_r2 = (1 - A(svx, k, j, i)) * (A(ezj, k, j, i) - A(Ex3, k, j, i)) +
@@ -2176,11 +2202,12 @@ struct Oracle {
       dv = mclim(vq(2) - vq(1), vq(1) - vq(0));
       vR = vq(1) - 0.5 * dv;
     }
-    const double phi = dR * bR - dL * bL;
+    double phi = dR * bR - dL * bL;
+    if (g_plant == ORC_PLANT_RIEMANN_PHI) phi = -phi;
     return sgn_phi > 0 ? (aL * vL * bL + aR * vR * bR) + phi : (aL * vL * bL + aR * vR * bR) - phi;
   }
   void riemann_average() {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2]; k < end[2] + offs[2]; k++)
       for (int j = beg[1]; j < end[1] + offs[1]; j++)
         for (int i = beg[0]; i < end[0] + offs[0]; i++) {
@@ -2254,7 +2281,7 @@ struct Oracle {
   // EvolveMagField, evolveMagField.hpp:53-132 (Cartesian)
   void evolve_mag_field(double dtl) {
     if (!c.mhd || c.dim < 2) return;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (par_)
     for (int k = beg[2]; k < end[2] + offs[2]; k++)
       for (int j = beg[1]; j < end[1] + offs[1]; j++)
         for (int i = beg[0]; i < end[0] + offs[0]; i++) {
@@ -2545,6 +2572,11 @@ int orc_face_flux(const orc_cfg *cfg, int dir, const double *vL_in, const double
     emf7[4] = E.aR; emf7[5] = E.dL; emf7[6] = E.dR;
   }
   return wrote ? 1 : 0;
+}
+int orc_plant(int which) {
+  const int prev = g_plant;
+  g_plant = which;
+  return prev;
 }
 double orc_vanleer(double dvp, double dvm) { return vanleer(dvp, dvm); }
 double orc_mclim(double dvp, double dvm) { return mclim(dvp, dvm); }

@@ -100,6 +100,13 @@ void orc_set_t(void *o, double t);
 void orc_set_dt(void *o, double dt);
 long orc_get_ncycles(void *o);
 
+/* Negative controls: switch on one planted mistake (0 = none) so that a test can show
+ * that its check fails when a restated branch is wrong; returns the previous value. */
+enum { ORC_PLANT_NONE = 0, ORC_PLANT_HLLD_SBX = 1, ORC_PLANT_CONTACT_SV = 2,
+       ORC_PLANT_UCT0_INDEX = 3, ORC_PLANT_RIEMANN_PHI = 4, ORC_PLANT_PPM_EXTREMUM = 5,
+       ORC_PLANT_ROE_ALFVEN = 6 };
+int orc_plant(int which);
+
 /* Pointwise kernels (for unit tests of the algebra) */
 int orc_face_flux(const orc_cfg *cfg, int dir, const double *vL, const double *vR,
                   double *flux, double *cmax, double *emf7);

@@ -73,6 +73,7 @@ def lib():
             "orc_get_dt": (d, [vp]), "orc_set_t": (None, [vp, d]),
             "orc_set_dt": (None, [vp, d]), "orc_get_ncycles": (ctypes.c_long, [vp]),
             "orc_face_flux": (i, [cp, i, dp, dp, dp, dp, dp]),
+            "orc_plant": (i, [i]),
             "orc_vanleer": (d, [d, d]), "orc_mclim": (d, [d, d]), "orc_limo3lim": (d, [d, d, d]),
             "orc_ppm_states": (None, [d, d, d, d, d, dp, dp]),
             "orc_prim2cons_pt": (None, [cp, dp, dp]), "orc_cons2prim_pt": (None, [cp, dp, dp]),
@@ -304,6 +305,26 @@ def face_flux(cfg, d, vL, vR):
                                 vR.ctypes.data_as(dp), F.ctypes.data_as(dp),
                                 cmax.ctypes.data_as(dp), emf.ctypes.data_as(dp))
     return F, cmax[0], emf, bool(wrote)
+
+
+PLANTS = {"hlld_sbx": 1, "contact_sv": 2, "uct0_index": 3, "riemann_phi": 4,
+          "ppm_extremum": 5, "roe_alfven": 6}
+
+
+class planted:
+    """Context manager switching on one planted mistake of the oracle (a negative
+    control: a pin must fail while it is on)."""
+
+    def __init__(self, name):
+        self.which = PLANTS[name]
+
+    def __enter__(self):
+        self.prev = lib().orc_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant(self.prev)
+        return False
 
 
 def vanleer(dvp, dvm):
