@@ -32,7 +32,13 @@ EXPORTS = [
     "idfx_mlp_param_count", "idfx_mlp_create", "idfx_mlp_destroy", "idfx_mlp_set_params",
     "idfx_mlp_get_params", "idfx_mlp_loss_grad", "idfx_mlp_train_step", "idfx_mlp_loss",
     "idfx_ctx_set_mlp_params", "idfx_mlp_faces", "idfx_ctx_set_iso_sound_speed",
+    "idfx_enforce_boundary_dir", "idfx_reconstruct_normal_field",
+    "idfx_ctx_set_userdef_boundary", "idfx_ctx_set_time",
 ]
+
+# idfx_userdef_boundary_fn: (arg, dir, side, t, Vc, Vs, stream)
+USERDEF_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
 
 REGIONS = ["boundaries", "sweep_x", "sweep_y", "sweep_z", "corner_emf", "evolve_b",
            "cons_to_prim", "fused2d_y"]
@@ -103,6 +109,10 @@ def load():
         "idfx_ctx_set_mlp_params": [vp, ctypes.POINTER(ctypes.c_float)],
         "idfx_ctx_set_iso_sound_speed": [vp, vp],
         "idfx_mlp_faces": [vp, i, vp, vp, vp],
+        "idfx_enforce_boundary_dir": [vp, i, vp, vp, vp],
+        "idfx_reconstruct_normal_field": [vp, i, vp, vp],
+        "idfx_ctx_set_userdef_boundary": [vp, USERDEF_FN, vp],
+        "idfx_ctx_set_time": [vp, d],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -31,6 +31,7 @@ void register_limo3_mhd(Ops *t);
 void register_limo3_mhd_b(Ops *t);
 void register_mhd1(Ops *t);
 void register_iso(Ops *t);
+void register_iso3(Ops *t);
 
 Ops *ops_table() {
   static Ops *table = [] {
@@ -53,6 +54,7 @@ Ops *ops_table() {
     register_limo3_mhd_b(t);
     register_mhd1(t);
     register_iso(t);
+    register_iso3(t);
     return t;
   }();
   return table;
@@ -107,8 +109,17 @@ struct idfx_ctx {
   bool fused2d_ok = false;   // configuration supports it
   bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
   double *Vc2 = nullptr, *Vs2 = nullptr;
+  // enrolled user boundary (IDFX_BC_USERDEF sides) and the time it is called with
+  idfx_userdef_boundary_fn user_fn = nullptr;
+  void *user_arg = nullptr;
+  double t_bound = 0.0;
   const void *emf() const { return fp32 ? (const void *)&e32 : (const void *)&e64; }
 };
+
+namespace idfx {
+// the launch grid of a context, for the other translation units of the library
+const GridDesc &ctx_grid(const idfx_ctx *c) { return c->g; }
+}  // namespace idfx
 
 namespace {
 
@@ -504,18 +515,51 @@ int idfx_ctx_array(idfx_ctx *c, int which, double **ptr) {
   return *ptr ? IDFX_OK : IDFX_E_ARGS;
 }
 
+int idfx_enforce_boundary_dir(idfx_ctx *c, int dir, double *Vc, double *Vs, void *stream) {
+  if (!c || !Vc || (c->d.mhd && !Vs) || dir < 0 || dir >= c->d.dim) return IDFX_E_ARGS;
+  hipStream_t s = (hipStream_t)stream;
+  // built-in side types (user-defined sides are left alone by the kernel)
+  IDFX_TRY(check(c->ops->boundary(c->g, Vc, Vs, dir, c->d.lbound[dir], c->d.rbound[dir], s)));
+  for (int side = 0; side < 2; side++) {
+    const int bc = side == 0 ? c->d.lbound[dir] : c->d.rbound[dir];
+    if (bc != IDFX_BC_USERDEF) continue;
+    // boundary.hpp:316-332: no enrolled function is a configuration error
+    if (!c->user_fn) return IDFX_E_ARGS;
+    c->user_fn(c->user_arg, dir, side, c->t_bound, Vc, Vs, stream);
+  }
+  return IDFX_OK;
+}
+
+int idfx_reconstruct_normal_field(idfx_ctx *c, int dir, double *Vs, void *stream) {
+  if (!c || !c->d.mhd || !Vs || dir < 0 || dir >= c->d.dim) return IDFX_E_ARGS;
+  // every side that is not periodic, user-defined ones included (boundary.hpp:409-410)
+  const int recL = c->d.lbound[dir] != IDFX_BC_PERIODIC;
+  const int recR = c->d.rbound[dir] != IDFX_BC_PERIODIC;
+  if (!recL && !recR) return IDFX_OK;
+  return check(c->ops->normal_field(c->g, Vs, dir, recL, recR, (hipStream_t)stream));
+}
+
 int idfx_set_boundaries(idfx_ctx *c, double *Vc, double *Vs, void *stream) {
   if (!c || !Vc || (c->d.mhd && !Vs)) return IDFX_E_ARGS;
   hipStream_t s = (hipStream_t)stream;
   for (int dir = 0; dir < c->d.dim; dir++) {
-    IDFX_TRY(check(c->ops->boundary(c->g, Vc, Vs, dir, c->d.lbound[dir], c->d.rbound[dir], s)));
-    if (c->d.mhd) {
-      const int recL = c->d.lbound[dir] != IDFX_BC_PERIODIC && c->d.lbound[dir] != IDFX_BC_USERDEF;
-      const int recR = c->d.rbound[dir] != IDFX_BC_PERIODIC && c->d.rbound[dir] != IDFX_BC_USERDEF;
-      if (recL || recR) IDFX_TRY(check(c->ops->normal_field(c->g, Vs, dir, recL, recR, s)));
-    }
+    IDFX_TRY(idfx_enforce_boundary_dir(c, dir, Vc, Vs, stream));
+    if (c->d.mhd) IDFX_TRY(idfx_reconstruct_normal_field(c, dir, Vs, stream));
   }
   if (c->d.mhd) IDFX_TRY(check(c->ops->vc_field(c->g, Vs, Vc, s)));
+  return IDFX_OK;
+}
+
+int idfx_ctx_set_userdef_boundary(idfx_ctx *c, idfx_userdef_boundary_fn fn, void *arg) {
+  if (!c) return IDFX_E_ARGS;
+  c->user_fn = fn;
+  c->user_arg = arg;
+  return IDFX_OK;
+}
+
+int idfx_ctx_set_time(idfx_ctx *c, double t) {
+  if (!c) return IDFX_E_ARGS;
+  c->t_bound = t;
   return IDFX_OK;
 }
 
@@ -880,7 +924,11 @@ int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long nc
   // runs its first stage in place so that the last stage writes the caller's arrays
   const bool fused = c->fused2d_on && nstages >= 2 && Vs;
   const int first_fused = (nstages % 2 == 1) ? 1 : 0;
+  // stage times of the boundary calls (TimeIntegrator::Cycle advances t by dt after each
+  // stage, then t = wc t + w0 t0): t0, t0 + dt, and t0 + dt/2 for the third RK3 stage
+  const double tstage[3] = {t0, t0 + *dt, 0.25 * (t0 + *dt + *dt) + 0.75 * t0};
   for (int stage = 0; stage < nstages; stage++) {
+    c->t_bound = tstage[stage];
     if (!fused || stage < first_fused) {
       IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0,
                              stage == nstages - 1, stage == 0, s, stage > 0));

@@ -166,7 +166,11 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
       sb[s] = (float)Vb[pf + (s - 3) * sd];
     }
     float x[12];
-    int fin = mlp_features6(sa, x) && mlp_features6(sb, x + 6);
+    // both feature sets are always written (no short circuit): the BF16 conversion below
+    // reads all twelve even when a stencil is not finite
+    const bool fa = mlp_features6(sa, x);
+    const bool fb = mlp_features6(sb, x + 6);
+    int fin = fa && fb;
     // the face's eight variables sit in lanes r, r + 16, r + 32, r + 48
     fin &= __shfl_xor(fin, 16);
     fin &= __shfl_xor(fin, 32);

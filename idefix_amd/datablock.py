@@ -216,6 +216,29 @@ class DataBlock:
                    "stage")
         return newdt.value
 
+    def set_userdef_boundary(self, fn):
+        """Boundary::EnrollUserDefBoundary: fn(dir, side, t, Vc, Vs) fills the ghost zones
+        of a user-defined side (side 0 left, 1 right) of direction dir; Vc and Vs are the
+        torch tensors the calling entry point works on (the block's own arrays on the
+        per-kernel schedule that user-defined sides select), and its work goes on the
+        current stream.  None unenrolls."""
+        if fn is None:
+            self._ufn = None
+            capi.check(self._L.idfx_ctx_set_userdef_boundary(self.ctx, capi.USERDEF_FN(0), None),
+                       "set_userdef_boundary")
+            return
+
+        def trampoline(arg, d, side, t, vc, vs, stream):
+            assert vc == self.Vc.data_ptr() and (vs or 0) == (self.Vs.data_ptr() if self.mhd else 0)
+            fn(d, side, t, self.Vc, self.Vs)
+        self._ufn = capi.USERDEF_FN(trampoline)
+        capi.check(self._L.idfx_ctx_set_userdef_boundary(self.ctx, self._ufn, None),
+                   "set_userdef_boundary")
+
+    def set_time(self, t):
+        """Time passed to the user boundary by set_boundaries() and stage()."""
+        capi.check(self._L.idfx_ctx_set_time(self.ctx, t), "set_time")
+
     def set_iso_sound_speed(self, cs):
         """Per-cell isothermal sound speed ("csiso userdef"): an array of shape
         (nz, ny, nx) over all cells, ghosts included (numpy or torch), kept on the device

@@ -52,7 +52,8 @@ extern "C" {
 #define IDFX_EMF_UCT_CONTACT 2
 #define IDFX_EMF_UCT_HLL 3
 #define IDFX_EMF_UCT_HLLD 4
-/* [Boundary] types (boundary.hpp:286-385); USERDEF sides are left to the caller */
+/* [Boundary] types (boundary.hpp:286-385); USERDEF sides are filled by the function enrolled
+ * with idfx_ctx_set_userdef_boundary */
 #define IDFX_BC_PERIODIC 0
 #define IDFX_BC_OUTFLOW 1
 #define IDFX_BC_REFLECTIVE 2
@@ -114,9 +115,33 @@ const char *idfx_strerror(int status);
  * returned through the double* slot; IDFX_RECON_MLP contexts only). */
 int idfx_ctx_array(idfx_ctx *ctx, int which, double **ptr);
 
-/* Boundary::SetBoundaries (boundary.hpp:239-282) for the built-in side types, with
- * ReconstructNormalField (:407-530) and ReconstructVcField(Vc) (:388-404). */
+/* Boundary::SetBoundaries (boundary.hpp:239-282): per direction EnforceBoundaryDir
+ * (built-in sides, and the enrolled user function on IDFX_BC_USERDEF sides) then
+ * ReconstructNormalField (:407-530) on every non-periodic side; last ReconstructVcField(Vc)
+ * (:388-404).  IDFX_E_ARGS when a side is IDFX_BC_USERDEF and no function is enrolled
+ * (the reference's "No function has been enrolled" error, :325-330). */
 int idfx_set_boundaries(idfx_ctx *ctx, double *Vc, double *Vs, void *stream);
+
+/* Boundary::EnforceBoundaryDir (boundary.hpp:285-385) for one direction: periodic, outflow
+ * and reflective sides by kernel, IDFX_BC_USERDEF sides by the enrolled function. */
+int idfx_enforce_boundary_dir(idfx_ctx *ctx, int dir, double *Vc, double *Vs, void *stream);
+
+/* Boundary::ReconstructNormalField (boundary.hpp:407-530) for one direction: the normal
+ * face field of the ghost faces of every non-periodic side from div B = 0. */
+int idfx_reconstruct_normal_field(idfx_ctx *ctx, int dir, double *Vs, void *stream);
+
+/* Boundary::EnrollUserDefBoundary (boundary.hpp:548-553) and the call in
+ * EnforceBoundaryDir (:316-332, :362-377): fn(arg, dir, side, t, Vc, Vs, stream) fills the
+ * ghost zones of side (0 left, 1 right) of direction dir, enqueuing its work on `stream`
+ * (the stream of the calling entry point).  t is the stage time: idfx_cycle passes t0,
+ * t0 + dt (and t0 + dt/2 in the third RK3 stage), as TimeIntegrator::Cycle does
+ * (timeIntegrator.cpp:263-333); callers
+ * of idfx_stage / idfx_set_boundaries set it with idfx_ctx_set_time.  fn == NULL
+ * unenrolls.  Contexts with user-defined sides run the per-kernel schedule. */
+typedef void (*idfx_userdef_boundary_fn)(void *arg, int dir, int side, double t, double *Vc,
+                                         double *Vs, void *stream);
+int idfx_ctx_set_userdef_boundary(idfx_ctx *ctx, idfx_userdef_boundary_fn fn, void *arg);
+int idfx_ctx_set_time(idfx_ctx *ctx, double t);
 
 /* Fluid::ConvertPrimToCons over all cells (convertConsToPrim.hpp:152-189). */
 int idfx_prim_to_cons(idfx_ctx *ctx, const double *Vc, double *Uc, void *stream);

@@ -1487,6 +1487,8 @@ struct Oracle {
   // threads only pay off on blocks of some size: small (1-D) blocks run serially, which
   // also keeps them fast when other processes compete for the cores
   bool par_ = true;
+  orc_userdef_fn user_fn = nullptr;
+  void *user_arg = nullptr;
 
   explicit Oracle(const orc_cfg &cfg) : c(cfg), V(cfg) {
     for (int d = 0; d < 3; d++) {
@@ -1762,6 +1764,7 @@ struct Oracle {
         if (bc == ORC_BC_PERIODIC) enforce_periodic(dir, side);
         else if (bc == ORC_BC_OUTFLOW) enforce_outflow(dir, side);
         else if (bc == ORC_BC_REFLECTIVE) enforce_reflective(dir, side);
+        else if (bc == ORC_BC_USERDEF && user_fn) user_fn(user_arg, dir, side, t);
       }
       if (c.mhd) reconstruct_normal_field(dir);
     }
@@ -2504,6 +2507,10 @@ double *orc_coord(void *p, int dir, int kind) {
   }
 }
 void orc_set_boundaries(void *p) { static_cast<Oracle *>(p)->set_boundaries(); }
+void orc_set_userdef(void *p, orc_userdef_fn fn, void *arg) {
+  static_cast<Oracle *>(p)->user_fn = fn;
+  static_cast<Oracle *>(p)->user_arg = arg;
+}
 int orc_set_iso_sound_speed(void *p, const double *cs) {
   Oracle *o = static_cast<Oracle *>(p);
   if (!o->c.isothermal) return 1;

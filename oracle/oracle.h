@@ -26,7 +26,7 @@ enum { ORC_SOLVER_HLLC = 1, ORC_SOLVER_HLLD = 2, ORC_SOLVER_ROE = 3, ORC_SOLVER_
 enum { ORC_EMF_ARITHMETIC = 0, ORC_EMF_UCT0 = 1, ORC_EMF_UCT_CONTACT = 2,
        ORC_EMF_UCT_HLL = 3, ORC_EMF_UCT_HLLD = 4 };
 /* Boundary types ([Boundary] X?-beg / X?-end). */
-enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_REFLECTIVE = 2 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_REFLECTIVE = 2, ORC_BC_USERDEF = 3 };
 
 /* Arrays exposed for inspection. */
 enum { ORC_VC = 0, ORC_VS = 1, ORC_UC = 2, ORC_INVDT = 3, ORC_CMAX = 4, ORC_FLUX = 5,
@@ -69,6 +69,12 @@ int orc_nthreads(int n);  /* set OpenMP threads (returns the previous value) */
 void orc_layout(void *o, int *np_tot, int *nghost, int *nvar, int *vs_shape);
 double *orc_array(void *o, int which, long *nelem);
 double *orc_coord(void *o, int dir, int kind); /* kind 0:x 1:xl 2:xr 3:dx */
+
+/* User-defined sides (boundary.hpp:316-332): fn(arg, dir, side, t) fills the ghost zones of
+ * that side in the arrays orc_array exposes; called inside SetBoundaries, before the normal
+ * field of the direction is reconstructed. */
+typedef void (*orc_userdef_fn)(void *arg, int dir, int side, double t);
+void orc_set_userdef(void *o, orc_userdef_fn fn, void *arg);
 
 /* Individual steps of the stage (same granularity as the reference kernels) */
 void orc_set_boundaries(void *o);

@@ -17,7 +17,8 @@ _LIB = os.path.join(_HERE, "liboracle.so")
 RECON = {"constant": 1, "plm": 2, "linear": 2, "limo3": 3, "ppm": 4, "parabolic": 4}
 SOLVER = {"hllc": 1, "hlld": 2, "roe": 3, "hll": 4, "tvdlf": 5}
 EMF = {"arithmetic": 0, "uct0": 1, "uct_contact": 2, "uct_hll": 3, "uct_hlld": 4}
-BC = {"periodic": 0, "outflow": 1, "reflective": 2}
+BC = {"periodic": 0, "outflow": 1, "reflective": 2, "userdef": 3}
+USERDEF_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double)
 
 ARRAYS = ["Vc", "Vs", "Uc", "InvDt", "cMax", "Flux", "ex", "ey", "ez", "ezi", "ezj",
           "exj", "exk", "eyi", "eyk", "svx", "svy", "svz", "axL", "axR", "ayL", "ayR",
@@ -60,6 +61,7 @@ def lib():
             "orc_layout": (None, [vp, ip, ip, ip, ip]),
             "orc_array": (dp, [vp, i, lp]), "orc_coord": (dp, [vp, i, i]),
             "orc_set_boundaries": (None, [vp]), "orc_prim_to_cons": (None, [vp]),
+            "orc_set_userdef": (None, [vp, USERDEF_FN, vp]),
             "orc_set_iso_sound_speed": (i, [vp, dp]),
             "orc_cons_to_prim": (None, [vp]), "orc_reset_stage": (None, [vp]),
             "orc_calc_flux": (None, [vp, i]), "orc_calc_rhs": (None, [vp, i, d]),
@@ -224,6 +226,16 @@ class Oracle:
 
     def copy_begin(self):
         lib().orc_copy_begin(self._h)
+
+    def set_userdef(self, fn):
+        """Enrolls fn(dir, side, t) for the user-defined sides: it writes the ghost zones of
+        that side into self.Vc / self.Vs (numpy views of the oracle's arrays)."""
+        if fn is None:
+            self._ufn = None
+            lib().orc_set_userdef(self._h, USERDEF_FN(0), None)
+            return
+        self._ufn = USERDEF_FN(lambda arg, d, side, t: fn(d, side, t))
+        lib().orc_set_userdef(self._h, self._ufn, None)
 
     def set_iso_sound_speed(self, cs):
         """Per-cell isothermal sound speed ([k][j][i] over the whole array, ghosts

@@ -1,0 +1,245 @@
+"""GPU checks of the configured workloads at small size and of the long runs, through the
+C ABI: the 3-D Kelvin-Helmholtz state, FP32 stencil arithmetic in 3-D against the FP64
+oracle, the learned limiter in 3-D, the reference's linear-wave test, the Sod and
+Orszag-Tang runs to their end times, the 3-D and FP32 per-cell isothermal sound speed
+and the fused learned path against the per-kernel schedule.
+
+Long-run tolerance: the GPU path repeats the oracle's FP64 operations in the same order,
+so a long run stays within 1e-12 relative per cell of the oracle (it is bit-identical in
+practice); L1 = mean |GPU - oracle| over the interior is stated per test.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from idefix_amd import problems
+from oracle import oracle as O
+from tests.exact_riemann import sample
+from tests.pairs import cfg_kwargs, compare_state, make_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _cycle_both(o, g, ncycles):
+    for _ in range(ncycles):
+        assert o.cycle() == 0
+        g.cycle()
+        assert g.t == pytest.approx(o.t, rel=1e-14, abs=0)
+        assert g.dt == pytest.approx(o.dt, rel=1e-12, abs=0)
+
+
+def _kh(gg):
+    return problems.kelvin_helmholtz(gg)
+
+
+def test_kelvin_helmholtz_3d_fp64():
+    """The Kelvin-Helmholtz configuration (PPM, HLLD, CT, RK2 on [0,1]^2 x [0,1/2]) at
+    32 x 32 x 16."""
+    kw = cfg_kwargs(3, (32, 32, 16), emf="uct_contact", recon="ppm", cfl=0.4, nstages=2,
+                    xend=(1.0, 1.0, 0.5), comps=3)
+    o, g = make_pair(kw, _kh)
+    o.set_boundaries()
+    _cycle_both(o, g, 4)
+    compare_state(o, g, what="KH 3-D")
+
+
+@pytest.mark.parametrize("case", ["turbulence", "kelvin_helmholtz"])
+def test_fp32_stencil_3d_close_to_fp64(case):
+    """FP32 reconstruction, Riemann solver and EMFs in 3-D (the FP32 legs of the 3-D
+    turbulence and Kelvin-Helmholtz configurations) stay within single-precision error of
+    the FP64 oracle: max |GPU - oracle| <= 1e-4 x field scale after 4 cycles, with the
+    FP64 accumulation keeping div B at round-off."""
+    if case == "turbulence":
+        kw = cfg_kwargs(3, (24, 20, 16), emf="uct_contact", recon="ppm", cfl=0.4,
+                        nstages=3, comps=3)
+        ic = problems.turbulence
+    else:
+        kw = cfg_kwargs(3, (32, 32, 16), emf="uct_contact", recon="ppm", cfl=0.4,
+                        nstages=2, xend=(1.0, 1.0, 0.5), comps=3)
+        ic = _kh
+    o, g = make_pair(kw, ic, precision="fp32")
+    o.set_boundaries()
+    for _ in range(4):
+        assert o.cycle() == 0
+        g.cycle()
+    assert g.dt == pytest.approx(o.dt, rel=1e-5)
+    Vc_g, _ = g.download()
+    for n in range(o.nvar):
+        a = o.interior(Vc_g)[n]
+        b = o.interior(o.Vc)[n]
+        assert np.max(np.abs(a - b)) <= 1e-4 * max(1.0, np.max(np.abs(b))), n
+    assert g.check_divb() < 1e-11
+
+
+def _mlp3d(recon, precision="fp64"):
+    from idefix_amd.datablock import DataBlock
+    g = DataBlock(3, (20, 16, 12), comps=3, mhd=True, recon=recon, solver="hlld",
+                  emf="uct_contact", precision=precision, cfl=0.4, nstages=3)
+    Vc, Vs = problems.turbulence(g, seed=11)
+    g.upload(Vc, Vs)
+    return g
+
+
+def test_learned_3d_without_network_tracks_ppm():
+    """The 3-D learned mode without a network takes the PPM weights on every interface:
+    after three RK3 cycles it matches the PPM run to the float precision of the stored
+    weights (an index or direction error would show at O(1))."""
+    a = _mlp3d("mlp")
+    a.set_mlp_params(None)
+    b = _mlp3d("ppm")
+    for _ in range(3):
+        a.cycle()
+        b.cycle()
+    Va, _ = a.download()
+    Vb, _ = b.download()
+    ia, ib = a.interior(Va), b.interior(Vb)
+    scale = np.max(np.abs(ib), axis=(1, 2, 3), keepdims=True)
+    assert np.max(np.abs(ia - ib) / scale) < 1e-5
+
+
+def test_learned_3d_with_network_conserves_and_keeps_div_b():
+    """With a network loaded the 3-D learned mode (FP32 stencil) conserves mass, momentum
+    and energy in the periodic box and keeps div B at round-off: the weights only change
+    the interface states inside the flux form."""
+    import torch
+    from idefix_amd import learned
+    g = _mlp3d("mlp", "fp32")
+    p = learned.init_params(5)
+    p = p + np.random.default_rng(6).normal(0, 0.05, p.shape).astype(np.float32)
+    g.set_mlp_params(p)
+
+    def totals():
+        b, e = g.beg, g.end
+        U = g.Uc[:, b[2]:e[2], b[1]:e[1], b[0]:e[0]].double()
+        return [float(torch.sum(U[n])) for n in range(5)]
+    g.cycle()
+    t1 = totals()
+    for _ in range(3):
+        g.cycle()
+    t4 = totals()
+    for a, b in zip(t1, t4):
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(a))
+    assert g.check_divb() < 1e-10
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_linear_wave_reference_test(mode):
+    """test/MHD/LinearWaveTest on the GPU: 64x32x32, Roe, PLM, uct_contact, CFL 0.9, one
+    wave period; every cycle's dt and the final state equal the oracle's (1e-12), and the
+    GPU result meets the reference's bound error * 1e6 < 3e-2."""
+    kw = cfg_kwargs(3, (64, 32, 32), solver="roe", emf="uct_contact", recon="plm", cfl=0.9,
+                    nstages=3 if mode == 2 else 2, xend=(3.0, 1.5, 1.5), comps=3)
+    period = []
+
+    def ic(gg):
+        Vc, Vs, per = problems.linear_wave_3d(gg, mode)
+        period.append(per)
+        return Vc, Vs
+    o, g = make_pair(kw, ic)
+    o.set_boundaries()
+    g.set_boundaries()
+    V0 = g.download()
+    assert o.run(period[0]) == 0
+    g.run(period[0])
+    assert g.ncycles == o.ncycles and g.t == pytest.approx(o.t, abs=1e-14)
+    compare_state(o, g, what="linear wave mode %d" % mode)
+    err = problems.linear_wave_error(V0, g.download(), g.beg, g.end)
+    assert err * 1e6 < 3e-2, err
+
+
+@pytest.mark.parametrize("solver", ["hllc", "roe"])
+def test_sod_to_end_time(solver):
+    """The Sod configuration (1000 cells, PLM, RK2, CFL 0.8) run to t = 0.2 on the GPU:
+    the final state equals the oracle's (L1 <= 1e-14, max rel 1e-12) and meets the
+    reference's acceptance bound mean |p - p_exact| < 2e-3
+    (test/HD/sod/python/testidefix.py:71-77)."""
+    kw = cfg_kwargs(1, (1000,), mhd=False, solver=solver, recon="plm", bc="outflow",
+                    gamma=1.4, cfl=0.8)
+    o, g = make_pair(kw, problems.sod)
+    o.set_boundaries()
+    assert o.run(0.2) == 0
+    g.run(0.2)
+    assert g.ncycles == o.ncycles
+    compare_state(o, g, what="Sod to t=0.2")
+    Vc, _ = g.download()
+    x = g.x[0][g.beg[0]:g.end[0]]
+    pg = Vc[2, 0, 0, g.beg[0]:g.end[0]]
+    po = o.interior(o.Vc)[2, 0, 0]
+    assert np.mean(np.abs(pg - po)) <= 1e-14
+    exact = sample(x, 0.2, 0.5, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), gamma=1.4)
+    assert np.mean(np.abs(pg - exact[2])) < 2e-3
+
+
+def test_orszag_tang_to_end_time():
+    """Orszag-Tang (PLM, HLLD, UCT-HLLD, RK2, CFL 0.6) at 256^2 to t = 0.5 (about 960
+    cycles, through the formation of the shocks): every cell within 1e-12 relative of the
+    oracle, L1 over the interior <= 1e-13 x field scale, div B at round-off.  (512^2 takes
+    the oracle about 11 minutes on 8 cores; 256^2 is the long-run check.)"""
+    kw = cfg_kwargs(2, (256, 256), emf="uct_hlld", recon="plm", cfl=0.6)
+    o, g = make_pair(kw, problems.orszag_tang)
+    O.set_threads(min(16, os.cpu_count() or 1))
+    o.set_boundaries()
+    assert o.run(0.5) == 0
+    g.run(0.5)
+    assert g.ncycles == o.ncycles and g.t == pytest.approx(0.5, abs=1e-14)
+    compare_state(o, g, what="OT 256^2 to t=0.5")
+    Vc, _ = g.download()
+    for n in range(o.nvar):
+        a, b = o.interior(Vc)[n], o.interior(o.Vc)[n]
+        assert np.mean(np.abs(a - b)) <= 1e-13 * max(1.0, np.max(np.abs(b)))
+    assert g.check_divb() < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_iso_cell_sound_speed_3d(precision):
+    """Per-cell isothermal sound speed in 3-D (z faces of the march kernel) and with FP32
+    stencil arithmetic: FP64 within 1e-12 of the oracle, FP32 within 1e-4 x scale."""
+    kw = cfg_kwargs(3, (16, 12, 10), mhd=True, solver="hlld", recon="plm", emf="uct_contact",
+                    cfl=0.4, comps=3, csiso=1.0)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=43, amp=0.2),
+                     precision=precision)
+    nx, ny, nz = o.np_tot
+    x = o.coord(0, "x")[None, None, :]
+    z = o.coord(2, "x")[:, None, None]
+    cs = (1.0 + 0.3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z)) * np.ones((nz, ny, nx))
+    o.set_iso_sound_speed(cs)
+    g.set_iso_sound_speed(cs)
+    o.set_boundaries()
+    for _ in range(3):
+        assert o.cycle() == 0
+        g.cycle()
+    if precision == "fp64":
+        compare_state(o, g, what="iso cs(x,z) 3-D")
+    else:
+        Vc_g, _ = g.download()
+        for n in range(o.nvar):
+            a, b = o.interior(Vc_g)[n], o.interior(o.Vc)[n]
+            assert np.max(np.abs(a - b)) <= 1e-4 * max(1.0, np.max(np.abs(b)))
+
+
+def test_fused_learned_path_bitwise_equals_per_kernel():
+    """With a network loaded, the learned limiter through the fused 2-D y kernel and
+    through the per-kernel schedule gives bit-identical interiors and dt."""
+    from idefix_amd import learned
+    from idefix_amd.datablock import DataBlock
+    p = learned.init_params(8)
+    p = p + np.random.default_rng(9).normal(0, 0.05, p.shape).astype(np.float32)
+    blocks = []
+    for level in (1, 0):
+        g = DataBlock(2, (133, 97), comps=3, mhd=True, recon="mlp", solver="hlld",
+                      emf="uct_contact", precision="fp32", cfl=0.4, nstages=3)
+        g.set_fusion(level)
+        Vc, Vs = problems.turbulence(g, seed=4)
+        g.upload(Vc, Vs)
+        g.set_mlp_params(p)
+        blocks.append(g)
+    for _ in range(4):
+        for g in blocks:
+            g.cycle()
+        assert blocks[0].dt == blocks[1].dt
+    (a_c, a_s), (b_c, b_s) = blocks[0].download(), blocks[1].download()
+    b, e = blocks[0].beg, blocks[0].end
+    assert np.array_equal(a_c[:, :, b[1]:e[1], b[0]:e[0]], b_c[:, :, b[1]:e[1], b[0]:e[0]])
+    assert np.array_equal(a_s[0][:, b[1]:e[1], b[0]:e[0] + 1], b_s[0][:, b[1]:e[1], b[0]:e[0] + 1])
+    assert np.array_equal(a_s[1][:, b[1]:e[1] + 1, b[0]:e[0]], b_s[1][:, b[1]:e[1] + 1, b[0]:e[0]])
