@@ -33,7 +33,7 @@ EXPORTS = [
     "idfx_mlp_get_params", "idfx_mlp_loss_grad", "idfx_mlp_train_step", "idfx_mlp_loss",
     "idfx_ctx_set_mlp_params", "idfx_mlp_faces", "idfx_ctx_set_iso_sound_speed",
     "idfx_enforce_boundary_dir", "idfx_reconstruct_normal_field",
-    "idfx_ctx_set_userdef_boundary", "idfx_ctx_set_time",
+    "idfx_ctx_set_userdef_boundary", "idfx_ctx_set_time", "idfx_compute_timestep_noalloc",
 ]
 
 # idfx_userdef_boundary_fn: (arg, dir, side, t, Vc, Vs, stream)
@@ -113,6 +113,7 @@ def load():
         "idfx_reconstruct_normal_field": [vp, i, vp, vp],
         "idfx_ctx_set_userdef_boundary": [vp, USERDEF_FN, vp],
         "idfx_ctx_set_time": [vp, d],
+        "idfx_compute_timestep_noalloc": [vp, vp, dp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

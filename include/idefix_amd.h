@@ -186,6 +186,17 @@ int idfx_evolve_stage(idfx_ctx *ctx, double t, double dt, const double *Vc, doub
  * 1/InvDt.  Blocking (returns a host scalar). */
 int idfx_compute_timestep(idfx_ctx *ctx, const double *InvDt, double *dtmin, void *stream);
 
+/* No counterpart in the reference: idfx_compute_timestep for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result as idfx_compute_timestep (blocking), but it calls no allocating routine:
+ * its reduction scratch is a static device array of about 32 KiB per device (4096 per-block
+ * partial maxima and one result word), so a block may hold at most 2^28 interior cells
+ * (IDFX_E_ARGS beyond, with a message on stderr).  Every call on a device shares that
+ * device's array: calls that overlap on one GPU (several host threads, or streams running
+ * concurrently) race and must be serialised by the caller. */
+int idfx_compute_timestep_noalloc(idfx_ctx *ctx, const double *InvDt, double *dtmin,
+                                  void *stream);
+
 /* StateContainer::AddAndStore (stateContainer.cpp:75-104): U = wl*U + wr*U0, n doubles. */
 int idfx_add_and_store(idfx_ctx *ctx, double wl, double wr, double *U, const double *U0,
                        long long n, void *stream);
