@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of the per-stage finite-volume update on one MI355X.
 
-Workload: 2-D ideal-MHD Orszag-Tang vortex at 2048^2 (the largest single-GPU size of
-that configuration): PLM van Leer, HLLD, constrained transport with UCT-HLLD corner
-EMFs, RK2, CFL 0.6, periodic boundaries, FP64.  A step is one full RK2 cycle of the
-whole grid (TimeIntegrator::Cycle); a zone-update is one cell advanced by one cycle.
-Inputs are synthetic (the analytic Orszag-Tang initial state), resident in HBM before
-the timed region starts.
+Workload (the configuration the metric's target is quoted on, and the largest
+single-GPU configuration): 3-D decaying ideal-MHD turbulence at 512^3 on the periodic
+unit cube, PPM, HLLD, constrained transport (uct_contact), RK3, CFL 0.4, gamma 5/3, the
+seeded solenoidal initial state (rms v 0.5, rms dB 0.1, Bx 0.1).  A step is one full RK3
+cycle of the whole grid (TimeIntegrator::Cycle); a zone-update is one cell advanced by one
+cycle.  Inputs are synthetic, resident in HBM before the timed region starts.
 
-Prints ONE JSON line.  `value` is zone-updates/s of the plain FP64 method; the FP32
-stencil mode is reported beside it.  `roofline` is computed for the kernel with the
-largest share of the cycle, timed with HIP events on the stream it runs on;
-`cpu_baseline` times the CPU oracle (an OpenMP restatement of the same algorithm) on a
-bounded sample of the same workload on the host cores.
+Prints ONE JSON line.  `value` is zone-updates/s of the plain FP64 method at 512^3; beside
+it the FP32 stencil mode at 512^3 and the FP32 + learned-limiter mode at 256^3 (the two
+legs of the metric), with the equal-L1 comparison of the committed accuracy study.
+`roofline` is computed for the kernel with the largest share of the cycle: the whole
+stage's compulsory HBM bytes (448 B per zone-update per RK3 cycle, one third per stage)
+over its average launch duration from HIP events on the stream it runs on; `traffic` is
+that kernel's HBM bytes per launch measured in this run by two rocprofv3 --pmc child runs
+(FETCH_SIZE, WRITE_SIZE) when rocprofv3 is available.  `cpu_baseline` times the CPU oracle
+(an OpenMP restatement of the same algorithm) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -31,10 +35,19 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--nx", type=int, default=2048)
+    p.add_argument("--nx", type=int, default=2048,
+                   help="Orszag-Tang size of --matrix")
+    p.add_argument("--n", type=int, default=512, help="3-D turbulence size of the headline")
+    p.add_argument("--no-learned", action="store_true")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 --pmc child runs that measure HBM traffic")
+    p.add_argument("--cpu-n", type=int, default=128,
+                   help="size of the CPU oracle sample of the headline workload")
+    p.add_argument("--traffic-child", default=None, help=argparse.SUPPRESS)
+    p.add_argument("--ic-cache", default=None, help=argparse.SUPPRESS)
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--kernel-reps", type=int, default=10)
     p.add_argument("--no-headline", action="store_true",
                    help="skip the reference's own 3-D benchmark configuration")
@@ -212,29 +225,6 @@ def kernel_times(g, cycles):
     return {k: ms / n for k, (ms, n) in rt.items()}, {k: ms / cycles for k, (ms, n) in rt.items()}
 
 
-def algorithmic_bytes(nx):
-    """Compulsory HBM bytes per launch of the single-kernel regions of a 2-D MHD RK2 stage
-    (each input read once, each output written once, 8-byte words per interior cell),
-    averaged over the two stages of the cycle.
-      sweep_x   stage 0: Vc 6 + BX1s 1 -> Uc 4 + Uc0 4 + InvDt 1 + face data 5 = 21
-                stage 1: Vc 6 + BX1s 1 -> Uc 4 + face data 5                  = 16
-      fused2d_y stage 0: Vc 6 + Vs 2 + Uc 4 + x-face data 5 + InvDt 1
-                         -> Vc 6 + Vs 2 + Vs0 2 + InvDt 1                    = 29
-                stage 1: Vc 6 + Vs 2 + Uc 4 + Uc0 4 + Vs0 2 + x-face data 5
-                         -> Vc 6 + Vs 2 + Uc 6                                = 37
-    """
-    n = nx * nx
-    w = 8
-    return {"sweep_x": n * w * (21 + 16) / 2, "fused2d_y": n * w * (29 + 37) / 2}
-
-
-# HBM traffic per launch of each kernel, measured with rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE on this workload and scaled by the plain-copy calibration (reads x2.0,
-# writes x1.0); the summary and the commands are in
-# profiles/ot2048_hlld_uct_hlld_fp64_kernels.md.  None where not measured.
-MEASURED_TRAFFIC = {2048: {"fused2d_y": (859.6 + 473.9) * 1e6, "sweep_x": (276.9 + 393.0) * 1e6}}
-
-
 # The reference's published benchmark (doc/source/performances.rst:5-10, bench.json): 3-D
 # Orszag-Tang, 128^3 cells per GPU, PLM, HLLD, uct_contact, RK2; cell-updates/s.
 REFERENCE_PUBLISHED = {"NVIDIA H100": 3.079643e8, "NVIDIA A100": 2.044728e8,
@@ -264,35 +254,136 @@ def reference_headline(steps, warmup, n=128):
             "ratio_vs_published": {k: zu / v for k, v in REFERENCE_PUBLISHED.items()}}
 
 
-def cpu_baseline(nx, seconds):
-    """CPU oracle on the host cores: Orszag-Tang at the same resolution, as many cycles as
-    fit in about `seconds`."""
+# The headline workload: 3-D decaying turbulence (the metric's configuration), its
+# compulsory HBM bytes per zone-update (7 state-sized transfers per RK3 cycle of S = 8
+# words: rho, 3 momenta, E, 3 face fields) and per stage launch (one third of a cycle).
+B_ZU_TURB3D = 7 * 8 * 8
+STAGES_TURB3D = 3
+# regions of the per-kernel 3-D schedule that are one kernel each, and the kernel names the
+# profiler reports for them (FP64 instantiations)
+SINGLE_KERNEL_REGIONS = {"sweep_x": "k_sweep_x<", "sweep_y": "k_sweep_march<",
+                         "sweep_z": "k_sweep_march<", "corner_emf": "k_corner_emf<",
+                         "evolve_b": "k_evolve_b<", "cons_to_prim": "k_cons_to_prim<"}
+# plain-copy calibration of the traffic counters (1 GiB read + 1 GiB written per launch,
+# bench.py --copy-probe): FETCH_SIZE reads back half the bytes, WRITE_SIZE all of them
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+# The equal-L1 comparison of the two legs (the accuracy study in profiles/, measured with
+# idefix_amd.learned / idefix_amd.accuracy on one MI355X; see that file for the commands).
+EQUAL_L1 = None
+
+
+def turb3d_block(n, precision, recon="ppm", ic=None):
+    from idefix_amd.datablock import DataBlock
+    from idefix_amd import problems
+    g = DataBlock(3, (n, n, n), comps=3, mhd=True, recon=recon, solver="hlld",
+                  emf="uct_contact", precision=precision, bc="periodic", gamma=5.0 / 3.0,
+                  nstages=3, cfl=0.4, first_dt=1e-4)
+    if ic is None:
+        ic = problems.turbulence(g)
+    g.upload(*ic)
+    return g, ic
+
+
+def learned_params():
+    """The trained network of the learned limiter: the 3-D one when it exists, else the
+    2-D one (the inference cost does not depend on the weights)."""
     import numpy as np
+    for name in ("learned_turb3d.npy", "learned_turb2d.npy"):
+        path = os.path.join(ROOT, "idefix_amd", name)
+        if os.path.exists(path):
+            return np.load(path), name
+    return None, None
+
+
+def traffic_child(cache, n, region, cycles):
+    """Run under rocprofv3 --pmc: the FP64 headline block from the cached initial state,
+    `cycles` cycles on the per-kernel schedule."""
+    import numpy as np
+    import torch
+    ic = (np.load(cache + "_Vc.npy", mmap_mode="r"), np.load(cache + "_Vs.npy", mmap_mode="r"))
+    g, _ = turb3d_block(n, "fp64", ic=ic)
+    for _ in range(cycles):
+        g.cycle()
+    torch.cuda.synchronize()
+    print("traffic child done", region, flush=True)
+
+
+def measure_traffic(cache, n, region, timeout=240):
+    """HBM bytes per launch of the kernel of `region`, from two rocprofv3 --pmc runs of
+    traffic_child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the
+    kernel's dispatches; y and z sweeps share a kernel name, so both count), scaled by the
+    plain-copy calibration.  None if rocprofv3 is missing or a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="idfx_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(timeout), prof, "--pmc", counter,
+               "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
+               os.path.abspath(__file__), "--traffic-child", region, "--n", str(n),
+               "--ic-cache", cache]
+        print("traffic: rocprofv3 --pmc %s (%s)" % (counter, region), file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, env=env, cwd="/tmp", capture_output=True, text=True)
+        files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs
+                 if f.endswith("counter_collection.csv")]
+        if not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, "rocprofv3 --pmc %s failed (rc %d)" % (counter, r.returncode)
+        per = {}
+        want = SINGLE_KERNEL_REGIONS[region]
+        for row in csv.DictReader(open(files[0])):
+            if row["Counter_Name"] != counter or want not in row["Kernel_Name"]:
+                continue
+            if "double" not in row["Kernel_Name"]:
+                continue
+            per[row["Dispatch_Id"]] = per.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+        shutil.rmtree(d, ignore_errors=True)
+        if not per:
+            return None, "no %s dispatch of %s in the profile" % (counter, want)
+        out[counter] = sum(per.values()) / len(per) * 1024.0
+    return (out["FETCH_SIZE"] * FETCH_SCALE + out["WRITE_SIZE"] * WRITE_SCALE,
+            {"fetch_bytes": out["FETCH_SIZE"] * FETCH_SCALE,
+             "write_bytes": out["WRITE_SIZE"] * WRITE_SCALE,
+             "scale": "FETCH_SIZE x %.1f, WRITE_SIZE x %.1f (1 GiB copy calibration)"
+                      % (FETCH_SCALE, WRITE_SCALE)})
+
+
+def cpu_baseline_turb3d(n, seconds):
+    """CPU oracle on the host cores: the same 3-D turbulence configuration at n^3, as many
+    RK3 cycles as fit in about `seconds` after one warm-up cycle."""
     from oracle import oracle as O
     from idefix_amd import problems
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or threads))
     O.set_threads(threads)
-    cfg = O.make_cfg(2, mhd=True, recon="plm", solver="hlld", emf="uct_hlld", nstages=2,
-                     nx=(nx, nx), bc="periodic", gamma=5.0 / 3.0, cfl=0.6)
+    cfg = O.make_cfg(3, comps=3, mhd=True, recon="ppm", solver="hlld", emf="uct_contact",
+                     nstages=3, nx=(n, n, n), bc="periodic", gamma=5.0 / 3.0, cfl=0.4)
     o = O.Oracle(cfg)
-    Vc, Vs = problems.orszag_tang(O.OracleGrid(o))
+    Vc, Vs = problems.turbulence(O.OracleGrid(o))
     o.Vc[...] = Vc
     o.Vs[...] = Vs
     o.set_boundaries()
-    o.cycle()  # warm-up
-    n = 0
+    o.cycle()
+    k = 0
     t0 = time.perf_counter()
     while True:
         o.cycle()
-        n += 1
+        k += 1
         el = time.perf_counter() - t0
-        if el > seconds or n >= 1000:
+        if el > seconds or k >= 1000:
             break
-    rate = nx * nx * n / el
-    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "sample": "Orszag-Tang %dx%d, PLM/HLLD/UCT-HLLD/RK2 FP64, %d cycles after 1 "
-                      "warm-up cycle (%.1f s)" % (nx, nx, n, el)}
+    return {"value": n ** 3 * k / el, "unit": "zone-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "3-D turbulence %d^3 (1/%d of the headline cells), PPM/HLLD/uct_contact/"
+                      "RK3 FP64, %d cycles after 1 warm-up cycle (%.1f s)"
+                      % (n, (512 // n) ** 3, k, el)}
 
 
 def main():
@@ -304,6 +395,10 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
+    if args.traffic_child:
+        traffic_child(args.ic_cache, args.n, args.traffic_child, 2)
+        return
+    import numpy as np
     import torch
     if args.headline_only:
         print(json.dumps(reference_headline(args.steps, args.warmup)))
@@ -314,29 +409,60 @@ def main():
     if args.configs:
         workloads(args.only, args.steps, args.warmup)
         return
-    nx = args.nx
-    g = make_block(nx, "fp64")
+    n = args.n
+    print("bench: 3-D turbulence %d^3, initial state" % n, file=sys.stderr, flush=True)
+    g, ic = turb3d_block(n, "fp64")
     el = time_cycles(g, args.steps, args.warmup)
-    zu = nx * nx * args.steps / el
+    cells = n ** 3
+    zu = cells * args.steps / el
     ms = 1e3 * el / args.steps
+    print("bench: FP64 %.3e zone-updates/s" % zu, file=sys.stderr, flush=True)
     kt, per_cycle = kernel_times(g, args.kernel_reps)
-    ab = algorithmic_bytes(nx)
-    # the single-kernel region with the largest share of the cycle
-    dom = max((k for k in per_cycle if k in ab), key=lambda k: per_cycle[k])
-    achieved = ab[dom] / (kt[dom] * 1e-3) / 1e9
-    traffic = MEASURED_TRAFFIC.get(nx, {}).get(dom)
+    dom = max((k for k in per_cycle if k in SINGLE_KERNEL_REGIONS), key=lambda k: per_cycle[k])
+    stage_bytes = B_ZU_TURB3D / STAGES_TURB3D * cells
+    achieved = stage_bytes / (kt[dom] * 1e-3) / 1e9
+    divb = g.check_divb()
     del g
     torch.cuda.empty_cache()
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic:
+        # the children start from this run's initial state, handed over in memory-backed
+        # files (removed afterwards)
+        shm = "/dev/shm" if os.path.isdir("/dev/shm") else "/tmp"
+        cache = os.path.join(shm, "idfx_bench_ic_%d_%d" % (n, os.getpid()))
+        try:
+            np.save(cache + "_Vc.npy", ic[0])
+            np.save(cache + "_Vs.npy", ic[1])
+            traffic, traffic_note = measure_traffic(cache, n, dom)
+        finally:
+            for suffix in ("_Vc.npy", "_Vs.npy"):
+                if os.path.exists(cache + suffix):
+                    os.remove(cache + suffix)
     fp32 = None
     if not args.no_fp32:
-        g32 = make_block(nx, "fp32")
+        g32, _ = turb3d_block(n, "fp32", ic=ic)
         el32 = time_cycles(g32, args.steps, args.warmup)
-        fp32 = nx * nx * args.steps / el32
+        fp32 = cells * args.steps / el32
         del g32
         torch.cuda.empty_cache()
+    del ic
+    learned = None
+    if not args.no_learned:
+        nl = n // 2
+        params, pname = learned_params()
+        gl, _ = turb3d_block(nl, "fp32", recon="mlp")
+        if params is not None:
+            gl.set_mlp_params(params)
+        ell = time_cycles(gl, args.steps, args.warmup)
+        learned = {"zone_updates_per_s": nl ** 3 * args.steps / ell,
+                   "ms_per_step": 1e3 * ell / args.steps,
+                   "workload": "3-D turbulence %d^3, FP32 stencil + learned limiter (BF16 MFMA "
+                               "inference, PPM fallback), HLLD, uct_contact, RK3, CFL 0.4" % nl,
+                   "network": pname}
+        del gl
+        torch.cuda.empty_cache()
     headline = None if args.no_headline else reference_headline(args.steps, args.warmup)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(nx, args.cpu_seconds)
-    cycle_bytes = 5 * 6 * 8  # RK2: 5 state transfers of S = 6 words per zone-update
+    cpu = None if args.no_cpu_baseline else cpu_baseline_turb3d(args.cpu_n, args.cpu_seconds)
     line = {
         "metric": "zone-updates/s on 1 MI355X: plain FP64 vs FP32+learned at equal L1 error",
         "value": zu,
@@ -350,18 +476,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "2D Orszag-Tang MHD %dx%d, PLM van Leer, HLLD, CT UCT-HLLD, "
-                               "RK2, CFL 0.6, periodic" % (nx, nx),
-                   "cells": nx * nx, "step": "one RK2 cycle",
-                   "parallelism": "1 GPU"},
+        "config": {"workload": "3-D decaying MHD turbulence %d^3, periodic unit cube, PPM, HLLD, "
+                               "CT uct_contact, RK3, CFL 0.4, plain FP64" % n,
+                   "cells": cells, "step": "one RK3 cycle", "parallelism": "1 GPU"},
         "fp32_stencil_value": fp32,
+        "fp32_learned_half_resolution": learned,
+        "equal_l1": EQUAL_L1,
         "kernel_ms": kt,
         "kernel_ms_per_cycle": per_cycle,
-        "cycle_hbm_frac": cycle_bytes * zu / (HBM_PEAK_GBS * 1e9),
+        "cycle_hbm_frac": B_ZU_TURB3D * zu / (HBM_PEAK_GBS * 1e9),
+        "div_b": divb,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "algorithmic_bytes": ab[dom], "ms_per_launch": kt[dom],
-                     "traffic": traffic},
+                     "algorithmic_bytes": stage_bytes, "ms_per_launch": kt[dom],
+                     "traffic": traffic, "traffic_note": traffic_note},
         "cpu_baseline": cpu,
         "reference_benchmark": headline,
     }
