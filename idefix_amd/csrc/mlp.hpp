@@ -6,14 +6,17 @@
 // primitives in the face frame (rho, v_n, v_t, v_b, p, B_n, B_t, B_b).  Features:
 // per variable c = (V[i-1] + V[i]) / 2, s = max |V_j - c|, x_j = (V_j - c) / s (0 when
 // s = 0), ordered [variable][cell].  Network 48 -> 64 -> 64 -> 16 with
-// gelu(z) = z sigmoid(1.702 z) on the hidden layers and sigmoid outputs, the limiter
-// weights w (2v: wL_v, 2v+1: wR_v); the interface states are
-// vL = V[i-1] + wL (V[i] - V[i-1]), vR = V[i] - wR (V[i] - V[i-1]).  Per variable the
-// weights must lie in the TVD region of slope limiters (Sweby 1984): each state's
-// deviation from its cell value has the sign of the cell's other one-sided difference
-// and does not exceed it, and 0 <= w <= 1.  A variable that fails, any variable of an
-// interface with non-finite input, and every interface when no network is loaded take
-// the weights of PPM (the member w = (v_PPM - V[i-1]) / (V[i] - V[i-1])).
+// gelu(z) = z sigmoid(1.702 z) on the hidden layers and linear outputs z3 that correct
+// the limiter weights of PPM: wL_v = wL_PPM + z3[2v], wR_v = wR_PPM - z3[2v] (output 2v+1
+// unused), where the PPM member of the family is w_PPM = (v_PPM - V[i-1]) / (V[i] - V[i-1]);
+// the interface states are vL = V[i-1] + wL (V[i] - V[i-1]), vR = V[i] - wR (V[i] - V[i-1]).
+// The correction shifts both states of a variable by the same amount, so the jump
+// vR - vL that sets the Riemann solver's upwind dissipation stays PPM's, and the BF16
+// rounding of the inputs perturbs only the shift.  Per variable the weights must lie in the TVD region of slope
+// limiters (Sweby 1984): each state's deviation from its cell value has the sign of the
+// cell's other one-sided difference and does not exceed it, and 0 <= w <= 1.  A variable
+// that fails, any variable of an interface with non-finite input, and every interface
+// when no network is loaded take the PPM weights.
 //
 // MFMA mapping (v_mfma_f32_16x16x16_bf16): a layer computes Z^T = W^T H^T with the
 // weights as the A operand (rows = output units) and the activations as the B operand
@@ -214,17 +217,17 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
             *reinterpret_cast<const mlp_bf16x4 *>(&wsh[W3 + rr * 64 + 16 * ks + 4 * g4]);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[ks], acc, 0, 0, 0);
       }
-      wal = mlp_sigmoid(acc[0] + blds[128 + 4 * g4]);
-      war = mlp_sigmoid(acc[1] + blds[128 + 4 * g4 + 1]);
-      wbl = mlp_sigmoid(acc[2] + blds[128 + 4 * g4 + 2]);
-      wbr = mlp_sigmoid(acc[3] + blds[128 + 4 * g4 + 3]);
+      // the network's corrections of the four PPM weights this lane holds
+      wal = acc[0] + blds[128 + 4 * g4];
+      war = acc[1] + blds[128 + 4 * g4 + 1];
+      wbl = acc[2] + blds[128 + 4 * g4 + 2];
+      wbr = acc[3] + blds[128 + 4 * g4 + 3];
     }
     const bool use_net = fin && net.valid;
-    const bool need_a = !(use_net && mlp_tvd_ok(sa, wal, war));
-    const bool need_b = !(use_net && mlp_tvd_ok(sb, wbl, wbr));
-    // PPM weights where the check fails.  The limiter's quotients use v_rcp_f32 (1 ulp);
-    // a non-finite one only skips a limiter step, since every comparison with NaN or inf
+    // PPM weights of both variables.  The limiter's quotients use v_rcp_f32 (1 ulp); a
+    // non-finite one only skips a limiter step, since every comparison with NaN or inf
     // there keeps the unlimited value.
+    float pal, par, pbl, pbr;
     if constexpr (DIR == IDIR) {
       // x faces: the lanes of a group hold consecutive faces along x, so the PPM values
       // of a cell serve both of its faces: every lane computes them for its own cell
@@ -246,37 +249,24 @@ __global__ __launch_bounds__(256) void k_mlp_infer(GridDesc g, const double *Vc,
       }
       carry_a = __shfl(ra, 16 * g4 + 15);
       carry_b = __shfl(rb, 16 * g4 + 15);
-      float ppl, ppr;
-      mlp_ppm_weights(sa, pos_a, pa, la, ppl, ppr);
-      if (need_a) {
-        wal = ppl;
-        war = ppr;
-      }
-      mlp_ppm_weights(sb, false, pb, lb, ppl, ppr);
-      if (need_b) {
-        wbl = ppl;
-        wbr = ppr;
-      }
+      mlp_ppm_weights(sa, pos_a, pa, la, pal, par);
+      mlp_ppm_weights(sb, false, pb, lb, pbl, pbr);
     } else {
       // y and z faces: neighbouring lanes are neighbours across the faces, not along
-      // them.  One pass serves every lane that needs the fallback for one of its
-      // variables (a if it fails, else b), a second only lanes that need both.
-      if (need_a || need_b) {
-        float sx[6];
-#pragma unroll
-        for (int t = 0; t < 6; t++) sx[t] = need_a ? sa[t] : sb[t];
-        float l, r;
-        mlp_ppm(sx, need_a && pos_a, l, r);
-        if (need_a) {
-          wal = l;
-          war = r;
-        } else {
-          wbl = l;
-          wbr = r;
-        }
-      }
-      if (need_a && need_b) mlp_ppm(sb, false, wbl, wbr);
+      // them, so each lane computes the PPM values of both cells of each variable
+      mlp_ppm(sa, pos_a, pal, par);
+      mlp_ppm(sb, false, pbl, pbr);
     }
+    // corrected weights where they pass the check, else PPM's: the correction of variable
+    // v is output 2v, a common shift of both states (wL + z, wR - z), which keeps PPM's
+    // jump vR - vL at the interface; output 2v + 1 is not used
+    const float cal = pal + wal, car = par - wal, cbl = pbl + wbl, cbr = pbr - wbl;
+    const bool ok_a = use_net && mlp_tvd_ok(sa, cal, car);
+    const bool ok_b = use_net && mlp_tvd_ok(sb, cbl, cbr);
+    wal = ok_a ? cal : pal;
+    war = ok_a ? car : par;
+    wbl = ok_b ? cbl : pbl;
+    wbr = ok_b ? cbr : pbr;
     if (face_ok) {
       W[(4 * g4) * g.sv + pf] = wal;
       W[(4 * g4 + 1) * g.sv + pf] = war;

@@ -3,10 +3,12 @@
 // include/idefix_amd.h (idfx_mlp_*).  No counterpart in the reference.
 //
 // Parameters are packed as in oracle/mlp.py: W1 [48][64], b1 [64], W2 [64][64], b2 [64],
-// W3 [64][16], b3 [16] (row-major [in][out]), 8336 floats.  A batch is X [B][48]
-// (features) and TAU [B][8] (normalised targets (t - c) / s).  Loss = mean over rows of
-// sum_v eL_v^2 + eR_v^2 with eL = x2 + wL (x3 - x2) - tau, eR = x3 - wR (x3 - x2) - tau and
-// limiter weights w = sigmoid(z3).
+// W3 [64][16], b3 [16] (row-major [in][out]), 8336 floats.  A batch row of X holds
+// MLP_ROW = 72 floats: the 48 features, the interface's 16 PPM weights wp and 8 per-variable
+// loss weights om; TAU [B][8] holds the normalised targets (t - c) / s.  The network's 16
+// outputs z3 shift both PPM states of a variable, wL = wpL + z3[2v], wR = wpR - z3[2v].  Loss = mean over rows of
+// sum_v om_v^2 (eL_v^2 + eR_v^2) with eL = x2 + wL (x3 - x2) - tau,
+// eR = x3 - wR (x3 - x2) - tau.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,7 +23,7 @@ constexpr int NF = 48, NH = 64, NO = 16;
 constexpr int P_W1 = 0, P_B1 = P_W1 + NF * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + NH * NH,
               P_W3 = P_B2 + NH, P_B3 = P_W3 + NH * NO, P_N = P_B3 + NO;
 constexpr float GELU_K = 1.702f;
-constexpr float W_SCALE = 1.0f, W_SHIFT = 0.0f;  // limiter weight w = sigmoid(z3)
+constexpr int ROW = 72, ROW_WP = 48, ROW_OM = 64;  // batch row: features | wp | om
 constexpr int WG_ROWS = 512;  // rows per block of the weight-gradient reduction
 
 __device__ __forceinline__ float sig(float z) { return 1.0f / (1.0f + expf(-z)); }
@@ -31,20 +33,21 @@ __device__ __forceinline__ float gelu_d(float z) {
   return s + GELU_K * z * s * (1.0f - s);
 }
 
-// Z = A W + b, H = gelu(Z) (or Z when act == 0); one thread per (row, output unit).
-__global__ __launch_bounds__(256) void k_dense_fwd(const float *A, int K, const float *W,
+// Z = A W + b, H = gelu(Z) (or Z when act == 0); A has K columns in rows of lda floats;
+// one thread per (row, output unit).
+__global__ __launch_bounds__(256) void k_dense_fwd(const float *A, int K, int lda, const float *W,
                                                    const float *b, int N, float *Z, float *H,
                                                    int act, int B) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)B * N) return;
   const int r = (int)(t / N), j = (int)(t % N);
   float z = b[j];
-  for (int k = 0; k < K; k++) z += A[(long long)r * K + k] * W[k * N + j];
+  for (int k = 0; k < K; k++) z += A[(long long)r * lda + k] * W[k * N + j];
   Z[t] = z;
   if (H) H[t] = act ? gelu(z) : z;
 }
 
-// Output layer: y = sigmoid(z3), the row's loss, and dZ3 = dloss/dz3 (1/B included).
+// Output layer: wL = wpL + z3[2v], wR = wpR - z3[2v], the row's loss, and dZ3 = dloss/dz3 (1/B included).
 __global__ __launch_bounds__(256) void k_out_loss(const float *Z3, const float *X,
                                                   const float *TAU, int B, float *dZ3,
                                                   float *rowloss) {
@@ -52,18 +55,20 @@ __global__ __launch_bounds__(256) void k_out_loss(const float *Z3, const float *
   if (r >= B) return;
   float l = 0.0f;
   const float inv = 1.0f / (float)B;
+  const float *xr = X + (long long)r * ROW;
   for (int v = 0; v < 8; v++) {
-    const float x2 = X[(long long)r * NF + 6 * v + 2], x3 = X[(long long)r * NF + 6 * v + 3];
+    const float x2 = xr[6 * v + 2], x3 = xr[6 * v + 3];
     const float d = x3 - x2;
     const float tau = TAU[(long long)r * 8 + v];
-    const float yl = sig(Z3[(long long)r * NO + 2 * v]);
-    const float yr = sig(Z3[(long long)r * NO + 2 * v + 1]);
-    const float wl = W_SCALE * yl + W_SHIFT, wr = W_SCALE * yr + W_SHIFT;
+    const float om2 = xr[ROW_OM + v] * xr[ROW_OM + v];
+    const float z = Z3[(long long)r * NO + 2 * v];
+    const float wl = xr[ROW_WP + 2 * v] + z;
+    const float wr = xr[ROW_WP + 2 * v + 1] - z;
     const float el = x2 + wl * d - tau;
     const float er = x3 - wr * d - tau;
-    l += el * el + er * er;
-    dZ3[(long long)r * NO + 2 * v] = W_SCALE * 2.0f * el * d * inv * yl * (1.0f - yl);
-    dZ3[(long long)r * NO + 2 * v + 1] = -W_SCALE * 2.0f * er * d * inv * yr * (1.0f - yr);
+    l += om2 * (el * el + er * er);
+    dZ3[(long long)r * NO + 2 * v] = om2 * 2.0f * (el + er) * d * inv;
+    dZ3[(long long)r * NO + 2 * v + 1] = 0.0f;
   }
   rowloss[r] = l;
 }
@@ -79,11 +84,12 @@ __global__ __launch_bounds__(256) void k_dense_bwd(const float *dZ, int N, const
   dZp[t] = s * gelu_d(Zp[t]);
 }
 
-// Partial weight and bias gradients of one layer over WG_ROWS rows per block:
+// Partial weight and bias gradients of one layer over WG_ROWS rows per block (A has K
+// columns in rows of lda floats):
 // out[blk][off + k N + n] = sum_r A[r][k] dZ[r][n] (k < K), out[blk][off + K N + n] =
 // sum_r dZ[r][n].
-__global__ __launch_bounds__(256) void k_wgrad(const float *A, int K, const float *dZ, int N,
-                                               int B, float *partial, int off) {
+__global__ __launch_bounds__(256) void k_wgrad(const float *A, int K, int lda, const float *dZ,
+                                               int N, int B, float *partial, int off) {
   __shared__ float sa[16][NH + 1];
   __shared__ float sz[16][NH];
   const int total = (K + 1) * N;
@@ -97,7 +103,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *A, int K, const floa
     for (int e = threadIdx.x; e < 16 * NH; e += 256) {
       const int rr = e / NH, c = e % NH;
       const int r = rt + rr;
-      sa[rr][c] = (r < r1 && c < K) ? A[(long long)r * K + c] : 0.0f;
+      sa[rr][c] = (r < r1 && c < K) ? A[(long long)r * lda + c] : 0.0f;
       sz[rr][c] = (r < r1 && c < N) ? dZ[(long long)r * N + c] : 0.0f;
     }
     if (threadIdx.x < 16) sa[threadIdx.x][NH] = (rt + threadIdx.x < r1) ? 1.0f : 0.0f;
@@ -185,12 +191,12 @@ int mlp_alloc(idfx_mlp *h, float **p, long long n) {
 // forward + loss + backward + gradient into h->G (asynchronous on s)
 int grad_enqueue(idfx_mlp *h, const float *X, const float *TAU, int B, hipStream_t s) {
   const float *P = h->P;
-  hipLaunchKernelGGL(k_dense_fwd, dim3(nblocks((long long)B * NH)), dim3(256), 0, s, X, NF,
+  hipLaunchKernelGGL(k_dense_fwd, dim3(nblocks((long long)B * NH)), dim3(256), 0, s, X, NF, ROW,
                      P + P_W1, P + P_B1, NH, h->Z1, h->H1, 1, B);
   hipLaunchKernelGGL(k_dense_fwd, dim3(nblocks((long long)B * NH)), dim3(256), 0, s, h->H1, NH,
-                     P + P_W2, P + P_B2, NH, h->Z2, h->H2, 1, B);
+                     NH, P + P_W2, P + P_B2, NH, h->Z2, h->H2, 1, B);
   hipLaunchKernelGGL(k_dense_fwd, dim3(nblocks((long long)B * NO)), dim3(256), 0, s, h->H2, NH,
-                     P + P_W3, P + P_B3, NO, h->Z3, (float *)nullptr, 0, B);
+                     NH, P + P_W3, P + P_B3, NO, h->Z3, (float *)nullptr, 0, B);
   hipLaunchKernelGGL(k_out_loss, dim3(nblocks(B)), dim3(256), 0, s, h->Z3, X, TAU, B, h->dZ3,
                      h->rowloss);
   hipLaunchKernelGGL(k_dense_bwd, dim3(nblocks((long long)B * NH)), dim3(256), 0, s, h->dZ3, NO,
@@ -198,9 +204,12 @@ int grad_enqueue(idfx_mlp *h, const float *X, const float *TAU, int B, hipStream
   hipLaunchKernelGGL(k_dense_bwd, dim3(nblocks((long long)B * NH)), dim3(256), 0, s, h->dZ2, NH,
                      P + P_W2, NH, h->Z1, h->dZ1, B);
   const int nblk = (B + WG_ROWS - 1) / WG_ROWS;
-  hipLaunchKernelGGL(k_wgrad, dim3(nblk), dim3(256), 0, s, X, NF, h->dZ1, NH, B, h->partial, P_W1);
-  hipLaunchKernelGGL(k_wgrad, dim3(nblk), dim3(256), 0, s, h->H1, NH, h->dZ2, NH, B, h->partial, P_W2);
-  hipLaunchKernelGGL(k_wgrad, dim3(nblk), dim3(256), 0, s, h->H2, NH, h->dZ3, NO, B, h->partial, P_W3);
+  hipLaunchKernelGGL(k_wgrad, dim3(nblk), dim3(256), 0, s, X, NF, ROW, h->dZ1, NH, B, h->partial,
+                     P_W1);
+  hipLaunchKernelGGL(k_wgrad, dim3(nblk), dim3(256), 0, s, h->H1, NH, NH, h->dZ2, NH, B,
+                     h->partial, P_W2);
+  hipLaunchKernelGGL(k_wgrad, dim3(nblk), dim3(256), 0, s, h->H2, NH, NH, h->dZ3, NO, B,
+                     h->partial, P_W3);
   hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(P_N)), dim3(256), 0, s, h->partial, nblk, h->G);
   hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, s, h->rowloss, B, h->loss);
   return hipGetLastError() == hipSuccess ? IDFX_OK : IDFX_E_LAUNCH;

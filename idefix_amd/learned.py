@@ -83,7 +83,8 @@ class Trainer:
         return p
 
     def loss_grad(self, X, tau):
-        """X, tau: float32 cuda tensors [B, 48], [B, 8].  Returns (loss, grad numpy)."""
+        """X, tau: float32 cuda tensors [B, 72] (48 features, the 16 PPM weights, 8 loss
+        weights) and [B, 8] (normalised targets).  Returns (loss, grad numpy)."""
         g = np.zeros(param_count(), dtype=np.float32)
         loss = ctypes.c_double()
         capi.check(self._L.idfx_mlp_loss_grad(self.h, ctypes.c_void_p(X.data_ptr()),
@@ -179,9 +180,12 @@ def face_targets(V, bx, by, f):
     return tx[FRAME[0]], ty[FRAME[1]]
 
 
-def samples(Vc, targets, d):
-    """Features [F, 48] and normalised targets [F, 8] (float32, torch) of every coarse
-    interface along d (periodic stencils), from coarse primitives Vc [8, N, N]."""
+def samples(Vc, targets, d, wp):
+    """Training rows of every coarse interface along d (periodic stencils), from coarse
+    primitives Vc [8, N, N], the fine interface values `targets` [8, N, N] and the PPM
+    weights of the same interfaces wp [16, N, N] (face index = cell right of the face):
+    rows [F, 72] = features (48) | PPM weights (16) | loss weights (8, set to 1 here),
+    normalised targets [F, 8] and the stencil scales s [F, 8] (float32, torch)."""
     import torch
     Vf = Vc[FRAME[d]]
     dim = 2 if d == 0 else 1
@@ -195,7 +199,24 @@ def samples(Vc, targets, d):
     # targets outside [-1, 1] (beyond the stencil's range) are unreachable by any weight
     # and would dominate the squared loss; clip them
     tau = torch.where(s > 0, (t - c) / safe, torch.zeros_like(s)).clamp(-1.0, 1.0)
-    return x.reshape(-1, 48).float(), tau.float()
+    w = wp.permute(1, 2, 0).reshape(-1, 16).to(x.dtype)
+    rows = torch.cat([x.reshape(-1, 48), w, torch.ones_like(s)], dim=1)
+    return rows.float(), tau.float(), s.float()
+
+
+def ppm_face_weights(Vc, bxc, byc):
+    """PPM weights [16, N, N] of the x and y interfaces of a coarse periodic state, from the
+    inference kernel itself with no network loaded (the fallback path)."""
+    g = coarse_block(Vc, bxc, byc, "mlp", "fp32")
+    g.set_boundaries()
+    g.set_mlp_params(None)
+    b, e = g.beg, g.end
+    out = []
+    for d in (0, 1):
+        W = g.mlp_faces(d)
+        out.append(W[:, 0, b[1]:e[1], b[0]:e[0]].double().clone())
+    del g
+    return out
 
 
 def coarse_block(Vc, bxc, byc, recon, precision, nstages=3, cfl=0.4, emf="uct_contact",
@@ -285,6 +306,11 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default="learned_out")
+    ap.add_argument("--no-calibration", action="store_true",
+                    help="keep the a-priori network (no a-posteriori output scales)")
+    ap.add_argument("--loss-weights", choices=("unit", "abs"), default="unit",
+                    help="per-sample loss weights: 1 (scale-free interface errors) or the "
+                         "stencil scale over the variable's mean scale (absolute errors)")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     gamma = 5.0 / 3.0
@@ -294,22 +320,33 @@ def main(argv=None):
     f = a.factor
     N = a.fine // f
     # ---- training samples -----------------------------------------------------------
-    Xs, Ts = [], []
+    Xs, Ts, Ss = [], [], []
+    calib = []  # (coarse initial state, coarse reference at tend) of each training seed
     for seed in a.seeds:
-        for (t, V, bx, by) in fine_run(a.fine, seed, times, log=log):
+        snaps = fine_run(a.fine, seed, times, log=log)
+        calib.append((coarse_state(*snaps[0][1:], f, gamma), coarse_state(*snaps[-1][1:], f, gamma)))
+        for (t, V, bx, by) in snaps:
             Vc, bxc, byc = coarse_state(V, bx, by, f, gamma)
             tx, ty = face_targets(V, bx, by, f)
+            wps = ppm_face_weights(Vc, bxc, byc)
             for d, tgt in ((0, tx), (1, ty)):
-                X, T = samples(Vc, tgt, d)
+                X, T, S = samples(Vc, tgt, d, wps[d])
                 Xs.append(X)
                 Ts.append(T)
+                Ss.append(S)
         torch.cuda.empty_cache()
     X = torch.cat(Xs)
     T = torch.cat(Ts)
-    del Xs, Ts
+    S = torch.cat(Ss)
+    del Xs, Ts, Ss
+    if a.loss_weights == "abs":
+        # absolute interface errors, each variable on the scale of its mean local variation
+        X[:, 64:72] = S / S.mean(0, keepdim=True).clamp_min(1e-30)
     log("samples: %d" % X.shape[0])
     # ---- training -------------------------------------------------------------------
-    tr = Trainer(a.batch, init_params(0))
+    p0 = unpack(init_params(0))
+    p0["W3"][:] = 0.0  # the network starts as PPM (zero correction)
+    tr = Trainer(a.batch, pack(p0))
     gen = torch.Generator(device="cuda")
     gen.manual_seed(1234)
     hist = []
@@ -323,16 +360,25 @@ def main(argv=None):
             hist.append((step, tr.loss()))
             log("step %d loss %.5f" % hist[-1])
     params = tr.params()
+    np.save(os.path.join(a.out, "params_apriori.npy"), params)
+    # ---- a-posteriori calibration on the training seeds --------------------------------
+    alpha = np.zeros(8) if a.no_calibration else None
+    if not a.no_calibration:
+        alpha, cal_log = calibrate(params, calib, a.tend, log)
+        params = scale_outputs(params, alpha)
     np.save(os.path.join(a.out, "params.npy"), params)
     # the van Leer member's loss on the same data, for reference
     vl = van_leer_loss(X[:200000], T[:200000])
+    ppm = ppm_loss(X[:200000], T[:200000])
     # ---- evaluation on the held-out seed ---------------------------------------------
     fine = fine_run(a.fine, a.test_seed, [a.tend], log=log)
     (_, V0, bx0, by0), (_, V1, bx1, by1) = fine[0], fine[-1]
     ref = coarse_state(V1, bx1, by1, f, gamma)
     rep = {"fine": a.fine, "coarse": N, "factor": f, "tend": a.tend, "train_seeds": a.seeds,
            "test_seed": a.test_seed, "samples": int(X.shape[0]), "loss_history": hist,
-           "van_leer_loss": vl, "runs": {}}
+           "van_leer_loss": vl, "ppm_loss": ppm, "loss_weights": a.loss_weights,
+           "output_scales": [float(v) for v in alpha] if alpha is not None else None,
+           "calibration": None if a.no_calibration else cal_log, "runs": {}}
     init = coarse_state(V0, bx0, by0, f, gamma)
     cases = [("learned_fp32_%d" % N, N, "mlp", "fp32", params),
              ("plain_ppm_fp64_%d" % N, N, "ppm", "fp64", None),
@@ -353,10 +399,64 @@ def main(argv=None):
     log("done")
 
 
+def scale_outputs(params, alpha):
+    """The network with the correction of variable v (output 2v) scaled by alpha[v]."""
+    P = unpack(params)
+    for v in range(8):
+        P["W3"][:, 2 * v] *= alpha[v]
+        P["b3"][2 * v] *= alpha[v]
+    return pack(P)
+
+
+def calibrate(params, calib, tend, log):
+    """A-posteriori calibration of the a-priori network: per-variable scales alpha of its
+    corrections that minimise the mean L1 error of coarse learned runs against the
+    block-averaged fine solutions of the training seeds (alpha = 0 is PPM, so the result
+    is never worse than PPM on these seeds).  A line search of a common scale, then one
+    coordinate sweep per variable."""
+    def err(alpha):
+        p = scale_outputs(params, alpha)
+        e = []
+        for init, ref in calib:
+            Vo, bxo, byo, _, _ = run_coarse(*init, tend, "mlp", "fp32", p)
+            e.append(l1_error(Vo, bxo, byo, *ref)[0])
+        return float(np.mean(e))
+    hist = []
+    best_a, best_e = None, None
+    for c in (0.0, 0.15, 0.3, 0.5, 0.75, 1.0):
+        e = err(np.full(8, c))
+        hist.append(([c] * 8, e))
+        log("calibration: common scale %.2f -> L1 %.4e" % (c, e))
+        if best_e is None or e < best_e:
+            best_a, best_e = np.full(8, c), e
+    for v in range(8):
+        for m in (0.0, 0.5, 1.5, 2.0):
+            trial = best_a.copy()
+            trial[v] = best_a[v] * m if best_a[v] > 0 else 0.3 * m
+            e = err(trial)
+            hist.append((list(trial), e))
+            if e < best_e:
+                best_a, best_e = trial, e
+        log("calibration: variable %d scale %.3f -> L1 %.4e" % (v, best_a[v], best_e))
+    return best_a, hist
+
+
+def ppm_loss(X, T):
+    """Training loss of the PPM member (zero correction) on a batch."""
+    x = X[:, :48].double().reshape(-1, 8, 6)
+    wp = X[:, 48:64].double()
+    om2 = X[:, 64:72].double() ** 2
+    jump = x[..., 3] - x[..., 2]
+    eL = x[..., 2] + wp[:, 0::2] * jump - T.double()
+    eR = x[..., 3] - wp[:, 1::2] * jump - T.double()
+    return float((om2 * (eL ** 2 + eR ** 2)).sum(1).mean())
+
+
 def van_leer_loss(X, T):
     """Training loss of the van Leer member on a batch (features are shift/scale
     normalised, so the member's weights follow from the features themselves)."""
-    x = X.double().reshape(-1, 8, 6)
+    om2 = X[:, 64:72].double() ** 2
+    x = X[:, :48].double().reshape(-1, 8, 6)
     a, b, c, d = x[..., 1], x[..., 2], x[..., 3], x[..., 4]
 
     def vl(dp, dm):
@@ -375,7 +475,7 @@ def van_leer_loss(X, T):
     tau = T.double()
     eL = b + wl * jump - tau
     eR = c - wr * jump - tau
-    return float(((eL ** 2 + eR ** 2).sum(1)).mean())
+    return float((om2 * (eL ** 2 + eR ** 2)).sum(1).mean())
 
 
 if __name__ == "__main__":

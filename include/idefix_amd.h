@@ -266,11 +266,15 @@ int idfx_ctx_region_times(idfx_ctx *ctx, double *ms, long *count, int n);
  * An interface x_{i-1/2} sees the stencil i-3 .. i+2 of the 8 face-frame primitives
  * (rho, v_n, v_t, v_b, p, B_n, B_t, B_b).  Features per variable: (V_j - c) / s with
  * c = (V[i-1] + V[i]) / 2, s = max_j |V_j - c|.  Network 48 -> 64 -> 64 -> 16, hidden
- * activation z * sigmoid(1.702 z), sigmoid outputs wL_v (2v) and wR_v (2v+1); states
- * vL = V[i-1] + wL (V[i] - V[i-1]), vR = V[i] - wR (V[i] - V[i-1]).  Parameters are
- * packed FP32, row-major [in][out]: W1 48x64, b1 64, W2 64x64, b2 64, W3 64x16, b3 16
- * (idfx_mlp_param_count() floats).  Callers: setups that run IDFX_RECON_MLP and the
- * training pipeline that fits the network to block-averaged high-resolution runs. */
+ * activation z * sigmoid(1.702 z), linear outputs z3 that shift the PPM member of the
+ * limiter family: wL_v = wL_PPM + z3[2v], wR_v = wR_PPM - z3[2v] (output 2v+1 unused),
+ * w_PPM = (v_PPM - V[i-1]) / (V[i] - V[i-1]); states vL = V[i-1] + wL (V[i] - V[i-1]),
+ * vR = V[i] - wR (V[i] - V[i-1]), so the jump vR - vL stays PPM's.  Weights outside the
+ * TVD region fall back to PPM's.
+ * Parameters are packed FP32, row-major [in][out]: W1 48x64, b1 64, W2 64x64, b2 64,
+ * W3 64x16, b3 16 (idfx_mlp_param_count() floats).  Callers: setups that run
+ * IDFX_RECON_MLP and the training pipeline that fits the network to block-averaged
+ * high-resolution runs. */
 typedef struct idfx_mlp idfx_mlp;
 int idfx_mlp_param_count(void);
 /* Training state (FP32 parameters, Adam moments, activations for batches up to
@@ -280,7 +284,9 @@ int idfx_mlp_destroy(idfx_mlp *h);
 /* Host arrays of idfx_mlp_param_count() floats; set_params also resets Adam. */
 int idfx_mlp_set_params(idfx_mlp *h, const float *params);
 int idfx_mlp_get_params(idfx_mlp *h, float *params);
-/* X: device [B][48] features, tau: device [B][8] normalised targets (t - c) / s.
+/* X: device [B][72] rows: the 48 features, the interface's 16 PPM weights and 8 loss
+ * weights om_v; tau: device [B][8] normalised targets (t - c) / s.  Loss = mean over rows
+ * of sum_v om_v^2 (eL_v^2 + eR_v^2), eL = vL/s-error of the left state, eR of the right.
  * loss_grad: blocking; mean loss and the gradient of every parameter (host arrays).
  * train_step: one Adam step (lr, beta 0.9 / 0.999, eps 1e-8), asynchronous;
  * idfx_mlp_loss returns the loss of the last step's batch (blocking). */

@@ -10,11 +10,12 @@ Features (48): per variable, c = (V[i-1] + V[i]) / 2 and s = max_j |V_j - c| ove
 stencil; x_j = (V_j - c) / s (0 when s = 0); ordered [variable][stencil cell].
 
 Network: z1 = x W1 + b1 (48 -> 64), h1 = gelu(z1); z2 = h1 W2 + b2 (64 -> 64),
-h2 = gelu(z2); z3 = h2 W3 + b3 (64 -> 16), y = sigmoid(z3), with
+h2 = gelu(z2); z3 = h2 W3 + b3 (64 -> 16), linear outputs, with
 gelu(z) = z * sigmoid(1.702 z) (Hendrycks & Gimpel 2016, sigmoid form).
 
-Output: y in (0, 1)^16 are the limiter weights w: w[2v] = wL_v, w[2v+1] = wR_v per
-variable v, and the interface states are
+Output: z3 corrects the limiter weights of PPM by a common shift of the two states of
+each variable v: wL_v = wL_PPM + z3[2v], wR_v = wR_PPM - z3[2v] (output 2v+1 unused;
+the jump vR - vL stays PPM's), and the interface states are
     vL_v = V_v[i-1] + wL_v (V_v[i] - V_v[i-1]),   vR_v = V_v[i] - wR_v (V_v[i] - V_v[i-1]).
 Van Leer PLM is the member wL = dvm / (dvm + dvp) (monotone data) or 0 (extremum); PPM is
 the member w = (v_PPM - V[i-1]) / (V[i] - V[i-1]) (0 where V[i] = V[i-1]).  Each
@@ -26,8 +27,9 @@ V[i+1] - V[i], and 0 <= w <= 1.  A variable that fails, or has non-finite weight
 the PPM weights (the fallback; van Leer with `fallback="plm"`).
 
 Training target: the interface value t_v of a finer solution; loss per interface
-sum_v ((vL_v - t_v)/s_v)^2 + ((vR_v - t_v)/s_v)^2 (both terms 0 when s_v = 0), mean
-over the batch, with the unchecked weights.
+sum_v om_v^2 [((vL_v - t_v)/s_v)^2 + ((vR_v - t_v)/s_v)^2] (both terms 0 when s_v = 0),
+mean over the batch, with the unchecked weights; om_v are per-sample loss weights (1 for
+the scale-free loss, s_v / sigma_v to weigh absolute errors with a per-variable scale).
 
 Inference on the GPU runs with BF16 inputs, weights and hidden activations and FP32
 accumulation; `forward(..., bf16=True)` rounds at the same points.
@@ -102,23 +104,22 @@ def forward(P, x, bf16_mode=False):
         z2 = (h1.astype(np.float64) @ W2.astype(np.float64)).astype(f) + b2
         h2 = bf16(gelu(z2).astype(f))
         z3 = (h2.astype(np.float64) @ W3.astype(np.float64)).astype(f) + b3
-        y = sigmoid(z3.astype(f)).astype(f)
-        return y, None
+        return z3, None
     z1 = x @ P["W1"] + P["b1"]
     h1 = gelu(z1)
     z2 = h1 @ P["W2"] + P["b2"]
     h2 = gelu(z2)
     z3 = h2 @ P["W3"] + P["b3"]
-    y = sigmoid(z3)
-    return y, (x, z1, h1, z2, h2, z3, y)
+    return z3, (x, z1, h1, z2, h2, z3)
 
 
-W_SCALE, W_SHIFT = 1.0, 0.0
-
-
-def weights(y):
-    """Limiter weights from the network outputs (the identity: w = y)."""
-    return W_SCALE * y + W_SHIFT
+def weights(y, wp):
+    """Limiter weights from the network outputs y and the PPM weights wp: output 2v shifts
+    both states of variable v, wL = wpL + y[2v], wR = wpR - y[2v] (output 2v+1 unused)."""
+    w = np.array(wp, dtype=np.float64, copy=True)
+    w[..., 0::2] = wp[..., 0::2] + y[..., 0::2]
+    w[..., 1::2] = wp[..., 1::2] - y[..., 0::2]
+    return w
 
 
 def face_states(S, w):
@@ -177,21 +178,21 @@ def ppm_weights(S):
     return w
 
 
-def loss_and_grad(P, x, tau):
+def loss_and_grad(P, x, tau, wp, om=None):
     """Loss (mean over the batch) and parameter gradients.  x: [B, 48] features,
-    tau: [B, 8] normalised targets (t - c) / s (0 where s = 0)."""
+    tau: [B, 8] normalised targets (t - c) / s (0 where s = 0), wp: [B, 16] PPM weights,
+    om: [B, 8] loss weights (1 when None)."""
     B = x.shape[0]
-    y, (x, z1, h1, z2, h2, z3, _) = forward(P, x)
+    om2 = np.ones((B, 8)) if om is None else np.asarray(om, dtype=np.float64) ** 2
+    y, (x, z1, h1, z2, h2, z3) = forward(P, x)
     xv = x.reshape(B, 8, 6)
     d = xv[:, :, 3] - xv[:, :, 2]
-    w = weights(y)
+    w = weights(y, wp)
     eL = xv[:, :, 2] + w[:, 0::2] * d - tau
     eR = xv[:, :, 3] - w[:, 1::2] * d - tau
-    loss = np.sum(eL ** 2 + eR ** 2) / B
-    dy = np.zeros_like(y)
-    dy[:, 0::2] = W_SCALE * 2.0 * eL * d / B
-    dy[:, 1::2] = -W_SCALE * 2.0 * eR * d / B
-    dz3 = dy * y * (1.0 - y)
+    loss = np.sum(om2 * (eL ** 2 + eR ** 2)) / B
+    dz3 = np.zeros_like(y)
+    dz3[:, 0::2] = om2 * 2.0 * (eL + eR) * d / B
     g = {"W3": h2.T @ dz3, "b3": dz3.sum(0)}
     dh2 = dz3 @ P["W3"].T
     dz2 = dh2 * gelu_grad(z2)

@@ -66,12 +66,18 @@ def test_samples_follow_the_feature_specification():
     V, bx, by = _fine(16, seed=3)
     Vc, _, _ = learned.coarse_state(V, bx, by, 2, 5.0 / 3.0)
     tgt = torch.rand((8, 8, 8), dtype=torch.float64)
-    X, T = learned.samples(Vc, tgt, 0)
+    wp = torch.rand((16, 8, 8), dtype=torch.float64)
+    X, T, Sc = learned.samples(Vc, tgt, 0, wp)
+    assert X.shape == (64, 72) and T.shape == (64, 8) and Sc.shape == (64, 8)
     # face (J=2, I=5): stencil cells I-3 .. I+2 along x in face-frame order
     S = Vc[learned.FRAME[0]][:, 2, :].numpy()
     st = np.stack([S[:, (5 - 3 + s) % 8] for s in range(6)], axis=-1)
     x, c, s = mlp.features(st)
     k = 2 * 8 + 5
-    assert np.allclose(X[k].numpy(), x.astype(np.float32), atol=1e-6)
+    assert np.allclose(X[k, :48].numpy(), x.astype(np.float32), atol=1e-6)
+    # the PPM weights of the same face, and unit loss weights
+    assert np.allclose(X[k, 48:64].numpy(), wp[:, 2, 5].numpy().astype(np.float32))
+    assert np.all(X[k, 64:72].numpy() == 1.0)
+    assert np.allclose(Sc[k].numpy(), s, rtol=1e-6)
     tau = np.clip((tgt[:, 2, 5].numpy()[learned.FRAME[0]] - c) / s, -1, 1)
     assert np.allclose(T[k].numpy(), tau, atol=1e-6)
