@@ -263,7 +263,8 @@ STAGES_TURB3D = 3
 # profiler reports for them (FP64 instantiations)
 SINGLE_KERNEL_REGIONS = {"sweep_x": "k_sweep_x<", "sweep_y": "k_sweep_march<",
                          "sweep_z": "k_sweep_march<", "corner_emf": "k_corner_emf<",
-                         "evolve_b": "k_evolve_b<", "cons_to_prim": "k_cons_to_prim<"}
+                         "evolve_b": "k_evolve_b<", "cons_to_prim": "k_cons_to_prim<",
+                         "ct_fused": "k_ct3d<"}
 # plain-copy calibration of the traffic counters (1 GiB read + 1 GiB written per launch,
 # bench.py --copy-probe): FETCH_SIZE reads back half the bytes, WRITE_SIZE all of them
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0

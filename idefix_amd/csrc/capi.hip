@@ -108,6 +108,8 @@ struct idfx_ctx {
   MlpNet net{};
   bool fused2d_ok = false;   // configuration supports it
   bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
+  bool fused3d_ok = false;   // 3-D MHD with uct_contact: fused CT kernel (k_ct3d)
+  bool fused3d_on = false;
   double *Vc2 = nullptr, *Vs2 = nullptr;
   // enrolled user boundary (IDFX_BC_USERDEF sides) and the time it is called with
   idfx_userdef_boundary_fn user_fn = nullptr;
@@ -450,6 +452,8 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
       c->fused2d_ok = c->fused2d_on = st == IDFX_OK;
     }
   }
+  if (st == IDFX_OK && d.mhd && d.dim == 3 && d.emf == IDFX_EMF_UCT_CONTACT && ops->ct3d)
+    c->fused3d_ok = c->fused3d_on = true;
   if (st == IDFX_OK) {
     void *h = nullptr;
     if (hipHostMalloc(&h, 64, hipHostMallocDefault) != hipSuccess) st = IDFX_E_ALLOC;
@@ -750,13 +754,19 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
     }
   }
   if (c->d.mhd && dim >= 2) {
-    {
-      Region r(c, IDFX_REGION_CORNER_EMF, s);
-      IDFX_TRY(check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, s)));
-    }
     const int mode = nstages > 1 ? (stage == 0 ? 1 : 2) : 0;
-    Region r(c, IDFX_REGION_EVOLVE_B, s);
-    IDFX_TRY(check(c->ops->evolve_b(c->g, Vs, c->Vs0, c->emf(), dt, mode, wc, w0, s)));
+    if (c->fused3d_on) {
+      // CalcCornerEMF + EvolveMagField in one kernel (the edge arrays are not written)
+      Region r(c, IDFX_REGION_CT_FUSED, s);
+      IDFX_TRY(check(c->ops->ct3d(c->g, Vc, Vs, c->Vs0, c->emf(), dt, mode, wc, w0, s)));
+    } else {
+      {
+        Region r(c, IDFX_REGION_CORNER_EMF, s);
+        IDFX_TRY(check(c->ops->corner_emf(c->g, Vc, Vs, c->emf(), c->d.emf, s)));
+      }
+      Region r(c, IDFX_REGION_EVOLVE_B, s);
+      IDFX_TRY(check(c->ops->evolve_b(c->g, Vs, c->Vs0, c->emf(), dt, mode, wc, w0, s)));
+    }
   } else if (c->d.mhd && stage > 0) {
     // 1-D MHD keeps Vs constant; AddAndStore still applies (a no-op on equal values)
   }
@@ -893,8 +903,9 @@ int idfx_mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, void *strea
 
 int idfx_ctx_set_fusion(idfx_ctx *c, int level) {
   if (!c || level < 0 || level > 1) return IDFX_E_ARGS;
-  if (level == 1 && !c->fused2d_ok) return IDFX_E_ARGS;
-  c->fused2d_on = level == 1;
+  if (level == 1 && !c->fused2d_ok && !c->fused3d_ok) return IDFX_E_ARGS;
+  c->fused2d_on = level == 1 && c->fused2d_ok;
+  c->fused3d_on = level == 1 && c->fused3d_ok;
   return IDFX_OK;
 }
 

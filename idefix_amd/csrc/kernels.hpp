@@ -713,6 +713,38 @@ __device__ __forceinline__ void cell_emf(const GridDesc &g, const double *Vc, in
   E3 = (vx2 * bx1 - vx1 * bx2);
 }
 
+// The three uct_contact edge EMFs of a 3-D block at edge index (k, j, i): ex at
+// (i, j-1/2, k-1/2), ey at (i-1/2, j, k-1/2), ez at (i-1/2, j-1/2, k)
+// (CalcContactAverage, calcCornerEmf.hpp:228-282).  E1/E2/E3[dk][dj][di] are the cell
+// EMFs of cells (k-1+dk, j-1+dj, i-1+di); the face data follow the argument names
+// (km = k-1, jm = j-1, im = i-1).  Shared by k_corner_emf and k_ct3d, so both compute the
+// same values with the same operations.
+template <typename T>
+struct ContactFaces {
+  T svx, svx_jm, svx_km, svy, svy_im, svy_km, svz, svz_jm, svz_im;
+  T ezi, ezi_jm, ezj, ezj_im, exj, exj_km, exk, exk_jm, eyi, eyi_km, eyk, eyk_im;
+};
+template <typename T>
+__device__ __forceinline__ void contact_edges3(const ContactFaces<T> &f, const T (&E1)[2][2][2],
+                                               const T (&E2)[2][2][2], const T (&E3)[2][2][2],
+                                               T &ex, T &ey, T &ez) {
+  const T ez_l2 = (1 - f.svx_jm) * (f.ezj - E3[1][0][1]) + (f.svx_jm) * (f.ezj_im - E3[1][0][0]);
+  const T ez_r2 = (1 - f.svx) * (f.ezj - E3[1][1][1]) + (f.svx) * (f.ezj_im - E3[1][1][0]);
+  const T ez_l1 = (1 - f.svy_im) * (f.ezi - E3[1][1][0]) + (f.svy_im) * (f.ezi_jm - E3[1][0][0]);
+  const T ez_r1 = (1 - f.svy) * (f.ezi - E3[1][1][1]) + (f.svy) * (f.ezi_jm - E3[1][0][1]);
+  ez = T(0.25) * (ez_l2 + ez_r2 + ez_l1 + ez_r1 + f.ezi + f.ezi_jm + f.ezj + f.ezj_im);
+  const T ex_l3 = (1 - f.svy_km) * (f.exk - E1[0][1][1]) + (f.svy_km) * (f.exk_jm - E1[0][0][1]);
+  const T ex_r3 = (1 - f.svy) * (f.exk - E1[1][1][1]) + (f.svy) * (f.exk_jm - E1[1][0][1]);
+  const T ex_l2 = (1 - f.svz_jm) * (f.exj - E1[1][0][1]) + (f.svz_jm) * (f.exj_km - E1[0][0][1]);
+  const T ex_r2 = (1 - f.svz) * (f.exj - E1[1][1][1]) + (f.svz) * (f.exj_km - E1[0][1][1]);
+  ex = T(0.25) * (ex_l3 + ex_r3 + ex_l2 + ex_r2 + f.exj + f.exj_km + f.exk + f.exk_jm);
+  const T ey_l3 = (1 - f.svx_km) * (f.eyk - E2[0][1][1]) + (f.svx_km) * (f.eyk_im - E2[0][1][0]);
+  const T ey_r3 = (1 - f.svx) * (f.eyk - E2[1][1][1]) + (f.svx) * (f.eyk_im - E2[1][1][0]);
+  const T ey_l1 = (1 - f.svz_im) * (f.eyi - E2[1][1][0]) + (f.svz_im) * (f.eyi_km - E2[0][1][0]);
+  const T ey_r1 = (1 - f.svz) * (f.eyi - E2[1][1][1]) + (f.svz) * (f.eyi_km - E2[0][1][1]);
+  ey = T(0.25) * (ey_l3 + ey_r3 + ey_l1 + ey_r1 + f.eyi + f.eyi_km + f.eyk + f.eyk_im);
+}
+
 // One edge term of CalcRiemannAverage (calcRiemannEmf.hpp:62-428): B and the upwinded
 // transverse velocity reconstructed (MC limiter, or PPM for ORDER 4) from the left and
 // right cells of the edge.  b[s], v[s] hold the face values at offset s-3 from the
@@ -788,38 +820,46 @@ __global__ __launch_bounds__(256) void k_corner_emf(GridDesc g, const double *Vc
         for (int di = 0; di < 2; di++)
           cell_emf<C, T>(g, Vc, k - D3 + dk, j - 1 + dj, i - 1 + di, E1[dk][dj][di],
                          E2[dk][dj][di], E3[dk][dj][di]);
-    const int K = D3;  // index of cell k in the [dk] axis
-    const T ez_l2 = (1 - at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i) - E3[K][0][1]) +
-                    (at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i - 1) - E3[K][0][0]);
-    const T ez_r2 = (1 - at(e.svx, k, j, i)) * (at(e.ezj, k, j, i) - E3[K][1][1]) +
-                    (at(e.svx, k, j, i)) * (at(e.ezj, k, j, i - 1) - E3[K][1][0]);
-    const T ez_l1 = (1 - at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j, i) - E3[K][1][0]) +
-                    (at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j - 1, i) - E3[K][0][0]);
-    const T ez_r1 = (1 - at(e.svy, k, j, i)) * (at(e.ezi, k, j, i) - E3[K][1][1]) +
-                    (at(e.svy, k, j, i)) * (at(e.ezi, k, j - 1, i) - E3[K][0][1]);
-    e.ez[p] = T(0.25) * (ez_l2 + ez_r2 + ez_l1 + ez_r1 + at(e.ezi, k, j, i) +
-                         at(e.ezi, k, j - 1, i) + at(e.ezj, k, j, i) + at(e.ezj, k, j, i - 1));
     if constexpr (C::DIM == 3) {
-      const T ex_l3 = (1 - at(e.svy, k - 1, j, i)) * (at(e.exk, k, j, i) - E1[0][1][1]) +
-                      (at(e.svy, k - 1, j, i)) * (at(e.exk, k, j - 1, i) - E1[0][0][1]);
-      const T ex_r3 = (1 - at(e.svy, k, j, i)) * (at(e.exk, k, j, i) - E1[1][1][1]) +
-                      (at(e.svy, k, j, i)) * (at(e.exk, k, j - 1, i) - E1[1][0][1]);
-      const T ex_l2 = (1 - at(e.svz, k, j - 1, i)) * (at(e.exj, k, j, i) - E1[1][0][1]) +
-                      (at(e.svz, k, j - 1, i)) * (at(e.exj, k - 1, j, i) - E1[0][0][1]);
-      const T ex_r2 = (1 - at(e.svz, k, j, i)) * (at(e.exj, k, j, i) - E1[1][1][1]) +
-                      (at(e.svz, k, j, i)) * (at(e.exj, k - 1, j, i) - E1[0][1][1]);
-      e.ex[p] = T(0.25) * (ex_l3 + ex_r3 + ex_l2 + ex_r2 + at(e.exj, k, j, i) +
-                           at(e.exj, k - 1, j, i) + at(e.exk, k, j, i) + at(e.exk, k, j - 1, i));
-      const T ey_l3 = (1 - at(e.svx, k - 1, j, i)) * (at(e.eyk, k, j, i) - E2[0][1][1]) +
-                      (at(e.svx, k - 1, j, i)) * (at(e.eyk, k, j, i - 1) - E2[0][1][0]);
-      const T ey_r3 = (1 - at(e.svx, k, j, i)) * (at(e.eyk, k, j, i) - E2[1][1][1]) +
-                      (at(e.svx, k, j, i)) * (at(e.eyk, k, j, i - 1) - E2[1][1][0]);
-      const T ey_l1 = (1 - at(e.svz, k, j, i - 1)) * (at(e.eyi, k, j, i) - E2[1][1][0]) +
-                      (at(e.svz, k, j, i - 1)) * (at(e.eyi, k - 1, j, i) - E2[0][1][0]);
-      const T ey_r1 = (1 - at(e.svz, k, j, i)) * (at(e.eyi, k, j, i) - E2[1][1][1]) +
-                      (at(e.svz, k, j, i)) * (at(e.eyi, k - 1, j, i) - E2[0][1][1]);
-      e.ey[p] = T(0.25) * (ey_l3 + ey_r3 + ey_l1 + ey_r1 + at(e.eyi, k, j, i) +
-                           at(e.eyi, k - 1, j, i) + at(e.eyk, k, j, i) + at(e.eyk, k, j, i - 1));
+      ContactFaces<T> f;
+      f.svx = at(e.svx, k, j, i);
+      f.svx_jm = at(e.svx, k, j - 1, i);
+      f.svx_km = at(e.svx, k - 1, j, i);
+      f.svy = at(e.svy, k, j, i);
+      f.svy_im = at(e.svy, k, j, i - 1);
+      f.svy_km = at(e.svy, k - 1, j, i);
+      f.svz = at(e.svz, k, j, i);
+      f.svz_jm = at(e.svz, k, j - 1, i);
+      f.svz_im = at(e.svz, k, j, i - 1);
+      f.ezi = at(e.ezi, k, j, i);
+      f.ezi_jm = at(e.ezi, k, j - 1, i);
+      f.ezj = at(e.ezj, k, j, i);
+      f.ezj_im = at(e.ezj, k, j, i - 1);
+      f.exj = at(e.exj, k, j, i);
+      f.exj_km = at(e.exj, k - 1, j, i);
+      f.exk = at(e.exk, k, j, i);
+      f.exk_jm = at(e.exk, k, j - 1, i);
+      f.eyi = at(e.eyi, k, j, i);
+      f.eyi_km = at(e.eyi, k - 1, j, i);
+      f.eyk = at(e.eyk, k, j, i);
+      f.eyk_im = at(e.eyk, k, j, i - 1);
+      T ex, ey, ez;
+      contact_edges3(f, E1, E2, E3, ex, ey, ez);
+      e.ex[p] = ex;
+      e.ey[p] = ey;
+      e.ez[p] = ez;
+    } else {
+      const int K = D3;  // index of cell k in the [dk] axis
+      const T ez_l2 = (1 - at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i) - E3[K][0][1]) +
+                      (at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i - 1) - E3[K][0][0]);
+      const T ez_r2 = (1 - at(e.svx, k, j, i)) * (at(e.ezj, k, j, i) - E3[K][1][1]) +
+                      (at(e.svx, k, j, i)) * (at(e.ezj, k, j, i - 1) - E3[K][1][0]);
+      const T ez_l1 = (1 - at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j, i) - E3[K][1][0]) +
+                      (at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j - 1, i) - E3[K][0][0]);
+      const T ez_r1 = (1 - at(e.svy, k, j, i)) * (at(e.ezi, k, j, i) - E3[K][1][1]) +
+                      (at(e.svy, k, j, i)) * (at(e.ezi, k, j - 1, i) - E3[K][0][1]);
+      e.ez[p] = T(0.25) * (ez_l2 + ez_r2 + ez_l1 + ez_r1 + at(e.ezi, k, j, i) +
+                           at(e.ezi, k, j - 1, i) + at(e.ezj, k, j, i) + at(e.ezj, k, j, i - 1));
     }
   } else {
     // uct_hll / uct_hlld: CalcRiemannAverage
@@ -934,6 +974,170 @@ __global__ __launch_bounds__(256) void k_evolve_b(GridDesc g, double *Vs, double
     const double upd = old + rhs[n];
     if (mode == 1) Vs0[q] = old;
     Vs[q] = (mode == 2) ? wc * upd + w0 * Vs0[q] : upd;
+  }
+}
+
+// CalcCornerEMF (uct_contact) and EvolveMagField of a 3-D block in one kernel, for the
+// stage path (the reference-granularity entry points keep k_corner_emf + k_evolve_b).
+// A block of 8 waves owns a tile of 63 face columns (i) x 7 face rows (j) and marches
+// along k over a segment of face planes.  Lane l of wave w computes the three edge EMFs at
+// edge (k, j0 + w, i0 + l) of each plane k with the arithmetic of k_corner_emf
+// (contact_edges3), so the tile's 64 x 8 edges cover its faces and the one halo column
+// and row they need; the edges of plane k stay in registers (own column), reach the
+// neighbouring column by a shuffle and the neighbouring row through LDS (three plane
+// buffers, one barrier per plane), and the face data and cell EMFs of plane k - 1 are
+// the previous step's plane k.  After plane k, the faces of plane k - 1 are updated with
+// the arithmetic of k_evolve_b (mode / wc / w0 as there), in place: every face has one
+// owner and no other block reads Vs.  Only faces the next stage reads are updated: each
+// face field over [beg, end] along its own direction and [beg, end) across; the edge
+// arrays ex, ey, ez are not written.
+// Cell EMFs of one cell from 32-bit offsets (the arithmetic of cell_emf).
+template <class C, typename T>
+__device__ __forceinline__ void cell_emf32(const double *Vc, long long sv, unsigned p, T &E1,
+                                           T &E2, T &E3) {
+  const T vx1 = (T)ld32(Vc + C::MX1 * sv, p), vx2 = (T)ld32(Vc + C::MX2 * sv, p);
+  const T vx3 = (T)ld32(Vc + C::MX3 * sv, p);
+  const T bx1 = (T)ld32(Vc + C::BX1 * sv, p), bx2 = (T)ld32(Vc + C::BX2 * sv, p);
+  const T bx3 = (T)ld32(Vc + C::BX3 * sv, p);
+  E1 = (vx3 * bx2 - vx2 * bx3);
+  E2 = (vx1 * bx3 - vx3 * bx1);
+  E3 = (vx2 * bx1 - vx1 * bx2);
+}
+
+// One face-field update of k_evolve_b's arithmetic (mode 0: Vs += rhs; 1: Vs0 = Vs, then
+// Vs += rhs; 2: Vs = wc (Vs + rhs) + w0 Vs0).
+__device__ __forceinline__ void evolve_face(double *Vs, double *Vs0, unsigned q, double rhs,
+                                            int mode, double wc, double w0) {
+  const double old = ld32(Vs, q);
+  const double upd = old + rhs;
+  if (mode == 1) st32(Vs0, q, old);
+  st32(Vs, q, (mode == 2) ? wc * upd + w0 * ld32(Vs0, q) : upd);
+}
+
+template <class C, typename T>
+__global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, double *Vs,
+                                              double *Vs0, EmfArrays<T> e, double dt, int mode,
+                                              double wc, double w0, int ntj, int nseg,
+                                              int kseg) {
+  static_assert(C::MHD && C::DIM == 3 && C::COMP == 3, "k_ct3d is the 3-D MHD CT");
+  constexpr int TI = 63, TJ = 7;
+  __shared__ T lez[3][8][64], lex[3][8][64];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int seg = blockIdx.x % nseg;
+  const int tile = blockIdx.x / nseg;
+  const int ti = tile / ntj, tj = tile % ntj;
+  const int i = g.beg[0] + TI * ti + l, j = g.beg[1] + TJ * tj + w;
+  const int ks = g.beg[2] + kseg * seg;
+  const int ke = min(ks + kseg, g.end[2] + 1);  // face planes [ks, ke)
+  const int klast = min(ke, g.end[2]);          // last edge plane
+  const bool edge_ok = i <= g.end[0] && j <= g.end[1];
+  const unsigned sy = (unsigned)g.nt[0], sz = (unsigned)g.nt[0] * (unsigned)g.nt[1];
+  const unsigned vy = (unsigned)g.vsn[0], vz = (unsigned)g.vsn[0] * (unsigned)g.vsn[1];
+  double *Vs1 = Vs + g.svs, *Vs2 = Vs + 2 * g.svs;
+  double *Vs01 = Vs0 + g.svs, *Vs02 = Vs0 + 2 * g.svs;
+  // plane k - 1 data carried from the previous step
+  T E1m0 = 0, E1m1 = 0, E2m0 = 0, E2m1 = 0;  // E1 of cells (k-1, j-1+dj, i), E2 of (k-1, j, i-1+di)
+  T svx_m = 0, svy_m = 0, exj_m = 0, eyi_m = 0;
+  if (edge_ok) {
+    const unsigned p = (unsigned)idx3(g, ks - 1, j, i);
+    T a1, a2, a3;
+    cell_emf32<C, T>(Vc, g.sv, p - sy, E1m0, a2, a3);
+    cell_emf32<C, T>(Vc, g.sv, p, E1m1, E2m1, a3);
+    cell_emf32<C, T>(Vc, g.sv, p - 1, a1, E2m0, a3);
+    svx_m = ld32(e.svx, p);
+    svy_m = ld32(e.svy, p);
+    exj_m = ld32(e.exj, p);
+    eyi_m = ld32(e.eyi, p);
+  }
+  T pex = 0, pey = 0, pez = 0;  // edges of the previous plane, own column
+  for (int k = ks; k <= klast; k++) {
+    T cex = 0, cey = 0, cez = 0;
+    if (edge_ok) {
+      const unsigned p = (unsigned)idx3(g, k, j, i);
+      T E1[2][2][2], E2[2][2][2], E3[2][2][2];
+      cell_emf32<C, T>(Vc, g.sv, p - sy - 1, E1[1][0][0], E2[1][0][0], E3[1][0][0]);
+      cell_emf32<C, T>(Vc, g.sv, p - sy, E1[1][0][1], E2[1][0][1], E3[1][0][1]);
+      cell_emf32<C, T>(Vc, g.sv, p - 1, E1[1][1][0], E2[1][1][0], E3[1][1][0]);
+      cell_emf32<C, T>(Vc, g.sv, p, E1[1][1][1], E2[1][1][1], E3[1][1][1]);
+      E1[0][0][1] = E1m0;
+      E1[0][1][1] = E1m1;
+      E2[0][1][0] = E2m0;
+      E2[0][1][1] = E2m1;
+      ContactFaces<T> f;
+      f.svx = ld32(e.svx, p);
+      f.svx_jm = ld32(e.svx, p - sy);
+      f.svx_km = svx_m;
+      f.svy = ld32(e.svy, p);
+      f.svy_im = ld32(e.svy, p - 1);
+      f.svy_km = svy_m;
+      f.svz = ld32(e.svz, p);
+      f.svz_jm = ld32(e.svz, p - sy);
+      f.svz_im = ld32(e.svz, p - 1);
+      f.ezi = ld32(e.ezi, p);
+      f.ezi_jm = ld32(e.ezi, p - sy);
+      f.ezj = ld32(e.ezj, p);
+      f.ezj_im = ld32(e.ezj, p - 1);
+      f.exj = ld32(e.exj, p);
+      f.exj_km = exj_m;
+      f.exk = ld32(e.exk, p);
+      f.exk_jm = ld32(e.exk, p - sy);
+      f.eyi = ld32(e.eyi, p);
+      f.eyi_km = eyi_m;
+      f.eyk = ld32(e.eyk, p);
+      f.eyk_im = ld32(e.eyk, p - 1);
+      contact_edges3(f, E1, E2, E3, cex, cey, cez);
+      E1m0 = E1[1][0][1];
+      E1m1 = E1[1][1][1];
+      E2m0 = E2[1][1][0];
+      E2m1 = E2[1][1][1];
+      svx_m = f.svx;
+      svy_m = f.svy;
+      exj_m = f.exj;
+      eyi_m = f.eyi;
+    }
+    lez[k % 3][w][l] = cez;
+    lex[k % 3][w][l] = cex;
+    __syncthreads();
+    if (k > ks) {
+      // faces of plane kk = k - 1 from the edges of planes kk (p*) and k (c*)
+      const int kk = k - 1;
+      const T ez_ip = __shfl_down(pez, 1);
+      const T ey_ip = __shfl_down(pey, 1);
+      const T ez_jp = lez[kk % 3][w < 7 ? w + 1 : w][l];
+      const T ex_jp = lex[kk % 3][w < 7 ? w + 1 : w][l];
+      const bool own = l < TI && w < TJ;
+      const unsigned q = (unsigned)kk * vz + (unsigned)j * vy + (unsigned)i;
+      if (own && i <= g.end[0] && j < g.end[1] && kk < g.end[2]) {  // BX1s
+        double r = 0.0 - dt / g.dx[1][j] * ((double)ez_jp - (double)pez);
+        r = r + dt / g.dx[2][kk] * ((double)cey - (double)pey);
+        evolve_face(Vs, Vs0, q, r, mode, wc, w0);
+      }
+      if (own && i < g.end[0] && j <= g.end[1] && kk < g.end[2]) {  // BX2s
+        double r = dt / g.dx[0][i] * ((double)ez_ip - (double)pez);
+        r = r - dt / g.dx[2][kk] * ((double)cex - (double)pex);
+        evolve_face(Vs1, Vs01, q, r, mode, wc, w0);
+      }
+      if (own && i < g.end[0] && j < g.end[1]) {  // BX3s at face plane kk
+        const double r = -dt / g.dx[0][i] * ((double)ey_ip - (double)pey) +
+                         dt / g.dx[1][j] * ((double)ex_jp - (double)pex);
+        evolve_face(Vs2, Vs02, q, r, mode, wc, w0);
+      }
+    }
+    pex = cex;
+    pey = cey;
+    pez = cez;
+  }
+  if (ke == g.end[2] + 1) {
+    // the top face plane of the block (BX3s at end) from the last edge plane
+    const int kk = g.end[2];
+    const T ey_ip = __shfl_down(pey, 1);
+    const T ex_jp = lex[kk % 3][w < 7 ? w + 1 : w][l];
+    if (l < TI && w < TJ && i < g.end[0] && j < g.end[1]) {
+      const double r = -dt / g.dx[0][i] * ((double)ey_ip - (double)pey) +
+                       dt / g.dx[1][j] * ((double)ex_jp - (double)pex);
+      const unsigned q = (unsigned)kk * vz + (unsigned)j * vy + (unsigned)i;
+      evolve_face(Vs2, Vs02, q, r, mode, wc, w0);
+    }
   }
 }
 

@@ -41,6 +41,10 @@ struct Ops {
                     double *Uc, double *InvDt, double dt, hipStream_t s) = nullptr;
   hipError_t (*divb)(const GridDesc &g, const double *Vs, unsigned long long *out,
                      hipStream_t s) = nullptr;
+  // 3-D MHD only: CalcCornerEMF (uct_contact) + EvolveMagField in one kernel (k_ct3d)
+  hipError_t (*ct3d)(const GridDesc &g, const double *Vc, double *Vs, double *Vs0,
+                     const void *emf, double dt, int mode, double wc, double w0,
+                     hipStream_t s) = nullptr;
   // 2-D MHD only: y sweep + corner EMF + face-B update + ConsToPrim in one kernel
   hipError_t (*fused2d_y)(const GridDesc &g, const SweepArgs &a, const void *emf,
                           Fused2dArgs f, int target_waves, hipStream_t s) = nullptr;
@@ -198,6 +202,28 @@ hipError_t launch_evolve_b(const GridDesc &g, double *Vs, double *Vs0, const voi
     return hipGetLastError();
   }
   return hipSuccess;
+}
+
+template <class C, typename T>
+hipError_t launch_ct3d(const GridDesc &g, const double *Vc, double *Vs, double *Vs0,
+                       const void *emf, double dt, int mode, double wc, double w0,
+                       hipStream_t s) {
+  if constexpr (C::MHD && C::DIM == 3) {
+    // tiles of 63 x 7 faces in (i, j); k segments so that the grid holds about 2048
+    // blocks of 8 waves (several rounds of the chip's resident blocks)
+    const int nfi = g.end[0] - g.beg[0] + 1, nfj = g.end[1] - g.beg[1] + 1;
+    const int nfk = g.end[2] - g.beg[2] + 1;
+    const int nti = (nfi + 62) / 63, ntj = (nfj + 6) / 7;
+    const long long tiles = (long long)nti * ntj;
+    int nseg = (int)std::max<long long>(1, std::min<long long>(nfk, (2048 + tiles - 1) / tiles));
+    const int kseg = (nfk + nseg - 1) / nseg;
+    nseg = (nfk + kseg - 1) / kseg;
+    hipLaunchKernelGGL((k_ct3d<C, T>), dim3((unsigned)(tiles * nseg)), dim3(512), 0, s, g, Vc,
+                       Vs, Vs0, *static_cast<const EmfArrays<T> *>(emf), dt, mode, wc, w0, ntj,
+                       nseg, kseg);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
 }
 
 template <class C>
@@ -446,6 +472,7 @@ Ops make_ops() {
   o.rhs = &launch_rhs<C>;
   o.divb = &launch_divb<C>;
   if constexpr (C::MHD && C::DIM == 2) o.fused2d_y = &launch_fused2d_y<C, T>;
+  if constexpr (C::MHD && C::DIM == 3) o.ct3d = &launch_ct3d<C, T>;
   if constexpr (C::ORDER == RECON_MLP) o.mlp_infer = &launch_mlp<C>;
   o.find_shock = &launch_find_shock<C>;
   return o;

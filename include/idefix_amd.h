@@ -226,9 +226,12 @@ int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long 
 
 /* No counterpart in the reference: selects the kernel schedule idfx_cycle uses.
  * level 0 runs each reference step as its own kernel (sweeps, CalcCornerEMF,
- * EvolveMagField, ConvertConsToPrim); level 1 (the default where supported: 2-D MHD
- * without user-defined boundaries) fuses the y sweep, corner EMF, face-field update,
- * AddAndStore and ConvertConsToPrim into one kernel.  Both give identical results.
+ * EvolveMagField, ConvertConsToPrim); level 1 (the default where supported) fuses, in 2-D
+ * MHD without user-defined boundaries, the y sweep, corner EMF, face-field update,
+ * AddAndStore and ConvertConsToPrim into one kernel, and in 3-D MHD with uct_contact the
+ * corner EMF and the face-field update (with its AddAndStore) into one kernel, which then
+ * leaves the edge EMF arrays unwritten and the face fields of ghost columns as they were.
+ * Both levels give identical interior cells and faces.
  * Returns IDFX_E_ARGS for level 1 where the configuration does not support it. */
 int idfx_ctx_set_fusion(idfx_ctx *ctx, int level);
 
@@ -242,7 +245,8 @@ enum {
   IDFX_REGION_EVOLVE_B = 5,     /* EvolveMagField (+ AddAndStore of Vs) */
   IDFX_REGION_CONS_TO_PRIM = 6, /* AddAndStore of Uc + ConvertConsToPrim */
   IDFX_REGION_FUSED2D_Y = 7,    /* fused 2-D second half (fusion level 1) */
-  IDFX_NREGIONS = 8
+  IDFX_REGION_CT_FUSED = 8,     /* fused 3-D CalcCornerEMF + EvolveMagField (fusion level 1) */
+  IDFX_NREGIONS = 9
 };
 
 /* idfx::pushRegion / popRegion with the profiler enabled (global.cpp:66-95, profiler.cpp):
