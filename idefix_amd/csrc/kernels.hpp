@@ -1023,8 +1023,13 @@ __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, doub
   constexpr int TI = 63, TJ = 7;
   __shared__ T lez[3][8][64], lex[3][8][64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int seg = blockIdx.x % nseg;
-  const int tile = blockIdx.x / nseg;
+  // work-groups go to the 8 XCDs in turn (block b on XCD b % 8): give each XCD a
+  // contiguous run of (tile, segment) indices, so that neighbouring tiles, which read each
+  // other's halo rows and columns, share an L2
+  const int nb = gridDim.x, xcd = blockIdx.x % 8, q8 = nb / 8, r8 = nb % 8;
+  const int blk = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + blockIdx.x / 8;
+  const int seg = blk % nseg;
+  const int tile = blk / nseg;
   const int ti = tile / ntj, tj = tile % ntj;
   const int i = g.beg[0] + TI * ti + l, j = g.beg[1] + TJ * tj + w;
   const int ks = g.beg[2] + kseg * seg;
