@@ -269,9 +269,18 @@ SINGLE_KERNEL_REGIONS = {"sweep_x": "k_sweep_x<", "sweep_y": "k_sweep_march<",
 # bench.py --copy-probe): FETCH_SIZE reads back half the bytes, WRITE_SIZE all of them
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
-# The equal-L1 comparison of the two legs (the accuracy study in profiles/, measured with
-# idefix_amd.learned / idefix_amd.accuracy on one MI355X; see that file for the commands).
-EQUAL_L1 = None
+# The equal-L1 comparison of the two legs: the learned-limiter accuracy study
+# (python -m idefix_amd.learned --fine 2048 --factor 8 ..., one MI355X): 2-D decaying
+# turbulence, held-out seed, L1 at t = 1 against the 2048^2 solution block-averaged to
+# 256^2.  No 3-D network was trained.  The learned 256^2 run does not reach the error of
+# the plain run at twice the resolution, so no equal-L1 wall-time ratio exists.
+EQUAL_L1 = {"study": "2-D decaying MHD turbulence, seeds 1-3 trained, seed 4 held out, "
+                     "L1 at t = 1 vs block-averaged 2048^2",
+            "learned_fp32_256": 2.1035e-3, "plain_ppm_fp64_256": 2.1046e-3,
+            "plain_ppm_fp64_512": 4.1035e-4, "wall_s": {"learned_fp32_256": 0.58,
+                                                        "plain_ppm_fp64_256": 0.48,
+                                                        "plain_ppm_fp64_512": 1.29},
+            "equal_l1_reached": False, "wall_time_ratio_at_equal_l1": None}
 
 
 def turb3d_block(n, precision, recon="ppm", ic=None):

@@ -18,16 +18,23 @@
 // pytools/idfx_test.py:132-135).
 //
 // Pinning status (checks that pin each function independently of this restatement;
-// the tests that run them are named in tests/):
-//   PLM van Leer / MC limiters ........ closed forms + brute-force monotonicity (test_oracle_limiters)
-//   PPM getPPMStates .................. exact on quadratics away from extrema, monotonicity (same)
+// the tests that run them are named in tests/; test_oracle_pins.py also runs each of its
+// checks against a planted mistake, orc_plant below, and requires it to fail):
+//   PLM van Leer / MC limiters ........ closed forms + brute-force monotonicity (test_oracle_pointwise)
+//   PPM getPPMStates .................. exact on cubics, on quadratics with an interior extremum,
+//                                        fourth order at smooth extrema, no new extrema (pointwise, pins)
 //   K_PrimToCons / K_ConsToPrim ........ round trip, pressure floor branch (test_oracle_pointwise)
-//   K_Flux, HLLC, HLLD, Roe HD ......... consistency F(U,U)=F(U), upwind limits, Galilean (same)
+//   K_Flux, HLLC, HLL, TVDLF, Roe HD ... consistency, upwind limits, jump conditions, Rusanov form
+//   HLLD (double-star included) ........ exact moving contacts with B_n != 0 and rotational
+//                                        discontinuities (test_oracle_pins)
+//   UCT-HLL / UCT-HLLD stores .......... coefficient identity with the solver's flux (pins)
+//   CalcCornerEMF ...................... uct_contact/uct0 reduce to 1-D, field-loop energy,
+//                                        Londrillo-Del Zanna edge formula (pins); div B, conservation
 //   HLLC + PLM + RK2 + outflow ......... Sod exact solution (Toro table 4.2 star state) (test_oracle_sod)
 //   HLLD + CT (all EMF schemes) ........ div B at round-off, conservation, oblique linear fast
 //                                        wave convergence (test_oracle_mhd)
+//   Roe MHD + CT + RK2/RK3 in 3-D ...... the reference's four-mode linear-wave test (pins)
 //   dt reduction ....................... brute-force min (test_oracle_pointwise)
-//   Roe MHD ............................ consistency, mirror symmetry, linear fast wave (test_oracle_*)
 #include "oracle.h"
 
 #include <cmath>
