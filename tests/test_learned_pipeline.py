@@ -81,3 +81,47 @@ def test_samples_follow_the_feature_specification():
     assert np.allclose(Sc[k].numpy(), s, rtol=1e-6)
     tau = np.clip((tgt[:, 2, 5].numpy()[learned.FRAME[0]] - c) / s, -1, 1)
     assert np.allclose(T[k].numpy(), tau, atol=1e-6)
+
+
+def test_coarse_state3_conserves_and_keeps_div_b():
+    from idefix_amd import learned3d
+    gamma = 5.0 / 3.0
+    n, f = 16, 4
+    rng = np.random.default_rng(3)
+    V = torch.tensor(rng.uniform(0.5, 1.5, (8, n, n, n)))
+    # divergence-free faces from an edge-centred vector potential (periodic): B = curl A
+    A = [rng.normal(0, 1, (n, n, n)) for _ in range(3)]   # Ax, Ay, Az on x, y, z edges
+    d = lambda a, ax: (np.roll(a, -1, axis=ax) - a) * n    # forward difference
+    bx = np.zeros((n, n, n + 1))
+    by = np.zeros((n, n + 1, n))
+    bz = np.zeros((n + 1, n, n))
+    bx[:, :, :n] = d(A[2], 1) - d(A[1], 0)      # dAz/dy - dAy/dz  (axes: 0=z, 1=y, 2=x)
+    by[:, :n, :] = d(A[0], 0) - d(A[2], 2)      # dAx/dz - dAz/dx
+    bz[:n] = d(A[1], 2) - d(A[0], 1)            # dAy/dx - dAx/dy
+    bx[:, :, n], by[:, n, :], bz[n] = bx[:, :, 0], by[:, 0, :], bz[0]
+    div = (bx[:, :, 1:] - bx[:, :, :-1]) + (by[:, 1:] - by[:, :-1]) + (bz[1:] - bz[:-1])
+    assert np.abs(div).max() < 1e-10 * np.abs(bx).max()
+    V[5] = torch.tensor(0.5 * (bx[..., :-1] + bx[..., 1:]))
+    V[6] = torch.tensor(0.5 * (by[:, :-1] + by[:, 1:]))
+    V[7] = torch.tensor(0.5 * (bz[:-1] + bz[1:]))
+    bx, by, bz = torch.tensor(bx), torch.tensor(by), torch.tensor(bz)
+    Vc, bxc, byc, bzc = learned3d.coarse_state3(V, bx, by, bz, f, gamma)
+    m = f ** 3
+    rho, v, p, B = V[0], V[1:4], V[4], V[5:8]
+    E = p / (gamma - 1) + 0.5 * rho * (v * v).sum(0) + 0.5 * (B * B).sum(0)
+    rc, vc, pc, Bc = Vc[0], Vc[1:4], Vc[4], Vc[5:8]
+    assert float(rc.sum()) * m == pytest.approx(float(rho.sum()), rel=1e-13)
+    for c in range(3):
+        assert float((rc * vc[c]).sum()) * m == pytest.approx(float((rho * v[c]).sum()), rel=1e-12, abs=1e-12)
+    Ec = pc / (gamma - 1) + 0.5 * rc * (vc * vc).sum(0) + 0.5 * (Bc * Bc).sum(0)
+    assert float(Ec.sum()) * m == pytest.approx(float(E.sum()), rel=1e-12)
+    # the coarse faces are the mean of their f^2 fine faces, and div B stays zero
+    assert float(bxc[1, 2, 3]) == pytest.approx(float(bx[f:2 * f, 2 * f:3 * f, 3 * f].mean()), rel=1e-14)
+    assert float(bzc[2, 0, 1]) == pytest.approx(float(bz[2 * f, 0:f, f:2 * f].mean()), rel=1e-14)
+    divc = (bxc[..., 1:] - bxc[..., :-1]) + (byc[:, 1:] - byc[:, :-1]) + (bzc[1:] - bzc[:-1])
+    assert float(divc.abs().max()) < 1e-10 * float(bxc.abs().max())
+    # the wrong pooling (faces of every fine plane, not every f-th) breaks div B
+    bx_bad = bx[:, :, :n].reshape(n // f, f, n // f, f, n // f, f).mean((1, 3, 5))
+    bx_bad = torch.cat([bx_bad, bx_bad[..., :1]], dim=-1)
+    div_bad = (bx_bad[..., 1:] - bx_bad[..., :-1]) + (byc[:, 1:] - byc[:, :-1]) + (bzc[1:] - bzc[:-1])
+    assert float(div_bad.abs().max()) > 1e-3 * float(bxc.abs().max())

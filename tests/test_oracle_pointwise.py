@@ -66,9 +66,12 @@ def test_limo3_limited_regime_properties():
     """Above the smoothness threshold the limiter lies in [0, 1.5], never exceeds twice the
     slope ratio on monotone data (no new extremum at the face), and reconstructs linear
     data exactly in both regimes."""
+    r = np.random.default_rng(17)  # own stream: the result must not depend on test order
     for _ in range(3000):
-        dvp, dvm = rng.normal(size=2)
-        dx = 10 ** rng.uniform(-3, -1)  # differences of order 1: the limited regime
+        dvp, dvm = r.normal(size=2)
+        dx = 10 ** r.uniform(-3, -1)  # differences of order 1: the limited regime
+        if (dvm ** 2 + dvp ** 2) / (0.1 * dx) ** 2 <= 1.0:
+            continue  # a rare draw inside the smooth regime (unlimited third order)
         lim = O.limo3lim(dvp, dvm, dx)
         th = dvm / dvp
         assert 0.0 <= lim <= 1.5
