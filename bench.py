@@ -269,18 +269,28 @@ SINGLE_KERNEL_REGIONS = {"sweep_x": "k_sweep_x<", "sweep_y": "k_sweep_march<",
 # bench.py --copy-probe): FETCH_SIZE reads back half the bytes, WRITE_SIZE all of them
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
-# The equal-L1 comparison of the two legs: the learned-limiter accuracy study
-# (python -m idefix_amd.learned --fine 2048 --factor 8 ..., one MI355X): 2-D decaying
+# The equal-L1 comparison of the two legs: the learned-limiter accuracy studies on one
+# MI355X.  2-D (python -m idefix_amd.learned --fine 2048 --factor 8 ...): decaying
 # turbulence, held-out seed, L1 at t = 1 against the 2048^2 solution block-averaged to
-# 256^2.  No 3-D network was trained.  The learned 256^2 run does not reach the error of
-# the plain run at twice the resolution, so no equal-L1 wall-time ratio exists.
-EQUAL_L1 = {"study": "2-D decaying MHD turbulence, seeds 1-3 trained, seed 4 held out, "
-                     "L1 at t = 1 vs block-averaged 2048^2",
-            "learned_fp32_256": 2.1035e-3, "plain_ppm_fp64_256": 2.1046e-3,
-            "plain_ppm_fp64_512": 4.1035e-4, "wall_s": {"learned_fp32_256": 0.58,
-                                                        "plain_ppm_fp64_256": 0.48,
-                                                        "plain_ppm_fp64_512": 1.29},
-            "equal_l1_reached": False, "wall_time_ratio_at_equal_l1": None}
+# 256^2.  3-D (python -m idefix_amd.learned3d --fine 512 --coarse 128 --tend 0.5 --seed 4):
+# the 2-D network on 3-D turbulence, L1 at t = 0.5 against 512^3 block-averaged to 128^3.
+# In neither does the learned run reach the error of the plain run at twice the
+# resolution, so no equal-L1 wall-time ratio exists.
+EQUAL_L1 = {"study_2d": {"case": "2-D decaying MHD turbulence, seeds 1-3 trained, seed 4 "
+                                 "held out, L1 at t = 1 vs block-averaged 2048^2",
+                         "L1": {"learned_fp32_256": 2.1035e-3, "plain_ppm_fp64_256": 2.1046e-3,
+                                "plain_ppm_fp64_512": 4.1035e-4},
+                         "wall_s": {"learned_fp32_256": 0.58, "plain_ppm_fp64_256": 0.48,
+                                    "plain_ppm_fp64_512": 1.29}},
+            "study_3d": {"case": "3-D decaying MHD turbulence, seed 4, 2-D network, L1 at "
+                                 "t = 0.5 vs 512^3 block-averaged to 128^3",
+                         "L1": {"learned_fp32_128": 6.5057e-3, "plain_ppm_fp64_128": 6.5146e-3,
+                                "learned_fp32_256": 1.7048e-3, "plain_ppm_fp64_256": 1.7099e-3},
+                         "wall_s": {"learned_fp32_128": 5.50, "plain_ppm_fp64_128": 2.96,
+                                    "learned_fp32_256": 71.58, "plain_ppm_fp64_256": 35.96},
+                         "note": "the learned 128^3 run was the first and includes module load"},
+            "equal_l1_reached": False, "wall_time_ratio_at_equal_l1": None,
+            "source": "profiles/learned_turb2d.md, profiles/learned_turb3d.md"}
 
 
 def turb3d_block(n, precision, recon="ppm", ic=None):
