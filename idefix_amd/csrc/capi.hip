@@ -510,8 +510,12 @@ int idfx_ctx_array(idfx_ctx *c, int which, double **ptr) {
     *ptr = reinterpret_cast<double *>(c->mlp ? c->mlpW[which - 25] : nullptr);
     return *ptr ? IDFX_OK : IDFX_E_ARGS;
   }
+  // svx, svy, svz hold byte codes (twice the upwind weight), returned through the slot
   double *const arr[24] = {c->e64.ex,  c->e64.ey,  c->e64.ez,  c->e64.ezi, c->e64.ezj, c->e64.exj,
-                           c->e64.exk, c->e64.eyi, c->e64.eyk, c->e64.svx, c->e64.svy, c->e64.svz,
+                           c->e64.exk, c->e64.eyi, c->e64.eyk,
+                           reinterpret_cast<double *>(c->e64.svx),
+                           reinterpret_cast<double *>(c->e64.svy),
+                           reinterpret_cast<double *>(c->e64.svz),
                            c->e64.axL, c->e64.axR, c->e64.ayL, c->e64.ayR, c->e64.azL, c->e64.azR,
                            c->e64.dxL, c->e64.dxR, c->e64.dyL, c->e64.dyR, c->e64.dzL, c->e64.dzR};
   if (which < 0 || which >= 24) return IDFX_E_ARGS;

@@ -136,7 +136,7 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
         lds[S_AD + 3][tid] = ld_row(e.dxR, c0 - 1);
       } else {
         lds[S_E1][tid] = ld_row(e.ezi, c0 - 1);
-        if constexpr (emf == EMF_UCT_CONTACT) lds[S_S1][tid] = ld_row(e.svx, c0 - 1);
+        if constexpr (emf == EMF_UCT_CONTACT) lds[S_S1][tid] = sv_val<T>(ld32(e.svx, (unsigned)(c0 - 1) * sd + col));
       }
     }
     for (int f = c0; f <= c1; f++) {
@@ -278,13 +278,13 @@ void k_fused2d_y(GridDesc g, SweepArgs a, EmfArrays<T> e, Fused2dArgs s, MarchGe
             const T a4 = T(2) * ezj_m - T(0.5) * (E3_mm + E3_m);
             ez = T(0.25) * (a1 + a2 + a3 + a4);
           } else {
-            const T svx_c = xf(e.svx, f);
+            const T svx_c = sv_val<T>(ld32(e.svx, (unsigned)f * sd + col));
             T svx_m;
             if constexpr (CARRY) {
               svx_m = lds[S_S1][tid];
               lds[S_S1][tid] = svx_c;
             } else {
-              svx_m = xf(e.svx, f - 1);
+              svx_m = sv_val<T>(ld32(e.svx, (unsigned)(f - 1) * sd + col));
             }
             const T svy_c = E.sv, svy_m = lane_at(E.sv, lane, -1);
             const T ez_l2 = (1 - svx_m) * (ezj_c - E3_cm) + (svx_m) * (ezj_m - E3_mm);

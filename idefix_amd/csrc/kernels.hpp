@@ -38,7 +38,7 @@ template <typename T>
 struct EmfArrays {
   T *ex, *ey, *ez;               // corner (edge) EMFs
   T *ezi, *ezj, *exj, *exk, *eyi, *eyk;  // face EMFs (Et/Eb of each sweep)
-  T *svx, *svy, *svz;            // uct_contact upwind weights
+  unsigned char *svx, *svy, *svz;  // uct_contact upwind weights, as codes (sv_code)
   T *axL, *axR, *ayL, *ayR, *azL, *azR;  // uct_hll(d) coefficients
   T *dxL, *dxR, *dyL, *dyR, *dzL, *dzR;
 };
@@ -84,6 +84,18 @@ __device__ __forceinline__ T ld32(const T *base, unsigned i) {
 template <typename T>
 __device__ __forceinline__ void st32(T *base, unsigned i, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (unsigned)(i * (unsigned)sizeof(T))) = v;
+}
+
+// The uct_contact upwind weight of a face is 0, 1/2 or 1 (K_StoreContact,
+// storeFlux.hpp:39-43); it is stored as the byte 2 w (1 byte per face instead of 8) and
+// decoded exactly.
+template <typename T>
+__device__ __forceinline__ unsigned char sv_code(T w) {
+  return (unsigned char)(T(2) * w);
+}
+template <typename T>
+__device__ __forceinline__ T sv_val(unsigned char c) {
+  return T(0.5) * T(c);
 }
 
 // The equation-of-state constant a Riemann solver takes at the face between cells pl and
@@ -152,7 +164,8 @@ __device__ __forceinline__ void store_face_emf(const EmfArrays<T> &e, unsigned p
         st32(et_of<T, DIR>(e), p, E.Et);
         if constexpr (C::DIM == 3) st32(eb_of<T, DIR>(e), p, E.Eb);
       }
-      if (emf == EMF_UCT_CONTACT) st32(DIR == 0 ? e.svx : DIR == 1 ? e.svy : e.svz, p, E.sv);
+      if (emf == EMF_UCT_CONTACT)
+        st32(DIR == 0 ? e.svx : DIR == 1 ? e.svy : e.svz, p, sv_code(E.sv));
     } else {
       if constexpr (C::COMP > 1) {
         st32(et_of<T, DIR>(e), p, E.Et);
@@ -785,6 +798,9 @@ __global__ __launch_bounds__(256) void k_corner_emf(GridDesc g, const double *Vc
   const int k = g.beg[2] + t / ((long long)ni * nj);
   const long long p = idx3(g, k, j, i);
   auto at = [&](const T *arr, int kk, int jj, int ii) { return arr[idx3(g, kk, jj, ii)]; };
+  auto sva = [&](const unsigned char *arr, int kk, int jj, int ii) {
+    return sv_val<T>(arr[idx3(g, kk, jj, ii)]);
+  };
   if (emf == EMF_ARITHMETIC || emf == EMF_UCT0) {
     // CalcArithmeticAverage (:62-78); uct0 first rewrites the face EMFs (:150-172)
     const T w = T(0.25);
@@ -822,15 +838,15 @@ __global__ __launch_bounds__(256) void k_corner_emf(GridDesc g, const double *Vc
                          E2[dk][dj][di], E3[dk][dj][di]);
     if constexpr (C::DIM == 3) {
       ContactFaces<T> f;
-      f.svx = at(e.svx, k, j, i);
-      f.svx_jm = at(e.svx, k, j - 1, i);
-      f.svx_km = at(e.svx, k - 1, j, i);
-      f.svy = at(e.svy, k, j, i);
-      f.svy_im = at(e.svy, k, j, i - 1);
-      f.svy_km = at(e.svy, k - 1, j, i);
-      f.svz = at(e.svz, k, j, i);
-      f.svz_jm = at(e.svz, k, j - 1, i);
-      f.svz_im = at(e.svz, k, j, i - 1);
+      f.svx = sva(e.svx, k, j, i);
+      f.svx_jm = sva(e.svx, k, j - 1, i);
+      f.svx_km = sva(e.svx, k - 1, j, i);
+      f.svy = sva(e.svy, k, j, i);
+      f.svy_im = sva(e.svy, k, j, i - 1);
+      f.svy_km = sva(e.svy, k - 1, j, i);
+      f.svz = sva(e.svz, k, j, i);
+      f.svz_jm = sva(e.svz, k, j - 1, i);
+      f.svz_im = sva(e.svz, k, j, i - 1);
       f.ezi = at(e.ezi, k, j, i);
       f.ezi_jm = at(e.ezi, k, j - 1, i);
       f.ezj = at(e.ezj, k, j, i);
@@ -850,14 +866,14 @@ __global__ __launch_bounds__(256) void k_corner_emf(GridDesc g, const double *Vc
       e.ez[p] = ez;
     } else {
       const int K = D3;  // index of cell k in the [dk] axis
-      const T ez_l2 = (1 - at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i) - E3[K][0][1]) +
-                      (at(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i - 1) - E3[K][0][0]);
-      const T ez_r2 = (1 - at(e.svx, k, j, i)) * (at(e.ezj, k, j, i) - E3[K][1][1]) +
-                      (at(e.svx, k, j, i)) * (at(e.ezj, k, j, i - 1) - E3[K][1][0]);
-      const T ez_l1 = (1 - at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j, i) - E3[K][1][0]) +
-                      (at(e.svy, k, j, i - 1)) * (at(e.ezi, k, j - 1, i) - E3[K][0][0]);
-      const T ez_r1 = (1 - at(e.svy, k, j, i)) * (at(e.ezi, k, j, i) - E3[K][1][1]) +
-                      (at(e.svy, k, j, i)) * (at(e.ezi, k, j - 1, i) - E3[K][0][1]);
+      const T ez_l2 = (1 - sva(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i) - E3[K][0][1]) +
+                      (sva(e.svx, k, j - 1, i)) * (at(e.ezj, k, j, i - 1) - E3[K][0][0]);
+      const T ez_r2 = (1 - sva(e.svx, k, j, i)) * (at(e.ezj, k, j, i) - E3[K][1][1]) +
+                      (sva(e.svx, k, j, i)) * (at(e.ezj, k, j, i - 1) - E3[K][1][0]);
+      const T ez_l1 = (1 - sva(e.svy, k, j, i - 1)) * (at(e.ezi, k, j, i) - E3[K][1][0]) +
+                      (sva(e.svy, k, j, i - 1)) * (at(e.ezi, k, j - 1, i) - E3[K][0][0]);
+      const T ez_r1 = (1 - sva(e.svy, k, j, i)) * (at(e.ezi, k, j, i) - E3[K][1][1]) +
+                      (sva(e.svy, k, j, i)) * (at(e.ezi, k, j - 1, i) - E3[K][0][1]);
       e.ez[p] = T(0.25) * (ez_l2 + ez_r2 + ez_l1 + ez_r1 + at(e.ezi, k, j, i) +
                            at(e.ezi, k, j - 1, i) + at(e.ezj, k, j, i) + at(e.ezj, k, j, i - 1));
     }
@@ -1049,8 +1065,8 @@ __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, doub
     cell_emf32<C, T>(Vc, g.sv, p - sy, E1m0, a2, a3);
     cell_emf32<C, T>(Vc, g.sv, p, E1m1, E2m1, a3);
     cell_emf32<C, T>(Vc, g.sv, p - 1, a1, E2m0, a3);
-    svx_m = ld32(e.svx, p);
-    svy_m = ld32(e.svy, p);
+    svx_m = sv_val<T>(ld32(e.svx, p));
+    svy_m = sv_val<T>(ld32(e.svy, p));
     exj_m = ld32(e.exj, p);
     eyi_m = ld32(e.eyi, p);
   }
@@ -1069,15 +1085,15 @@ __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, doub
       E2[0][1][0] = E2m0;
       E2[0][1][1] = E2m1;
       ContactFaces<T> f;
-      f.svx = ld32(e.svx, p);
-      f.svx_jm = ld32(e.svx, p - sy);
+      f.svx = sv_val<T>(ld32(e.svx, p));
+      f.svx_jm = sv_val<T>(ld32(e.svx, p - sy));
       f.svx_km = svx_m;
-      f.svy = ld32(e.svy, p);
-      f.svy_im = ld32(e.svy, p - 1);
+      f.svy = sv_val<T>(ld32(e.svy, p));
+      f.svy_im = sv_val<T>(ld32(e.svy, p - 1));
       f.svy_km = svy_m;
-      f.svz = ld32(e.svz, p);
-      f.svz_jm = ld32(e.svz, p - sy);
-      f.svz_im = ld32(e.svz, p - 1);
+      f.svz = sv_val<T>(ld32(e.svz, p));
+      f.svz_jm = sv_val<T>(ld32(e.svz, p - sy));
+      f.svz_im = sv_val<T>(ld32(e.svz, p - 1));
       f.ezi = ld32(e.ezi, p);
       f.ezi_jm = ld32(e.ezi, p - sy);
       f.ezj = ld32(e.ezj, p);
@@ -1219,44 +1235,49 @@ __global__ __launch_bounds__(256) void k_prim_to_cons(GridDesc g, const double *
 // ReconstructVcField (boundary.hpp:388-404) + AddAndStore of the cell state
 // (stateContainer.cpp:75-104) + ConvertConsToPrim (convertConsToPrim.hpp:105-149), over
 // the box [lo, hi).  flags: 1 = combine Uc = wc*Uc + w0*Uc0 first, 2 = write Uc back.
+// A block of 256 threads covers a run of 256 cells along x: blockIdx.x = run + nxb (j -
+// lo1), blockIdx.y = k - lo2.  Offsets are 32-bit (a block holds fewer than 2^29 cells and
+// faces per component).
 template <class C>
 __global__ __launch_bounds__(256) void k_cons_to_prim(GridDesc g, double *Uc, const double *Uc0,
                                                       double *Vc, const double *Vs, int lo0,
-                                                      int lo1, int lo2, int n0, int n1, int n2,
+                                                      int lo1, int lo2, int n0, int nxb,
                                                       int flags, double wc, double w0) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n0 * n1 * n2) return;
-  const int i = lo0 + t % n0;
-  const int j = lo1 + (t / n0) % n1;
-  const int k = lo2 + t / ((long long)n0 * n1);
-  const long long p = idx3(g, k, j, i);
+  const int run = blockIdx.x % nxb;
+  const int i = lo0 + 256 * run + threadIdx.x;
+  if (i >= lo0 + n0) return;
+  const int j = lo1 + blockIdx.x / nxb;
+  const int k = lo2 + blockIdx.y;
+  const unsigned p = (unsigned)idx3(g, k, j, i);
   double u[C::NVAR], v[C::NVAR];
 #pragma unroll
   for (int n = 0; n < C::NVAR; n++) {
     if (C::is_ct_field(n)) continue;
-    u[n] = Uc[n * g.sv + p];
-    if (flags & 1) u[n] = wc * u[n] + w0 * Uc0[n * g.sv + p];
+    u[n] = ld32(Uc + n * g.sv, p);
+    if (flags & 1) u[n] = wc * u[n] + w0 * ld32(Uc0 + n * g.sv, p);
   }
   if constexpr (C::MHD) {
-    u[C::BX1] = 0.5 * (Vs[idxs(g, 0, k, j, i)] + Vs[idxs(g, 0, k, j, i + 1)]);
+    const unsigned vy = (unsigned)g.vsn[0], vz = (unsigned)g.vsn[0] * (unsigned)g.vsn[1];
+    const unsigned q = (unsigned)k * vz + (unsigned)j * vy + (unsigned)i;
+    u[C::BX1] = 0.5 * (ld32(Vs, q) + ld32(Vs, q + 1));
     if constexpr (C::DIM >= 2)
-      u[C::BX2] = 0.5 * (Vs[idxs(g, 1, k, j, i)] + Vs[idxs(g, 1, k, j + 1, i)]);
+      u[C::BX2] = 0.5 * (ld32(Vs + g.svs, q) + ld32(Vs + g.svs, q + vy));
     if constexpr (C::DIM >= 3)
-      u[C::BX3] = 0.5 * (Vs[idxs(g, 2, k, j, i)] + Vs[idxs(g, 2, k + 1, j, i)]);
+      u[C::BX3] = 0.5 * (ld32(Vs + 2 * g.svs, q) + ld32(Vs + 2 * g.svs, q + vz));
   }
   cons2prim<C>(u, v, g.gamma);
 #pragma unroll
-  for (int n = 0; n < C::NVAR; n++) Vc[n * g.sv + p] = v[n];
+  for (int n = 0; n < C::NVAR; n++) st32(Vc + n * g.sv, p, v[n]);
   if (flags & 2) {
 #pragma unroll
-    for (int n = 0; n < C::NVAR; n++) Uc[n * g.sv + p] = u[n];
+    for (int n = 0; n < C::NVAR; n++) st32(Uc + n * g.sv, p, u[n]);
   }
   // tracers: AddAndStore, then Tracer::ConvertConsToPrim (tracer.cpp:13-28)
   for (int n = C::NVAR; n < C::NVAR + g.ntr; n++) {
-    double ut = Uc[n * g.sv + p];
-    if (flags & 1) ut = wc * ut + w0 * Uc0[n * g.sv + p];
-    Vc[n * g.sv + p] = ut / u[C::RHO];
-    if (flags & 2) Uc[n * g.sv + p] = ut;
+    double ut = ld32(Uc + n * g.sv, p);
+    if (flags & 1) ut = wc * ut + w0 * ld32(Uc0 + n * g.sv, p);
+    st32(Vc + n * g.sv, p, ut / u[C::RHO]);
+    if (flags & 2) st32(Uc + n * g.sv, p, ut);
   }
 }
 

@@ -238,9 +238,10 @@ hipError_t launch_cons_to_prim(const GridDesc &g, double *Uc, const double *Uc0,
                                const double *Vs, const int *lo, const int *hi, int flags,
                                double wc, double w0, hipStream_t s) {
   const int n0 = hi[0] - lo[0], n1 = hi[1] - lo[1], n2 = hi[2] - lo[2];
-  const long long n = (long long)n0 * n1 * n2;
-  hipLaunchKernelGGL((k_cons_to_prim<C>), dim3(blocks_for(n)), dim3(256), 0, s, g, Uc, Uc0, Vc,
-                     Vs, lo[0], lo[1], lo[2], n0, n1, n2, flags, wc, w0);
+  if (n0 <= 0 || n1 <= 0 || n2 <= 0) return hipSuccess;
+  const int nxb = (n0 + 255) / 256;  // runs of 256 cells along x per row
+  hipLaunchKernelGGL((k_cons_to_prim<C>), dim3(nxb * n1, n2), dim3(256), 0, s, g, Uc, Uc0, Vc,
+                     Vs, lo[0], lo[1], lo[2], n0, nxb, flags, wc, w0);
   return hipGetLastError();
 }
 

@@ -195,16 +195,19 @@ class DataBlock:
         return out.value
 
     def ctx_array(self, which):
-        """Context-owned scratch array (face/corner EMFs, cMax) as a torch view."""
+        """Context-owned scratch array (face/corner EMFs, cMax) as a torch view; the
+        uct_contact upwind weights (9..11, stored as byte codes 2 w) as a float64 copy."""
         ptr = ctypes.c_void_p()
         capi.check(self._L.idfx_ctx_array(self.ctx, which, ctypes.byref(ptr)), "ctx_array")
         shape = (self.np_tot[2], self.np_tot[1], self.np_tot[0])
+        code = 9 <= which <= 11
 
         class _View:  # zero-copy wrapper through the CUDA array interface
-            __cuda_array_interface__ = {"shape": shape, "typestr": "<f8",
+            __cuda_array_interface__ = {"shape": shape, "typestr": "|u1" if code else "<f8",
                                         "data": (ptr.value, False), "version": 2,
                                         "strides": None}
-        return self.torch.as_tensor(_View(), device=self.device)
+        t = self.torch.as_tensor(_View(), device=self.device)
+        return t.double() * 0.5 if code else t
 
     # ---- driver ---------------------------------------------------------------------
     def stage(self, stage, dt, skip_boundaries=False, write_uc=True, want_dt=False):

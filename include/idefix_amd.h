@@ -111,6 +111,8 @@ const char *idfx_strerror(int status);
 /* Device pointer of a context-owned array for user hooks: which = 0..23 in the order
  * ex ey ez ezi ezj exj exk eyi eyk svx svy svz axL axR ayL ayR azL azR dxL dxR dyL dyR
  * dzL dzR (constrainedTransport.hpp members), 24 = cMax (fluid.hpp cMax).  FP64 only.
+ * svx, svy, svz (9..11) are unsigned char [np_tot cells] holding twice the uct_contact
+ * upwind weight (0, 1, 2 for 0, 1/2, 1), returned through the double* slot.
  * 25..27: the learned limiter's interface weights of x, y, z (float [16][np_tot cells],
  * returned through the double* slot; IDFX_RECON_MLP contexts only). */
 int idfx_ctx_array(idfx_ctx *ctx, int which, double **ptr);
