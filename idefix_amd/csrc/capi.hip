@@ -110,6 +110,9 @@ struct idfx_ctx {
   bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
   bool fused3d_ok = false;   // 3-D MHD with uct_contact: fused CT kernel (k_ct3d)
   bool fused3d_on = false;
+  // 3-D adiabatic MHD without per-cell options or user-defined sides: idfx_cycle's stages
+  // ping-pong between Vc and Vc2, the z sweep writing the next state (SW_FINAL)
+  bool fused3d_pp_ok = false;
   double *Vc2 = nullptr, *Vs2 = nullptr;
   // enrolled user boundary (IDFX_BC_USERDEF sides) and the time it is called with
   idfx_userdef_boundary_fn user_fn = nullptr;
@@ -454,6 +457,16 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
   }
   if (st == IDFX_OK && d.mhd && d.dim == 3 && d.emf == IDFX_EMF_UCT_CONTACT && ops->ct3d)
     c->fused3d_ok = c->fused3d_on = true;
+  if (st == IDFX_OK && c->fused3d_ok && ops->prs_from_faces && !d.isothermal &&
+      d.ntracer == 0 && !d.shock_flattening) {
+    bool user = false;
+    for (int q = 0; q < 3; q++)
+      user = user || d.lbound[q] == IDFX_BC_USERDEF || d.rbound[q] == IDFX_BC_USERDEF;
+    if (!user) {
+      st = alloc_n(c, &c->Vc2, c->n3 * c->nvar);
+      c->fused3d_pp_ok = st == IDFX_OK;
+    }
+  }
   if (st == IDFX_OK) {
     void *h = nullptr;
     if (hipHostMalloc(&h, 64, hipHostMallocDefault) != hipSuccess) st = IDFX_E_ALLOC;
@@ -708,9 +721,13 @@ static int stage_boundaries(idfx_ctx *c, double *Vc, double *Vs, bool cell_field
   return idfx_set_boundaries(c, Vc, Vs, s);
 }
 
+// VcOut (3-D ping-pong schedule, fused3d_pp_ok): the z sweep writes the stage's new state
+// into VcOut (rho, v, E - kinetic energy) and k_prs_from_faces completes it after the face
+// update, in place of ConsToPrim; Vc is only read.
 static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double *Vc, double *Vs,
                          double *Uc, double *InvDt, int skip_boundaries, int write_uc,
-                         bool want_dt, hipStream_t s, bool consistent = false) {
+                         bool want_dt, hipStream_t s, bool consistent = false,
+                         double *VcOut = nullptr) {
   static const double WC[2][2] = {{0.5, 0.0}, {0.25, 2.0 / 3.0}};
   static const double W0[2][2] = {{0.5, 0.0}, {0.75, 1.0 / 3.0}};
   if (!skip_boundaries) {
@@ -752,7 +769,12 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
                            InvDt ? InvDt : c->InvDt, c->dtmax);
     } else {
       IDFX_TRY(shock_step(c, dir, Vc, s));
+      if (VcOut && dir == dim - 1)
+        flags |= SW_FINAL | (stage > 0 ? SW_RK : 0) | (write_uc ? SW_WRITE_UC : 0);
       SweepArgs a = sweep_args(c, dir, dt, Vc, Vs, Uc, InvDt, flags);
+      a.VcOut = VcOut;
+      a.wc = wc;
+      a.w0 = w0;
       IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
       IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
     }
@@ -783,7 +805,10 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
   const int hi[3] = {c->g.end[0], c->g.end[1], c->g.end[2]};
   const int flags = (stage > 0 ? 1 : 0) | (write_uc ? 2 : 0);
   Region r(c, IDFX_REGION_CONS_TO_PRIM, s);
-  IDFX_TRY(check(c->ops->cons_to_prim(c->g, Uc, c->Uc0, Vc, Vs, lo, hi, flags, wc, w0, s)));
+  if (VcOut)
+    IDFX_TRY(check(c->ops->prs_from_faces(c->g, VcOut, Vs, Uc, write_uc, s)));
+  else
+    IDFX_TRY(check(c->ops->cons_to_prim(c->g, Uc, c->Uc0, Vc, Vs, lo, hi, flags, wc, w0, s)));
   return IDFX_OK;
 }
 
@@ -938,13 +963,20 @@ int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long nc
   // fused 2-D stages ping-pong between (Vc, Vs) and the context's pair; an odd stage count
   // runs its first stage in place so that the last stage writes the caller's arrays
   const bool fused = c->fused2d_on && nstages >= 2 && Vs;
+  // 3-D: the same alternation between (Vc, Vs) and (Vc2, Vs), Vs updated in place
+  const bool pp3 = c->fused3d_on && c->fused3d_pp_ok && nstages >= 2 && Vs;
   const int first_fused = (nstages % 2 == 1) ? 1 : 0;
   // stage times of the boundary calls (TimeIntegrator::Cycle advances t by dt after each
   // stage, then t = wc t + w0 t0): t0, t0 + dt, and t0 + dt/2 for the third RK3 stage
   const double tstage[3] = {t0, t0 + *dt, 0.25 * (t0 + *dt + *dt) + 0.75 * t0};
   for (int stage = 0; stage < nstages; stage++) {
     c->t_bound = tstage[stage];
-    if (!fused || stage < first_fused) {
+    if (pp3 && stage >= first_fused) {
+      const bool fwd = (stage - first_fused) % 2 == 0;
+      IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, fwd ? Vc : c->Vc2, Vs, Uc, InvDt, 0,
+                             stage == nstages - 1, stage == 0, s, stage > 0,
+                             fwd ? c->Vc2 : Vc));
+    } else if (!fused || stage < first_fused) {
       IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0,
                              stage == nstages - 1, stage == 0, s, stage > 0));
     } else {

@@ -51,6 +51,12 @@ enum : int {
   SW_INVDT_ADD = 8,    // InvDt += term
   SW_DT_REDUCE = 16,   // fold max(InvDt) of the updated cells into *dtmax
   SW_NO_INVDT_STORE = 32,  // keep InvDt in registers only (requires SW_INVDT_SET)
+  // last sweep of a 3-D stage on the ping-pong schedule (the FIN kernels): the updated
+  // cell state goes through AddAndStore and the field-independent part of ConsToPrim
+  // into VcOut instead of back into Uc
+  SW_FINAL = 64,
+  SW_RK = 128,         // with SW_FINAL: U = wc U + w0 U0 (stages after the first)
+  SW_WRITE_UC = 256,   // with SW_FINAL: also store the combined U into Uc
 };
 
 struct SweepArgs {
@@ -70,6 +76,8 @@ struct SweepArgs {
   double dtdV;
   const float *mlpw;  // learned-limiter interface weights of this direction (RECON_MLP)
   const unsigned char *shock;  // shock flags per cell (shock flattening) or nullptr
+  double *VcOut;      // SW_FINAL: output primitive state
+  double wc, w0;      // SW_RK weights
 };
 
 // Element i of an array whose base is uniform across the wave, addressed as base + 32-bit
@@ -343,7 +351,7 @@ __device__ __forceinline__ void tracer_update(const GridDesc &g, const SweepArgs
 // Cartesian, with the area correction of :156-194 applied to both faces first).  XTRA
 // compiles in the optional per-cell work (passive tracers); without it the kernels carry
 // no trace of the options.
-template <class C, int DIR, bool XTRA, typename T>
+template <class C, int DIR, bool XTRA, bool FIN = false, typename T>
 __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs &a, int k,
                                               int j, int i, const T *Fm, const T *Fp,
                                               T cmaxm, T cmaxp) {
@@ -372,13 +380,46 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     for (int n = 0; n < C::NVAR; n++)
       if (!C::is_ct_field(n)) u[n] = ld32((const double *)a.Uc + n * g.sv, p);
   }
+  double u0[C::NVAR];  // FIN: the stage's begin state, loaded before any store of this cell
+  if constexpr (FIN) {
+    if (a.flags & SW_RK) {
 #pragma unroll
-  for (int n = 0; n < C::NVAR; n++) {
-    if (C::is_ct_field(n)) continue;
-    const double fp = (double)Fp[n] * A;
-    const double fm = (double)Fm[n] * A;
-    const double rhs = -dtdV * (fp - fm);
-    st32(a.Uc + n * g.sv, p, u[n] + rhs);
+      for (int n = 0; n < C::NVAR; n++)
+        if (!C::is_ct_field(n)) u0[n] = ld32((const double *)a.Uc0 + n * g.sv, p);
+    }
+  }
+  if constexpr (FIN) {
+    // the stage's last sweep (ping-pong schedule): the new state, its RK combination as
+    // in AddAndStore, and rho, v and E - kinetic energy of ConsToPrim into VcOut; the
+    // pressure and the cell field follow once the faces are updated (k_prs_from_faces)
+    double un[C::NVAR], vo[C::NVAR];
+#pragma unroll
+    for (int n = 0; n < C::NVAR; n++) {
+      if (C::is_ct_field(n)) continue;
+      const double fp = (double)Fp[n] * A;
+      const double fm = (double)Fm[n] * A;
+      const double rhs = -dtdV * (fp - fm);
+      un[n] = u[n] + rhs;
+      if (a.flags & SW_RK) un[n] = a.wc * un[n] + a.w0 * u0[n];
+    }
+    if (a.flags & SW_WRITE_UC) {
+#pragma unroll
+      for (int n = 0; n < C::NVAR; n++)
+        if (!C::is_ct_field(n)) st32(a.Uc + n * g.sv, p, un[n]);
+    }
+    cons2prim_part<C>(un, vo);
+#pragma unroll
+    for (int n = 0; n < C::NVAR; n++)
+      if (!C::is_ct_field(n)) st32(a.VcOut + n * g.sv, p, vo[n]);
+  } else {
+#pragma unroll
+    for (int n = 0; n < C::NVAR; n++) {
+      if (C::is_ct_field(n)) continue;
+      const double fp = (double)Fp[n] * A;
+      const double fm = (double)Fm[n] * A;
+      const double rhs = -dtdV * (fp - fm);
+      st32(a.Uc + n * g.sv, p, u[n] + rhs);
+    }
   }
   if constexpr (XTRA) {
     if (g.ntr > 0) tracer_update<C, DIR>(g, a, p, dtdV, A, (double)Fm[C::RHO], (double)Fp[C::RHO]);
@@ -513,7 +554,7 @@ template <class C>
 constexpr int march_waves_per_eu() {
   return (march_lds_window<C>() && C::DIM == 2 && C::NVAR < 8) ? 3 : 2;
 }
-template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA>
+template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA, bool FIN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(march_waves_per_eu<C>())))
 void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
                                                      MarchGeom m) {
@@ -641,7 +682,7 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (f > c0 && rhs_lane) {
           const int cf = f - 1;
           const int ci = ic, cj = (DIR == JDIR) ? cf : jj, ck = (DIR == JDIR) ? kk : cf;
-          const double inv = cell_update<C, DIR, XTRA>(g, a, ck, cj, ci, Fprev, F, cprev, cmax);
+          const double inv = cell_update<C, DIR, XTRA, FIN>(g, a, ck, cj, ci, Fprev, F, cprev, cmax);
           const unsigned long long b = __double_as_longlong(inv);
           red = b > red ? b : red;
         }
@@ -1278,6 +1319,40 @@ __global__ __launch_bounds__(256) void k_cons_to_prim(GridDesc g, double *Uc, co
     if (flags & 1) ut = wc * ut + w0 * ld32(Uc0 + n * g.sv, p);
     st32(Vc + n * g.sv, p, ut / u[C::RHO]);
     if (flags & 2) st32(Uc + n * g.sv, p, ut);
+  }
+}
+
+// Second half of ConsToPrim on the ping-pong schedule (adiabatic MHD): the cell field of
+// the interior cells from the updated faces (ReconstructVcField, boundary.hpp:388-404),
+// the pressure from E - kinetic energy left in the energy slot by the last sweep
+// (prs_from_part), and with write_uc the cell field into Uc as k_cons_to_prim does.
+// Grid as k_cons_to_prim's.
+template <class C>
+__global__ __launch_bounds__(256) void k_prs_from_faces(GridDesc g, double *Vc, const double *Vs,
+                                                        double *Uc, int n0, int nxb,
+                                                        int write_uc) {
+  if constexpr (C::MHD && !C::ISO && C::DIM == 3) {
+    const int run = blockIdx.x % nxb;
+    const int i = g.beg[0] + 256 * run + threadIdx.x;
+    if (i >= g.beg[0] + n0) return;
+    const int j = g.beg[1] + blockIdx.x / nxb;
+    const int k = g.beg[2] + blockIdx.y;
+    const unsigned p = (unsigned)idx3(g, k, j, i);
+    const unsigned vy = (unsigned)g.vsn[0], vz = (unsigned)g.vsn[0] * (unsigned)g.vsn[1];
+    const unsigned q = (unsigned)k * vz + (unsigned)j * vy + (unsigned)i;
+    double b[3];
+    b[0] = 0.5 * (ld32(Vs, q) + ld32(Vs, q + 1));
+    b[1] = 0.5 * (ld32(Vs + g.svs, q) + ld32(Vs + g.svs, q + vy));
+    b[2] = 0.5 * (ld32(Vs + 2 * g.svs, q) + ld32(Vs + 2 * g.svs, q + vz));
+    st32(Vc + C::ENG * g.sv, p, prs_from_part<C>(ld32(Vc + C::ENG * g.sv, p), b, g.gamma));
+    st32(Vc + C::BX1 * g.sv, p, b[0]);
+    st32(Vc + C::BX2 * g.sv, p, b[1]);
+    st32(Vc + C::BX3 * g.sv, p, b[2]);
+    if (write_uc) {
+      st32(Uc + C::BX1 * g.sv, p, b[0]);
+      st32(Uc + C::BX2 * g.sv, p, b[1]);
+      st32(Uc + C::BX3 * g.sv, p, b[2]);
+    }
   }
 }
 

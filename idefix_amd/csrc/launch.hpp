@@ -41,6 +41,9 @@ struct Ops {
                     double *Uc, double *InvDt, double dt, hipStream_t s) = nullptr;
   hipError_t (*divb)(const GridDesc &g, const double *Vs, unsigned long long *out,
                      hipStream_t s) = nullptr;
+  // 3-D adiabatic MHD only: pressure and cell field after a SW_FINAL sweep
+  hipError_t (*prs_from_faces)(const GridDesc &g, double *Vc, const double *Vs, double *Uc,
+                               int write_uc, hipStream_t s) = nullptr;
   // 3-D MHD only: CalcCornerEMF (uct_contact) + EvolveMagField in one kernel (k_ct3d)
   hipError_t (*ct3d)(const GridDesc &g, const double *Vc, double *Vs, double *Vs0,
                      const void *emf, double dt, int mode, double wc, double w0,
@@ -87,7 +90,7 @@ inline int march_segment(long long columns, int nd, long long slots) {
   return bt;
 }
 
-template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA>
+template <class C, typename T, int DIR, bool FUSED, int EMF, bool XTRA, bool FIN = false>
 hipError_t launch_sweep_xtra(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
                             int target_waves, hipStream_t s) {
   if constexpr (DIR == IDIR) {
@@ -109,12 +112,12 @@ hipError_t launch_sweep_xtra(const GridDesc &g, const SweepArgs &a, const EmfArr
     const int nd = g.end[DIR] - g.beg[DIR];
     const long long columns = (long long)m.nxb * (m.thi - m.tlo);
     static const long long slots =
-        resident_waves(k_sweep_march<C, T, DIR, FUSED, EMF, XTRA>, target_waves);
+        resident_waves(k_sweep_march<C, T, DIR, FUSED, EMF, XTRA, FIN>, target_waves);
     m.ts = march_segment(columns, nd, slots);
     m.nseg = (nd + m.ts - 1) / m.ts;
     const long long waves = columns * m.nseg;
-    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED, EMF, XTRA>), dim3(blocks_for(waves, 4)), dim3(256),
-                       0, s, g, a, e, m);
+    hipLaunchKernelGGL((k_sweep_march<C, T, DIR, FUSED, EMF, XTRA, FIN>), dim3(blocks_for(waves, 4)),
+                       dim3(256), 0, s, g, a, e, m);
   }
   return hipGetLastError();
 }
@@ -124,6 +127,17 @@ hipError_t launch_sweep_xtra(const GridDesc &g, const SweepArgs &a, const EmfArr
 template <class C, typename T, int DIR, bool FUSED, int EMF>
 hipError_t launch_sweep_dir(const GridDesc &g, const SweepArgs &a, const EmfArrays<T> &e,
                             int target_waves, hipStream_t s) {
+  if (a.flags & SW_FINAL) {
+    // the z sweep of the 3-D ping-pong schedule (adiabatic MHD, uct_contact, no per-cell
+    // options)
+    if constexpr (DIR == KDIR && C::DIM == 3 && C::MHD && !C::ISO && FUSED &&
+                  EMF == EMF_UCT_CONTACT) {
+      if (a.shock != nullptr || g.ntr > 0) return hipErrorInvalidValue;
+      return launch_sweep_xtra<C, T, DIR, FUSED, EMF, false, true>(g, a, e, target_waves, s);
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (a.shock != nullptr || g.ntr > 0)
     return launch_sweep_xtra<C, T, DIR, FUSED, EMF, true>(g, a, e, target_waves, s);
   return launch_sweep_xtra<C, T, DIR, FUSED, EMF, false>(g, a, e, target_waves, s);
@@ -242,6 +256,16 @@ hipError_t launch_cons_to_prim(const GridDesc &g, double *Uc, const double *Uc0,
   const int nxb = (n0 + 255) / 256;  // runs of 256 cells along x per row
   hipLaunchKernelGGL((k_cons_to_prim<C>), dim3(nxb * n1, n2), dim3(256), 0, s, g, Uc, Uc0, Vc,
                      Vs, lo[0], lo[1], lo[2], n0, nxb, flags, wc, w0);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t launch_prs_from_faces(const GridDesc &g, double *Vc, const double *Vs, double *Uc,
+                                 int write_uc, hipStream_t s) {
+  const int n0 = g.end[0] - g.beg[0], n1 = g.end[1] - g.beg[1], n2 = g.end[2] - g.beg[2];
+  const int nxb = (n0 + 255) / 256;
+  hipLaunchKernelGGL((k_prs_from_faces<C>), dim3(nxb * n1, n2), dim3(256), 0, s, g, Vc, Vs, Uc,
+                     n0, nxb, write_uc);
   return hipGetLastError();
 }
 
@@ -474,6 +498,7 @@ Ops make_ops() {
   o.divb = &launch_divb<C>;
   if constexpr (C::MHD && C::DIM == 2) o.fused2d_y = &launch_fused2d_y<C, T>;
   if constexpr (C::MHD && C::DIM == 3) o.ct3d = &launch_ct3d<C, T>;
+  if constexpr (C::MHD && C::DIM == 3 && !C::ISO) o.prs_from_faces = &launch_prs_from_faces<C>;
   if constexpr (C::ORDER == RECON_MLP) o.mlp_infer = &launch_mlp<C>;
   o.find_shock = &launch_find_shock<C>;
   return o;

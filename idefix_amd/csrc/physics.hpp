@@ -240,6 +240,34 @@ __device__ __forceinline__ void cons2prim(const double *u, double *v, double gam
   }
 }
 
+// cons2prim above split in two for a schedule that knows the face field only later
+// (adiabatic MHD): cons2prim_part gives rho, v and E - kinetic energy (in the energy
+// slot) with cons2prim's operations, and prs_from_part finishes the pressure once the
+// cell field is known: ((E - kin) - mag) (gamma - 1) and the floor, as cons2prim does.
+template <class C>
+__device__ __forceinline__ void cons2prim_part(const double *u, double *v) {
+  static_assert(C::MHD && !C::ISO, "the split conversion is for adiabatic MHD");
+  v[C::RHO] = u[C::RHO];
+  v[C::MX1] = u[C::MX1] / u[C::RHO];
+  if constexpr (C::COMP >= 2) v[C::MX2] = u[C::MX2] / u[C::RHO];
+  if constexpr (C::COMP >= 3) v[C::MX3] = u[C::MX3] / u[C::RHO];
+  double m2 = u[C::MX1] * u[C::MX1];
+  if constexpr (C::COMP >= 2) m2 = m2 + u[C::MX2] * u[C::MX2];
+  if constexpr (C::COMP >= 3) m2 = m2 + u[C::MX3] * u[C::MX3];
+  const double kin = 0.5 / u[C::RHO] * m2;
+  v[C::ENG] = u[C::ENG] - kin;
+}
+template <class C>
+__device__ __forceinline__ double prs_from_part(double e_part, const double *b, double gamma) {
+  double b2 = b[0] * b[0];
+  if constexpr (C::COMP >= 2) b2 = b2 + b[1] * b[1];
+  if constexpr (C::COMP >= 3) b2 = b2 + b[2] * b[2];
+  const double mag = 0.5 * b2;
+  double p = (e_part - mag) * (gamma - 1.0);
+  if (p <= 0.0) p = 1.0e-5;
+  return p;
+}
+
 // ------------------------------------------------------------------------------------
 // K_Flux<Phys,DIR>, src/fluid/RiemannSolver/flux.hpp:28-95
 // ------------------------------------------------------------------------------------

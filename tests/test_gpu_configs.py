@@ -247,9 +247,11 @@ def test_fused_learned_path_bitwise_equals_per_kernel():
 
 @pytest.mark.parametrize("precision,nstages", [("fp64", 3), ("fp32", 2)])
 def test_fused_3d_ct_bitwise_equals_per_kernel(precision, nstages):
-    """3-D stages with the fused corner EMF + face update (k_ct3d, fusion level 1) and with
-    one kernel per reference step (level 0): bit-identical interior cells and interior
-    faces and the same dt, on a block whose extents leave partial tiles in i, j and k."""
+    """3-D stages with the fused corner EMF + face update (k_ct3d) and the ping-pong state
+    (the z sweep writes the next state, k_prs_from_faces finishes ConsToPrim), fusion
+    level 1, and with one kernel per reference step (level 0): bit-identical interior
+    cells, interior faces, conservative state and dt, on a block whose extents leave
+    partial tiles in i, j and k."""
     from idefix_amd.datablock import DataBlock
     nx = (70, 23, 41)
     blocks = []
@@ -267,6 +269,8 @@ def test_fused_3d_ct_bitwise_equals_per_kernel(precision, nstages):
     (a_c, a_s), (b_c, b_s) = blocks[0].download(), blocks[1].download()
     b, e = blocks[0].beg, blocks[0].end
     assert np.array_equal(blocks[0].interior(a_c), blocks[1].interior(b_c))
+    ua, ub = (g.Uc.cpu().numpy() for g in blocks)
+    assert np.array_equal(blocks[0].interior(ua), blocks[1].interior(ub))
     from tests.pairs import interior_faces
     for fa, fb in zip(interior_faces(blocks[0], a_s), interior_faces(blocks[0], b_s)):
         assert np.array_equal(fa, fb)
