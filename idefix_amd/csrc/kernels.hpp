@@ -354,7 +354,8 @@ __device__ __forceinline__ void tracer_update(const GridDesc &g, const SweepArgs
 template <class C, int DIR, bool XTRA, bool FIN = false, typename T>
 __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs &a, int k,
                                               int j, int i, const T *Fm, const T *Fp,
-                                              T cmaxm, T cmaxp) {
+                                              T cmaxm, T cmaxp,
+                                              const double *u0pre = nullptr) {
   const unsigned p = (unsigned)idx3(g, k, j, i);
   const double dtdV = a.uniform ? a.dtdV : a.dt / cell_dV<C::DIM>(g, k, j, i);
   const double A = face_A<C::DIM, DIR>(g, k, j, i);
@@ -380,12 +381,15 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
     for (int n = 0; n < C::NVAR; n++)
       if (!C::is_ct_field(n)) u[n] = ld32((const double *)a.Uc + n * g.sv, p);
   }
-  double u0[C::NVAR];  // FIN: the stage's begin state, loaded before any store of this cell
+  // FIN: the stage's begin state (loaded by the caller ahead of the Riemann solve, or
+  // here before any store of this cell)
+  double u0[C::NVAR];
   if constexpr (FIN) {
     if (a.flags & SW_RK) {
 #pragma unroll
       for (int n = 0; n < C::NVAR; n++)
-        if (!C::is_ct_field(n)) u0[n] = ld32((const double *)a.Uc0 + n * g.sv, p);
+        if (!C::is_ct_field(n))
+          u0[n] = u0pre ? u0pre[n] : ld32((const double *)a.Uc0 + n * g.sv, p);
     }
   }
   if constexpr (FIN) {
@@ -444,11 +448,13 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
 // lane itself, and cell c takes the flux of face c+1 from lane l+1.  A wave therefore
 // solves 63 faces and updates the 62 cells between them.
 // ------------------------------------------------------------------------------------
-// occupancy target of the x sweep: 4 waves per SIMD (<= 128 VGPRs) in 2-D, 3 in 3-D (the
-// 8-variable 2-D MHD state spills a little at 128 but still runs faster than at 3 waves)
+// occupancy target of the x sweep: 4 waves per SIMD (<= 128 VGPRs).  The 8-variable MHD
+// states spill a little at 128 (2-D; 3-D PPM in FP64: 36 B per lane) but still run faster
+// than at 3 waves (3-D PPM HLLD at 512^3: 11.7 against 12.4 ms); the other 3-D
+// reconstructions in FP64 spill more (up to 64 B) and stay at 3.
 template <class C>
 constexpr int x_waves_per_eu() {
-  return C::DIM == 3 ? 3 : 4;
+  return (C::DIM == 3 && C::ORDER != RECON_PPM) ? 3 : 4;
 }
 template <class C, typename T, bool FUSED, int EMF, bool XTRA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(x_waves_per_eu<C>())))
@@ -640,6 +646,17 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         for (int n = 0; n < NV; n++)
           win[NW - 1][n] = (T)ld32(a.Vc + n * g.sv, col + (unsigned)(f + H) * sd);
       }
+      // FIN: the begin state of cell f - 1 (updated at the end of this step), loaded now
+      // so that its latency overlaps the reconstruction and the Riemann solve
+      double u0pre[NV];
+      if constexpr (FIN) {
+        if ((a.flags & SW_RK) && f > c0 && rhs_lane) {
+#pragma unroll
+          for (int n = 0; n < NV; n++)
+            if (!C::is_ct_field(n))
+              u0pre[n] = ld32((const double *)a.Uc0 + n * g.sv, col + (unsigned)(f - 1) * sd);
+        }
+      }
       T vL[NV], vR[NV], vr_next[NV];
       const bool flat = XTRA && recon_flat<C>(a, col + (unsigned)f * sd);
 #pragma unroll
@@ -682,7 +699,8 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (f > c0 && rhs_lane) {
           const int cf = f - 1;
           const int ci = ic, cj = (DIR == JDIR) ? cf : jj, ck = (DIR == JDIR) ? kk : cf;
-          const double inv = cell_update<C, DIR, XTRA, FIN>(g, a, ck, cj, ci, Fprev, F, cprev, cmax);
+          const double inv =
+              cell_update<C, DIR, XTRA, FIN>(g, a, ck, cj, ci, Fprev, F, cprev, cmax, u0pre);
           const unsigned long long b = __double_as_longlong(inv);
           red = b > red ? b : red;
         }
