@@ -114,6 +114,7 @@ struct idfx_ctx {
   // ping-pong between Vc and Vc2, the z sweep writing the next state (SW_FINAL)
   bool fused3d_pp_ok = false;
   double *Vc2 = nullptr, *Vs2 = nullptr;
+  double *Vc3 = nullptr;  // 3-D ping-pong with three stages: the first stage's output
   // enrolled user boundary (IDFX_BC_USERDEF sides) and the time it is called with
   idfx_userdef_boundary_fn user_fn = nullptr;
   void *user_arg = nullptr;
@@ -966,16 +967,34 @@ int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long nc
   // 3-D: the same alternation between (Vc, Vs) and (Vc2, Vs), Vs updated in place
   const bool pp3 = c->fused3d_on && c->fused3d_pp_ok && nstages >= 2 && Vs;
   const int first_fused = (nstages % 2 == 1) ? 1 : 0;
+  // 3-D ping-pong inputs and outputs per stage: RK2 Vc -> Vc2 -> Vc; RK3 Vc -> Vc3 -> Vc2
+  // -> Vc (Vc3 allocated at the first such cycle; without it the first RK3 stage runs in
+  // place on the per-kernel ConsToPrim and the others alternate Vc -> Vc2 -> Vc)
+  if (pp3 && nstages == 3 && !c->Vc3 && alloc_n(c, &c->Vc3, c->n3 * c->nvar) != IDFX_OK) {
+    c->Vc3 = nullptr;
+    (void)hipGetLastError();  // the failed allocation must not surface as a launch error
+  }
+  double *pin[3] = {Vc, c->Vc2, Vc}, *pout[3] = {c->Vc2, Vc, nullptr};
+  int pp_first = 0;
+  if (nstages == 3) {
+    if (c->Vc3) {
+      pin[0] = Vc, pout[0] = c->Vc3;
+      pin[1] = c->Vc3, pout[1] = c->Vc2;
+      pin[2] = c->Vc2, pout[2] = Vc;
+    } else {
+      pp_first = 1;
+      pin[1] = Vc, pout[1] = c->Vc2;
+      pin[2] = c->Vc2, pout[2] = Vc;
+    }
+  }
   // stage times of the boundary calls (TimeIntegrator::Cycle advances t by dt after each
   // stage, then t = wc t + w0 t0): t0, t0 + dt, and t0 + dt/2 for the third RK3 stage
   const double tstage[3] = {t0, t0 + *dt, 0.25 * (t0 + *dt + *dt) + 0.75 * t0};
   for (int stage = 0; stage < nstages; stage++) {
     c->t_bound = tstage[stage];
-    if (pp3 && stage >= first_fused) {
-      const bool fwd = (stage - first_fused) % 2 == 0;
-      IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, fwd ? Vc : c->Vc2, Vs, Uc, InvDt, 0,
-                             stage == nstages - 1, stage == 0, s, stage > 0,
-                             fwd ? c->Vc2 : Vc));
+    if (pp3 && stage >= pp_first) {
+      IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, pin[stage], Vs, Uc, InvDt, 0,
+                             stage == nstages - 1, stage == 0, s, stage > 0, pout[stage]));
     } else if (!fused || stage < first_fused) {
       IDFX_TRY(stage_enqueue(c, stage, nstages, *dt, Vc, Vs, Uc, InvDt, 0,
                              stage == nstages - 1, stage == 0, s, stage > 0));
