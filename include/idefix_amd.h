@@ -218,10 +218,12 @@ int idfx_stage(idfx_ctx *ctx, int stage, int nstages, double dt, double cfl, dou
 /* TimeIntegrator::Cycle (timeIntegrator.cpp:246-414) on the fused path: all stages and
  * the dt update (CFL_max_var, first-cycle check, dt < 1e-15).  *t and *dt are updated;
  * ncycles is the index of this cycle.  On return the interior cells and interior faces of
- * Vc/Vs hold the new state; with the fused 2-D MHD schedule (idfx_ctx_set_fusion) the
- * ghost zones hold the boundary values of an earlier stage, which the next stage's
- * boundary fill rewrites before any use, and the corner/y-face EMF scratch arrays are not
- * written. */
+ * Vc/Vs (and Uc) hold the new state; with the fused schedules (idfx_ctx_set_fusion: 2-D
+ * MHD, and 3-D adiabatic MHD with uct_contact, no tracers, no shock flattening, no
+ * user-defined sides, whose stages alternate between Vc and context-owned state arrays)
+ * the ghost zones hold the boundary values of an earlier stage, which the next stage's
+ * boundary fill rewrites before any use, and the corner (2-D also y-face) EMF scratch
+ * arrays are not written.  The first 3-D RK3 cycle allocates one more state array. */
 int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long ncycles,
                double *t, double *dt, double *Vc, double *Vs, double *Uc, double *InvDt,
                void *stream);
@@ -232,8 +234,11 @@ int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long 
  * MHD without user-defined boundaries, the y sweep, corner EMF, face-field update,
  * AddAndStore and ConvertConsToPrim into one kernel, and in 3-D MHD with uct_contact the
  * corner EMF and the face-field update (with its AddAndStore) into one kernel, which then
- * leaves the edge EMF arrays unwritten and the face fields of ghost columns as they were.
- * Both levels give identical interior cells and faces.
+ * leaves the edge EMF arrays unwritten and the face fields of ghost columns as they were;
+ * for adiabatic 3-D MHD (no tracers, no shock flattening, no user-defined sides) the last
+ * sweep also applies AddAndStore and the field-independent part of ConvertConsToPrim into
+ * the next state array, and a pointwise kernel adds pressure and cell field.
+ * Both levels give identical interior cells, faces and Uc.
  * Returns IDFX_E_ARGS for level 1 where the configuration does not support it. */
 int idfx_ctx_set_fusion(idfx_ctx *ctx, int level);
 
