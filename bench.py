@@ -79,6 +79,27 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8, "launches": 10}))
 
 
+def copy_bandwidth():
+    """Measured device copy bandwidth (GB/s): a 1 GiB float64 copy (1 GiB read + 1 GiB
+    written) 10 times after one warm-up, timed with events on torch's stream, where the
+    copies run.  The STREAM-like practical ceiling beside the vendor peak."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 10 * 2 * n * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def make_block(nx, precision, solver="hlld", recon="plm", emf="uct_hlld", ntracer=0):
     from idefix_amd.datablock import DataBlock
     from idefix_amd import problems
@@ -444,6 +465,8 @@ def main():
     divb = g.check_divb()
     del g
     torch.cuda.empty_cache()
+    copy_gbs = copy_bandwidth()
+    print("bench: device copy %.0f GB/s" % copy_gbs, file=sys.stderr, flush=True)
     traffic, traffic_note = None, "not measured (--no-traffic)"
     if not args.no_traffic:
         # the children start from this run's initial state, handed over in memory-backed
@@ -509,7 +532,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes": stage_bytes, "ms_per_launch": kt[dom],
-                     "traffic": traffic, "traffic_note": traffic_note},
+                     "traffic": traffic, "traffic_note": traffic_note,
+                     "copy_gbs": copy_gbs,
+                     "frac_of_copy": achieved / copy_gbs if copy_gbs else None},
         "cpu_baseline": cpu,
         "reference_benchmark": headline,
     }

@@ -277,6 +277,34 @@ def test_fused_3d_ct_bitwise_equals_per_kernel(precision, nstages):
     assert blocks[0].check_divb() < 1e-11
 
 
+def test_fused_3d_learned_bitwise_equals_per_kernel():
+    """3-D learned limiter with a network: the fused CT + ping-pong stages and the
+    per-kernel schedule give bit-identical interiors, faces and dt (the interface weights
+    are computed from each stage's input state on both schedules)."""
+    from idefix_amd import learned
+    from idefix_amd.datablock import DataBlock
+    p = learned.init_params(5)
+    p = p + np.random.default_rng(6).normal(0, 0.05, p.shape).astype(np.float32)
+    blocks = []
+    for level in (1, 0):
+        g = DataBlock(3, (40, 27, 19), comps=3, mhd=True, recon="mlp", solver="hlld",
+                      emf="uct_contact", precision="fp32", cfl=0.4, nstages=3)
+        g.set_fusion(level)
+        Vc, Vs = problems.turbulence(g, seed=8)
+        g.upload(Vc, Vs)
+        g.set_mlp_params(p)
+        blocks.append(g)
+    for _ in range(3):
+        for g in blocks:
+            g.cycle()
+        assert blocks[0].dt == blocks[1].dt
+    (a_c, a_s), (b_c, b_s) = blocks[0].download(), blocks[1].download()
+    assert np.array_equal(blocks[0].interior(a_c), blocks[1].interior(b_c))
+    from tests.pairs import interior_faces
+    for fa, fb in zip(interior_faces(blocks[0], a_s), interior_faces(blocks[0], b_s)):
+        assert np.array_equal(fa, fb)
+
+
 def test_fused_3d_region_is_used():
     from idefix_amd.datablock import DataBlock
     g = DataBlock(3, (16, 12, 10), comps=3, emf="uct_contact", recon="plm", cfl=0.4)
