@@ -69,6 +69,11 @@ __global__ void k_fill(double *p, long long n, double v) {
 
 using namespace idfx;
 
+// Interior cells from which the fused CT kernel replaces k_corner_emf + k_evolve_b at
+// fusion level 1 (3-D uct_contact, MI355X, per stage: 128^3 0.19 against 0.12 ms, 256^3
+// 1.10 against 1.00 ms, 384^3 3.52 against 3.44 ms, 512^3 7.84 against 8.28 ms)
+constexpr long long CT3D_MIN_CELLS = 1LL << 26;
+
 struct idfx_ctx {
   idfx_desc d;
   GridDesc g;
@@ -110,6 +115,9 @@ struct idfx_ctx {
   bool fused2d_on = false;   // selected (idfx_ctx_set_fusion)
   bool fused3d_ok = false;   // 3-D MHD with uct_contact: fused CT kernel (k_ct3d)
   bool fused3d_on = false;
+  // the fused CT kernel (k_ct3d) rather than k_corner_emf + k_evolve_b: fusion level 2, or
+  // level 1 on blocks of at least CT3D_MIN_CELLS interior cells, where it is the faster one
+  bool ct3d_use = false;
   // 3-D adiabatic MHD without per-cell options or user-defined sides: idfx_cycle's stages
   // ping-pong between Vc and Vc2, the z sweep writing the next state (SW_FINAL)
   bool fused3d_pp_ok = false;
@@ -456,8 +464,10 @@ int idfx_ctx_create(const idfx_desc *desc, const double *dx1, const double *dx2,
       c->fused2d_ok = c->fused2d_on = st == IDFX_OK;
     }
   }
-  if (st == IDFX_OK && d.mhd && d.dim == 3 && d.emf == IDFX_EMF_UCT_CONTACT && ops->ct3d)
+  if (st == IDFX_OK && d.mhd && d.dim == 3 && d.emf == IDFX_EMF_UCT_CONTACT && ops->ct3d) {
     c->fused3d_ok = c->fused3d_on = true;
+    c->ct3d_use = (long long)d.nx[0] * d.nx[1] * d.nx[2] >= CT3D_MIN_CELLS;
+  }
   if (st == IDFX_OK && c->fused3d_ok && ops->prs_from_faces && !d.isothermal &&
       d.ntracer == 0 && !d.shock_flattening) {
     bool user = false;
@@ -782,7 +792,7 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
   }
   if (c->d.mhd && dim >= 2) {
     const int mode = nstages > 1 ? (stage == 0 ? 1 : 2) : 0;
-    if (c->fused3d_on) {
+    if (c->fused3d_on && c->ct3d_use) {
       // CalcCornerEMF + EvolveMagField in one kernel (the edge arrays are not written)
       Region r(c, IDFX_REGION_CT_FUSED, s);
       IDFX_TRY(check(c->ops->ct3d(c->g, Vc, Vs, c->Vs0, c->emf(), dt, mode, wc, w0, s)));
@@ -932,10 +942,12 @@ int idfx_mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, void *strea
 }
 
 int idfx_ctx_set_fusion(idfx_ctx *c, int level) {
-  if (!c || level < 0 || level > 1) return IDFX_E_ARGS;
-  if (level == 1 && !c->fused2d_ok && !c->fused3d_ok) return IDFX_E_ARGS;
-  c->fused2d_on = level == 1 && c->fused2d_ok;
-  c->fused3d_on = level == 1 && c->fused3d_ok;
+  if (!c || level < 0 || level > 2) return IDFX_E_ARGS;
+  if (level >= 1 && !c->fused2d_ok && !c->fused3d_ok) return IDFX_E_ARGS;
+  c->fused2d_on = level >= 1 && c->fused2d_ok;
+  c->fused3d_on = level >= 1 && c->fused3d_ok;
+  c->ct3d_use = c->fused3d_on &&
+                (level == 2 || (long long)c->d.nx[0] * c->d.nx[1] * c->d.nx[2] >= CT3D_MIN_CELLS);
   return IDFX_OK;
 }
 

@@ -280,8 +280,10 @@ class DataBlock:
         return W
 
     def set_fusion(self, level):
-        """Kernel schedule of cycle(): 0 = one kernel per reference step, 1 = fused 2-D MHD
-        second half (default where supported).  Results are identical."""
+        """Kernel schedule of cycle(): 0 = one kernel per reference step, 1 = the fused
+        schedules (2-D MHD second half; 3-D ping-pong stage, with the fused CT kernel on
+        blocks of at least 2^26 interior cells), the default where supported, 2 = as 1 with
+        the fused 3-D CT kernel at every size.  Results are identical."""
         capi.check(self._L.idfx_ctx_set_fusion(self.ctx, int(level)), "set_fusion")
 
     def set_profiling(self, on):

@@ -238,8 +238,11 @@ int idfx_cycle(idfx_ctx *ctx, int nstages, double cfl, double cfl_max_var, long 
  * for adiabatic 3-D MHD (no tracers, no shock flattening, no user-defined sides) the last
  * sweep also applies AddAndStore and the field-independent part of ConvertConsToPrim into
  * the next state array, and a pointwise kernel adds pressure and cell field.
- * Both levels give identical interior cells, faces and Uc.
- * Returns IDFX_E_ARGS for level 1 where the configuration does not support it. */
+ * In 3-D, level 1 takes the fused CT kernel on blocks of at least 2^26 interior cells
+ * and k_corner_emf + k_evolve_b below (the faster choice on MI355X); level 2 takes the
+ * fused CT kernel at every size.  All levels give identical interior cells, faces and Uc.
+ * Returns IDFX_E_ARGS for levels 1 and 2 where the configuration supports neither fused
+ * schedule, and for other levels. */
 int idfx_ctx_set_fusion(idfx_ctx *ctx, int level);
 
 /* Regions of idfx_cycle / idfx_stage timed by the region profiler. */

@@ -247,15 +247,16 @@ def test_fused_learned_path_bitwise_equals_per_kernel():
 
 @pytest.mark.parametrize("precision,nstages", [("fp64", 3), ("fp32", 2)])
 def test_fused_3d_ct_bitwise_equals_per_kernel(precision, nstages):
-    """3-D stages with the fused corner EMF + face update (k_ct3d) and the ping-pong state
-    (the z sweep writes the next state, k_prs_from_faces finishes ConsToPrim), fusion
-    level 1, and with one kernel per reference step (level 0): bit-identical interior
-    cells, interior faces, conservative state and dt, on a block whose extents leave
-    partial tiles in i, j and k."""
+    """3-D stages on the ping-pong state (the z sweep writes the next state,
+    k_prs_from_faces finishes ConsToPrim) with the fused corner EMF + face update (k_ct3d,
+    fusion level 2) and with k_corner_emf + k_evolve_b (level 1 at this size), and with
+    one kernel per reference step (level 0): bit-identical interior cells, interior faces,
+    conservative state and dt, on a block whose extents leave partial tiles in i, j and
+    k."""
     from idefix_amd.datablock import DataBlock
     nx = (70, 23, 41)
     blocks = []
-    for level in (1, 0):
+    for level in (2, 1, 0):
         g = DataBlock(3, nx, comps=3, emf="uct_contact", recon="ppm", cfl=0.4,
                       nstages=nstages, precision=precision)
         g.set_fusion(level)
@@ -265,15 +266,16 @@ def test_fused_3d_ct_bitwise_equals_per_kernel(precision, nstages):
     for _ in range(3):
         for g in blocks:
             g.cycle()
-        assert blocks[0].dt == blocks[1].dt
-    (a_c, a_s), (b_c, b_s) = blocks[0].download(), blocks[1].download()
-    b, e = blocks[0].beg, blocks[0].end
-    assert np.array_equal(blocks[0].interior(a_c), blocks[1].interior(b_c))
-    ua, ub = (g.Uc.cpu().numpy() for g in blocks)
-    assert np.array_equal(blocks[0].interior(ua), blocks[1].interior(ub))
+        assert blocks[0].dt == blocks[2].dt and blocks[1].dt == blocks[2].dt
     from tests.pairs import interior_faces
-    for fa, fb in zip(interior_faces(blocks[0], a_s), interior_faces(blocks[0], b_s)):
-        assert np.array_equal(fa, fb)
+    ref_c, ref_s = blocks[2].download()
+    ref_u = blocks[2].Uc.cpu().numpy()
+    for g in blocks[:2]:
+        a_c, a_s = g.download()
+        assert np.array_equal(g.interior(a_c), g.interior(ref_c))
+        assert np.array_equal(g.interior(g.Uc.cpu().numpy()), g.interior(ref_u))
+        for fa, fb in zip(interior_faces(g, a_s), interior_faces(g, ref_s)):
+            assert np.array_equal(fa, fb)
     assert blocks[0].check_divb() < 1e-11
 
 
@@ -286,7 +288,7 @@ def test_fused_3d_learned_bitwise_equals_per_kernel():
     p = learned.init_params(5)
     p = p + np.random.default_rng(6).normal(0, 0.05, p.shape).astype(np.float32)
     blocks = []
-    for level in (1, 0):
+    for level in (2, 0):
         g = DataBlock(3, (40, 27, 19), comps=3, mhd=True, recon="mlp", solver="hlld",
                       emf="uct_contact", precision="fp32", cfl=0.4, nstages=3)
         g.set_fusion(level)
@@ -311,7 +313,13 @@ def test_fused_3d_region_is_used():
     Vc, Vs = problems.turbulence(g, seed=3)
     g.upload(Vc, Vs)
     g.set_profiling(True)
+    g.cycle()  # level 1 on a small block: the two CT kernels
+    rt = g.region_times()
+    assert "ct_fused" not in rt and "corner_emf" in rt and "evolve_b" in rt
+    g.set_fusion(2)
     g.cycle()
     rt = g.region_times()
     g.set_profiling(False)
     assert "ct_fused" in rt and "corner_emf" not in rt and "evolve_b" not in rt
+    with pytest.raises(Exception):
+        g.set_fusion(3)
