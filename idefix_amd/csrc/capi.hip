@@ -123,6 +123,7 @@ struct idfx_ctx {
   bool fused3d_pp_ok = false;
   double *Vc2 = nullptr, *Vs2 = nullptr;
   double *Vc3 = nullptr;  // 3-D ping-pong with three stages: the first stage's output
+  bool vc3_failed = false;  // its allocation failed once: RK3 keeps the in-place first stage
   // enrolled user boundary (IDFX_BC_USERDEF sides) and the time it is called with
   idfx_userdef_boundary_fn user_fn = nullptr;
   void *user_arg = nullptr;
@@ -982,8 +983,10 @@ int idfx_cycle(idfx_ctx *c, int nstages, double cfl, double cfl_max_var, long nc
   // 3-D ping-pong inputs and outputs per stage: RK2 Vc -> Vc2 -> Vc; RK3 Vc -> Vc3 -> Vc2
   // -> Vc (Vc3 allocated at the first such cycle; without it the first RK3 stage runs in
   // place on the per-kernel ConsToPrim and the others alternate Vc -> Vc2 -> Vc)
-  if (pp3 && nstages == 3 && !c->Vc3 && alloc_n(c, &c->Vc3, c->n3 * c->nvar) != IDFX_OK) {
+  if (pp3 && nstages == 3 && !c->Vc3 && !c->vc3_failed &&
+      alloc_n(c, &c->Vc3, c->n3 * c->nvar) != IDFX_OK) {
     c->Vc3 = nullptr;
+    c->vc3_failed = true;
     (void)hipGetLastError();  // the failed allocation must not surface as a launch error
   }
   double *pin[3] = {Vc, c->Vc2, Vc}, *pout[3] = {c->Vc2, Vc, nullptr};
