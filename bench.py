@@ -282,10 +282,16 @@ B_ZU_TURB3D = 7 * 8 * 8
 STAGES_TURB3D = 3
 # regions of the per-kernel 3-D schedule that are one kernel each, and the kernel names the
 # profiler reports for them (FP64 instantiations)
-SINGLE_KERNEL_REGIONS = {"sweep_x": "k_sweep_x<", "sweep_y": "k_sweep_march<",
-                         "sweep_z": "k_sweep_march<", "corner_emf": "k_corner_emf<",
-                         "evolve_b": "k_evolve_b<", "cons_to_prim": "k_cons_to_prim<",
-                         "ct_fused": "k_ct3d<"}
+# region -> (kernel name prefix, further substring of the demangled name): the y and z
+# sweeps are one template, told apart by its direction argument
+SINGLE_KERNEL_REGIONS = {"sweep_x": ("k_sweep_x<", ""),
+                         "sweep_y": ("k_sweep_march<", ", double, 1, "),
+                         "sweep_z": ("k_sweep_march<", ", double, 2, "),
+                         "corner_emf": ("k_corner_emf<", ""), "evolve_b": ("k_evolve_b<", ""),
+                         "cons_to_prim": ("k_cons_to_prim<", ""), "ct_fused": ("k_ct3d<", "")}
+# FP64 VALU issue rate of the device: a wave64 FP64 instruction occupies a SIMD (16 lanes)
+# for 4 cycles; 256 CUs x 4 SIMDs at 2.4 GHz
+VALU_SIMDS, VALU_CLOCK_HZ, VALU_CYCLES_PER_INST = 1024, 2.4e9, 4
 # plain-copy calibration of the traffic counters (1 GiB read + 1 GiB written per launch,
 # bench.py --copy-probe): FETCH_SIZE reads back half the bytes, WRITE_SIZE all of them
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
@@ -339,7 +345,7 @@ def learned_params():
 
 def traffic_child(cache, n, region, cycles):
     """Run under rocprofv3 --pmc: the FP64 headline block from the cached initial state,
-    `cycles` cycles on the per-kernel schedule."""
+    `cycles` cycles of the same schedule as the timed run."""
     import numpy as np
     import torch
     ic = (np.load(cache + "_Vc.npy", mmap_mode="r"), np.load(cache + "_Vs.npy", mmap_mode="r"))
@@ -353,18 +359,19 @@ def traffic_child(cache, n, region, cycles):
 def measure_traffic(cache, n, region, timeout=240):
     """HBM bytes per launch of the kernel of `region`, from two rocprofv3 --pmc runs of
     traffic_child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the
-    kernel's dispatches; y and z sweeps share a kernel name, so both count), scaled by the
-    plain-copy calibration.  None if rocprofv3 is missing or a run fails."""
+    kernel's dispatches), scaled by the plain-copy calibration, and its VALU instructions
+    per launch from a third run (SQ_INSTS_VALU).  (None, reason, None) if rocprofv3 is
+    missing or a run fails."""
     import csv
     import shutil
     import subprocess
     import tempfile
     prof = shutil.which("rocprofv3")
     if prof is None:
-        return None, "rocprofv3 not found"
+        return None, "rocprofv3 not found", None
     env = dict(os.environ, TMPDIR="/tmp")
     out = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
         d = tempfile.mkdtemp(prefix="idfx_pmc_", dir="/tmp")
         cmd = ["timeout", "-s", "KILL", str(timeout), prof, "--pmc", counter,
                "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
@@ -376,24 +383,26 @@ def measure_traffic(cache, n, region, timeout=240):
                  if f.endswith("counter_collection.csv")]
         if not files:
             shutil.rmtree(d, ignore_errors=True)
-            return None, "rocprofv3 --pmc %s failed (rc %d)" % (counter, r.returncode)
+            return None, "rocprofv3 --pmc %s failed (rc %d)" % (counter, r.returncode), None
         per = {}
-        want = SINGLE_KERNEL_REGIONS[region]
+        prefix, sub = SINGLE_KERNEL_REGIONS[region]
         for row in csv.DictReader(open(files[0])):
-            if row["Counter_Name"] != counter or want not in row["Kernel_Name"]:
+            name = row["Kernel_Name"]
+            if row["Counter_Name"] != counter or prefix not in name or sub not in name:
                 continue
-            if "double" not in row["Kernel_Name"]:
+            if "double" not in name:
                 continue
             per[row["Dispatch_Id"]] = per.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
         shutil.rmtree(d, ignore_errors=True)
         if not per:
-            return None, "no %s dispatch of %s in the profile" % (counter, want)
-        out[counter] = sum(per.values()) / len(per) * 1024.0
-    return (out["FETCH_SIZE"] * FETCH_SCALE + out["WRITE_SIZE"] * WRITE_SCALE,
-            {"fetch_bytes": out["FETCH_SIZE"] * FETCH_SCALE,
-             "write_bytes": out["WRITE_SIZE"] * WRITE_SCALE,
+            return None, "no %s dispatch of %s in the profile" % (counter, prefix + sub), None
+        out[counter] = sum(per.values()) / len(per)
+    return (out["FETCH_SIZE"] * 1024.0 * FETCH_SCALE + out["WRITE_SIZE"] * 1024.0 * WRITE_SCALE,
+            {"fetch_bytes": out["FETCH_SIZE"] * 1024.0 * FETCH_SCALE,
+             "write_bytes": out["WRITE_SIZE"] * 1024.0 * WRITE_SCALE,
              "scale": "FETCH_SIZE x %.1f, WRITE_SIZE x %.1f (1 GiB copy calibration)"
-                      % (FETCH_SCALE, WRITE_SCALE)})
+                      % (FETCH_SCALE, WRITE_SCALE)},
+            out["SQ_INSTS_VALU"])
 
 
 def cpu_baseline_turb3d(n, seconds):
@@ -467,7 +476,7 @@ def main():
     torch.cuda.empty_cache()
     copy_gbs = copy_bandwidth()
     print("bench: device copy %.0f GB/s" % copy_gbs, file=sys.stderr, flush=True)
-    traffic, traffic_note = None, "not measured (--no-traffic)"
+    traffic, traffic_note, valu_insts = None, "not measured (--no-traffic)", None
     if not args.no_traffic:
         # the children start from this run's initial state, handed over in memory-backed
         # files (removed afterwards)
@@ -476,7 +485,7 @@ def main():
         try:
             np.save(cache + "_Vc.npy", ic[0])
             np.save(cache + "_Vs.npy", ic[1])
-            traffic, traffic_note = measure_traffic(cache, n, dom)
+            traffic, traffic_note, valu_insts = measure_traffic(cache, n, dom)
         finally:
             for suffix in ("_Vc.npy", "_Vs.npy"):
                 if os.path.exists(cache + suffix):
@@ -534,7 +543,14 @@ def main():
                      "algorithmic_bytes": stage_bytes, "ms_per_launch": kt[dom],
                      "traffic": traffic, "traffic_note": traffic_note,
                      "copy_gbs": copy_gbs,
-                     "frac_of_copy": achieved / copy_gbs if copy_gbs else None},
+                     "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
+                     "valu": None if valu_insts is None else {
+                         "insts_per_launch": valu_insts,
+                         "busy": valu_insts * VALU_CYCLES_PER_INST
+                                 / (VALU_SIMDS * VALU_CLOCK_HZ * kt[dom] * 1e-3),
+                         "note": "SQ_INSTS_VALU per launch (rocprofv3 --pmc child run) x 4 "
+                                 "cycles / (1024 SIMDs x 2.4 GHz x launch time): the FP64 "
+                                 "VALU issue fraction, the kernel's actual bound"}},
         "cpu_baseline": cpu,
         "reference_benchmark": headline,
     }
