@@ -70,8 +70,9 @@ __global__ void k_fill(double *p, long long n, double v) {
 using namespace idfx;
 
 // Interior cells from which the fused CT kernel replaces k_corner_emf + k_evolve_b at
-// fusion level 1 (3-D uct_contact, MI355X, per stage: 128^3 0.19 against 0.12 ms, 256^3
-// 1.10 against 1.00 ms, 384^3 3.52 against 3.44 ms, 512^3 7.84 against 8.28 ms)
+// fusion level 1 (3-D uct_contact, MI355X, per stage, fused against the two kernels:
+// 128^3 0.19 against 0.12 ms, 256^3 1.10 against 0.87-0.92 ms, 384^3 3.52 against 3.25
+// ms, 512^3 7.84 against 8.34 ms)
 constexpr long long CT3D_MIN_CELLS = 1LL << 26;
 
 struct idfx_ctx {

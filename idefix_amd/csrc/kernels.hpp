@@ -126,6 +126,16 @@ __device__ __forceinline__ long long idxs(const GridDesc &g, int n, int k, int j
   return (((long long)n * g.vsn[2] + k) * g.vsn[1] + j) * g.vsn[0] + i;
 }
 
+// Work-groups are dispatched to the 8 XCDs in turn (block b on XCD b % 8), each with its
+// own L2.  This maps the dispatched block to a logical block so that each XCD takes a
+// contiguous run of logical blocks: kernels whose threads read the neighbouring rows and
+// planes of the blocks before them then find them in their own L2.  A bijection on
+// [0, gridDim.x).
+__device__ __forceinline__ int xcd_contiguous_block() {
+  const int nb = gridDim.x, xcd = blockIdx.x % 8, q8 = nb / 8, r8 = nb % 8;
+  return (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + blockIdx.x / 8;
+}
+
 // Cartesian volume and areas, makeGeometry.cpp:67-70, :146-147, :178-179, :201-202
 template <int DIM>
 __device__ __forceinline__ double cell_dV(const GridDesc &g, int k, int j, int i) {
@@ -850,7 +860,9 @@ __global__ __launch_bounds__(256) void k_corner_emf(GridDesc g, const double *Vc
   constexpr int D3 = C::DIM == 3;
   const int ni = g.end[0] + 1 - g.beg[0], nj = g.end[1] + 1 - g.beg[1];
   const int nk = g.end[2] + D3 - g.beg[2];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // edges in i, j, k order; the rows and planes below an edge are read by this block and
+  // the ones just before it, kept on one XCD
+  const long long t = (long long)xcd_contiguous_block() * blockDim.x + threadIdx.x;
   if (t >= (long long)ni * nj * nk) return;
   const int i = g.beg[0] + t % ni;
   const int j = g.beg[1] + (t / ni) % nj;
@@ -1026,7 +1038,8 @@ __global__ __launch_bounds__(256) void k_evolve_b(GridDesc g, double *Vs, double
   constexpr int D3 = C::DIM == 3;
   const int ni = g.end[0] + 1 - g.beg[0], nj = g.end[1] + 1 - g.beg[1];
   const int nk = g.end[2] + D3 - g.beg[2];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // faces in i, j, k order; the edges of the next row and plane stay on one XCD
+  const long long t = (long long)xcd_contiguous_block() * blockDim.x + threadIdx.x;
   if (t >= (long long)ni * nj * nk) return;
   const int i = g.beg[0] + t % ni;
   const int j = g.beg[1] + (t / ni) % nj;
@@ -1098,11 +1111,9 @@ __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, doub
   constexpr int TI = 63, TJ = 7;
   __shared__ T lez[3][8][64], lex[3][8][64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // work-groups go to the 8 XCDs in turn (block b on XCD b % 8): give each XCD a
-  // contiguous run of (tile, segment) indices, so that neighbouring tiles, which read each
-  // other's halo rows and columns, share an L2
-  const int nb = gridDim.x, xcd = blockIdx.x % 8, q8 = nb / 8, r8 = nb % 8;
-  const int blk = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + blockIdx.x / 8;
+  // each XCD takes a contiguous run of (tile, segment) indices, so that neighbouring
+  // tiles, which read each other's halo rows and columns, share an L2
+  const int blk = xcd_contiguous_block();
   const int seg = blk % nseg;
   const int tile = blk / nseg;
   const int ti = tile / ntj, tj = tile % ntj;

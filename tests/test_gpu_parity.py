@@ -55,6 +55,19 @@ def test_mhd3d_cycles(emf, recon, solver):
     compare_state(o, g, what="3D %s %s %s" % (solver, emf, recon))
 
 
+@pytest.mark.parametrize("emf", ["arithmetic", "uct0"])
+@pytest.mark.parametrize("nstages", [2, 3])
+def test_mhd3d_arithmetic_uct0_cycles(emf, nstages):
+    """The two corner-EMF averages the 3-D cases above leave out (calcCornerEmf.hpp:14-126),
+    RK2 and RK3."""
+    kw = cfg_kwargs(3, (16, 12, 10), emf=emf, recon="plm", cfl=0.4, nstages=nstages,
+                    solver="hlld")
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=5, amp=0.2))
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    compare_state(o, g, what="3D hlld %s RK%d" % (emf, nstages))
+
+
 @pytest.mark.parametrize("solver", ["hllc", "roe"])
 @pytest.mark.parametrize("recon", ["plm", "ppm"])
 def test_sod_cycles(solver, recon):
