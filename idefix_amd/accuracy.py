@@ -6,6 +6,10 @@ fields for the components held on faces) and the throughput of each run.
     python -m idefix_amd.accuracy --config ot --n 512 --tend 0.5
     python -m idefix_amd.accuracy --config turb3d --n 256 --tend 1.0
     python -m idefix_amd.accuracy --config kh --n 128 --tend 2.0
+
+Runs too long for one process pair (512^3 to t = 1) run one precision at a time and keep
+full-resolution planes of the final state (`--precision fp64 --save-planes a.npz`, then
+fp32 into b.npz); `--compare a.npz b.npz` gives the L1 difference over those planes.
 """
 import argparse
 import json
@@ -73,13 +77,60 @@ def l1(ga, gb):
     return float(np.mean(errs)), errs
 
 
+def sample_planes(g, nplanes=2):
+    """Full-resolution planes of the final interior cell state of a 3-D block (all
+    variables, the cell field being the face average): k at (q + 1/2) / nplanes of the
+    interior, q = 0 .. nplanes - 1.  Returns (k indices, array [nplanes, nvar, ny, nx])."""
+    b, e = g.beg, g.end
+    nz = e[2] - b[2]
+    ks = [b[2] + int((q + 0.5) * nz / nplanes) for q in range(nplanes)]
+    planes = np.stack([g.Vc[:, k, b[1]:e[1], b[0]:e[0]].cpu().numpy() for k in ks])
+    return ks, planes
+
+
+def compare_planes(fa, fb):
+    """L1 difference between two saved plane files: mean over variables of the mean
+    absolute difference over the sampled cells."""
+    a, b = np.load(fa), np.load(fb)
+    assert np.array_equal(a["k"], b["k"]) and a["planes"].shape == b["planes"].shape
+    assert abs(float(a["t"]) - float(b["t"])) < 1e-12
+    pa, pb = a["planes"], b["planes"]
+    per = [float(np.abs(pa[:, n] - pb[:, n]).mean()) for n in range(pa.shape[1])]
+    return {"L1": float(np.mean(per)), "L1_per_variable": per, "t": float(a["t"]),
+            "planes_k": [int(k) for k in a["k"]],
+            "a": {k: float(a[k]) for k in ("cycles", "wall_s", "zone_updates_per_s", "divB")},
+            "b": {k: float(b[k]) for k in ("cycles", "wall_s", "zone_updates_per_s", "divB")}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", choices=sorted(CONFIGS), default="ot")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--tend", type=float, default=0.5)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--precision", choices=("both", "fp64", "fp32"), default="both",
+                    help="one precision only: with --save-planes, keep planes of the result")
+    ap.add_argument("--save-planes", default=None,
+                    help="npz file for full-resolution planes of the final state (3-D)")
+    ap.add_argument("--compare", nargs=2, default=None, metavar=("A", "B"),
+                    help="L1 difference between two --save-planes files (no GPU)")
     a = ap.parse_args(argv)
+    if a.compare:
+        rep = compare_planes(*a.compare)
+        print(json.dumps(rep))
+        return
+    if a.precision != "both":
+        g, r = run(a.config, a.n, a.tend, a.precision)
+        r["divB"] = g.check_divb()
+        print(json.dumps({"config": a.config, "n": a.n, "tend": a.tend,
+                          "precision": a.precision, **r}), flush=True)
+        if a.save_planes:
+            ks, planes = sample_planes(g)
+            os.makedirs(os.path.dirname(a.save_planes) or ".", exist_ok=True)
+            np.savez(a.save_planes, k=np.array(ks), planes=planes, t=r["t"],
+                     cycles=r["cycles"], wall_s=r["wall_s"],
+                     zone_updates_per_s=r["zone_updates_per_s"], divB=r["divB"])
+        return
     g64, r64 = run(a.config, a.n, a.tend, "fp64")
     g32, r32 = run(a.config, a.n, a.tend, "fp32")
     err, per = l1(g32, g64)
