@@ -35,7 +35,9 @@ CONFIGS = {
 }
 
 
-def run(config, n, tend, precision, recon=None, params=None):
+def run(config, n, tend, precision, recon=None, params=None, perturb=0.0):
+    """perturb > 0 multiplies the initial cell state by 1 + perturb x (uniform in [-1, 1],
+    fixed seed): a control run that shows how a perturbation of that size grows."""
     import torch
     from .datablock import DataBlock
     from . import problems
@@ -44,6 +46,9 @@ def run(config, n, tend, precision, recon=None, params=None):
                   recon=recon or c["recon"], solver="hlld", emf=c["emf"], precision=precision,
                   nstages=c["nstages"], cfl=c["cfl"], xend=c["xend"])
     Vc, Vs = getattr(problems, c["ic"])(g)
+    if perturb > 0:
+        rng = np.random.default_rng(12345)
+        Vc = Vc * (1.0 + perturb * rng.uniform(-1.0, 1.0, Vc.shape))
     g.upload(Vc, Vs)
     if params is not None:
         g.set_mlp_params(params)
@@ -114,16 +119,18 @@ def main(argv=None):
                     help="npz file for full-resolution planes of the final state (3-D)")
     ap.add_argument("--compare", nargs=2, default=None, metavar=("A", "B"),
                     help="L1 difference between two --save-planes files (no GPU)")
+    ap.add_argument("--perturb", type=float, default=0.0,
+                    help="relative random perturbation of the initial cell state (control)")
     a = ap.parse_args(argv)
     if a.compare:
         rep = compare_planes(*a.compare)
         print(json.dumps(rep))
         return
     if a.precision != "both":
-        g, r = run(a.config, a.n, a.tend, a.precision)
+        g, r = run(a.config, a.n, a.tend, a.precision, perturb=a.perturb)
         r["divB"] = g.check_divb()
         print(json.dumps({"config": a.config, "n": a.n, "tend": a.tend,
-                          "precision": a.precision, **r}), flush=True)
+                          "precision": a.precision, "perturb": a.perturb, **r}), flush=True)
         if a.save_planes:
             ks, planes = sample_planes(g)
             os.makedirs(os.path.dirname(a.save_planes) or ".", exist_ok=True)
