@@ -19,6 +19,15 @@ Run (repository root, one GPU):
 A run at 512^3 is the largest a 288 GB device holds with the per-kernel arrays; a 1024^3
 reference for a 256^3 / 512^3 pair does not fit (about 45 words of FP64 state, fluxes and
 EMF scratch per cell: 1030^3 x 45 x 8 B = 393 GB).
+
+`--train` trains a network on 3-D data instead (the pipeline of idefix_amd.learned with
+x, y and z interfaces): plain FP64 runs at --fine^3 of the training seeds, block-averaged
+to --coarse^3 at --snapshots times, interface targets from the fine solution, FP32
+training on the GPU, a-posteriori calibration of the output scales on the training seeds,
+then the evaluation above on the held-out seed with the trained network.  Everything
+runs in one process, since a fine state cannot leave the GPU between calls:
+    python -u -m idefix_amd.learned3d --train --fine 256 --coarse 64 --seeds 1 2 \
+        --seed 4 --tend 0.5 --out DIR
 """
 import numpy as np
 
@@ -114,6 +123,152 @@ def run_coarse3(state, tend, recon, precision, params=None):
     return out, wall, ncyc
 
 
+def face_targets3(V, bx, by, bz, f):
+    """Interface values of the fine solution on the coarse faces of a periodic 3-D block,
+    per direction in face-frame order [8, N, N, N] (face index = coarse cell right of the
+    face): the mean over the coarse face of (V(left) + V(right)) / 2 across the fine faces
+    it covers, and the face field itself for the normal component."""
+    import torch
+    from .learned import FRAME
+    nz, ny, nx = V.shape[1:]
+    Nz, Ny, Nx = nz // f, ny // f, nx // f
+    left = torch.roll(V, 1, dims=3)[..., ::f]
+    tx = (0.5 * (left + V[..., ::f])).reshape(8, Nz, f, Ny, f, Nx).mean((2, 4))
+    tx[5] = bx[:, :, ::f][..., :Nx].reshape(Nz, f, Ny, f, Nx).mean((1, 3))
+    down = torch.roll(V, 1, dims=2)[:, :, ::f, :]
+    ty = (0.5 * (down + V[:, :, ::f, :])).reshape(8, Nz, f, Ny, Nx, f).mean((2, 5))
+    ty[6] = by[:, ::f, :][:, :Ny].reshape(Nz, f, Ny, Nx, f).mean((1, 4))
+    back = torch.roll(V, 1, dims=1)[:, ::f]
+    tz = (0.5 * (back + V[:, ::f])).reshape(8, Nz, Ny, f, Nx, f).mean((3, 5))
+    tz[7] = bz[::f][:Nz].reshape(Nz, Ny, f, Nx, f).mean((2, 4))
+    return tx[FRAME[0]], ty[FRAME[1]], tz[FRAME[2]]
+
+
+def samples3(Vc, targets, d, wp):
+    """Training rows of every coarse interface along d of a periodic 3-D state, as
+    idefix_amd.learned.samples: features (48) | PPM weights (16) | loss weights (8),
+    normalised targets and stencil scales (float32)."""
+    import torch
+    from .learned import FRAME
+    Vf = Vc[FRAME[d]]
+    ax = 3 - d
+    S = torch.stack([torch.roll(Vf, 3 - s, dims=ax) for s in range(6)], dim=-1)  # [8,N,N,N,6]
+    S = S.permute(1, 2, 3, 0, 4).reshape(-1, 8, 6)
+    t = targets.permute(1, 2, 3, 0).reshape(-1, 8)
+    c = 0.5 * (S[..., 2] + S[..., 3])
+    sc = (S - c[..., None]).abs().amax(-1)
+    safe = torch.where(sc > 0, sc, torch.ones_like(sc))
+    x = torch.where(sc[..., None] > 0, (S - c[..., None]) / safe[..., None], torch.zeros_like(S))
+    tau = torch.where(sc > 0, (t - c) / safe, torch.zeros_like(sc)).clamp(-1.0, 1.0)
+    w = wp.permute(1, 2, 3, 0).reshape(-1, 16).to(x.dtype)
+    rows = torch.cat([x.reshape(-1, 48), w, torch.ones_like(sc)], dim=1)
+    return rows.float(), tau.float()
+
+
+def ppm_face_weights3(state):
+    """PPM weights [16, N, N, N] of the x, y, z interfaces of a coarse periodic 3-D state,
+    from the inference kernel with no network loaded."""
+    g = block3(state[0].shape[-1], "mlp", "fp32")
+    load_state3(g, *state)
+    g.set_boundaries()
+    g.set_mlp_params(None)
+    b, e = g.beg, g.end
+    out = [g.mlp_faces(d)[:, b[2]:e[2], b[1]:e[1], b[0]:e[0]].double().clone() for d in range(3)]
+    del g
+    return out
+
+
+def fine_run3(n, seed, times, log):
+    """Plain FP64 run of the 3-D turbulence at n^3; interior states (torch copies) at t = 0
+    and at each of `times`."""
+    g = block3(n, "ppm", "fp64")
+    from . import problems
+    Vc, Vs = problems.turbulence(g, seed=seed)
+    g.upload(Vc, Vs)
+    g.set_boundaries()
+    out = [tuple(x.clone() for x in interior3(g))]
+    for t in times:
+        g.run(t)
+        out.append(tuple(x.clone() for x in interior3(g)))
+        log("  fine %d^3 seed %d t=%.3f cycles=%d" % (n, seed, g.t, g.ncycles))
+    del g
+    return out
+
+
+def calibrate3(params, calib, tend, log):
+    """idefix_amd.learned.calibrate with 3-D coarse runs: per-variable output scales chosen
+    by the L1 error of coarse learned runs against the training seeds' references."""
+    from .learned import scale_outputs
+
+    def err(alpha):
+        p = scale_outputs(params, alpha)
+        e = []
+        for init, ref in calib:
+            st, _, _ = run_coarse3(init, tend, "mlp", "fp32", p)
+            e.append(l1_error3(*st, *ref)[0])
+        return float(np.mean(e))
+    best_a, best_e = None, None
+    for c in (0.0, 0.15, 0.3, 0.5, 0.75, 1.0):
+        e = err(np.full(8, c))
+        log("calibration: common scale %.2f -> L1 %.5e" % (c, e))
+        if best_e is None or e < best_e:
+            best_a, best_e = np.full(8, c), e
+    for v in range(8):
+        for m in (0.0, 0.5, 1.5, 2.0):
+            trial = best_a.copy()
+            trial[v] = best_a[v] * m if best_a[v] > 0 else 0.3 * m
+            e = err(trial)
+            if e < best_e:
+                best_a, best_e = trial, e
+        log("calibration: variable %d scale %.3f -> L1 %.5e" % (v, best_a[v], best_e))
+    return best_a
+
+
+def train3(a, log):
+    """Data, training and calibration on 3-D turbulence; returns the calibrated parameters
+    and a report of the training."""
+    import torch
+    from .learned import Trainer, init_params, pack, unpack, scale_outputs
+    f = a.fine // a.coarse
+    times = [a.tend * (q + 1) / a.snapshots for q in range(a.snapshots)]
+    Xs, Ts, calib = [], [], []
+    for seed in a.seeds:
+        snaps = fine_run3(a.fine, seed, times, log)
+        calib.append((coarse_state3(*snaps[0], f), coarse_state3(*snaps[-1], f)))
+        for st in snaps:
+            cs = coarse_state3(*st, f)
+            tg = face_targets3(*st, f)
+            wps = ppm_face_weights3(cs)
+            for d in range(3):
+                X, T = samples3(cs[0], tg[d], d, wps[d])
+                Xs.append(X)
+                Ts.append(T)
+        del snaps
+        torch.cuda.empty_cache()
+    X, T = torch.cat(Xs), torch.cat(Ts)
+    del Xs, Ts
+    log("samples: %d" % X.shape[0])
+    p0 = unpack(init_params(0))
+    p0["W3"][:] = 0.0  # starts as PPM
+    tr = Trainer(a.batch, pack(p0))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1234)
+    hist = []
+    for step in range(a.steps):
+        idx = torch.randint(0, X.shape[0], (a.batch,), device="cuda", generator=gen)
+        lr = a.lr * (0.5 * (1.0 + np.cos(np.pi * step / a.steps)))
+        tr.step(X[idx].contiguous(), T[idx].contiguous(), max(lr, 1e-5))
+        if step % 1000 == 0 or step == a.steps - 1:
+            hist.append((step, tr.loss()))
+            log("step %d loss %.5f" % hist[-1])
+    params = tr.params()
+    del X, T
+    torch.cuda.empty_cache()
+    alpha = calibrate3(params, calib, a.tend, log)
+    return scale_outputs(params, alpha), {"loss_history": hist, "output_scales":
+                                           [float(v) for v in alpha], "train_seeds": a.seeds}
+
+
 def main(argv=None):
     import argparse
     import json
@@ -131,11 +286,24 @@ def main(argv=None):
     ap.add_argument("--log-every", type=int, default=200,
                     help="progress line every this many cycles of the fine run")
     ap.add_argument("--out", default="learned_out/turb3d")
+    ap.add_argument("--train", action="store_true",
+                    help="train and calibrate a network on 3-D data first (see above)")
+    ap.add_argument("--seeds", type=int, nargs="+", default=[1, 2])
+    ap.add_argument("--snapshots", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     t_start = time.perf_counter()
     log = lambda s: print("[%.0fs] %s" % (time.perf_counter() - t_start, s), flush=True)
-    params = np.load(a.params)
+    train_rep = None
+    if a.train:
+        assert a.seed not in a.seeds, "the evaluation seed must be held out"
+        params, train_rep = train3(a, log)
+        np.save(os.path.join(a.out, "params3d.npy"), params)
+    else:
+        params = np.load(a.params)
     N, f = a.coarse, a.fine // a.coarse
     assert N * f == a.fine and f >= 2
     # ---- reference --------------------------------------------------------------------
@@ -159,7 +327,8 @@ def main(argv=None):
     del g
     torch.cuda.empty_cache()
     rep = {"fine": a.fine, "coarse": N, "tend": a.tend, "seed": a.seed,
-           "params": os.path.basename(a.params),
+           "params": "trained here (3-D)" if a.train else os.path.basename(a.params),
+           "training": train_rep,
            "fine_run": {"wall_s": fine_wall, "cycles": fine_cyc}, "runs": {}}
     cases = [("learned_fp32_%d" % N, N, "mlp", "fp32", params),
              ("ppm_via_mlp_fp32_%d" % N, N, "mlp", "fp32", None),

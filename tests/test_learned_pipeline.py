@@ -125,3 +125,56 @@ def test_coarse_state3_conserves_and_keeps_div_b():
     bx_bad = torch.cat([bx_bad, bx_bad[..., :1]], dim=-1)
     div_bad = (bx_bad[..., 1:] - bx_bad[..., :-1]) + (byc[:, 1:] - byc[:, :-1]) + (bzc[1:] - bzc[:-1])
     assert float(div_bad.abs().max()) > 1e-3 * float(bxc.abs().max())
+
+
+def test_face_targets3_and_samples3_on_linear_data():
+    """3-D interface targets are exact for data linear along the face normal (away from
+    the periodic wrap), and the 3-D samples follow the 2-D specification: stencil
+    features of the face-frame variables and normalised targets."""
+    from idefix_amd import learned3d
+    n, f = 16, 4
+    N = n // f
+    x = (np.arange(n) + 0.5) / n
+    for d in range(3):
+        V = torch.zeros((8, n, n, n), dtype=torch.float64)
+        shape = [1, 1, 1]
+        shape[2 - d] = n
+        for q in range(8):
+            V[q] = torch.tensor((1.0 + 0.1 * q + 0.3 * x).reshape(shape)).expand(n, n, n)
+        bx = torch.zeros((n, n, n + 1), dtype=torch.float64)
+        by = torch.zeros((n, n + 1, n), dtype=torch.float64)
+        bz = torch.zeros((n + 1, n, n), dtype=torch.float64)
+        tg = learned3d.face_targets3(V, bx, by, bz, f)[d]
+        xf = np.arange(N) * f / n
+        frame = learned.FRAME[d]
+        normal_b = 5 + d
+        for qf in range(8):
+            q = frame[qf]
+            if q == normal_b:
+                continue
+            want = 1.0 + 0.1 * q + 0.3 * xf
+            got = np.moveaxis(tg[qf].numpy(), 2 - d, -1)[..., 1:]
+            assert np.allclose(got, np.broadcast_to(want[1:], got.shape), rtol=0, atol=1e-14)
+        # samples of a coarse state linear along d, with the exact face values as
+        # targets: features (j - 2.5) / 2.5, normalised targets 0
+        Nc = 8
+        Vc = torch.zeros((8, Nc, Nc, Nc), dtype=torch.float64)
+        xc = (np.arange(Nc) + 0.5) / Nc
+        shp = [1, 1, 1]
+        shp[2 - d] = Nc
+        for q in range(8):
+            Vc[q] = torch.tensor((1.0 + 0.1 * q + 0.3 * xc).reshape(shp)).expand(Nc, Nc, Nc)
+        face = [0, 0, 0]
+        face[2 - d] = 4  # stencil cells 1 .. 6, no periodic wrap
+        tgt = torch.zeros((8, Nc, Nc, Nc), dtype=torch.float64)
+        for qf, q in enumerate(learned.FRAME[d]):
+            tgt[qf] = 1.0 + 0.1 * q + 0.3 * (4.0 / Nc)
+        wp = torch.zeros((16, Nc, Nc, Nc), dtype=torch.float64)
+        rows, tau = learned3d.samples3(Vc, tgt, d, wp)
+        assert rows.shape == (Nc ** 3, 72) and tau.shape == (Nc ** 3, 8)
+        r = rows.reshape(Nc, Nc, Nc, 72)[tuple(face)]
+        feats = r[:48].reshape(8, 6).numpy()
+        want = (np.arange(6) - 2.5) / 2.5
+        assert np.allclose(feats, np.broadcast_to(want, (8, 6)), atol=1e-6)
+        assert np.allclose(r[64:72].numpy(), 1.0)
+        assert np.allclose(tau.reshape(Nc, Nc, Nc, 8)[tuple(face)].numpy(), 0.0, atol=1e-5)
