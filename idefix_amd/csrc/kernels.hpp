@@ -365,7 +365,8 @@ template <class C, int DIR, bool XTRA, bool FIN = false, typename T>
 __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs &a, int k,
                                               int j, int i, const T *Fm, const T *Fp,
                                               T cmaxm, T cmaxp,
-                                              const double *u0pre = nullptr) {
+                                              const double *u0pre = nullptr,
+                                              const double *upre = nullptr) {
   const unsigned p = (unsigned)idx3(g, k, j, i);
   const double dtdV = a.uniform ? a.dtdV : a.dt / cell_dV<C::DIM>(g, k, j, i);
   const double A = face_A<C::DIM, DIR>(g, k, j, i);
@@ -387,9 +388,10 @@ __device__ __forceinline__ double cell_update(const GridDesc &g, const SweepArgs
         if (!C::is_ct_field(n)) u[n] = ld32((const double *)a.Uc + n * g.sv, p);
     }
   } else {
+    // upre: the cell's accumulated state, loaded by the caller ahead of the Riemann solve
 #pragma unroll
     for (int n = 0; n < C::NVAR; n++)
-      if (!C::is_ct_field(n)) u[n] = ld32((const double *)a.Uc + n * g.sv, p);
+      if (!C::is_ct_field(n)) u[n] = upre ? upre[n] : ld32((const double *)a.Uc + n * g.sv, p);
   }
   // FIN: the stage's begin state (loaded by the caller ahead of the Riemann solve, or
   // here before any store of this cell)
@@ -667,6 +669,18 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
               u0pre[n] = ld32((const double *)a.Uc0 + n * g.sv, col + (unsigned)(f - 1) * sd);
         }
       }
+      // the other 3-D sweeps without per-cell options, except Roe's: the accumulated state
+      // of cell f - 1 (the register room this needs is there only in those)
+      double upre[NV];
+      constexpr bool PRE_U = FUSED && !FIN && !XTRA && C::DIM == 3 && C::SOLVER != SOLVER_ROE;
+      if constexpr (PRE_U) {
+        if (f > c0 && rhs_lane) {
+#pragma unroll
+          for (int n = 0; n < NV; n++)
+            if (!C::is_ct_field(n))
+              upre[n] = ld32((const double *)a.Uc + n * g.sv, col + (unsigned)(f - 1) * sd);
+        }
+      }
       T vL[NV], vR[NV], vr_next[NV];
       const bool flat = XTRA && recon_flat<C>(a, col + (unsigned)f * sd);
 #pragma unroll
@@ -709,8 +723,8 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
         if (f > c0 && rhs_lane) {
           const int cf = f - 1;
           const int ci = ic, cj = (DIR == JDIR) ? cf : jj, ck = (DIR == JDIR) ? kk : cf;
-          const double inv =
-              cell_update<C, DIR, XTRA, FIN>(g, a, ck, cj, ci, Fprev, F, cprev, cmax, u0pre);
+          const double inv = cell_update<C, DIR, XTRA, FIN>(g, a, ck, cj, ci, Fprev, F, cprev,
+                                                            cmax, u0pre, PRE_U ? upre : nullptr);
           const unsigned long long b = __double_as_longlong(inv);
           red = b > red ? b : red;
         }
