@@ -669,10 +669,12 @@ void k_sweep_march(GridDesc g, SweepArgs a, EmfArrays<T> e,
               u0pre[n] = ld32((const double *)a.Uc0 + n * g.sv, col + (unsigned)(f - 1) * sd);
         }
       }
-      // the other 3-D sweeps without per-cell options, except Roe's: the accumulated state
-      // of cell f - 1 (the register room this needs is there only in those)
+      // the 3-D sweeps without per-cell options, except Roe's and the final FP64 one: the
+      // accumulated state of cell f - 1 (the register room this needs is there only in
+      // those; the FP32 sweeps have it at their 3 waves per SIMD)
       double upre[NV];
-      constexpr bool PRE_U = FUSED && !FIN && !XTRA && C::DIM == 3 && C::SOLVER != SOLVER_ROE;
+      constexpr bool PRE_U = FUSED && !XTRA && C::DIM == 3 && C::SOLVER != SOLVER_ROE &&
+                             (!FIN || std::is_same<T, float>::value);
       if constexpr (PRE_U) {
         if (f > c0 && rhs_lane) {
 #pragma unroll
