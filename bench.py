@@ -176,7 +176,8 @@ WORKLOADS = [
 def workloads(only, steps, warmup):
     """Throughput of each configured workload at its full size: `warmup` untimed cycles,
     then `steps` timed cycles from the workload's initial state.  The learned mode uses
-    the network trained on the 2-D turbulence (idefix_amd/learned_turb2d.npy)."""
+    the network trained on data of the workload's dimension (idefix_amd/learned_turb2d.npy
+    for 1-D and 2-D, idefix_amd/learned_turb3d.npy for 3-D)."""
     import numpy as np
     import torch
     from idefix_amd.datablock import DataBlock
@@ -207,7 +208,8 @@ def workloads(only, steps, warmup):
                 ic = getattr(problems, w["ic"])(g)
             g.upload(*ic)
             if recon == "mlp":
-                g.set_mlp_params(np.load(os.path.join(ROOT, "idefix_amd", "learned_turb2d.npy")))
+                net = "learned_turb3d.npy" if w["dim"] == 3 else "learned_turb2d.npy"
+                g.set_mlp_params(np.load(os.path.join(ROOT, "idefix_amd", net)))
             setup_s = time.perf_counter() - t0
             el = time_cycles(g, steps, warmup)
             zu = cells * steps / el
