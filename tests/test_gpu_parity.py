@@ -621,3 +621,37 @@ def test_iso_cell_sound_speed(dim, nx, mhd, solver, recon, emf, flat):
     g.set_iso_sound_speed(None)
     _cycle_both(o, g, 2)
     compare_state(o, g, what="iso cs constant again %dD %s" % (dim, solver))
+
+
+@pytest.mark.parametrize("dim,nx,recon,emf,nstages,fusion", [
+    (3, (16, 12, 10), "ppm", "uct_contact", 3, None),  # ping-pong: k_prs_from_faces
+    (3, (16, 12, 10), "plm", "uct_contact", 3, 0),     # k_cons_to_prim
+    (2, (24, 20), "plm", "uct_contact", 3, None),      # fused y kernel
+    (2, (24, 20), "ppm", "uct_hlld", 2, None),
+    (1, (64,), "plm", "uct_contact", 2, None),
+])
+def test_pressure_floor(dim, nx, recon, emf, nstages, fusion):
+    """ConsToPrim's pressure floor (convertConsToPrim.hpp:40-51: p <= 0 -> SMALL_PRESSURE_FIX
+    = 1e-5, the re-synced energy local to the kernel) on every path that converts: a
+    magnetised random state with its pressure scaled by 1e-4 drives (E - kin - mag) below
+    zero in some cells of every cycle.  The oracle must floor cells, the GPU must floor the
+    same cells, and the whole state must match."""
+    comps = 2 if dim == 1 else 3
+    kw = cfg_kwargs(dim, nx, emf=emf, recon=recon, cfl=0.4, nstages=nstages, comps=comps,
+                    solver="hlld")
+
+    def ic(gg):
+        Vc, Vs = problems.random_mhd(gg, seed=5, amp=0.3)
+        Vc[comps + 1] *= 1e-4
+        return Vc, Vs
+
+    o, g = make_pair(kw, ic)
+    if fusion is not None:
+        g.set_fusion(fusion)
+    o.set_boundaries()
+    _cycle_both(o, g, 3)
+    p_o = o.interior(o.Vc)[comps + 1]
+    p_g = o.interior(g.download()[0])[comps + 1]
+    assert np.sum(p_o == 1e-5) > 0, "the state does not reach the floor"
+    assert np.array_equal(p_o == 1e-5, p_g == 1e-5)
+    compare_state(o, g, what="pressure floor %dD %s %s" % (dim, recon, emf))
