@@ -1,5 +1,6 @@
 """GPU checks of the configured workloads at small size and of the long runs, through the
-C ABI: the 3-D Kelvin-Helmholtz state, FP32 stencil arithmetic in 3-D against the FP64
+C ABI: the 3-D Kelvin-Helmholtz state (also one cycle at its full size, and one cycle
+on the 3-D turbulence workload's full 512^3 grid), FP32 stencil arithmetic in 3-D against the FP64
 oracle, the learned limiter in 3-D, the reference's linear-wave test, the Sod and
 Orszag-Tang runs to their end times, the 3-D and FP32 per-cell isothermal sound speed
 and the fused learned path against the per-kernel schedule.
@@ -42,6 +43,57 @@ def test_kelvin_helmholtz_3d_fp64():
     o.set_boundaries()
     _cycle_both(o, g, 4)
     compare_state(o, g, what="KH 3-D")
+
+
+def test_kelvin_helmholtz_full_size_cycle():
+    """The Kelvin-Helmholtz workload at its full size, 512 x 512 x 256 = 2^26 cells (PPM,
+    HLLD, uct_contact, RK2, CFL 0.4), through the schedule its timed runs take: the
+    ping-pong stages, the fused corner-EMF + face-update kernel that blocks of 2^26 cells
+    and more use, the pressure from the updated faces.  One cycle equals the oracle's to
+    1e-12 per cell (interior cells and faces), dt is the oracle's and div B stays at
+    round-off.  The oracle's cycle takes about half a minute on 16 host threads; the
+    host holds about 55 GB."""
+    import psutil
+    if psutil.virtual_memory().available < 80e9:
+        pytest.skip("needs about 55 GB of free host memory for the oracle at full size")
+    kw = cfg_kwargs(3, (512, 512, 256), emf="uct_contact", recon="ppm", cfl=0.4,
+                    nstages=2, xend=(1.0, 1.0, 0.5), comps=3)
+    o, g = make_pair(kw, _kh)
+    o.set_boundaries()
+    _cycle_both(o, g, 1)
+    assert g.ncycles == 1
+    compare_state(o, g, what="KH 512x512x256")
+    assert g.check_divb() < 1e-11
+
+
+def test_turbulence_grid_full_size_cycle():
+    """The 3-D turbulence workload's grid and schedule at full size, 512^3 (PPM, HLLD,
+    uct_contact, RK3, CFL 0.4: three-array ping-pong stages, the fused corner-EMF +
+    face-update kernel, the pressure from the updated faces), from a random magnetised
+    state (the seeded turbulence state is checked against the oracle at small sizes; its
+    512^3 synthesis would double the test's host time).  One RK3 cycle equals the
+    oracle's to 1e-12 per cell and dt is the oracle's.  The random face field is not
+    solenoidal, so the CT check is that every cell keeps its divergence to round-off.
+    About a minute of oracle time on 16 host threads; the host holds about 110 GB."""
+    import psutil
+    if psutil.virtual_memory().available < 150e9:
+        pytest.skip("needs about 110 GB of free host memory for the oracle at full size")
+    kw = cfg_kwargs(3, (512, 512, 512), emf="uct_contact", recon="ppm", cfl=0.4,
+                    nstages=3, comps=3)
+    o, g = make_pair(kw, lambda gg: problems.random_mhd(gg, seed=7, amp=0.2))
+    div0 = _face_div(o, o.Vs)
+    o.set_boundaries()
+    _cycle_both(o, g, 1)
+    compare_state(o, g, what="turbulence grid 512^3")
+    assert np.max(np.abs(_face_div(o, g.download()[1]) - div0)) < 1e-12
+
+
+def _face_div(o, Vs):
+    """dx x div B of every interior cell of a grid with equal spacings: the sum over the
+    directions of the face-field difference across the cell."""
+    from tests.pairs import interior_faces
+    f = interior_faces(o, Vs)
+    return np.diff(f[0], axis=2) + np.diff(f[1], axis=1) + np.diff(f[2], axis=0)
 
 
 @pytest.mark.parametrize("case", ["turbulence", "kelvin_helmholtz"])
