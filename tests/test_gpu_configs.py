@@ -88,6 +88,33 @@ def test_turbulence_grid_full_size_cycle():
     assert np.max(np.abs(_face_div(o, g.download()[1]) - div0)) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["ot2048", "turb2d2048", "turb3d256"])
+def test_workload_full_size_cycles(case):
+    """The other configured workloads at their full sizes from their own initial states:
+    2-D Orszag-Tang 2048^2 (PLM, HLLD, UCT-HLLD, RK2, CFL 0.6; the fused 2-D y kernel)
+    and 2-D turbulence 2048^2 (PPM, HLLD, uct_contact, RK3, CFL 0.4) for three cycles, 3-D
+    turbulence 256^3 (PPM, HLLD, uct_contact, RK3, CFL 0.4; ping-pong stages with the
+    corner-EMF and face-update kernels of blocks below 2^26 cells) for one: equal to the
+    oracle to 1e-12 per cell, dt equal every cycle, div B at round-off."""
+    if case == "ot2048":
+        kw = cfg_kwargs(2, (2048, 2048), emf="uct_hlld", recon="plm", cfl=0.6, nstages=2,
+                        comps=2)
+        ic, ncycles = problems.orszag_tang, 3
+    elif case == "turb2d2048":
+        kw = cfg_kwargs(2, (2048, 2048), emf="uct_contact", recon="ppm", cfl=0.4,
+                        nstages=3, comps=3)
+        ic, ncycles = problems.turbulence, 3
+    else:
+        kw = cfg_kwargs(3, (256, 256, 256), emf="uct_contact", recon="ppm", cfl=0.4,
+                        nstages=3, comps=3)
+        ic, ncycles = problems.turbulence, 1
+    o, g = make_pair(kw, ic)
+    o.set_boundaries()
+    _cycle_both(o, g, ncycles)
+    compare_state(o, g, what="%s full size" % case)
+    assert g.check_divb() < 1e-11
+
+
 def _face_div(o, Vs):
     """dx x div B of every interior cell of a grid with equal spacings: the sum over the
     directions of the face-field difference across the cell."""
