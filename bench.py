@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor peak
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, vendor peak
+# one interface through the 48 -> 64 -> 64 -> 16 network: 8192 multiply-adds
+MLP_FLOP_PER_FACE = 2 * (48 * 64 + 64 * 64 + 64 * 16)
 
 
 def parse():
@@ -524,6 +527,20 @@ def main():
                    "workload": "3-D turbulence %d^3, FP32 stencil + learned limiter (BF16 MFMA "
                                "inference, PPM fallback), HLLD, uct_contact, RK3, CFL 0.4" % nl,
                    "network": pname}
+        # MFMA rate of the inference kernel (region timer, HIP events on its stream): one
+        # launch per direction covers the faces beg..end along it and every transverse
+        # index, ghosts included
+        kl, _ = kernel_times(gl, 3)
+        if "mlp" in kl:
+            faces = (nl + 1) * gl.np_tot[1] * gl.np_tot[2]
+            flop = MLP_FLOP_PER_FACE * faces
+            tf = flop / (kl["mlp"] * 1e-3) / 1e12
+            learned["mlp_roofline"] = {
+                "bound": "mfma", "kernel": "k_mlp_infer", "achieved": tf,
+                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / BF16_PEAK_TFLOPS,
+                "flop_per_launch": flop, "interfaces_per_launch": faces,
+                "ms_per_launch": kl["mlp"]}
+            learned["kernel_ms"] = kl
         del gl
         torch.cuda.empty_cache()
     headline = None if args.no_headline else reference_headline(args.steps, args.warmup)

@@ -41,7 +41,7 @@ USERDEF_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
 
 REGIONS = ["boundaries", "sweep_x", "sweep_y", "sweep_z", "corner_emf", "evolve_b",
-           "cons_to_prim", "fused2d_y", "ct_fused"]
+           "cons_to_prim", "fused2d_y", "ct_fused", "mlp"]
 
 
 class IdfxDesc(ctypes.Structure):

@@ -244,12 +244,8 @@ SweepArgs sweep_args(idfx_ctx *c, int dir, double dt, const double *Vc, const do
 }
 
 // Learned limiter: interface weights of direction dir from Vc into the context's array
-// (or W when given), before a sweep of that direction.
-int mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, hipStream_t s) {
-  if (!c->mlp) return IDFX_OK;
-  return c->ops->mlp_infer(dir, c->g, Vc, c->net, W ? W : c->mlpW[dir], s) == hipSuccess
-             ? IDFX_OK : IDFX_E_LAUNCH;
-}
+// (or W when given), before a sweep of that direction; its own profiler region.
+int mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, hipStream_t s);
 
 bool needs_unfused(const idfx_ctx *c) {
   // the 1-D Roe flux keeps the previous flux at strong shocks (roeHD.hpp:315-328)
@@ -333,6 +329,13 @@ struct Region {
     if (slot >= 0) hipEventRecord(c->ev_stop[slot], s);
   }
 };
+
+int mlp_faces(idfx_ctx *c, int dir, const double *Vc, float *W, hipStream_t s) {
+  if (!c->mlp) return IDFX_OK;
+  Region r(c, IDFX_REGION_MLP, s);
+  return c->ops->mlp_infer(dir, c->g, Vc, c->net, W ? W : c->mlpW[dir], s) == hipSuccess
+             ? IDFX_OK : IDFX_E_LAUNCH;
+}
 
 }  // namespace
 
@@ -764,6 +767,8 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
       flags |= (dir == 0) ? SW_INVDT_SET : SW_INVDT_ADD;
       if (dir == dim - 1 && want_dt) flags |= SW_DT_REDUCE;
     }
+    // the learned limiter's weights first, outside the sweep's region
+    if (!needs_unfused(c)) IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
     Region r(c, IDFX_REGION_SWEEP_X + dir, s);
     if (needs_unfused(c)) {
       // reference sequence: PrimToCons, (begin copy), CalcFlux, CalcRightHandSide
@@ -788,7 +793,6 @@ static int stage_enqueue(idfx_ctx *c, int stage, int nstages, double dt, double 
       a.VcOut = VcOut;
       a.wc = wc;
       a.w0 = w0;
-      IDFX_TRY(mlp_faces(c, dir, Vc, nullptr, s));
       IDFX_TRY(check(c->ops->sweep(dir, true, c->g, a, c->emf(), c->target_waves, s)));
     }
   }

@@ -256,7 +256,8 @@ enum {
   IDFX_REGION_CONS_TO_PRIM = 6, /* AddAndStore of Uc + ConvertConsToPrim */
   IDFX_REGION_FUSED2D_Y = 7,    /* fused 2-D second half (fusion level 1) */
   IDFX_REGION_CT_FUSED = 8,     /* fused 3-D CalcCornerEMF + EvolveMagField (fusion level 1) */
-  IDFX_NREGIONS = 9
+  IDFX_REGION_MLP = 9,          /* learned limiter: interface weights of one direction */
+  IDFX_NREGIONS = 10
 };
 
 /* idfx::pushRegion / popRegion with the profiler enabled (global.cpp:66-95, profiler.cpp):
