@@ -402,3 +402,18 @@ def test_fused_3d_region_is_used():
     assert "ct_fused" in rt and "corner_emf" not in rt and "evolve_b" not in rt
     with pytest.raises(Exception):
         g.set_fusion(3)
+
+
+def test_learned_inference_has_its_own_region():
+    """The region profiler times the learned limiter's inference apart from the sweeps:
+    one launch per direction and stage in the 3-D learned mode (RK3: 9 per cycle), none in
+    the plain mode."""
+    g = _mlp3d("mlp", "fp32")
+    g.set_profiling(True)
+    g.cycle()
+    rt = g.region_times()
+    assert rt["mlp"][1] == 9 and rt["sweep_x"][1] == 3 and rt["mlp"][0] > 0.0
+    p = _mlp3d("ppm")
+    p.set_profiling(True)
+    p.cycle()
+    assert "mlp" not in p.region_times()
