@@ -1117,6 +1117,14 @@ __device__ __forceinline__ void evolve_face(double *Vs, double *Vs0, unsigned q,
   if (mode == 1) st32(Vs0, q, old);
   st32(Vs, q, (mode == 2) ? wc * upd + w0 * ld32(Vs0, q) : upd);
 }
+// The same with the face's old value (and, in mode 2, its RK begin value) loaded earlier.
+__device__ __forceinline__ void evolve_face_pre(double *Vs, double *Vs0, unsigned q, double old,
+                                                double old0, double rhs, int mode, double wc,
+                                                double w0) {
+  const double upd = old + rhs;
+  if (mode == 1) st32(Vs0, q, old);
+  st32(Vs, q, (mode == 2) ? wc * upd + w0 * old0 : upd);
+}
 
 template <class C, typename T>
 __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, double *Vs,
@@ -1157,7 +1165,34 @@ __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, doub
     eyi_m = ld32(e.eyi, p);
   }
   T pex = 0, pey = 0, pez = 0;  // edges of the previous plane, own column
+  // dt / dx of the lane's column and row (clamped indices: lanes past the block's end
+  // compute no face)
+  const double rdx = dt / g.dx[0][min(i, g.nt[0] - 1)], rdy = dt / g.dx[1][min(j, g.nt[1] - 1)];
+  const double mrdx = -dt / g.dx[0][min(i, g.nt[0] - 1)];
+  const bool own = l < TI && w < TJ;
   for (int k = ks; k <= klast; k++) {
+    // the faces of plane k - 1 this lane updates in this step: their old values (and RK
+    // begin values) are loaded before the edge work and the barrier; only this lane
+    // writes them, later in this step
+    const int kk = k - 1;
+    const unsigned q = (unsigned)kk * vz + (unsigned)j * vy + (unsigned)i;
+    const bool u1 = k > ks && own && i <= g.end[0] && j < g.end[1] && kk < g.end[2];
+    const bool u2 = k > ks && own && i < g.end[0] && j <= g.end[1] && kk < g.end[2];
+    const bool u3 = k > ks && own && i < g.end[0] && j < g.end[1];
+    double o1 = 0.0, o2 = 0.0, o3 = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0;
+    if (u1) {
+      o1 = ld32(Vs, q);
+      if (mode == 2) z1 = ld32(Vs0, q);
+    }
+    if (u2) {
+      o2 = ld32(Vs1, q);
+      if (mode == 2) z2 = ld32(Vs01, q);
+    }
+    if (u3) {
+      o3 = ld32(Vs2, q);
+      if (mode == 2) z3 = ld32(Vs02, q);
+    }
+    const double rdz = (k > ks && kk < g.end[2]) ? dt / g.dx[2][kk] : 0.0;
     T cex = 0, cey = 0, cez = 0;
     if (edge_ok) {
       const unsigned p = (unsigned)idx3(g, k, j, i);
@@ -1207,27 +1242,23 @@ __global__ __launch_bounds__(512) void k_ct3d(GridDesc g, const double *Vc, doub
     __syncthreads();
     if (k > ks) {
       // faces of plane kk = k - 1 from the edges of planes kk (p*) and k (c*)
-      const int kk = k - 1;
       const T ez_ip = __shfl_down(pez, 1);
       const T ey_ip = __shfl_down(pey, 1);
       const T ez_jp = lez[kk % 3][w < 7 ? w + 1 : w][l];
       const T ex_jp = lex[kk % 3][w < 7 ? w + 1 : w][l];
-      const bool own = l < TI && w < TJ;
-      const unsigned q = (unsigned)kk * vz + (unsigned)j * vy + (unsigned)i;
-      if (own && i <= g.end[0] && j < g.end[1] && kk < g.end[2]) {  // BX1s
-        double r = 0.0 - dt / g.dx[1][j] * ((double)ez_jp - (double)pez);
-        r = r + dt / g.dx[2][kk] * ((double)cey - (double)pey);
-        evolve_face(Vs, Vs0, q, r, mode, wc, w0);
+      if (u1) {  // BX1s
+        double r = 0.0 - rdy * ((double)ez_jp - (double)pez);
+        r = r + rdz * ((double)cey - (double)pey);
+        evolve_face_pre(Vs, Vs0, q, o1, z1, r, mode, wc, w0);
       }
-      if (own && i < g.end[0] && j <= g.end[1] && kk < g.end[2]) {  // BX2s
-        double r = dt / g.dx[0][i] * ((double)ez_ip - (double)pez);
-        r = r - dt / g.dx[2][kk] * ((double)cex - (double)pex);
-        evolve_face(Vs1, Vs01, q, r, mode, wc, w0);
+      if (u2) {  // BX2s
+        double r = rdx * ((double)ez_ip - (double)pez);
+        r = r - rdz * ((double)cex - (double)pex);
+        evolve_face_pre(Vs1, Vs01, q, o2, z2, r, mode, wc, w0);
       }
-      if (own && i < g.end[0] && j < g.end[1]) {  // BX3s at face plane kk
-        const double r = -dt / g.dx[0][i] * ((double)ey_ip - (double)pey) +
-                         dt / g.dx[1][j] * ((double)ex_jp - (double)pex);
-        evolve_face(Vs2, Vs02, q, r, mode, wc, w0);
+      if (u3) {  // BX3s at face plane kk
+        const double r = mrdx * ((double)ey_ip - (double)pey) + rdy * ((double)ex_jp - (double)pex);
+        evolve_face_pre(Vs2, Vs02, q, o3, z3, r, mode, wc, w0);
       }
     }
     pex = cex;
