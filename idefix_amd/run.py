@@ -1,15 +1,37 @@
 """Command-line driver with the reference's main loop (src/main.cpp:120-260):
 
-    python -m idefix_amd.run -i idefix.ini --setup orszag_tang [-restart [n]]
+    python -m idefix_amd.run -i idefix.ini --setup orszag_tang|path/setup.py [-restart [n]]
                              [-maxcycles N] [-nowrite] [--recon plm|ppm] [--dir DIR]
 
 Reads the input file (Input, src/input.cpp), builds the DataBlock, creates the initial
-state with the named setup (the reference compiles one setup.cpp per test; here the
-setups are functions of idefix_amd.problems) or restarts from a dump, then cycles to
-tstop with the dt clipping of main.cpp:187 and calls the output cadence after every
-cycle.  Options keep the reference's spelling (-i, -restart, -maxcycles, -nowrite).
+state with the setup or restarts from a dump, then cycles to tstop with the dt clipping
+of main.cpp:187 and calls the output cadence after every cycle.  Options keep the
+reference's spelling (-i, -restart, -maxcycles, -nowrite).
+
+The setup is a built-in one (functions of idefix_amd.problems reproducing reference
+tests) or a user's setup file, the counterpart of the setup.cpp the reference compiles
+with each problem (src/setup.hpp:23-30):
+
+    MHD = True          # definitions.hpp: MHD or hydro (default True)
+    COMPONENTS = 3      # definitions.hpp: COMPONENTS (default: the grid's dimensions)
+
+    def setup(data, inp):
+        # Setup::Setup: enroll functions, e.g. data.set_userdef_boundary(fn) for
+        # "userdef" sides (fn(dir, side, t, Vc, Vs) fills their ghost zones) and
+        # data.set_iso_sound_speed(cs) for "csiso userdef"; inp holds the input file's
+        # blocks ({block: {entry: [values]}})
+        ...
+
+    def init_flow(data):
+        # Setup::InitFlow: return (Vc, Vs) as numpy arrays of the block's shapes
+        ...
+
+setup() is optional; init_flow() is required.  It runs once per run, before the
+initial state or the restart, as the reference constructs Setup before both.
 """
 import argparse
+import importlib.util
+import os
 import sys
 import time
 
@@ -30,16 +52,42 @@ SETUPS = {
 }
 
 
+def load_setup(name):
+    """(init_flow, mhd, COMPONENTS or None, setup or None) of a built-in setup name or of
+    a user's setup file (*.py)."""
+    if name in SETUPS:
+        fn, mhd, comps = SETUPS[name]
+        return fn, mhd, comps, None
+    if name.endswith(".py"):
+        if not os.path.isfile(name):
+            raise FileNotFoundError("setup file %s not found" % name)
+        spec = importlib.util.spec_from_file_location("idefix_user_setup", name)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        init = getattr(mod, "init_flow", None)
+        if not callable(init):
+            raise ValueError("setup file %s defines no init_flow(data)" % name)
+        enroll = getattr(mod, "setup", None)
+        if enroll is not None and not callable(enroll):
+            raise ValueError("setup file %s: setup must be a function setup(data, inp)" % name)
+        comps = getattr(mod, "COMPONENTS", None)
+        return init, bool(getattr(mod, "MHD", True)), None if comps is None else int(comps), enroll
+    raise ValueError("unknown setup %r: one of %s, or a setup file *.py"
+                     % (name, ", ".join(sorted(SETUPS))))
+
+
 def build(blocks, setup, recon="plm", precision="fp64", comps=None):
     from .datablock import DataBlock
-    fn, mhd, comps0 = SETUPS[setup]
+    fn, mhd, comps0, enroll = load_setup(setup)
     comps = comps if comps is not None else comps0
     kw, run = ini.block_config(blocks, recon=recon, mhd=mhd, precision=precision, comps=comps)
-    if run.get("csiso_userdef"):
-        # none of the setups here enrolls a sound-speed function
-        raise NotImplementedError("csiso userdef: setup %r provides no per-cell sound speed "
-                                  "(DataBlock.set_iso_sound_speed)" % setup)
     g = DataBlock(**kw)
+    if enroll is not None:
+        enroll(g, blocks)
+    if run.get("csiso_userdef") and getattr(g, "_cs", None) is None:
+        # eos_isothermal.hpp:41-45: "userdef" without an enrolled sound speed
+        raise ValueError("csiso userdef: no user-defined isothermal sound speed has been "
+                         "enrolled (the setup calls data.set_iso_sound_speed)")
     periodic = [all(b == "periodic" for b in kw["bc"][d]) if d < kw["dim"] else True
                 for d in range(3)]
     return g, fn, run, periodic
@@ -53,7 +101,8 @@ def simulate(argv=None):
     """The main loop; returns (return code, DataBlock, Output)."""
     ap = argparse.ArgumentParser(prog="idefix_amd.run")
     ap.add_argument("-i", dest="ini", default="idefix.ini")
-    ap.add_argument("--setup", required=True, choices=sorted(SETUPS))
+    ap.add_argument("--setup", required=True,
+                    help="a built-in setup (%s) or a setup file *.py" % ", ".join(sorted(SETUPS)))
     ap.add_argument("--recon", default="plm", choices=["constant", "plm", "ppm"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--comps", type=int, default=None)

@@ -95,3 +95,94 @@ def test_driver_sod_hllc(tmp_path):
     Vc, _ = g.download()
     rho = Vc[0, 0, 0, g.beg[0]:g.end[0]]
     assert rho.min() > 0.12 and rho.max() < 1.0 + 1e-12
+
+
+# A user's setup file (the counterpart of a problem's setup.cpp): isothermal hydro with
+# "csiso userdef" and an enrolled inflow boundary on the left side.
+USER_SETUP = '''
+import numpy as np
+
+MHD = False
+
+
+def sound_speed(data):
+    nx, ny, nz = data.np_tot
+    x = data.x[0][None, None, :]
+    return (1.0 + 0.3 * np.sin(2.0 * np.pi * x)) * np.ones((nz, ny, nx))
+
+
+def inflow(d, side, t, Vc, Vs, ng):
+    if d == 0 and side == 0:
+        Vc[0, :, :, :ng] = 1.0
+        Vc[1, :, :, :ng] = 0.5
+
+
+def setup(data, inp):
+    ng = data.beg[0]
+    data.set_iso_sound_speed(sound_speed(data))
+    data.set_userdef_boundary(lambda d, side, t, Vc, Vs: inflow(d, side, t, Vc, Vs, ng))
+
+
+def init_flow(data):
+    nx, ny, nz = data.np_tot
+    x = data.x[0][None, None, :]
+    Vc = np.zeros((data.nvar, nz, ny, nx))
+    Vc[0] = np.where(x < 0.5, 1.0, 0.125) * np.ones((nz, ny, nx))
+    return Vc, None
+'''
+
+USER_INI = """
+[Grid]
+X1-grid    1  0.0  400  u  1.0
+[TimeIntegrator]
+CFL         0.8
+tstop       0.1
+first_dt    1.e-4
+nstages     2
+[Hydro]
+solver    hll
+csiso     userdef
+[Boundary]
+X1-beg    userdef
+X1-end    outflow
+"""
+
+
+def test_driver_user_setup_file(tmp_path):
+    """A user's setup file through the driver: its setup() enrolls a per-cell isothermal
+    sound speed ("csiso userdef") and a user-defined inflow side, its init_flow() sets the
+    state; the run to tstop equals the oracle's with the same functions enrolled, to 1e-12
+    per cell, cycle for cycle."""
+    import importlib.util
+    from types import SimpleNamespace
+    from tests.pairs import cfg_kwargs, compare_state, make_oracle
+    from oracle.oracle import OracleGrid
+    d = str(tmp_path)
+    setup_path = os.path.join(d, "setup.py")
+    with open(setup_path, "w") as fh:
+        fh.write(USER_SETUP)
+    ini_path = os.path.join(d, "idefix.ini")
+    with open(ini_path, "w") as fh:
+        fh.write(USER_INI)
+    rc, g, out = simulate(["-i", ini_path, "--setup", setup_path, "--dir", d, "-nowrite"])
+    assert rc == 0 and g.t == pytest.approx(0.1, abs=1e-15)
+
+    spec = importlib.util.spec_from_file_location("user_setup_for_oracle", setup_path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    kw = cfg_kwargs(1, (400,), mhd=False, solver="hll", recon="plm", cfl=0.8, nstages=2,
+                    bc=[("userdef", "outflow")], csiso=1.0)
+    o = make_oracle(kw)
+    og = OracleGrid(o)
+    data = SimpleNamespace(np_tot=o.np_tot, x=og.x, nvar=o.nvar, beg=o.beg)
+    o.set_iso_sound_speed(mod.sound_speed(data))
+    o.set_userdef(lambda dd, side, t: mod.inflow(dd, side, t, o.Vc, None, o.beg[0]))
+    Vc, _ = mod.init_flow(data)
+    o.Vc[...] = Vc
+    o.set_boundaries()
+    while o.t < 0.1:
+        if 0.1 - o.t < o.dt:
+            o.dt = 0.1 - o.t
+        assert o.cycle() == 0
+    assert o.ncycles == g.ncycles and g.t == o.t
+    compare_state(o, g, what="user setup file")

@@ -200,3 +200,30 @@ def test_vtk_snapshot_numbering_and_time(tmp_path):
     assert meta["TIME"] == pytest.approx(0.5)
     assert sorted(fields) == ["BX1", "BX2", "PRS", "RHO", "VX1", "VX2"]
     assert output.INT == 2
+
+
+def test_setup_loader(tmp_path):
+    """--setup takes a built-in setup or a user's setup file: init_flow() is required,
+    setup() optional, MHD and COMPONENTS mirror definitions.hpp."""
+    from idefix_amd.run import SETUPS, load_setup
+    fn, mhd, comps, enroll = load_setup("brio_wu")
+    assert fn is SETUPS["brio_wu"][0] and mhd and comps == 2 and enroll is None
+    good = tmp_path / "good.py"
+    good.write_text("MHD = False\nCOMPONENTS = 1\n"
+                    "def setup(data, inp):\n    pass\n"
+                    "def init_flow(data):\n    return None, None\n")
+    fn, mhd, comps, enroll = load_setup(str(good))
+    assert fn.__name__ == "init_flow" and mhd is False and comps == 1
+    assert enroll.__name__ == "setup"
+    bare = tmp_path / "bare.py"
+    bare.write_text("def init_flow(data):\n    return None, None\n")
+    fn, mhd, comps, enroll = load_setup(str(bare))
+    assert mhd is True and comps is None and enroll is None
+    bad = tmp_path / "bad.py"
+    bad.write_text("X = 1\n")
+    with pytest.raises(ValueError, match="init_flow"):
+        load_setup(str(bad))
+    with pytest.raises(FileNotFoundError):
+        load_setup(str(tmp_path / "missing.py"))
+    with pytest.raises(ValueError, match="unknown setup"):
+        load_setup("no_such_setup")
