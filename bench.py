@@ -372,13 +372,30 @@ def traffic_child(cache, n, region, cycles):
     print("traffic child done", region, flush=True)
 
 
+def kernel_counter_average(path, counter, region):
+    """Average per dispatch of `counter` over the FP64 dispatches of the kernel of
+    `region` in a rocprofv3 counter_collection.csv (rows of one dispatch are summed:
+    rocprofv3 may report a counter in several rows per dispatch); None if none."""
+    import csv
+    prefix, sub = SINGLE_KERNEL_REGIONS[region]
+    per = {}
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            name = row["Kernel_Name"]
+            if row["Counter_Name"] != counter or prefix not in name or sub not in name:
+                continue
+            if "double" not in name:
+                continue
+            per[row["Dispatch_Id"]] = per.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+    return sum(per.values()) / len(per) if per else None
+
+
 def measure_traffic(cache, n, region, timeout=240):
     """HBM bytes per launch of the kernel of `region`, from two rocprofv3 --pmc runs of
     traffic_child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the
     kernel's dispatches), scaled by the plain-copy calibration, and its VALU instructions
     per launch from a third run (SQ_INSTS_VALU).  (None, reason, None) if rocprofv3 is
     missing or a run fails."""
-    import csv
     import shutil
     import subprocess
     import tempfile
@@ -400,19 +417,12 @@ def measure_traffic(cache, n, region, timeout=240):
         if not files:
             shutil.rmtree(d, ignore_errors=True)
             return None, "rocprofv3 --pmc %s failed (rc %d)" % (counter, r.returncode), None
-        per = {}
-        prefix, sub = SINGLE_KERNEL_REGIONS[region]
-        for row in csv.DictReader(open(files[0])):
-            name = row["Kernel_Name"]
-            if row["Counter_Name"] != counter or prefix not in name or sub not in name:
-                continue
-            if "double" not in name:
-                continue
-            per[row["Dispatch_Id"]] = per.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+        avg = kernel_counter_average(files[0], counter, region)
         shutil.rmtree(d, ignore_errors=True)
-        if not per:
+        if avg is None:
+            prefix, sub = SINGLE_KERNEL_REGIONS[region]
             return None, "no %s dispatch of %s in the profile" % (counter, prefix + sub), None
-        out[counter] = sum(per.values()) / len(per)
+        out[counter] = avg
     return (out["FETCH_SIZE"] * 1024.0 * FETCH_SCALE + out["WRITE_SIZE"] * 1024.0 * WRITE_SCALE,
             {"fetch_bytes": out["FETCH_SIZE"] * 1024.0 * FETCH_SCALE,
              "write_bytes": out["WRITE_SIZE"] * 1024.0 * WRITE_SCALE,
