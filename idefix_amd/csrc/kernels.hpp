@@ -1092,7 +1092,9 @@ __global__ __launch_bounds__(256) void k_evolve_b(GridDesc g, double *Vs, double
 // buffers, one barrier per plane), and the face data and cell EMFs of plane k - 1 are
 // the previous step's plane k.  After plane k, the faces of plane k - 1 are updated with
 // the arithmetic of k_evolve_b (mode / wc / w0 as there), in place: every face has one
-// owner and no other block reads Vs.  Only faces the next stage reads are updated: each
+// owner and no other block reads Vs.  A lane loads the old values (and RK begin values)
+// of the faces it updates in a step at the top of that step, so their latency overlaps
+// the edge work and the barrier.  Only faces the next stage reads are updated: each
 // face field over [beg, end] along its own direction and [beg, end) across; the edge
 // arrays ex, ey, ez are not written.
 // Cell EMFs of one cell from 32-bit offsets (the arithmetic of cell_emf).
