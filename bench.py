@@ -465,8 +465,8 @@ def cpu_baseline_turb3d(n, seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box "
-              "here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; run with "
+              "--gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()

@@ -7,7 +7,7 @@
 // outflow / reflective boundaries and the RK driver.  It mirrors the reference's kernel
 // sequence one loop per reference kernel, so that every intermediate array can be
 // compared with the GPU path.  Only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg use it, always as the checker.
+// cpu_baseline leg use it, as the results the GPU path is compared with.
 //
 // Runtime configuration replaces the reference's compile-time macros: `dim` is
 // DIMENSIONS, `comps` is COMPONENTS, `order` is ORDER.  A term that the reference

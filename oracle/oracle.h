@@ -2,7 +2,8 @@
  * CPU oracle for the per-stage finite-volume update (TEST INFRASTRUCTURE ONLY).
  *
  * This is a plain CPU restatement of the reference's algorithm, used only by
- * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the reference
+ * results the GPU path is compared with (and, in the bench, as the CPU baseline).
  * It is never linked into, loaded by or called from the product library.
  *
  * Every function cites the reference file:line it restates (paths relative to the

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
-always as the checker of the GPU path.  It wraps oracle/liboracle.so (built by
+as the results the GPU path is compared with.  It wraps oracle/liboracle.so (built by
 oracle/Makefile) and exposes the solver arrays as numpy views in the reference's
 LayoutRight shapes: Vc/Uc [nvar][nz][ny][nx], Vs [dim][nz+KOFF][ny+JOFF][nx+IOFF].
 """
