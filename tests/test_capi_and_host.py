@@ -25,6 +25,17 @@ def test_header_declares_the_binding_list():
     assert declared_entry_points() == sorted(capi.EXPORTS)
 
 
+def test_region_names_follow_the_header_enum():
+    """capi.REGIONS[q] names region q of the header's IDFX_REGION_* enum, and there are
+    IDFX_NREGIONS of them (DataBlock.region_times reads them by index)."""
+    txt = open(os.path.join(ROOT, "include", "idefix_amd.h")).read()
+    enum = {name.lower(): int(v) for name, v in re.findall(r"IDFX_REGION_(\w+)\s*=\s*(\d+)", txt)}
+    n = int(re.search(r"IDFX_NREGIONS\s*=\s*(\d+)", txt).group(1))
+    assert len(capi.REGIONS) == n == len(enum)
+    for q, name in enumerate(capi.REGIONS):
+        assert enum[name] == q, (name, q)
+
+
 def test_library_exports_every_declared_symbol():
     L = capi.load()
     missing = [n for n in declared_entry_points() if not hasattr(L, n)]
